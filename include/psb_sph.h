@@ -1,0 +1,365 @@
+/*
+ * psb_sph.h -- C-ABI of the MI355X (gfx950) SPH / PCISPH timestep.
+ *
+ * Drop-in boundary for the CUDA launch-wrapper layer of PSB_Project
+ * (the `extern "C"` functions declared in src/common.cuh, UniformGrid.cuh,
+ * SphSystem.cuh, PciSphSystem.cuh, BoxCollision.cuh, ForceExt.cuh and
+ * CudaSystem.cuh).  Every function here names the reference wrapper it
+ * replaces.  Differences that are deliberate:
+ *
+ *   - every function returns a psb_status instead of void, and nothing is
+ *     left unchecked (the reference never calls cudaGetLastError);
+ *   - device arrays are FP32.  A "vec4" argument is a device array of
+ *     n float4 records (x, y, z, w; 16-byte stride, w unused on input
+ *     unless stated) replacing the reference's FP64 double3 AoS arrays
+ *     (24-byte stride); a scalar array is n floats;
+ *   - all work is ordered on the context's own HIP stream (default
+ *     priority) rather than the legacy default stream; functions that hand
+ *     results to the host synchronise that stream themselves;
+ *   - the per-particle material arrays of the reference (mass, radius, k,
+ *     restDensity, viscosity, threshold, surfaceTension) are uniform per
+ *     system and passed once in psb_sph_params;
+ *   - capacity, grid domain and neighbour-list width are arguments instead of
+ *     the constants 65,536 / 1 m^3 / 200 / 64 of the reference.
+ *
+ * No torch or HIP types appear in these signatures; a stream is exposed
+ * only as an opaque pointer (psb_context_stream).
+ */
+#ifndef PSB_SPH_H
+#define PSB_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes and per-step flags                                           */
+/* ------------------------------------------------------------------------ */
+typedef int32_t psb_status;
+#define PSB_OK                    0
+#define PSB_ERR_INVALID_ARGUMENT  1
+#define PSB_ERR_HIP               2  /* a HIP runtime call failed          */
+#define PSB_ERR_CAPACITY          3  /* n larger than the allocated capacity */
+#define PSB_ERR_NO_DEVICE         4
+#define PSB_ERR_OUT_OF_MEMORY     5
+#define PSB_ERR_STATE             6  /* call order (e.g. step before set)   */
+
+/* Bits of psb_step_stats.flags, raised by the kernels on the device.       */
+#define PSB_FLAG_NEIGHBOR_OVERFLOW  0x1u  /* > max_neighbors within h; list truncated */
+#define PSB_FLAG_OUT_OF_GRID        0x2u  /* a particle outside the grid domain (clamped to an edge cell) */
+#define PSB_FLAG_NONFINITE          0x4u  /* NaN/Inf position or density */
+#define PSB_FLAG_NOT_CONVERGED      0x8u  /* PCISPH hit max_iterations above tolerance */
+#define PSB_FLAG_ZERO_DENSITY       0x10u /* rho <= 0 for some particle */
+
+/* Text of the last error raised on this thread (never NULL). */
+const char* psb_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Parameters                                                                */
+/* ------------------------------------------------------------------------ */
+#define PSB_MODE_REFERENCE  0  /* reference semantics: linear EOS, 3 fixed PCISPH
+                                  iterations, per-particle delta, sigma forced to
+                                  0.0728 and g = -9.81 y in PCISPH, r = 0 box    */
+#define PSB_MODE_CONFIG     1  /* Tait EOS, tolerance loop, template delta,
+                                  box restitution, pressure clamped at 0        */
+
+#define PSB_SOLVER_WCSPH    0  /* SPHSystem::update      (SphSystem.cpp:273-297)   */
+#define PSB_SOLVER_PCISPH   1  /* PCI_SPHSystem::update  (PciSphSystem.cpp:163-204) */
+
+#define PSB_EOS_LINEAR      0  /* p = k (rho - rho0)            SphKernel.cu:25 */
+#define PSB_EOS_TAIT        1  /* p = B ((rho/rho0)^gamma - 1)                  */
+
+#define PSB_WARM_NONE       0  /* PCISPH starts from p = 0 (PciSphSystem_Kernel.cu:131) */
+#define PSB_WARM_PREVIOUS   1  /* PCISPH starts from the previous step's pressure       */
+#define PSB_WARM_MLP        2  /* PCISPH starts from the per-particle MLP prediction    */
+
+typedef struct psb_grid_desc {
+  float   min[3];     /* UniformGrid::Min (UniformGrid.cuh:18)            */
+  float   cell_size;  /* UniformGrid::sizeCell                            */
+  int32_t dims[3];    /* nbCellsX/Y/Z = ceil(extent / cell) (UniformGrid.cu:27-29) */
+  int32_t pad;
+} psb_grid_desc;
+
+/* Material (SimulationData_SPHSystem, SimulationData_SPHSystem.h:10-62),
+ * collider, external force and solver settings of one system.             */
+typedef struct psb_sph_params {
+  int32_t mode;             /* PSB_MODE_*   */
+  int32_t solver;           /* PSB_SOLVER_* */
+  int32_t eos;              /* PSB_EOS_*    */
+  int32_t warm_start;       /* PSB_WARM_*   */
+  float   mass;             /* particle mass m                               */
+  float   support_radius;   /* h, SPHParticle::interactionRadius             */
+  float   rest_density;     /* rho0                                          */
+  float   gas_stiffness;    /* k                                             */
+  float   viscosity;        /* mu                                            */
+  float   surface_tension;  /* sigma (0 disables the surface term)           */
+  float   threshold;        /* l, |n| threshold of the surface term          */
+  float   particle_radius;  /* radius used by the box collider               */
+  float   tait_gamma;       /* gamma of the Tait EOS (7)                     */
+  float   tait_b;           /* B of the Tait EOS; 0 -> k * rho0 / gamma       */
+  float   gravity[3];       /* ForceExt_Constante direction * amplitude      */
+  float   dt;               /* time step                                     */
+  int32_t box_enabled;      /* BoxCollision present                          */
+  float   box_min[3];       /* Box::getMin()                                 */
+  float   box_max[3];       /* Box::getMax()                                 */
+  float   elast;            /* restitution (ignored in PSB_MODE_REFERENCE)   */
+  psb_grid_desc grid;
+  int32_t max_neighbors;    /* ELL width of the neighbour list (reference: 200) */
+  float   tolerance;        /* PCISPH max compression error / rho0 (0.01)     */
+  int32_t min_iterations;   /* PCISPH minimum corrector iterations (3)        */
+  int32_t max_iterations;   /* PCISPH maximum corrector iterations (50)       */
+  float   delta;            /* PCISPH template scaling; 0 -> computed from a
+                               full cubic-lattice neighbourhood of spacing
+                               lattice_spacing                                */
+  float   lattice_spacing;  /* s, used for the template delta                 */
+} psb_sph_params;
+
+/* Fill *p with the reference defaults of SPHParticle / SimulationData_SPHSystem
+ * (SphParticle.cpp:7-22): water, h = 0.0457, rho0 = 998.29, k = 3,
+ * mu = 3.5, sigma = 0.0728, l = 7.065; PSB_MODE_REFERENCE, WCSPH, dt 0.01,
+ * grid [-0.5,0.5]^3 with cell = h, 200 neighbours, PCISPH 3 iterations.    */
+psb_status psb_default_params(psb_sph_params* p);
+
+/* Derived constants the kernels use (exposed for tests and the host API). */
+typedef struct psb_derived {
+  float h2, c_poly6, c_spiky, c_visc, c_surf_lap, c_surf_grad;
+  float mc_poly6;      /* mass * c_poly6                  */
+  float beta;          /* 2 (m dt / rho0)^2                */
+  float delta;         /* template PCISPH delta            */
+  float tait_b;
+  int32_t stencil;     /* R = ceil(h / cell) cells         */
+} psb_derived;
+psb_status psb_derive(const psb_sph_params* p, psb_derived* out);
+
+/* ------------------------------------------------------------------------ */
+/* L0: device and memory helpers  (common.cuh:1-25, common.cu:16-107)        */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_context_s* psb_context;
+
+/* cudaInit (common.cu:36-55): select a device and create the stream.       */
+psb_status psb_init(int32_t device, psb_context* out);
+psb_status psb_finalize(psb_context ctx);
+/* getCudaEnabledDeviceCount (common.cu:16) */
+psb_status psb_device_count(int32_t* count);
+void*      psb_context_stream(psb_context ctx);  /* hipStream_t as void*    */
+
+/* allocateArray / freeArray (common.cu:58-66) */
+psb_status psb_allocate_array(psb_context ctx, void** dev, size_t bytes);
+psb_status psb_free_array(psb_context ctx, void* dev);
+/* copyArrayToDevice / copyArrayFromDevice / copyArrayDeviceToDevice
+ * (common.cu:68-93): byte offset applies to the device side.              */
+psb_status psb_copy_array_to_device(psb_context ctx, void* dev, const void* host,
+                                    size_t offset, size_t bytes);
+psb_status psb_copy_array_from_device(psb_context ctx, void* host, const void* dev,
+                                      size_t offset, size_t bytes);
+psb_status psb_copy_array_device_to_device(psb_context ctx, void* dst, const void* src,
+                                           size_t offset, size_t bytes);
+psb_status psb_memset_array(psb_context ctx, void* dev, int32_t value, size_t bytes);
+/* threadSync (common.cu:95) */
+psb_status psb_thread_sync(psb_context ctx);
+/* computeGridSize (common.cu:102-107) with an explicit block size.        */
+psb_status psb_compute_grid_size(uint32_t n, uint32_t block,
+                                 uint32_t* num_blocks, uint32_t* num_threads);
+
+/* ------------------------------------------------------------------------ */
+/* Uniform grid  (UniformGrid.cuh:16-53, UniformGrid.cu:17-99)               */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_grid_s* psb_grid;
+
+/* createGrid (UniformGrid.cu:17-49): Min = centre - extent/2,
+ * dims = ceil(extent / cell).  `capacity` is the largest particle count
+ * psb_store_particles will be called with.                                */
+psb_status psb_create_grid(psb_context ctx, double xc, double yc, double zc,
+                           double length, double width, double depth,
+                           float cell_size, uint32_t capacity, psb_grid* out);
+/* Same with an explicit origin and cell counts. */
+psb_status psb_create_grid_desc(psb_context ctx, const psb_grid_desc* desc,
+                                uint32_t capacity, psb_grid* out);
+/* _finalizeGrid (UniformGrid.cu:53-61) */
+psb_status psb_finalize_grid(psb_grid grid);
+psb_status psb_grid_get_desc(psb_grid grid, psb_grid_desc* out);
+
+/* reinitCells + storeParticles (UniformGrid.cu:64-87, UniformGridKernel.cu:30-55).
+ * Replaces the atomic 64-slot buckets by a stable counting sort:
+ *   hash[i]  = cx + nx (cy + ny cz),  c* = floor((x* - Min*) / cell) clamped;
+ *   perm[k]  = index of the k-th particle in (hash, index) order;
+ *   start[c] = number of particles with hash < c, for c in [0, ncells];
+ *   the particles of cell c are perm[start[c] .. start[c+1]).
+ * pos4 is a device vec4 array.  Results stay on the device (get_arrays). */
+psb_status psb_store_particles(psb_grid grid, const float* pos4, uint32_t n);
+/* Device pointers owned by the grid (valid until the next store/finalize):
+ * hash[n], perm[n], cell_start[ncells + 1]; flags = device uint32.        */
+psb_status psb_grid_get_arrays(psb_grid grid, const uint32_t** hash,
+                               const uint32_t** perm, const uint32_t** cell_start,
+                               uint64_t* ncells);
+/* Gather src[perm[k]] -> dst[k] (the "reordered particle arrays").
+ * elem_floats = 1 (scalar arrays) or 4 (vec4 arrays).                      */
+psb_status psb_grid_reorder(psb_grid grid, const float* src, float* dst,
+                            uint32_t n, int32_t elem_floats);
+
+/* Neighbour list of the reference's partVoisine (SphKernel.cuh:6-10), in ELL
+ * layout: the k-th neighbour of particle i is list[k * n + i], k < count[i],
+ * in ascending (sorted) index order, self included; at most `capacity`
+ * entries per particle.                                                    */
+typedef struct psb_neighbors {
+  uint32_t* count;     /* device [n]             */
+  uint32_t* list;      /* device [capacity * n]  */
+  uint32_t  capacity;
+  uint32_t  n;
+} psb_neighbors;
+
+/* searchNeighbooring (UniformGrid.cu:91-99, UniformGridKernel.cu:58-95):
+ * for each particle of sorted_pos4 (ordered by the last psb_store_particles,
+ * e.g. via psb_grid_reorder), list every j with |x_i - x_j|^2 <= radius^2.
+ * Stencil = ceil(radius / cell) cells per side.  Raises
+ * PSB_FLAG_NEIGHBOR_OVERFLOW in *flags_out (host) if a list was truncated. */
+psb_status psb_search_neighbors(psb_grid grid, const float* sorted_pos4, uint32_t n,
+                                float radius, psb_neighbors nb, uint32_t* flags_out);
+
+/* ------------------------------------------------------------------------ */
+/* SPH kernels on caller arrays  (SphSystem.cuh:8-17)                         */
+/* ------------------------------------------------------------------------ */
+
+/* evaluate_densities_forces (SphSystem.cu:27-42 -> SphKernel.cu:9-91):
+ * density (poly6) + EOS, then pressure, viscosity and surface-tension force
+ * densities and their sum forces_accum4 = f_viscosity + f_surface + f_pressure.
+ * normals4, f_*4 may be NULL when not wanted.                               */
+psb_status psb_evaluate_densities_forces(psb_context ctx, const psb_sph_params* prm,
+                                         psb_neighbors nb, const float* pos4,
+                                         const float* vel4, uint32_t n,
+                                         float* density, float* pressure,
+                                         float* normals4, float* f_pressure4,
+                                         float* f_viscosity4, float* f_surface4,
+                                         float* forces_accum4);
+
+/* evaluate_ForceExt_Constante (ForceExt.cu:24-34 -> ForceExt_Constante.cu:15-24):
+ * accum4[i] += (direction * amplitude) * weight[i]  (the reference passes the
+ * density array as weight for SPH, SphSystem.cpp:312).                     */
+psb_status psb_force_constante(psb_context ctx, uint32_t n, float* accum4,
+                               const float* weight, const float direction[3],
+                               float amplitude);
+
+/* integrateSPHSystem (SphSystem.cu:6-14 -> SphKernel.cu:112-124): leapfrog
+ *   vel_av = vel_ap; vel_ap += (F / rho) dt; old_pos = new_pos;
+ *   new_pos += vel_ap dt.                                                    */
+psb_status psb_integrate_sph(psb_context ctx, float* vel_inter_av4, float* vel_inter_ap4,
+                             float* old_pos4, float* new_pos4, const float* forces4,
+                             const float* density, float dt, uint32_t n);
+
+/* interpolateSPHVelocities (SphSystem.cu:17-24 -> SphKernel.cu:128-135):
+ *   old_vel = new_vel; new_vel = (vel_av + vel_ap) / 2.                      */
+psb_status psb_interpolate_velocities(psb_context ctx, const float* vel_inter_av4,
+                                      const float* vel_inter_ap4, float* old_vel4,
+                                      float* new_vel4, uint32_t n);
+
+/* collisionSystem_Box_CUDA (BoxCollision.cu:10-18 -> BoxCollision_Kernel.cu:7-34):
+ * clamp new_pos into [Min + r, Max - r] per axis and remove (1 + elast) times
+ * the velocity component along the normalised sum of the hit face normals.
+ * The reference ignores elast (r = 0, :27); pass elast = 0 for that.
+ * old_vel4 is accepted and unused, as in the reference.                     */
+psb_status psb_collision_box(psb_context ctx, float* new_pos4, const float* old_vel4,
+                             float* new_vel4, float radius, float dt, uint32_t n,
+                             const float box_min[3], const float box_max[3],
+                             int32_t container, float elast);
+
+/* pci_SPH_update (PciSphSystem.cu:8-68): one PCISPH predict-correct solve.
+ * On entry pos4/vel_inter_ap4 etc. hold the start-of-step state (the
+ * reference's restore_* snapshot equals it); on return pos4 holds the final
+ * predicted position, old_pos4 the start position, vel_inter_av4 the start
+ * half-step velocity and vel_inter_ap4 the new one (collision and velocity
+ * interpolation are separate calls, as in PciSphSystem.cpp:193-197).
+ * PSB_MODE_REFERENCE runs exactly 3 iterations; PSB_MODE_CONFIG iterates
+ * until max(0, rho - rho0)/rho0 <= tolerance (min/max iterations).
+ * pressure is read as the initial pressure when warm_start != PSB_WARM_NONE.
+ * iterations/max_error (host, may be NULL) report the solve.               */
+psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_neighbors nb,
+                              float* old_pos4, float* pos4, const float* vel4,
+                              float* vel_inter_av4, float* vel_inter_ap4,
+                              float* density, float* pressure, float* normals4,
+                              float* f_viscosity4, float* f_surface4, float* f_pressure4,
+                              float* forces_accum4, uint32_t n,
+                              int32_t* iterations, float* max_error);
+
+/* ------------------------------------------------------------------------ */
+/* Fused system step (SPHSystem / PCI_SPHSystem)                              */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_system_s* psb_system;
+
+typedef struct psb_step_stats {
+  uint32_t flags;          /* PSB_FLAG_* raised during the last update call */
+  int32_t  iterations;     /* PCISPH corrector iterations of the last step  */
+  float    max_error;      /* max(0, rho - rho0)/rho0 at the last iteration */
+  int32_t  steps;          /* steps run by the last update call             */
+  int64_t  total_iterations; /* corrector iterations over the last update call */
+  float    worst_error;    /* largest final max_error over the last update call */
+  int32_t  max_neighbors_seen; /* largest neighbour count of the last step   */
+} psb_step_stats;
+
+/* SPHSystem::SPHSystem + init (SphSystem.cpp:6-113): allocate every device
+ * buffer for `capacity` particles under the given parameters.             */
+psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm,
+                             uint32_t capacity, psb_system* out);
+psb_status psb_system_destroy(psb_system sys);
+psb_status psb_system_get_params(psb_system sys, psb_sph_params* out);
+/* SPHSystem::init(particles) / _initialize (SphSystem.cpp:50-201): host FP64
+ * xyz triples (3 doubles per particle, the reference's m_h* layout).
+ * Any pointer may be NULL (zeros).  Resets pressure and statistics.       */
+psb_status psb_system_set_particles(psb_system sys, uint32_t n,
+                                    const double* old_pos, const double* new_pos,
+                                    const double* old_vel, const double* new_vel,
+                                    const double* vel_inter_av, const double* vel_inter_ap);
+/* Same from device vec4 arrays (no host round trip). */
+psb_status psb_system_set_particles_device(psb_system sys, uint32_t n,
+                                           const float* pos4, const float* vel4,
+                                           const float* vel_inter_ap4);
+uint32_t   psb_system_count(psb_system sys);
+/* SPHSystem::update / PCI_SPHSystem::update, `steps` times. */
+psb_status psb_system_update(psb_system sys, int32_t steps);
+psb_status psb_system_get_stats(psb_system sys, psb_step_stats* out);
+/* Host FP64 triples in the caller's original particle order (NULL skips):
+ * m_hPos[0], m_hPos[1], m_hVel[0], m_hVel[1], m_hVelInterAv, m_hVelInterAp,
+ * plus density and pressure (n doubles).                                   */
+psb_status psb_system_get_particles(psb_system sys, double* old_pos, double* new_pos,
+                                    double* old_vel, double* new_vel,
+                                    double* vel_inter_av, double* vel_inter_ap,
+                                    double* density, double* pressure);
+/* Same, FP32, sorted order of the last step, plus the particle ids. */
+psb_status psb_system_get_particles_f32(psb_system sys, float* pos4, float* vel4,
+                                        float* vel_inter_ap4, float* density,
+                                        float* pressure, uint32_t* ids);
+
+/* Device pointers of the system's sorted state (valid until the next update). */
+typedef struct psb_system_arrays {
+  const float*    pos4;          /* m_dPos[1] after the last step   */
+  const float*    old_pos4;      /* m_dPos[0]                       */
+  const float*    vel4;          /* m_dVel[1]                       */
+  const float*    old_vel4;      /* m_dVel[0]                       */
+  const float*    vel_inter_ap4; /* m_dVelInterAp                   */
+  const float*    vel_inter_av4; /* m_dVelInterAv                   */
+  const float*    density;
+  const float*    pressure;
+  const uint32_t* ids;           /* caller index of each sorted slot */
+  const uint32_t* hash;          /* grid hash in the previous order  */
+  const uint32_t* perm;
+  const uint32_t* cell_start;
+  psb_neighbors   neighbors;
+  uint64_t        ncells;
+} psb_system_arrays;
+psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
+
+/* Per-phase timing with HIP events on the system stream (off by default).
+ * Phase names: "hash_count", "scan", "scatter", "reorder", "neighbors_density",
+ * "forces", "nonpressure", "pci_density", "pci_force", plus "mlp" etc.    */
+psb_status psb_system_set_timing(psb_system sys, int32_t enable);
+/* Returns the number of phases; writes up to `cap` entries. */
+int32_t    psb_system_get_timing(psb_system sys, int32_t cap, const char** names,
+                                 double* total_ms, int64_t* launches);
+psb_status psb_system_reset_timing(psb_system sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSB_SPH_H */

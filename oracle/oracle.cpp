@@ -1,0 +1,844 @@
+// oracle.cpp -- CPU restatement of the PSB_Project SPH/PCISPH timestep.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg use it, as the checker.
+//
+// It restates the reference kernels (file:line cited per function) in plain
+// C++ with OpenMP over particles, templated on the scalar type:
+//   - R = float  : the canonical FP32 arithmetic.  Every operation is written
+//                  in a fixed order (no contraction: built with
+//                  -ffp-contract=off; fused multiply-adds only where std::fma
+//                  is written), so an FP32 GPU implementation following the
+//                  same order can be compared bit for bit;
+//   - R = double : the same algorithm in FP64, for accuracy reports.
+//
+// Two semantic modes (psb_sph_params.mode):
+//   PSB_MODE_REFERENCE  what the reference computes (linear EOS, exactly three
+//                       PCISPH iterations with a per-particle delta, sigma
+//                       forced to 0.0728 and g = -9.81 y inside PCISPH, the box
+//                       collider ignoring restitution);
+//   PSB_MODE_CONFIG     the configured semantics (Tait EOS, tolerance-driven
+//                       PCISPH with a template delta and clamped pressure,
+//                       restitution).
+//
+// The neighbour search replaces the reference's atomic 64-slot buckets by a
+// stable counting sort: the particles of a cell appear in ascending index
+// order, which is the order the reference's buckets hold when filled serially
+// (UniformGridKernel.cu:43-55).  Neighbour lists are in ascending sorted index
+// order (rows of the stencil visited z, then y, then x ascending).
+//
+// Parity status of each function (what pins it) is listed in DESIGN.md and in
+// tests/test_oracle_pinning.py.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/psb_sph.h"
+
+namespace {
+
+template <class R> struct V4 { R x, y, z, w; };
+
+// ---------------------------------------------------------------------------
+// Derived constants.  Computed in double from the float parameters, then
+// rounded once to R.  SphKernel.cu:19-20 (poly6), :59-60 (spiky gradient),
+// :64 (viscosity Laplacian), :69-76 (surface Laplacian / gradient);
+// PciSphSystem_Kernel.cu:44 (beta).
+// ---------------------------------------------------------------------------
+template <class R> struct Derived {
+  R h, h2, h2x3, c_poly6, c_spiky, c_visc, c_surf_lap, c_surf_grad;
+  R mc;           // m * c_poly6
+  R beta;         // 2 (m dt / rho0)^2
+  R delta;        // template delta
+  R tait_b, inv_rho0, rho0, k, kp;   // kp = -m / rho0^2
+  R cv, cs, cn;   // mu m c_visc, m c_surf_lap, m c_surf_grad
+  R m, dt, sigma, l, gx, gy, gz;
+  R box_lo[3], box_hi[3], one_plus_e;
+  int R_stencil;
+};
+
+double template_w2(double h, double s) {
+  // Sum over a cubic lattice of spacing s of |grad W_spiky|^2 for all
+  // offsets 0 < d < h (grad W = -45 (h-d)^2 / (pi h^6)).  Sum of grad W over
+  // the full neighbourhood is 0 by symmetry, so W1.W1 = 0.
+  const double c = 45.0 / (M_PI * std::pow(h, 6));
+  int n = (int)std::ceil(h / s) + 1;
+  double w2 = 0.0;
+  for (int a = -n; a <= n; ++a)
+    for (int b = -n; b <= n; ++b)
+      for (int e = -n; e <= n; ++e) {
+        double d = s * std::sqrt((double)(a * a + b * b + e * e));
+        if (d > 0.0 && d < h) { double g = c * (h - d) * (h - d); w2 += g * g; }
+      }
+  return w2;
+}
+
+template <class R> Derived<R> derive(const psb_sph_params& p) {
+  Derived<R> d;
+  const double h = p.support_radius, m = p.mass, rho0 = p.rest_density, dt = p.dt;
+  const double pi = M_PI;
+  d.h = (R)h;
+  d.h2 = (R)(h * h);
+  d.h2x3 = (R)(3.0 * h * h);
+  d.c_poly6 = (R)(315.0 / (64.0 * pi * std::pow(h, 9)));
+  d.c_spiky = (R)(-45.0 / (pi * std::pow(h, 6)));
+  d.c_visc = (R)(45.0 / (pi * std::pow(h, 6)));
+  d.c_surf_lap = (R)(-945.0 / (32.0 * pi * std::pow(h, 9)));
+  d.c_surf_grad = (R)(-945.0 / (32.0 * pi * std::pow(h, 9)));
+  d.mc = (R)(m * 315.0 / (64.0 * pi * std::pow(h, 9)));
+  d.beta = (R)(2.0 * std::pow(m * dt / rho0, 2));
+  const double beta = 2.0 * std::pow(m * dt / rho0, 2);
+  if (p.delta > 0.f) d.delta = (R)p.delta;
+  else {
+    double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
+    d.delta = (R)(1.0 / (beta * template_w2(h, s)));
+  }
+  const double gamma = p.tait_gamma > 0.f ? p.tait_gamma : 7.0;
+  d.tait_b = (R)(p.tait_b > 0.f ? (double)p.tait_b : (double)p.gas_stiffness * rho0 / gamma);
+  d.inv_rho0 = (R)(1.0 / rho0);
+  d.rho0 = (R)rho0;
+  d.k = (R)p.gas_stiffness;
+  d.kp = (R)(-m / (rho0 * rho0));
+  d.cv = (R)((double)p.viscosity * m * 45.0 / (pi * std::pow(h, 6)));
+  d.cs = (R)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));
+  d.cn = (R)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));
+  d.m = (R)m;
+  d.dt = (R)dt;
+  d.sigma = (R)p.surface_tension;
+  d.l = (R)p.threshold;
+  d.gx = (R)p.gravity[0]; d.gy = (R)p.gravity[1]; d.gz = (R)p.gravity[2];
+  for (int a = 0; a < 3; ++a) {
+    // BoxCollision_Kernel.cu:17-23 compares against Max - r and Min + r,
+    // both evaluated in float.
+    d.box_lo[a] = (R)(float)(p.box_min[a] + p.particle_radius);
+    d.box_hi[a] = (R)(float)(p.box_max[a] - p.particle_radius);
+  }
+  d.one_plus_e = (R)(p.mode == PSB_MODE_REFERENCE ? 1.0f : 1.0f + p.elast);
+  double rr = std::ceil((double)h / (double)p.grid.cell_size);
+  if ((rr - 1.0) * (double)p.grid.cell_size >= (double)h) rr -= 1.0;
+  d.R_stencil = (int)rr;
+  return d;
+}
+
+template <class R> inline R fmaR(R a, R b, R c) { return std::fma(a, b, c); }
+template <class R> inline R dot3(R ax, R ay, R az, R bx, R by, R bz) {
+  return fmaR(ax, bx, fmaR(ay, by, az * bz));
+}
+
+// ---------------------------------------------------------------------------
+// Grid hash: UniformGridKernel.cu:48-51 (cell = floor((x - Min)/cell),
+// flat index i + j nx + k nx ny).  Out-of-grid coordinates are clamped to the
+// edge cell and flagged (the reference writes out of bounds there).
+// ---------------------------------------------------------------------------
+inline int cell_coord(float x, float mn, float cell, int n, uint32_t& flags) {
+  float t = std::floor((x - mn) / cell);
+  if (!(t >= 0.0f && t < (float)n)) {
+    flags |= PSB_FLAG_OUT_OF_GRID;
+    t = (t >= (float)n) ? (float)(n - 1) : 0.0f;
+  }
+  return (int)t;
+}
+
+struct Grid {
+  float min[3], cell;
+  int nx, ny, nz;
+  uint64_t ncells() const { return (uint64_t)nx * ny * nz; }
+};
+
+Grid grid_from(const psb_grid_desc& g) {
+  Grid r;
+  for (int a = 0; a < 3; ++a) r.min[a] = g.min[a];
+  r.cell = g.cell_size; r.nx = g.dims[0]; r.ny = g.dims[1]; r.nz = g.dims[2];
+  return r;
+}
+
+inline void cell_of(const Grid& g, float x, float y, float z, int& cx, int& cy, int& cz,
+                    uint32_t& flags) {
+  cx = cell_coord(x, g.min[0], g.cell, g.nx, flags);
+  cy = cell_coord(y, g.min[1], g.cell, g.ny, flags);
+  cz = cell_coord(z, g.min[2], g.cell, g.nz, flags);
+}
+
+// Stable counting sort by hash: perm lists particle indices in (hash, index)
+// order, start[c] = #particles with hash < c.
+void grid_build(const Grid& g, uint32_t n, const float* x4, uint32_t* hash, uint32_t* perm,
+                uint32_t* start, uint32_t& flags) {
+  const uint64_t nc = g.ncells();
+  for (uint32_t i = 0; i < n; ++i) {
+    int cx, cy, cz;
+    cell_of(g, x4[4 * i], x4[4 * i + 1], x4[4 * i + 2], cx, cy, cz, flags);
+    if (!std::isfinite(x4[4 * i]) || !std::isfinite(x4[4 * i + 1]) || !std::isfinite(x4[4 * i + 2]))
+      flags |= PSB_FLAG_NONFINITE;
+    hash[i] = (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
+  }
+  std::vector<uint32_t> cnt(nc + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) cnt[hash[i]]++;
+  uint32_t run = 0;
+  for (uint64_t c = 0; c < nc; ++c) { start[c] = run; run += cnt[c]; }
+  start[nc] = run;
+  std::vector<uint32_t> fill(start, start + nc);
+  for (uint32_t i = 0; i < n; ++i) perm[fill[hash[i]]++] = i;
+}
+
+// searchNeighbooring restated (UniformGridKernel.cu:58-95): all j with
+// |x_i - x_j|^2 <= radius^2, self included, over (2R+1)^3 cells.  Rows of
+// the stencil are visited z-major, then y, then x ascending, i.e. ascending
+// sorted index.  Density of the same pairs (SphKernel.cu:9-27) is accumulated
+// when rho_sum != nullptr: S = sum (h^2 - d^2)^3.
+template <class R>
+uint32_t neighbors(const Grid& g, const uint32_t* start, uint32_t n, const V4<R>* xs,
+                   R radius2, int Rs, uint32_t cap, uint32_t* count, uint32_t* list,
+                   R* rho_sum, uint32_t* maxcnt) {
+  uint32_t flags = 0, mx = 0;
+#pragma omp parallel for schedule(static) reduction(| : flags) reduction(max : mx)
+  for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
+    const uint32_t p = (uint32_t)pi;
+    const V4<R> xi = xs[p];
+    int cx, cy, cz;
+    uint32_t f = 0;
+    cell_of(g, (float)xi.x, (float)xi.y, (float)xi.z, cx, cy, cz, f);
+    uint32_t cnt = 0;
+    R S = 0;
+    for (int dz = -Rs; dz <= Rs; ++dz) {
+      int z = cz + dz;
+      if (z < 0 || z >= g.nz) continue;
+      for (int dy = -Rs; dy <= Rs; ++dy) {
+        int y = cy + dy;
+        if (y < 0 || y >= g.ny) continue;
+        int x0 = std::max(cx - Rs, 0), x1 = std::min(cx + Rs, g.nx - 1);
+        uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * z);
+        uint32_t q0 = start[base + x0], q1 = start[base + x1 + 1];
+        for (uint32_t q = q0; q < q1; ++q) {
+          const V4<R> xj = xs[q];
+          R dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
+          R d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
+          if (d2 <= radius2) {
+            if (cnt < cap) list[(uint64_t)cnt * n + p] = q;
+            cnt++;
+            R w = radius2 - d2;
+            S = fmaR(w * w, w, S);
+          }
+        }
+      }
+    }
+    if (cnt > cap) f |= PSB_FLAG_NEIGHBOR_OVERFLOW;
+    count[p] = std::min(cnt, cap);
+    if (rho_sum) rho_sum[p] = S;
+    flags |= (f & PSB_FLAG_NEIGHBOR_OVERFLOW);
+    mx = std::max(mx, cnt);
+  }
+  if (maxcnt) *maxcnt = mx;
+  return flags;
+}
+
+// EOS: SphKernel.cu:25 (linear); Tait B((rho/rho0)^7 - 1) for PSB_EOS_TAIT.
+// PSB_MODE_CONFIG clamps negative pressure to zero.
+template <class R> inline R eos(const Derived<R>& d, const psb_sph_params& p, R rho) {
+  R pr;
+  if (p.eos == PSB_EOS_TAIT) {
+    R x = rho * d.inv_rho0;
+    R x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x7 = x6 * x;
+    pr = d.tait_b * (x7 - (R)1);
+  } else {
+    pr = d.k * (rho - d.rho0);
+  }
+  if (p.mode == PSB_MODE_CONFIG && pr < (R)0) pr = (R)0;
+  return pr;
+}
+
+// BoxCollision_Kernel.cu:7-34.  Clamp per axis (max side first, min side
+// overrides), normalise the summed face normals, remove (1+e) of the normal
+// velocity component.
+template <class R> inline void box_collide(const Derived<R>& d, V4<R>& x, V4<R>& v) {
+  R px = x.x, py = x.y, pz = x.z, nx = 0, ny = 0, nz = 0;
+  if (x.x >= d.box_hi[0]) { px = d.box_hi[0]; nx = -1; }
+  if (x.y >= d.box_hi[1]) { py = d.box_hi[1]; ny = -1; }
+  if (x.z >= d.box_hi[2]) { pz = d.box_hi[2]; nz = -1; }
+  if (x.x <= d.box_lo[0]) { px = d.box_lo[0]; nx = 1; }
+  if (x.y <= d.box_lo[1]) { py = d.box_lo[1]; ny = 1; }
+  if (x.z <= d.box_lo[2]) { pz = d.box_lo[2]; nz = 1; }
+  R ln2 = dot3(nx, ny, nz, nx, ny, nz);
+  if (ln2 > (R)0) {
+    R ln = std::sqrt(ln2);
+    nx = nx / ln; ny = ny / ln; nz = nz / ln;
+    x.x = px; x.y = py; x.z = pz;
+    R vn = d.one_plus_e * dot3(v.x, v.y, v.z, nx, ny, nz);
+    v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
+  }
+}
+
+// Pair sums of internalForces (SphKernel.cu:47-78) and of
+// compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:134-158)
+// for particle p; xs[q].w = p_q / rho_q^2 (pressure term, only when PRESS),
+// vs[q].w = 1 / rho_q.
+template <class R> struct PairSums { R fp[3], fv[3], fs, nn[3]; };
+
+template <class R, bool PRESS, bool SURF>
+inline PairSums<R> internal_sums(const Derived<R>& d, uint32_t p, uint32_t n,
+                                 const uint32_t* count, const uint32_t* list,
+                                 const V4<R>* xs, const V4<R>* vs) {
+  PairSums<R> s{};
+  const V4<R> xi = xs[p], vi = vs[p];
+  const R pvi = xi.w;
+  for (uint32_t k = 0; k < count[p]; ++k) {
+    uint32_t q = list[(uint64_t)k * n + p];
+    const V4<R> xj = xs[q], vj = vs[q];
+    R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    R d2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (d2 > (R)0) {
+      R dd = std::sqrt(d2);
+      R hd = d.h - dd;
+      R rinvj = vj.w;
+      if (PRESS) {
+        R invd = (R)1 / dd;
+        R gw = d.c_spiky * (hd * hd);
+        R c = gw * (pvi + xj.w);
+        s.fp[0] = fmaR(dx * invd, c, s.fp[0]);
+        s.fp[1] = fmaR(dy * invd, c, s.fp[1]);
+        s.fp[2] = fmaR(dz * invd, c, s.fp[2]);
+      }
+      R lv = hd * rinvj;
+      s.fv[0] = fmaR(vj.x - vi.x, lv, s.fv[0]);
+      s.fv[1] = fmaR(vj.y - vi.y, lv, s.fv[1]);
+      s.fv[2] = fmaR(vj.z - vi.z, lv, s.fv[2]);
+      if (SURF) {
+        R a = d.h2 - d2;
+        R b = d.h2x3 - (R)7 * d2;
+        s.fs = fmaR(a * b, rinvj, s.fs);
+        R an = (a * a) * rinvj;
+        s.nn[0] = fmaR(dx, an, s.nn[0]);
+        s.nn[1] = fmaR(dy, an, s.nn[1]);
+        s.nn[2] = fmaR(dz, an, s.nn[2]);
+      }
+    }
+  }
+  return s;
+}
+
+// Surface-tension finish (SphKernel.cu:80-88): n = sum * m c_grad;
+// if |n| >= l: f_S = (n * (fs * -sigma)) * (1/|n|), n <- n/|n|; else 0.
+template <class R>
+inline void surface_finish(const Derived<R>& d, const PairSums<R>& s, R sigma, R fS[3], R nrm[3]) {
+  R fsum = s.fs * d.cs;
+  R n0 = s.nn[0] * d.cn, n1 = s.nn[1] * d.cn, n2 = s.nn[2] * d.cn;
+  R ln = std::sqrt(dot3(n0, n1, n2, n0, n1, n2));
+  if (ln >= d.l) {
+    R tt = fsum * (-sigma);
+    R r = (R)1 / ln;
+    fS[0] = (n0 * tt) * r; fS[1] = (n1 * tt) * r; fS[2] = (n2 * tt) * r;
+    nrm[0] = n0 / ln; nrm[1] = n1 / ln; nrm[2] = n2 / ln;
+  } else {
+    fS[0] = fS[1] = fS[2] = 0;
+    nrm[0] = nrm[1] = nrm[2] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// System: sorted state of one scene, advanced by WCSPH or PCISPH steps.
+// ---------------------------------------------------------------------------
+template <class R> struct System {
+  psb_sph_params prm;
+  Derived<R> d;
+  Grid g;
+  uint32_t cap = 0, n = 0;
+  // A = start-of-step (sorted), B = end-of-step (same order)
+  std::vector<V4<R>> Ax, Av, Avh, Bx, Bv, Bvh;
+  std::vector<uint32_t> Aid, Bid, hash, perm, start, count, list;
+  std::vector<R> Ap, Bp, rho, pres, rho_pred;
+  std::vector<V4<R>> fa, fs4, anp, xp0, xp1;
+  uint32_t flags = 0;
+  int last_iters = 0;
+  R last_err = 0;
+  int64_t total_iters = 0;
+  R worst_err = 0;
+  int steps_done = 0;
+  uint32_t max_nbr_seen = 0;
+  std::vector<int> iter_hist;
+  std::vector<R> err_hist;
+
+  void init(const psb_sph_params& p, uint32_t capacity) {
+    prm = p; d = derive<R>(p); g = grid_from(p.grid); cap = capacity;
+    auto rs = [&](auto& v, size_t k) { v.assign(k, {}); };
+    rs(Ax, cap); rs(Av, cap); rs(Avh, cap); rs(Bx, cap); rs(Bv, cap); rs(Bvh, cap);
+    rs(fa, cap); rs(fs4, cap); rs(anp, cap); rs(xp0, cap); rs(xp1, cap);
+    Aid.assign(cap, 0); Bid.assign(cap, 0); hash.assign(cap, 0); perm.assign(cap, 0);
+    start.assign(g.ncells() + 1, 0); count.assign(cap, 0);
+    list.assign((size_t)cap * p.max_neighbors, 0);
+    Ap.assign(cap, 0); Bp.assign(cap, 0); rho.assign(cap, 0); pres.assign(cap, 0);
+    rho_pred.assign(cap, 0);
+  }
+
+  // storeParticles + reorder (B -> A by the stable order).
+  void build_grid() {
+    std::vector<float> x4(4 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+      x4[4 * i] = (float)Bx[i].x; x4[4 * i + 1] = (float)Bx[i].y; x4[4 * i + 2] = (float)Bx[i].z;
+      x4[4 * i + 3] = 0.f;
+    }
+    grid_build(g, n, x4.data(), hash.data(), perm.data(), start.data(), flags);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)n; ++k) {
+      uint32_t i = perm[k];
+      Ax[k] = Bx[i]; Av[k] = Bv[i]; Avh[k] = Bvh[i]; Aid[k] = Bid[i]; Ap[k] = Bp[i];
+    }
+  }
+
+  void neighbors_density() {
+    std::vector<R> S(n);
+    uint32_t mx = 0;
+    flags |= neighbors<R>(g, start.data(), n, Ax.data(), d.h2, d.R_stencil,
+                          (uint32_t)prm.max_neighbors, count.data(), list.data(), S.data(), &mx);
+    max_nbr_seen = std::max(max_nbr_seen, mx);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)n; ++k) rho[k] = S[k] * d.mc;
+  }
+
+  void finish_particle(uint32_t p, V4<R> x, V4<R> vh) {
+    // collide (SphSystem.cpp:323-328) then interpolate (SphKernel.cu:128-135)
+    if (prm.box_enabled) box_collide(d, x, vh);
+    Bx[p] = {x.x, x.y, x.z, 0};
+    Bvh[p] = {vh.x, vh.y, vh.z, 0};
+    const V4<R> va = Avh[p];
+    Bv[p] = {(va.x + vh.x) * (R)0.5, (va.y + vh.y) * (R)0.5, (va.z + vh.z) * (R)0.5, 0};
+    Bid[p] = Aid[p];
+  }
+
+  // SPHSystem::update (SphSystem.cpp:273-297): grid, densities+forces
+  // (SphKernel.cu:9-91), gravity as g*rho (SphSystem.cpp:312,
+  // ForceExt_Constante.cu:15-24), leapfrog (SphKernel.cu:112-124), box,
+  // interpolation.
+  void wcsph_step() {
+    build_grid();
+    neighbors_density();
+    const bool surf = prm.surface_tension != 0.f;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)n; ++k) {
+      R r = rho[k];
+      R pr = eos(d, prm, r);
+      pres[k] = pr;
+      Ax[k].w = pr / (r * r);
+      Av[k].w = (R)1 / r;
+    }
+    uint32_t zf = 0;
+#pragma omp parallel for schedule(static) reduction(| : zf)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      PairSums<R> s = surf ? internal_sums<R, true, true>(d, p, n, count.data(), list.data(), Ax.data(), Av.data())
+                           : internal_sums<R, true, false>(d, p, n, count.data(), list.data(), Ax.data(), Av.data());
+      const R ri = rho[p];
+      if (!(ri > (R)0)) zf |= PSB_FLAG_ZERO_DENSITY;
+      R cpf = -(d.m * ri);
+      R fP[3] = {s.fp[0] * cpf, s.fp[1] * cpf, s.fp[2] * cpf};
+      R fV[3] = {s.fv[0] * d.cv, s.fv[1] * d.cv, s.fv[2] * d.cv};
+      R fS[3] = {0, 0, 0}, nrm[3];
+      if (surf) surface_finish(d, s, d.sigma, fS, nrm);
+      R F[3];
+      F[0] = (fV[0] + fS[0]) + fP[0];
+      F[1] = (fV[1] + fS[1]) + fP[1];
+      F[2] = (fV[2] + fS[2]) + fP[2];
+      F[0] = F[0] + d.gx * ri; F[1] = F[1] + d.gy * ri; F[2] = F[2] + d.gz * ri;
+      R inv = (R)1 / ri;
+      V4<R> vh = Avh[p], x = Ax[p];
+      vh.x = vh.x + (F[0] * inv) * d.dt;
+      vh.y = vh.y + (F[1] * inv) * d.dt;
+      vh.z = vh.z + (F[2] * inv) * d.dt;
+      x.x = x.x + vh.x * d.dt; x.y = x.y + vh.y * d.dt; x.z = x.z + vh.z * d.dt;
+      finish_particle(p, x, vh);
+      Bp[p] = pres[p];
+    }
+    flags |= zf;
+    last_iters = 0; last_err = 0;
+  }
+
+  // PCI_SPHSystem::update (PciSphSystem.cpp:163-204) -> pci_SPH_update
+  // (PciSphSystem.cu:8-68).
+  void pcisph_step() {
+    build_grid();
+    neighbors_density();
+    const bool ref = prm.mode == PSB_MODE_REFERENCE;
+    const R sigma = ref ? (R)0.0728f : d.sigma;
+    const bool surf = ref ? true : (prm.surface_tension != 0.f);
+    const R gx = ref ? (R)0 : d.gx, gz = ref ? (R)0 : d.gz;
+    const R gy = ref ? (R)-9.81f : d.gy;
+    const bool warm = !ref && prm.warm_start == PSB_WARM_PREVIOUS;
+    // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)n; ++k) {
+      pres[k] = warm ? Ap[k] : (R)0;
+      Av[k].w = (R)1 / rho[k];
+    }
+    // non-pressure forces + first prediction (PciSphSystem_Kernel.cu:117-170,
+    // SphKernel.cu:95-109)
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      PairSums<R> s = surf ? internal_sums<R, false, true>(d, p, n, count.data(), list.data(), Ax.data(), Av.data())
+                           : internal_sums<R, false, false>(d, p, n, count.data(), list.data(), Ax.data(), Av.data());
+      const R ri = rho[p];
+      R fV[3] = {s.fv[0] * d.cv, s.fv[1] * d.cv, s.fv[2] * d.cv};
+      R fS[3] = {0, 0, 0}, nrm[3];
+      if (surf) surface_finish(d, s, sigma, fS, nrm);
+      const V4<R> x = Ax[p], vh0 = Avh[p];
+      V4<R> xp;
+      if (ref) {
+        // forcesAccum = (0, -9.81 rho, 0); F = ((Facc + fV) + fP) + fS
+        R fa0 = (R)0 + fV[0], fa1 = (gy * ri) + fV[1], fa2 = (R)0 + fV[2];
+        fa[p] = {fa0, fa1, fa2, 0};
+        fs4[p] = {fS[0], fS[1], fS[2], 0};
+        R F0 = (fa0 + (R)0) + fS[0], F1 = (fa1 + (R)0) + fS[1], F2 = (fa2 + (R)0) + fS[2];
+        R inv = (R)1 / ri;
+        R v0 = vh0.x + (F0 * inv) * d.dt, v1 = vh0.y + (F1 * inv) * d.dt, v2 = vh0.z + (F2 * inv) * d.dt;
+        xp = {x.x + v0 * d.dt, x.y + v1 * d.dt, x.z + v2 * d.dt, 0};
+      } else {
+        R inv = (R)1 / ri;
+        R a0 = (fV[0] + fS[0]) * inv + gx, a1 = (fV[1] + fS[1]) * inv + gy, a2 = (fV[2] + fS[2]) * inv + gz;
+        anp[p] = {a0, a1, a2, 0};
+        if (warm) {
+          xp = {x.x, x.y, x.z, 0};
+        } else {
+          R v0 = vh0.x + a0 * d.dt, v1 = vh0.y + a1 * d.dt, v2 = vh0.z + a2 * d.dt;
+          xp = {x.x + v0 * d.dt, x.y + v1 * d.dt, x.z + v2 * d.dt, 0};
+        }
+      }
+      xp0[p] = xp;
+    }
+    std::vector<V4<R>>* cur = &xp0;
+    std::vector<V4<R>>* nxt = &xp1;
+    if (warm) {
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < (int64_t)n; ++k) (*cur)[k].w = pres[k];
+      pci_force(*cur, *nxt, false);
+      std::swap(cur, nxt);
+    }
+    const int min_it = ref ? 3 : std::max(1, prm.min_iterations);
+    const int max_it = ref ? 3 : std::max(min_it, prm.max_iterations);
+    int it = 0;
+    R err = 0;
+    while (true) {
+      err = pci_density(*cur);
+      it++;
+      bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it;
+      if (fin && !ref && err > (R)prm.tolerance) flags |= PSB_FLAG_NOT_CONVERGED;
+      pci_force(*cur, *nxt, fin);
+      if (fin) break;
+      std::swap(cur, nxt);
+    }
+    last_iters = it; last_err = err;
+    total_iters += it; worst_err = std::max(worst_err, err);
+    iter_hist.push_back(it); err_hist.push_back(err);
+  }
+
+  // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
+  // in reference mode; template-delta pressure update in config mode.
+  // Returns max_i max(0, rho_i - rho0)/rho0.
+  R pci_density(std::vector<V4<R>>& X) {
+    const bool ref = prm.mode == PSB_MODE_REFERENCE;
+    R emax = 0;
+#pragma omp parallel for schedule(static) reduction(max : emax)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const V4<R> xi = X[p];
+      R S = 0, W1x = 0, W1y = 0, W1z = 0, W2 = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        uint32_t q = list[(uint64_t)k * n + p];
+        const V4<R> xj = X[q];
+        R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        R w = d.h2 - d2;
+        if (!ref) w = std::max(w, (R)0);
+        S = fmaR(w * w, w, S);
+        if (ref && d2 > (R)0) {
+          R dd = std::sqrt(d2);
+          R invd = (R)1 / dd;
+          R hd = d.h - dd;
+          R gw = d.c_spiky * (hd * hd);
+          R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+          W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+          W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        }
+      }
+      R r = S * d.mc;
+      R pr = pres[p];
+      R e = r - d.rho0;
+      if (ref) {
+        if (r > d.rho0 && W2 > (R)0)
+          pr = pr + e / (d.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
+        X[p].w = pr / (r * r);
+        rho_pred[p] = r;
+      } else {
+        pr = pr + d.delta * e;
+        if (pr < (R)0) pr = (R)0;
+        X[p].w = pr;
+        rho_pred[p] = r;
+      }
+      pres[p] = pr;
+      R err = (e > (R)0 ? e : (R)0) * d.inv_rho0;
+      emax = std::max(emax, err);
+    }
+    return emax;
+  }
+
+  // compute_Pressure_Force_PCI (PciSphSystem_Kernel.cu:88-114) +
+  // integrateSPH_LeapFrog_Forces (SphKernel.cu:95-109) from the restored
+  // start state (restore_Pos_Vel_PCI, :190-202).
+  void pci_force(std::vector<V4<R>>& X, std::vector<V4<R>>& Xn, bool fin) {
+    const bool ref = prm.mode == PSB_MODE_REFERENCE;
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const V4<R> xi = X[p];
+      R sx = 0, sy = 0, sz = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        uint32_t q = list[(uint64_t)k * n + p];
+        const V4<R> xj = X[q];
+        R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (ref) {
+          if (d2 > (R)0) {
+            R dd = std::sqrt(d2);
+            R invd = (R)1 / dd;
+            R hd = d.h - dd;
+            R gw = d.c_spiky * (hd * hd);
+            R c = gw * (xi.w + xj.w);
+            sx = fmaR(dx * invd, c, sx); sy = fmaR(dy * invd, c, sy); sz = fmaR(dz * invd, c, sz);
+          }
+        } else if (d2 > (R)0 && d2 < d.h2) {
+          R dd = std::sqrt(d2);
+          R invd = (R)1 / dd;
+          R hd = d.h - dd;
+          R gw = d.c_spiky * (hd * hd);
+          R c = (gw * invd) * (xi.w + xj.w);
+          sx = fmaR(dx, c, sx); sy = fmaR(dy, c, sy); sz = fmaR(dz, c, sz);
+        }
+      }
+      const V4<R> x0 = Ax[p], vh0 = Avh[p];
+      V4<R> vh, x;
+      if (ref) {
+        const R rp = rho_pred[p];
+        R cpf = -(d.m * rp);
+        R fP0 = sx * cpf, fP1 = sy * cpf, fP2 = sz * cpf;
+        const V4<R> a = fa[p], s = fs4[p];
+        R F0 = (a.x + fP0) + s.x, F1 = (a.y + fP1) + s.y, F2 = (a.z + fP2) + s.z;
+        R inv = (R)1 / rp;
+        vh = {vh0.x + (F0 * inv) * d.dt, vh0.y + (F1 * inv) * d.dt, vh0.z + (F2 * inv) * d.dt, 0};
+      } else {
+        const V4<R> a = anp[p];
+        R a0 = a.x + sx * d.kp, a1 = a.y + sy * d.kp, a2 = a.z + sz * d.kp;
+        vh = {vh0.x + a0 * d.dt, vh0.y + a1 * d.dt, vh0.z + a2 * d.dt, 0};
+      }
+      x = {x0.x + vh.x * d.dt, x0.y + vh.y * d.dt, x0.z + vh.z * d.dt, 0};
+      if (!fin) {
+        Xn[p].x = x.x; Xn[p].y = x.y; Xn[p].z = x.z;
+      } else {
+        finish_particle(p, x, vh);
+        Bp[p] = pres[p];
+      }
+    }
+  }
+
+  void update(int steps) {
+    flags = 0; total_iters = 0; worst_err = 0; steps_done = 0; max_nbr_seen = 0;
+    for (int s = 0; s < steps; ++s) {
+      if (prm.solver == PSB_SOLVER_PCISPH) pcisph_step(); else wcsph_step();
+      steps_done++;
+    }
+  }
+};
+
+struct Handle {
+  int prec;  // 32 or 64
+  System<float>* f = nullptr;
+  System<double>* d = nullptr;
+};
+
+template <class R>
+void set_particles(System<R>* s, uint32_t n, const float* pos4, const float* vel4, const float* vh4) {
+  s->n = n;
+  for (uint32_t i = 0; i < n; ++i) {
+    s->Bx[i] = {(R)pos4[4 * i], (R)pos4[4 * i + 1], (R)pos4[4 * i + 2], 0};
+    s->Bv[i] = vel4 ? V4<R>{(R)vel4[4 * i], (R)vel4[4 * i + 1], (R)vel4[4 * i + 2], 0} : V4<R>{0, 0, 0, 0};
+    s->Bvh[i] = vh4 ? V4<R>{(R)vh4[4 * i], (R)vh4[4 * i + 1], (R)vh4[4 * i + 2], 0} : V4<R>{0, 0, 0, 0};
+    s->Bid[i] = i;
+    s->Bp[i] = 0;
+  }
+  s->Ax = s->Bx; s->Av = s->Bv; s->Avh = s->Bvh; s->Aid = s->Bid;
+}
+
+template <class R>
+void get_particles(System<R>* s, float* pos4, float* vel4, float* vh4, float* old_pos4,
+                   float* old_vel4, float* vav4, float* rho, float* pres, uint32_t* ids) {
+  for (uint32_t i = 0; i < s->n; ++i) {
+    auto put = [&](float* dst, const V4<R>& v) {
+      if (!dst) return;
+      dst[4 * i] = (float)v.x; dst[4 * i + 1] = (float)v.y; dst[4 * i + 2] = (float)v.z; dst[4 * i + 3] = 0.f;
+    };
+    put(pos4, s->Bx[i]); put(vel4, s->Bv[i]); put(vh4, s->Bvh[i]);
+    put(old_pos4, s->Ax[i]); put(old_vel4, s->Av[i]); put(vav4, s->Avh[i]);
+    if (rho) rho[i] = (float)(s->prm.solver == PSB_SOLVER_PCISPH ? s->rho_pred[i] : s->rho[i]);
+    if (pres) pres[i] = (float)s->pres[i];
+    if (ids) ids[i] = s->Bid[i];
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C entry points (ctypes).  All arrays are host memory.
+// ===========================================================================
+extern "C" {
+
+int oracle_derive(const psb_sph_params* p, double* out /* 16 */) {
+  Derived<double> d = derive<double>(*p);
+  Derived<float> f = derive<float>(*p);
+  double v[16] = {f.h2, f.c_poly6, f.c_spiky, f.c_visc, f.c_surf_lap, f.c_surf_grad, f.mc,
+                  f.beta, f.delta, f.tait_b, (double)f.R_stencil, d.delta, d.beta, d.c_poly6,
+                  d.c_spiky, d.mc};
+  std::memcpy(out, v, sizeof(v));
+  return 0;
+}
+
+uint32_t oracle_grid_build(uint32_t n, const float* x4, const psb_grid_desc* gd, uint32_t* hash,
+                           uint32_t* perm, uint32_t* start) {
+  uint32_t flags = 0;
+  grid_build(grid_from(*gd), n, x4, hash, perm, start, flags);
+  return flags;
+}
+
+uint32_t oracle_neighbors(uint32_t n, const float* sorted_x4, const psb_grid_desc* gd,
+                          const uint32_t* start, float radius, uint32_t cap, uint32_t* count,
+                          uint32_t* list) {
+  Grid g = grid_from(*gd);
+  double rr = std::ceil((double)radius / (double)g.cell);
+  if ((rr - 1.0) * (double)g.cell >= (double)radius) rr -= 1.0;
+  return neighbors<float>(g, start, n, reinterpret_cast<const V4<float>*>(sorted_x4),
+                          radius * radius, (int)rr, cap, count, list, nullptr, nullptr);
+}
+
+// evaluate_densities_forces restated on a given neighbour list
+// (SphKernel.cu:9-91).  Arrays in the list's index space.
+int oracle_evaluate_densities_forces(const psb_sph_params* prm, uint32_t n, const uint32_t* count,
+                                     const uint32_t* list, const float* pos4, const float* vel4,
+                                     float* density, float* pressure, float* normals4,
+                                     float* fP4, float* fV4, float* fS4, float* F4) {
+  Derived<float> d = derive<float>(*prm);
+  std::vector<V4<float>> xs(n), vs(n);
+  std::memcpy(xs.data(), pos4, 16 * (size_t)n);
+  std::memcpy(vs.data(), vel4, 16 * (size_t)n);
+  for (uint32_t p = 0; p < n; ++p) {
+    float S = 0;
+    for (uint32_t k = 0; k < count[p]; ++k) {
+      uint32_t q = list[(uint64_t)k * n + p];
+      float dx = xs[p].x - xs[q].x, dy = xs[p].y - xs[q].y, dz = xs[p].z - xs[q].z;
+      float d2 = dot3(dx, dy, dz, dx, dy, dz);
+      float w = d.h2 - d2;
+      S = std::fma(w * w, w, S);
+    }
+    float r = S * d.mc;
+    density[p] = r;
+    float pr = eos(d, *prm, r);
+    pressure[p] = pr;
+  }
+  for (uint32_t p = 0; p < n; ++p) {
+    xs[p].w = pressure[p] / (density[p] * density[p]);
+    vs[p].w = 1.0f / density[p];
+  }
+  const bool surf = prm->surface_tension != 0.f;
+  for (uint32_t p = 0; p < n; ++p) {
+    PairSums<float> s = surf ? internal_sums<float, true, true>(d, p, n, count, list, xs.data(), vs.data())
+                             : internal_sums<float, true, false>(d, p, n, count, list, xs.data(), vs.data());
+    float ri = density[p];
+    float cpf = -(d.m * ri);
+    float fP[3] = {s.fp[0] * cpf, s.fp[1] * cpf, s.fp[2] * cpf};
+    float fV[3] = {s.fv[0] * d.cv, s.fv[1] * d.cv, s.fv[2] * d.cv};
+    float fS[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+    if (surf) surface_finish(d, s, d.sigma, fS, nrm);
+    for (int a = 0; a < 3; ++a) {
+      if (normals4) normals4[4 * p + a] = nrm[a];
+      if (fP4) fP4[4 * p + a] = fP[a];
+      if (fV4) fV4[4 * p + a] = fV[a];
+      if (fS4) fS4[4 * p + a] = fS[a];
+      if (F4) F4[4 * p + a] = (fV[a] + fS[a]) + fP[a];
+    }
+  }
+  return 0;
+}
+
+int oracle_collision_box(const psb_sph_params* prm, uint32_t n, float* pos4, float* vel4) {
+  Derived<float> d = derive<float>(*prm);
+  for (uint32_t i = 0; i < n; ++i) {
+    V4<float> x = {pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], pos4[4 * i + 3]};
+    V4<float> v = {vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2], vel4[4 * i + 3]};
+    box_collide(d, x, v);
+    pos4[4 * i] = x.x; pos4[4 * i + 1] = x.y; pos4[4 * i + 2] = x.z;
+    vel4[4 * i] = v.x; vel4[4 * i + 1] = v.y; vel4[4 * i + 2] = v.z;
+  }
+  return 0;
+}
+
+void* oracle_system_create(const psb_sph_params* prm, uint32_t capacity, int precision) {
+  Handle* h = new Handle();
+  h->prec = precision;
+  if (precision == 64) { h->d = new System<double>(); h->d->init(*prm, capacity); }
+  else { h->f = new System<float>(); h->f->init(*prm, capacity); }
+  return h;
+}
+
+void oracle_system_destroy(void* hp) {
+  Handle* h = (Handle*)hp;
+  delete h->f; delete h->d; delete h;
+}
+
+int oracle_system_set_particles(void* hp, uint32_t n, const float* pos4, const float* vel4,
+                                const float* vh4) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) { if (n > h->d->cap) return 3; set_particles(h->d, n, pos4, vel4, vh4); }
+  else { if (n > h->f->cap) return 3; set_particles(h->f, n, pos4, vel4, vh4); }
+  return 0;
+}
+
+int oracle_system_update(void* hp, int steps) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) h->d->update(steps); else h->f->update(steps);
+  return 0;
+}
+
+int oracle_system_get(void* hp, float* pos4, float* vel4, float* vh4, float* old_pos4,
+                      float* old_vel4, float* vav4, float* rho, float* pres, uint32_t* ids) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) get_particles(h->d, pos4, vel4, vh4, old_pos4, old_vel4, vav4, rho, pres, ids);
+  else get_particles(h->f, pos4, vel4, vh4, old_pos4, old_vel4, vav4, rho, pres, ids);
+  return 0;
+}
+
+// stats: flags, last_iters, last_err, total_iters, worst_err, steps, max_nbr
+int oracle_system_stats(void* hp, double* out /* 7 */) {
+  Handle* h = (Handle*)hp;
+  auto fill = [&](auto* s) {
+    out[0] = s->flags; out[1] = s->last_iters; out[2] = (double)s->last_err;
+    out[3] = (double)s->total_iters; out[4] = (double)s->worst_err; out[5] = s->steps_done;
+    out[6] = s->max_nbr_seen;
+  };
+  if (h->prec == 64) fill(h->d); else fill(h->f);
+  return 0;
+}
+
+// Sorted-order grid/neighbour arrays of the last step (for kernel-level parity).
+int oracle_system_arrays(void* hp, uint32_t* hash, uint32_t* perm, uint32_t* start,
+                         uint32_t* count, uint32_t* list) {
+  Handle* h = (Handle*)hp;
+  auto fill = [&](auto* s) {
+    if (hash) std::memcpy(hash, s->hash.data(), 4 * (size_t)s->n);
+    if (perm) std::memcpy(perm, s->perm.data(), 4 * (size_t)s->n);
+    if (start) std::memcpy(start, s->start.data(), 4 * (size_t)(s->g.ncells() + 1));
+    if (count) std::memcpy(count, s->count.data(), 4 * (size_t)s->n);
+    if (list) std::memcpy(list, s->list.data(), 4 * (size_t)s->n * s->prm.max_neighbors);
+  };
+  if (h->prec == 64) fill(h->d); else fill(h->f);
+  return 0;
+}
+
+}  // extern "C"

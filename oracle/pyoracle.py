@@ -1,0 +1,176 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  Build with `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB.exists():
+        args = ["make", "-s", "-C", str(HERE)] + (["-B"] if force else [])
+        subprocess.run(args, check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise ImportError(f"{LIB} is missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB))
+        vp = C.c_void_p
+        L.oracle_derive.argtypes = [vp, vp]
+        L.oracle_grid_build.restype = C.c_uint32
+        L.oracle_grid_build.argtypes = [C.c_uint32, vp, vp, vp, vp, vp]
+        L.oracle_neighbors.restype = C.c_uint32
+        L.oracle_neighbors.argtypes = [C.c_uint32, vp, vp, vp, C.c_float, C.c_uint32, vp, vp]
+        L.oracle_evaluate_densities_forces.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp,
+                                                       vp, vp, vp, vp]
+        L.oracle_collision_box.argtypes = [vp, C.c_uint32, vp, vp]
+        L.oracle_system_create.restype = vp
+        L.oracle_system_create.argtypes = [vp, C.c_uint32, C.c_int]
+        L.oracle_system_destroy.argtypes = [vp]
+        L.oracle_system_set_particles.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.oracle_system_update.argtypes = [vp, C.c_int]
+        L.oracle_system_get.argtypes = [vp] + [vp] * 9
+        L.oracle_system_stats.argtypes = [vp, vp]
+        L.oracle_system_arrays.argtypes = [vp, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def derive(params) -> np.ndarray:
+    out = np.zeros(16, np.float64)
+    lib().oracle_derive(C.addressof(params), out.ctypes.data)
+    return out
+
+
+def grid_build(x4: np.ndarray, gd):
+    x4 = np.ascontiguousarray(x4, np.float32)
+    n = x4.shape[0]
+    nc = gd.dims[0] * gd.dims[1] * gd.dims[2]
+    h = np.empty(n, np.uint32); perm = np.empty(n, np.uint32); start = np.empty(nc + 1, np.uint32)
+    flags = lib().oracle_grid_build(n, x4.ctypes.data, C.addressof(gd), h.ctypes.data,
+                                    perm.ctypes.data, start.ctypes.data)
+    return h, perm, start, flags
+
+
+def neighbors(sorted_x4: np.ndarray, gd, start: np.ndarray, radius: float, cap: int):
+    x = np.ascontiguousarray(sorted_x4, np.float32)
+    n = x.shape[0]
+    cnt = np.empty(n, np.uint32)
+    lst = np.zeros(cap * n, np.uint32)
+    flags = lib().oracle_neighbors(n, x.ctypes.data, C.addressof(gd), start.ctypes.data,
+                                   float(radius), cap, cnt.ctypes.data, lst.ctypes.data)
+    return cnt, lst.reshape(cap, n), flags
+
+
+def evaluate_densities_forces(params, count, lst, pos4, vel4):
+    n = pos4.shape[0]
+    out = {k: np.zeros((n, 4), np.float32) for k in ("normals", "fP", "fV", "fS", "F")}
+    rho = np.zeros(n, np.float32)
+    prs = np.zeros(n, np.float32)
+    lst = np.ascontiguousarray(lst, np.uint32)
+    lib().oracle_evaluate_densities_forces(
+        C.addressof(params), n, np.ascontiguousarray(count, np.uint32).ctypes.data, lst.ctypes.data,
+        np.ascontiguousarray(pos4, np.float32).ctypes.data,
+        np.ascontiguousarray(vel4, np.float32).ctypes.data, rho.ctypes.data, prs.ctypes.data,
+        out["normals"].ctypes.data, out["fP"].ctypes.data, out["fV"].ctypes.data,
+        out["fS"].ctypes.data, out["F"].ctypes.data)
+    out["density"] = rho
+    out["pressure"] = prs
+    return out
+
+
+def collision_box(params, pos4, vel4):
+    x = np.array(pos4, np.float32, copy=True)
+    v = np.array(vel4, np.float32, copy=True)
+    lib().oracle_collision_box(C.addressof(params), x.shape[0], x.ctypes.data, v.ctypes.data)
+    return x, v
+
+
+class OracleSystem:
+    """Oracle counterpart of psb_system (precision 32 = canonical FP32, 64 = FP64)."""
+
+    def __init__(self, params, capacity: int, precision: int = 32):
+        self.params = params
+        self.capacity = capacity
+        self.h = lib().oracle_system_create(C.addressof(params), capacity, precision)
+        self.n = 0
+
+    def set_particles(self, pos4, vel4=None, vh4=None):
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        self.n = pos4.shape[0]
+        v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        w = None if vh4 is None else np.ascontiguousarray(vh4, np.float32)
+        rc = lib().oracle_system_set_particles(self.h, self.n, pos4.ctypes.data, _p(v), _p(w))
+        assert rc == 0
+
+    def update(self, steps: int = 1):
+        lib().oracle_system_update(self.h, int(steps))
+
+    def state(self) -> dict:
+        n = self.n
+        o = {k: np.empty((n, 4), np.float32) for k in ("pos", "vel", "vh", "old_pos", "old_vel", "vav")}
+        o["density"] = np.empty(n, np.float32)
+        o["pressure"] = np.empty(n, np.float32)
+        o["ids"] = np.empty(n, np.uint32)
+        lib().oracle_system_get(self.h, o["pos"].ctypes.data, o["vel"].ctypes.data, o["vh"].ctypes.data,
+                                o["old_pos"].ctypes.data, o["old_vel"].ctypes.data,
+                                o["vav"].ctypes.data, o["density"].ctypes.data,
+                                o["pressure"].ctypes.data, o["ids"].ctypes.data)
+        return o
+
+    def stats(self) -> dict:
+        out = np.zeros(7, np.float64)
+        lib().oracle_system_stats(self.h, out.ctypes.data)
+        keys = ("flags", "iterations", "max_error", "total_iterations", "worst_error", "steps",
+                "max_neighbors_seen")
+        return {k: (int(v) if k not in ("max_error", "worst_error") else float(v))
+                for k, v in zip(keys, out)}
+
+    def arrays(self) -> dict:
+        n = self.n
+        gd = self.params.grid
+        nc = gd.dims[0] * gd.dims[1] * gd.dims[2]
+        cap = self.params.max_neighbors
+        o = {"hash": np.empty(n, np.uint32), "perm": np.empty(n, np.uint32),
+             "start": np.empty(nc + 1, np.uint32), "count": np.empty(n, np.uint32),
+             "list": np.empty(cap * n, np.uint32)}
+        lib().oracle_system_arrays(self.h, o["hash"].ctypes.data, o["perm"].ctypes.data,
+                                   o["start"].ctypes.data, o["count"].ctypes.data,
+                                   o["list"].ctypes.data)
+        o["list"] = o["list"].reshape(cap, n)
+        return o
+
+    def close(self):
+        if self.h:
+            lib().oracle_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(n)

@@ -1,0 +1,418 @@
+// grid.hip -- uniform grid: cell hash, stable counting sort, cell ranges,
+// reorder and neighbour search for gfx950.
+//
+// Replaces createGrid / reinitCells / storeParticles / searchNeighbooring
+// (UniformGrid.cu:17-99, UniformGridKernel.cu:12-95).  The reference inserts
+// particles into 64-slot cell buckets with atomicAdd (order depends on the
+// race, no overflow check) and writes a 200-wide neighbour list with a global
+// read-modify-write per hit.  Here:
+//
+//   1. k_hash_count   hash[i] = cell of x_i;  slot[i] = atomicAdd(count[c])
+//   2. k_scan_*       start = exclusive prefix sum of count over all cells
+//                     (start[c] = #particles with hash < c, start[ncells] = n);
+//                     the scan also re-zeroes count for the next build
+//   3. k_scatter      tmp[start[c] + slot[i]] = i      (order within a cell
+//                     still depends on the atomics)
+//   4. k_rank_reorder rank of i among its cell's tmp entries by index ->
+//                     dst = start[c] + rank: the stable (hash, index) order;
+//                     perm[dst] = i and the particle state is gathered here
+//
+// The dense lower-bound table start[] makes every stencil row of cells one
+// contiguous particle range [start[c0], start[c1 + 1]) because the hash is
+// x-fastest, so the neighbour kernel does two table loads per row.
+#include "psb_internal.h"
+
+namespace psb {
+
+__global__ __launch_bounds__(256) void k_hash_count(const float4* __restrict__ pos, uint32_t n,
+                                                    GridDev g, uint32_t* __restrict__ hash,
+                                                    uint32_t* __restrict__ slot,
+                                                    uint32_t* __restrict__ count,
+                                                    uint32_t* __restrict__ flags) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  uint32_t bad = 0;
+  int cx = cell_coord(p.x, g.minx, g.cell, g.nx, bad);
+  int cy = cell_coord(p.y, g.miny, g.cell, g.ny, bad);
+  int cz = cell_coord(p.z, g.minz, g.cell, g.nz, bad);
+  uint32_t c = (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
+  hash[i] = c;
+  slot[i] = atomicAdd(&count[c], 1u);
+  uint32_t f = 0;
+  if (bad) f |= PSB_FLAG_OUT_OF_GRID;
+  if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) f |= PSB_FLAG_NONFINITE;
+  if (f) atomicOr(flags, f);
+}
+
+// exclusive scan of one value per thread over a 256-thread block
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    uint32_t t = lds[w];
+    if (w < wid) woff += t;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return woff + x - v;
+}
+
+// per-tile sums of count
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ count,
+                                                     uint64_t ncells, uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t lds[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint32_t s = 0;
+  if (base + kScanTile <= ncells) {
+    const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      uint4 v = c4[threadIdx.x + 256 * j];
+      s += v.x + v.y + v.z + v.w;
+    }
+  } else {
+    for (uint64_t k = base + threadIdx.x; k < ncells; k += 256) s += count[k];
+  }
+  uint32_t tot;
+  block_excl_scan(s, lds, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// exclusive scan of the tile sums in one workgroup; start[ncells] = total
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ bsum, uint32_t nb,
+                                                    uint32_t* __restrict__ start, uint64_t ncells) {
+  __shared__ uint32_t lds[4];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+    uint32_t k = b0 + threadIdx.x;
+    uint32_t v = k < nb ? bsum[k] : 0u;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(v, lds, tot);
+    if (k < nb) bsum[k] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) start[ncells] = carry;
+}
+
+// start[c] = tile offset + in-tile exclusive prefix; count[c] = 0
+__global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ count, uint64_t ncells,
+                                                    const uint32_t* __restrict__ bsum,
+                                                    uint32_t* __restrict__ start) {
+  __shared__ uint32_t lds[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * 16;
+  uint32_t v[16];
+  const bool full = (uint64_t)(blockIdx.x + 1) * kScanTile <= ncells;
+  if (full) {
+    const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      uint4 t = c4[j];
+      v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = (base + j < ncells) ? count[base + j] : 0u;
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) { uint32_t t = v[j]; v[j] = s; s += t; }
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, lds, tot) + bsum[blockIdx.x];
+  if (full) {
+    uint4* s4 = reinterpret_cast<uint4*>(start + base);
+    uint4* c4 = reinterpret_cast<uint4*>(count + base);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      s4[j] = make_uint4(v[4 * j] + off, v[4 * j + 1] + off, v[4 * j + 2] + off, v[4 * j + 3] + off);
+      c4[j] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (base + j < ncells) { start[base + j] = v[j] + off; count[base + j] = 0u; }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ hash,
+                                                 const uint32_t* __restrict__ slot, uint32_t n,
+                                                 const uint32_t* __restrict__ start,
+                                                 uint32_t* __restrict__ tmp) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tmp[start[hash[i]] + slot[i]] = i;
+}
+
+// Stable rank within the cell + gather of the particle state.
+template <bool STATE>
+__global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict__ hash, uint32_t n,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint32_t* __restrict__ tmp,
+                                                      uint32_t* __restrict__ perm, ReorderArrays ra) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = hash[i];
+  const uint32_t s = start[c], e = start[c + 1];
+  uint32_t rank = 0;
+  if (e - s > 1)
+    for (uint32_t q = s; q < e; ++q) rank += (tmp[q] < i) ? 1u : 0u;
+  const uint32_t dst = s + rank;
+  perm[dst] = i;
+  if (STATE) {
+    ra.dx[dst] = ra.sx[i];
+    ra.dv[dst] = ra.sv[i];
+    ra.dvh[dst] = ra.svh[i];
+    ra.did[dst] = ra.sid[i];
+    ra.dp[dst] = ra.sp[i];
+  }
+}
+
+// searchNeighbooring (UniformGridKernel.cu:58-95) on sorted positions; the
+// density sum of the same pairs rides along (SphKernel.cu:9-27).
+template <bool DENS>
+__global__ __launch_bounds__(256) void k_neighbors(const float4* xs, uint32_t n, GridDev g,
+                                                   const uint32_t* __restrict__ start, float r2,
+                                                   uint32_t cap, uint32_t* __restrict__ ncount,
+                                                   uint32_t* __restrict__ nlist,
+                                                   float* __restrict__ rho, float mc,
+                                                   uint32_t* __restrict__ flags,
+                                                   uint32_t* __restrict__ maxcnt) {
+  const uint32_t p = particle_index();
+  uint32_t cnt = 0;
+  if (p < n) {
+    const float4 xi = xs[p];
+    uint32_t bad = 0;
+    const int cx = cell_coord(xi.x, g.minx, g.cell, g.nx, bad);
+    const int cy = cell_coord(xi.y, g.miny, g.cell, g.ny, bad);
+    const int cz = cell_coord(xi.z, g.minz, g.cell, g.nz, bad);
+    const int R = g.R;
+    const int x0 = max(cx - R, 0), x1 = min(cx + R, g.nx - 1);
+    float S = 0.f;
+    for (int dz = -R; dz <= R; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= g.nz) continue;
+      for (int dy = -R; dy <= R; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= g.ny) continue;
+        const uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+        const uint32_t q0 = start[base + x0], q1 = start[base + x1 + 1];
+        for (uint32_t q = q0; q < q1; ++q) {
+          const float4 xj = xs[q];
+          const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
+          const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
+          if (d2 <= r2) {
+            if (cnt < cap) nlist[(uint64_t)cnt * n + p] = q;
+            cnt++;
+            if (DENS) {
+              const float w = r2 - d2;
+              S = __builtin_fmaf(w * w, w, S);
+            }
+          }
+        }
+      }
+    }
+    ncount[p] = cnt < cap ? cnt : cap;
+    if (DENS) rho[p] = S * mc;
+    if (cnt > cap) atomicOr(flags, PSB_FLAG_NEIGHBOR_OVERFLOW);
+  }
+  if (maxcnt) {
+    uint32_t m = wave_max_u(cnt);
+    if ((threadIdx.x & 63u) == 0 && m) atomicMax(maxcnt, m);
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ src, float* __restrict__ dst,
+                                                const uint32_t* __restrict__ perm, uint32_t n) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (W == 4) reinterpret_cast<float4*>(dst)[k] = reinterpret_cast<const float4*>(src)[perm[k]];
+  else dst[k] = src[perm[k]];
+}
+
+psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
+                      hipStream_t s) {
+  GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+  const uint32_t nb = blocks_for(n);
+  if (n) {
+    hipLaunchKernelGGL(k_hash_count, dim3(nb), dim3(kBlock), 0, s, pos, n, gd, g->hash, g->slot,
+                       g->count, g->flags);
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, g->bsum, g->nscan, g->start, g->ncells);
+  hipLaunchKernelGGL(k_scan_apply, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum,
+                     g->start);
+  if (n) {
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, g->hash, g->slot, n, g->start, g->tmp);
+    if (ra) {
+      hipLaunchKernelGGL(k_rank_reorder<true>, dim3(nb), dim3(kBlock), 0, s, g->hash, n, g->start,
+                         g->tmp, g->perm, *ra);
+    } else {
+      ReorderArrays none{};
+      hipLaunchKernelGGL(k_rank_reorder<false>, dim3(nb), dim3(kBlock), 0, s, g->hash, n, g->start,
+                         g->tmp, g->perm, none);
+    }
+  }
+  PSB_HIP(hipGetLastError());
+  g->n = n;
+  return PSB_OK;
+}
+
+psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
+                            psb_neighbors nb, float* rho, float mc, uint32_t* flags,
+                            uint32_t* maxcnt, hipStream_t s) {
+  if (!n) return PSB_OK;
+  GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+  gd.R = R;
+  if (rho)
+    hipLaunchKernelGGL(k_neighbors<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, xs, n, gd,
+                       g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt);
+  else
+    hipLaunchKernelGGL(k_neighbors<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, xs, n, gd,
+                       g->start, r2, nb.capacity, nb.count, nb.list, nullptr, 0.f, flags, maxcnt);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+static psb_status grid_alloc(psb_grid g) {
+  g->ncells = (uint64_t)g->desc.dims[0] * g->desc.dims[1] * g->desc.dims[2];
+  g->nscan = (uint32_t)((g->ncells + kScanTile - 1) / kScanTile);
+  const size_t cap = g->capacity ? g->capacity : 1;
+  PSB_HIP(hipMalloc(&g->count, 4 * (g->nscan * (size_t)kScanTile)));
+  PSB_HIP(hipMalloc(&g->start, 4 * (g->ncells + 1)));
+  PSB_HIP(hipMalloc(&g->hash, 4 * cap));
+  PSB_HIP(hipMalloc(&g->slot, 4 * cap));
+  PSB_HIP(hipMalloc(&g->tmp, 4 * cap));
+  PSB_HIP(hipMalloc(&g->perm, 4 * cap));
+  PSB_HIP(hipMalloc(&g->bsum, 4 * (size_t)g->nscan));
+  PSB_HIP(hipMalloc(&g->flags, 64));
+  PSB_HIP(hipMemsetAsync(g->count, 0, 4 * (g->nscan * (size_t)kScanTile), g->ctx->stream));
+  PSB_HIP(hipMemsetAsync(g->flags, 0, 64, g->ctx->stream));
+  PSB_HIP(hipStreamSynchronize(g->ctx->stream));
+  return PSB_OK;
+}
+
+psb_status grid_create(psb_context ctx, const psb_grid_desc& d, uint32_t capacity, psb_grid* out) {
+  PSB_ARG(ctx && out, "null argument");
+  PSB_ARG(d.cell_size > 0.f && d.dims[0] > 0 && d.dims[1] > 0 && d.dims[2] > 0, "bad grid");
+  PSB_ARG((uint64_t)d.dims[0] * d.dims[1] * d.dims[2] < (1ull << 32) - 1, "grid too large");
+  PSB_HIP(hipSetDevice(ctx->device));
+  psb_grid g = new psb_grid_s();
+  g->ctx = ctx;
+  g->desc = d;
+  g->capacity = capacity;
+  psb_status st = grid_alloc(g);
+  if (st != PSB_OK) { psb_finalize_grid(g); return st; }
+  *out = g;
+  return PSB_OK;
+}
+
+}  // namespace psb
+
+using namespace psb;
+
+extern "C" {
+
+psb_status psb_create_grid(psb_context ctx, double xc, double yc, double zc, double length,
+                           double width, double depth, float cell_size, uint32_t capacity,
+                           psb_grid* out) {
+  PSB_ARG(cell_size > 0.f && length > 0 && width > 0 && depth > 0, "bad grid extent");
+  psb_grid_desc d{};
+  // createGrid (UniformGrid.cu:19-29): Min = C - extent/2 (stored as float),
+  // nbCells = ceil(extent / sizeCell)
+  d.min[0] = (float)(xc - length / 2);
+  d.min[1] = (float)(yc - width / 2);
+  d.min[2] = (float)(zc - depth / 2);
+  d.cell_size = cell_size;
+  d.dims[0] = (int32_t)std::ceil(length / cell_size);
+  d.dims[1] = (int32_t)std::ceil(width / cell_size);
+  d.dims[2] = (int32_t)std::ceil(depth / cell_size);
+  return grid_create(ctx, d, capacity, out);
+}
+
+psb_status psb_create_grid_desc(psb_context ctx, const psb_grid_desc* desc, uint32_t capacity,
+                                psb_grid* out) {
+  PSB_ARG(desc, "null desc");
+  return grid_create(ctx, *desc, capacity, out);
+}
+
+psb_status psb_finalize_grid(psb_grid g) {
+  if (!g) return PSB_OK;
+  if (g->ctx) { hipSetDevice(g->ctx->device); hipStreamSynchronize(g->ctx->stream); }
+  hipFree(g->count); hipFree(g->start); hipFree(g->hash); hipFree(g->slot);
+  hipFree(g->tmp); hipFree(g->perm); hipFree(g->bsum); hipFree(g->flags);
+  delete g;
+  return PSB_OK;
+}
+
+psb_status psb_grid_get_desc(psb_grid g, psb_grid_desc* out) {
+  PSB_ARG(g && out, "null argument");
+  *out = g->desc;
+  return PSB_OK;
+}
+
+psb_status psb_store_particles(psb_grid g, const float* pos4, uint32_t n) {
+  PSB_ARG(g && (pos4 || !n), "null argument");
+  if (n > g->capacity) { set_error("n exceeds grid capacity"); return PSB_ERR_CAPACITY; }
+  PSB_HIP(hipSetDevice(g->ctx->device));
+  PSB_HIP(hipMemsetAsync(g->flags, 0, 4, g->ctx->stream));
+  return grid_build(g, reinterpret_cast<const float4*>(pos4), n, nullptr, g->ctx->stream);
+}
+
+psb_status psb_grid_get_arrays(psb_grid g, const uint32_t** hash, const uint32_t** perm,
+                               const uint32_t** cell_start, uint64_t* ncells) {
+  PSB_ARG(g, "null grid");
+  if (hash) *hash = g->hash;
+  if (perm) *perm = g->perm;
+  if (cell_start) *cell_start = g->start;
+  if (ncells) *ncells = g->ncells;
+  return PSB_OK;
+}
+
+psb_status psb_grid_reorder(psb_grid g, const float* src, float* dst, uint32_t n,
+                            int32_t elem_floats) {
+  PSB_ARG(g && src && dst, "null argument");
+  PSB_ARG(elem_floats == 1 || elem_floats == 4, "elem_floats must be 1 or 4");
+  PSB_ARG(n == g->n, "n differs from the last psb_store_particles");
+  if (!n) return PSB_OK;
+  if (elem_floats == 4)
+    hipLaunchKernelGGL(k_gather<4>, dim3(blocks_for(n)), dim3(kBlock), 0, g->ctx->stream, src, dst,
+                       g->perm, n);
+  else
+    hipLaunchKernelGGL(k_gather<1>, dim3(blocks_for(n)), dim3(kBlock), 0, g->ctx->stream, src, dst,
+                       g->perm, n);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status psb_search_neighbors(psb_grid g, const float* sorted_pos4, uint32_t n, float radius,
+                                psb_neighbors nb, uint32_t* flags_out) {
+  PSB_ARG(g && (sorted_pos4 || !n), "null argument");
+  PSB_ARG(nb.count && nb.list && nb.capacity > 0, "bad neighbour buffers");
+  PSB_ARG(nb.n == n, "neighbors.n must equal n (ELL stride)");
+  PSB_ARG(n == g->n, "n differs from the last psb_store_particles");
+  PSB_ARG(radius > 0.f, "radius must be > 0");
+  psb_context ctx = g->ctx;
+  PSB_HIP(hipSetDevice(ctx->device));
+  PSB_HIP(hipMemsetAsync(ctx->d_flags, 0, 8, ctx->stream));
+  GridDev gd = grid_dev(g->desc, radius);
+  PSB_TRY(launch_neighbors(g, reinterpret_cast<const float4*>(sorted_pos4), n, radius * radius,
+                           gd.R, nb, nullptr, 0.f, ctx->d_flags, ctx->d_flags + 1, ctx->stream));
+  PSB_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+  PSB_HIP(hipMemcpyAsync(ctx->h_pinned + 2, g->flags, 4, hipMemcpyDeviceToHost, ctx->stream));
+  PSB_HIP(hipStreamSynchronize(ctx->stream));
+  if (flags_out) *flags_out = ctx->h_pinned[0] | ctx->h_pinned[2];
+  return PSB_OK;
+}
+
+}  // extern "C"

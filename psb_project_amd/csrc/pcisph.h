@@ -1,0 +1,65 @@
+// pcisph.h -- PCISPH device structures and launchers (internal).
+#pragma once
+#include "psb_internal.h"
+
+namespace psb {
+
+// Device-side state of one PCISPH solve.  The first 16 bytes are copied to
+// the host to learn whether (and when) the corrector finished.
+struct PciStatus {
+  int32_t final_iter;              // -1 until a force pass finalised the step
+  int32_t last_iters;              // corrector iterations of the last step
+  uint32_t last_err;               // float bits: max compression error, last step
+  uint32_t worst_err;              // float bits: largest final error since reset
+  unsigned long long total_iters;  // iterations since reset
+  uint32_t pad[2];
+  uint32_t err[kMaxIterSlots];     // float bits: max error of D(k)
+};
+
+struct PciBuffers {
+  float4* fa;    // reference mode: Facc + f_viscosity (force density)
+  float4* fs;    // reference mode: f_surface
+  float4* anp;   // config mode: non-pressure acceleration
+  float4* xp0;   // first predicted positions
+  const float4* ax;   // start-of-step positions
+  const float4* avh;  // start-of-step half-step velocities
+  const uint32_t* aid;
+};
+
+struct PciOut {
+  // fused step: end-of-step state
+  float4* bx; float4* bv; float4* bvh; uint32_t* bid; float* bp;
+  // standalone pci_SPH_update outputs
+  float4* pos; float4* old_pos; float4* vav; float4* vap;
+  float4* fP; float4* normals; float4* fV; float4* fS; float4* F;
+  uint32_t* flags;
+};
+
+psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s);
+psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
+                                  const uint32_t* cnt, const uint32_t* list, const float4* xs,
+                                  const float4* vs, const float* rho, const float4* avh,
+                                  const float* pres, const PciBuffers& b, const PciOut& o,
+                                  hipStream_t s);
+psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
+                              const uint32_t* list, float4* X, float* pres, float* rho_pred,
+                              PciStatus* st, int k, hipStream_t s);
+psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
+                            const uint32_t* list, const float4* X, float4* Xn,
+                            const float* rho_pred, const float* pres, const PciBuffers& b,
+                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s);
+
+// sph.hip
+psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, const float4* pos,
+                               const float4* vel, float* rho, float* pres, float4* xs, float4* vs,
+                               hipStream_t s);
+psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
+                                   const float* rho, float* pres, const float* pprev, float4* xs,
+                                   float4* vs, hipStream_t s);
+psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
+                                    const uint32_t* list, const float4* xs, const float4* vs,
+                                    const float* rho, const float* pres, const float4* avh,
+                                    const uint32_t* aid, float4* bx, float4* bv, float4* bvh,
+                                    uint32_t* bid, float* bp, uint32_t* flags, hipStream_t s);
+
+}  // namespace psb

@@ -1,0 +1,425 @@
+// pcisph.hip -- PCISPH predict-correct kernels for gfx950.
+//
+// Reference: pci_SPH_update (PciSphSystem.cu:8-68) and its kernels
+// compute_Density_PCI / compute_Pressure_PCI / compute_Pressure_Force_PCI /
+// compute_Viscosity_SurfaceTension_Force_PCI / restore_Pos_Vel_PCI
+// (PciSphSystem_Kernel.cu:36-202) and integrateSPH_LeapFrog_Forces
+// (SphKernel.cu:95-109).  The reference runs, per iteration, five kernels
+// separated by host synchronisations; here one iteration is two kernels:
+//
+//   D(k): predicted density at X (the predicted positions), pressure update,
+//         per-particle compression error max(0, rho - rho0)/rho0 reduced to one
+//         atomicMax per workgroup into err[k];
+//   F(k): pressure force at X, then either the next prediction (written to
+//         the other X buffer: the "restore" of the reference is implicit, the
+//         prediction always starts from the saved start state) or, on the
+//         final iteration, the leapfrog update + box + velocity interpolation.
+//
+// F(k) decides on the device whether iteration k was the last one
+// (iterations >= min and err[k] <= tolerance, or iterations == max) and
+// records it in PciStatus.final_iter; every later D/F launch of the same step
+// returns immediately, so the host can queue iterations ahead without
+// waiting for each error.
+#include <cstring>
+
+#include "psb_internal.h"
+#include "pcisph.h"
+#include "sph_kernels.h"
+
+namespace psb {
+
+__global__ void k_pci_reset(PciStatus* st) {
+  for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
+  if (threadIdx.x == 0) st->final_iter = -1;
+}
+
+__global__ void k_pci_stats_reset(PciStatus* st) {
+  if (threadIdx.x == 0) {
+    st->total_iters = 0;
+    st->worst_err = 0u;
+    st->last_iters = 0;
+    st->last_err = 0u;
+    st->final_iter = -1;
+  }
+}
+
+// compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:117-170)
+// + first prediction (integrateSPH_LeapFrog_Forces with f_P = 0).
+template <bool REF, bool SURF, bool WARM, bool OUT>
+__global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ list,
+                                                         const float4* __restrict__ xs,
+                                                         const float4* __restrict__ vs,
+                                                         const float* __restrict__ rho,
+                                                         const float4* __restrict__ avh,
+                                                         const float* __restrict__ pres,
+                                                         PciBuffers b, PciOut o) {
+  const uint32_t p = particle_index();
+  if (p >= n) return;
+  PairSums s = internal_sums<false, SURF>(c, p, n, cnt, list, xs, vs);
+  const float ri = rho[p];
+  const float sigma = REF ? 0.0728f : c.sigma;
+  float fV[3] = {s.fv[0] * c.cv, s.fv[1] * c.cv, s.fv[2] * c.cv};
+  float fS[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f};
+  if (SURF) surface_finish(c, s, sigma, fS, nrm);
+  const float4 x = xs[p], vh0 = avh[p];
+  float4 xp;
+  if (REF) {
+    // forcesAccum = (0, -9.81 rho, 0) (PciSphSystem_Kernel.cu:168);
+    // predictor F = ((Facc + fV) + fP) + fS with fP = 0 (:127, SphKernel.cu:103)
+    const float gy = -9.81f;
+    const float fa0 = 0.f + fV[0], fa1 = (gy * ri) + fV[1], fa2 = 0.f + fV[2];
+    b.fa[p] = make_float4(fa0, fa1, fa2, 0.f);
+    b.fs[p] = make_float4(fS[0], fS[1], fS[2], 0.f);
+    const float F0 = (fa0 + 0.f) + fS[0], F1 = (fa1 + 0.f) + fS[1], F2 = (fa2 + 0.f) + fS[2];
+    const float inv = 1.0f / ri;
+    const float v0 = vh0.x + (F0 * inv) * c.dt, v1 = vh0.y + (F1 * inv) * c.dt,
+                v2 = vh0.z + (F2 * inv) * c.dt;
+    xp = make_float4(x.x + v0 * c.dt, x.y + v1 * c.dt, x.z + v2 * c.dt, 0.f);
+    if (OUT) {
+      if (o.F) o.F[p] = make_float4(0.f, gy * ri, 0.f, 0.f);
+    }
+  } else {
+    const float inv = 1.0f / ri;
+    const float a0 = (fV[0] + fS[0]) * inv + c.gx, a1 = (fV[1] + fS[1]) * inv + c.gy,
+                a2 = (fV[2] + fS[2]) * inv + c.gz;
+    b.anp[p] = make_float4(a0, a1, a2, 0.f);
+    if (WARM) {
+      xp = make_float4(x.x, x.y, x.z, pres[p]);
+    } else {
+      const float v0 = vh0.x + a0 * c.dt, v1 = vh0.y + a1 * c.dt, v2 = vh0.z + a2 * c.dt;
+      xp = make_float4(x.x + v0 * c.dt, x.y + v1 * c.dt, x.z + v2 * c.dt, 0.f);
+    }
+    if (OUT) {
+      if (o.F) o.F[p] = make_float4(a0, a1, a2, 0.f);
+    }
+  }
+  b.xp0[p] = xp;
+  if (OUT) {
+    if (o.normals) o.normals[p] = make_float4(nrm[0], nrm[1], nrm[2], 0.f);
+    if (o.fV) o.fV[p] = make_float4(fV[0], fV[1], fV[2], 0.f);
+    if (o.fS) o.fS[p] = make_float4(fS[0], fS[1], fS[2], 0.f);
+  }
+}
+
+// compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
+// in reference mode; template-delta update p = max(0, p + delta (rho - rho0))
+// in config mode.  X[p].w <- p/rho^2 (reference) or p (config).
+template <bool REF>
+__global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
+                                                     const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ list, float4* X,
+                                                     float* __restrict__ pres,
+                                                     float* __restrict__ rho_pred,
+                                                     PciStatus* __restrict__ st, int k) {
+  __shared__ float red[4];
+  {
+    const int fi = st->final_iter;
+    if (fi >= 0 && fi < k) return;
+  }
+  const uint32_t p = particle_index();
+  float err = 0.f;
+  if (p < n) {
+    const float4 xi = X[p];
+    float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
+    const uint32_t m = cnt[p];
+    for (uint32_t kk = 0; kk < m; ++kk) {
+      const uint32_t q = list[(uint64_t)kk * n + p];
+      const float4 xj = X[q];
+      const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+      float w = c.h2 - d2;
+      if (!REF) w = fmaxf(w, 0.f);
+      S = __builtin_fmaf(w * w, w, S);
+      if (REF && d2 > 0.f) {
+        const float dd = sqrtf(d2);
+        const float invd = 1.0f / dd;
+        const float hd = c.h - dd;
+        const float gw = c.c_spiky * (hd * hd);
+        const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+        W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+      }
+    }
+    const float r = S * c.mc;
+    float pr = pres[p];
+    const float e = r - c.rho0;
+    if (REF) {
+      if (r > c.rho0 && W2 > 0.f) pr = pr + e / (c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
+      reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr / (r * r);
+    } else {
+      pr = pr + c.delta * e;
+      if (pr < 0.f) pr = 0.f;
+      reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr;
+    }
+    rho_pred[p] = r;
+    pres[p] = pr;
+    err = (e > 0.f ? e : 0.f) * c.inv_rho0;
+  }
+  err = wave_max<64>(err);
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = err;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0.f) atomicMax(&st->err[k], __float_as_uint(m));
+  }
+}
+
+// compute_Pressure_Force_PCI (PciSphSystem_Kernel.cu:88-114) +
+// integrateSPH_LeapFrog_Forces from the start state (SphKernel.cu:95-109,
+// restore_Pos_Vel_PCI :190-202).  warm_pass: the pass before the first
+// density evaluation that turns a warm-start pressure into a force.
+template <bool REF, bool STEP>
+__global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
+                                                   const uint32_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ list,
+                                                   const float4* __restrict__ X,
+                                                   float4* __restrict__ Xn,
+                                                   const float* __restrict__ rho_pred,
+                                                   const float* __restrict__ pres, PciBuffers b,
+                                                   PciStatus* __restrict__ st, int k,
+                                                   int warm_pass, PciOut o) {
+  bool fin = false;
+  float err = 0.f;
+  if (!warm_pass) {
+    const int fi = st->final_iter;
+    if (fi >= 0 && fi < k) return;
+    err = __uint_as_float(st->err[k]);
+    const int it = k + 1;
+    fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it;
+    if (fin && blockIdx.x == 0 && threadIdx.x == 0) {
+      st->final_iter = k;
+      st->last_iters = it;
+      st->last_err = __float_as_uint(err);
+      atomicAdd(&st->total_iters, (unsigned long long)it);
+      atomicMax(&st->worst_err, __float_as_uint(err));
+      if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
+    }
+  }
+  const uint32_t p = particle_index();
+  if (p >= n) return;
+  const float4 xi = X[p];
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  const uint32_t m = cnt[p];
+  for (uint32_t kk = 0; kk < m; ++kk) {
+    const uint32_t q = list[(uint64_t)kk * n + p];
+    const float4 xj = X[q];
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (REF) {
+      if (d2 > 0.f) {
+        const float dd = sqrtf(d2);
+        const float invd = 1.0f / dd;
+        const float hd = c.h - dd;
+        const float gw = c.c_spiky * (hd * hd);
+        const float cc = gw * (xi.w + xj.w);
+        sx = __builtin_fmaf(dx * invd, cc, sx);
+        sy = __builtin_fmaf(dy * invd, cc, sy);
+        sz = __builtin_fmaf(dz * invd, cc, sz);
+      }
+    } else if (d2 > 0.f && d2 < c.h2) {
+      const float dd = sqrtf(d2);
+      const float invd = 1.0f / dd;
+      const float hd = c.h - dd;
+      const float gw = c.c_spiky * (hd * hd);
+      const float cc = (gw * invd) * (xi.w + xj.w);
+      sx = __builtin_fmaf(dx, cc, sx);
+      sy = __builtin_fmaf(dy, cc, sy);
+      sz = __builtin_fmaf(dz, cc, sz);
+    }
+  }
+  const float4 x0 = b.ax[p], vh0 = b.avh[p];
+  float4 vh;
+  float fP0 = 0.f, fP1 = 0.f, fP2 = 0.f;
+  if (REF) {
+    const float rp = rho_pred[p];
+    const float cpf = -(c.m * rp);
+    fP0 = sx * cpf; fP1 = sy * cpf; fP2 = sz * cpf;
+    const float4 a = b.fa[p], s = b.fs[p];
+    const float F0 = (a.x + fP0) + s.x, F1 = (a.y + fP1) + s.y, F2 = (a.z + fP2) + s.z;
+    const float inv = 1.0f / rp;
+    vh = make_float4(vh0.x + (F0 * inv) * c.dt, vh0.y + (F1 * inv) * c.dt,
+                     vh0.z + (F2 * inv) * c.dt, 0.f);
+  } else {
+    const float4 a = b.anp[p];
+    fP0 = sx * c.kp; fP1 = sy * c.kp; fP2 = sz * c.kp;
+    const float a0 = a.x + fP0, a1 = a.y + fP1, a2 = a.z + fP2;
+    vh = make_float4(vh0.x + a0 * c.dt, vh0.y + a1 * c.dt, vh0.z + a2 * c.dt, 0.f);
+  }
+  float4 x = make_float4(x0.x + vh.x * c.dt, x0.y + vh.y * c.dt, x0.z + vh.z * c.dt, 0.f);
+  if (!fin) {
+    Xn[p] = x;
+    return;
+  }
+  if (STEP) {
+    finish_particle(c, p, x, vh, vh0, o.bx, o.bv, o.bvh);
+    o.bid[p] = b.aid[p];
+    o.bp[p] = pres[p];
+  } else {
+    // standalone pci_SPH_update: new state without collision / interpolation
+    o.pos[p] = make_float4(x.x, x.y, x.z, 0.f);
+    o.old_pos[p] = make_float4(x0.x, x0.y, x0.z, 0.f);
+    o.vav[p] = vh0;
+    o.vap[p] = vh;
+    if (o.fP) o.fP[p] = make_float4(fP0, fP1, fP2, 0.f);
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s) {
+  if (stats) hipLaunchKernelGGL(k_pci_stats_reset, dim3(1), dim3(64), 0, s, st);
+  hipLaunchKernelGGL(k_pci_reset, dim3(1), dim3(256), 0, s, st);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
+                                  const uint32_t* cnt, const uint32_t* list, const float4* xs,
+                                  const float4* vs, const float* rho, const float4* avh,
+                                  const float* pres, const PciBuffers& b, const PciOut& o,
+                                  hipStream_t s) {
+  if (!n) return PSB_OK;
+  const bool ref = c.mode == PSB_MODE_REFERENCE;
+  dim3 g(blocks_for(n)), t(kBlock);
+#define PSB_NP(R_, S_, W_, O_)                                                           \
+  hipLaunchKernelGGL((k_pci_nonpressure<R_, S_, W_, O_>), g, t, 0, s, n, c, cnt, list, xs, vs, \
+                     rho, avh, pres, b, o)
+  if (ref) {
+    if (out) PSB_NP(true, true, false, true); else PSB_NP(true, true, false, false);
+  } else if (surf) {
+    if (warm) { if (out) PSB_NP(false, true, true, true); else PSB_NP(false, true, true, false); }
+    else { if (out) PSB_NP(false, true, false, true); else PSB_NP(false, true, false, false); }
+  } else {
+    if (warm) { if (out) PSB_NP(false, false, true, true); else PSB_NP(false, false, true, false); }
+    else { if (out) PSB_NP(false, false, false, true); else PSB_NP(false, false, false, false); }
+  }
+#undef PSB_NP
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
+                              const uint32_t* list, float4* X, float* pres, float* rho_pred,
+                              PciStatus* st, int k, hipStream_t s) {
+  if (!n) return PSB_OK;
+  if (c.mode == PSB_MODE_REFERENCE)
+    hipLaunchKernelGGL(k_pci_density<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt, list,
+                       X, pres, rho_pred, st, k);
+  else
+    hipLaunchKernelGGL(k_pci_density<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt,
+                       list, X, pres, rho_pred, st, k);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
+                            const uint32_t* list, const float4* X, float4* Xn,
+                            const float* rho_pred, const float* pres, const PciBuffers& b,
+                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s) {
+  if (!n) return PSB_OK;
+  const bool ref = c.mode == PSB_MODE_REFERENCE;
+  dim3 g(blocks_for(n)), t(kBlock);
+  if (ref) {
+    if (step) hipLaunchKernelGGL((k_pci_force<true, true>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
+    else hipLaunchKernelGGL((k_pci_force<true, false>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
+  } else {
+    if (step) hipLaunchKernelGGL((k_pci_force<false, true>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
+    else hipLaunchKernelGGL((k_pci_force<false, false>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
+  }
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+}  // namespace psb
+
+using namespace psb;
+
+extern "C" {
+
+psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_neighbors nb,
+                              float* old_pos4, float* pos4, const float* vel4,
+                              float* vel_inter_av4, float* vel_inter_ap4, float* density,
+                              float* pressure, float* normals4, float* f_viscosity4,
+                              float* f_surface4, float* f_pressure4, float* forces_accum4,
+                              uint32_t n, int32_t* iterations, float* max_error) {
+  PSB_ARG(ctx && prm, "null argument");
+  PSB_ARG(n == 0 || (old_pos4 && pos4 && vel4 && vel_inter_av4 && vel_inter_ap4 && density &&
+                     pressure),
+          "null array");
+  PSB_ARG(nb.n == n && nb.count && nb.list, "neighbour list stride must equal n");
+  PSB_ARG(prm->warm_start != PSB_WARM_MLP, "the MLP warm start needs a psb_system");
+  Consts c;
+  PSB_TRY(make_consts(*prm, &c, nullptr));
+  if (!n) return PSB_OK;
+  PSB_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t v4 = 16 * (size_t)n;
+  PSB_TRY(ensure_scratch(ctx, 7 * v4 + sizeof(PciStatus) + 256 + 4 * (size_t)n));
+  char* base = (char*)ctx->scratch;
+  float4* xs = (float4*)base;
+  float4* vs = (float4*)(base + v4);
+  PciBuffers b{};
+  b.fa = (float4*)(base + 2 * v4);
+  b.fs = (float4*)(base + 3 * v4);
+  b.anp = (float4*)(base + 4 * v4);
+  b.xp0 = (float4*)(base + 5 * v4);
+  float4* xp1 = (float4*)(base + 6 * v4);
+  float* tmp_p = (float*)(base + 7 * v4);
+  PciStatus* st = (PciStatus*)(((uintptr_t)(base + 7 * v4 + 4 * (size_t)n) + 255) & ~(uintptr_t)255);
+  b.ax = reinterpret_cast<const float4*>(pos4);
+  b.avh = reinterpret_cast<const float4*>(vel_inter_ap4);
+  b.aid = nullptr;
+  const bool ref = prm->mode == PSB_MODE_REFERENCE;
+  const bool warm = !ref && prm->warm_start == PSB_WARM_PREVIOUS;
+  PSB_TRY(launch_pci_reset(st, true, s));
+  PSB_HIP(hipMemsetAsync(ctx->d_flags, 0, 4, s));
+  // start density (PciSphSystem.cu:17), scratch copies with v.w = 1/rho
+  PSB_TRY(launch_density_list(n, c, nb, reinterpret_cast<const float4*>(pos4),
+                              reinterpret_cast<const float4*>(vel4), density, tmp_p, xs, vs, s));
+  if (!warm) PSB_HIP(hipMemsetAsync(pressure, 0, 4 * (size_t)n, s));
+  PciOut o{};
+  o.normals = reinterpret_cast<float4*>(normals4);
+  o.fV = reinterpret_cast<float4*>(f_viscosity4);
+  o.fS = reinterpret_cast<float4*>(f_surface4);
+  o.F = reinterpret_cast<float4*>(forces_accum4);
+  o.fP = reinterpret_cast<float4*>(f_pressure4);
+  o.pos = reinterpret_cast<float4*>(pos4);
+  o.old_pos = reinterpret_cast<float4*>(old_pos4);
+  o.vav = reinterpret_cast<float4*>(vel_inter_av4);
+  o.vap = reinterpret_cast<float4*>(vel_inter_ap4);
+  o.flags = ctx->d_flags;
+  const bool surf = ref || prm->surface_tension != 0.f;
+  PSB_TRY(launch_pci_nonpressure(n, c, surf, warm, true, nb.count, nb.list, xs, vs, density,
+                                 b.avh, pressure, b, o, s));
+  float4* cur = b.xp0;
+  float4* nxt = xp1;
+  if (warm) {
+    PSB_TRY(launch_pci_force(n, c, false, nb.count, nb.list, cur, nxt, density, pressure, b, st, -1,
+                             1, o, s));
+    std::swap(cur, nxt);
+  }
+  int iters = 0;
+  float err = 0.f;
+  for (int k = 0; k < c.max_it; ++k) {
+    PSB_TRY(launch_pci_density(n, c, nb.count, nb.list, cur, pressure, density, st, k, s));
+    PSB_TRY(launch_pci_force(n, c, false, nb.count, nb.list, cur, nxt, density, pressure, b, st, k,
+                             0, o, s));
+    std::swap(cur, nxt);
+    if (k + 1 >= c.min_it) {
+      PSB_HIP(hipMemcpyAsync(ctx->h_pinned, st, 16, hipMemcpyDeviceToHost, s));
+      PSB_HIP(hipStreamSynchronize(s));
+      const PciStatus* hs = reinterpret_cast<const PciStatus*>(ctx->h_pinned);
+      if (hs->final_iter >= 0) {
+        iters = hs->last_iters;
+        std::memcpy(&err, &hs->last_err, 4);
+        break;
+      }
+    }
+  }
+  if (iterations) *iterations = iters;
+  if (max_error) *max_error = err;
+  return PSB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// psb_internal.h -- internal declarations of the gfx950 SPH/PCISPH library.
+//
+// Layout in HBM (all FP32, sorted particle order after each grid build):
+//   vec4 arrays (16 B/particle): positions, velocities, half-step velocities,
+//     predicted positions, per-particle force terms.  The w lane carries a
+//     per-particle scalar that neighbour passes gather together with xyz
+//     (p/rho^2, 1/rho or the PCISPH pressure), so one 16-byte load serves a
+//     neighbour.
+//   scalar arrays (4 B/particle): density, pressure, ids, grid hash/perm.
+//   cell table: start[ncells + 1] (u32), count[ncells] (u32).
+//   neighbour list: ELL, list[k * n + i] (u32), count[n].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/psb_sph.h"
+
+namespace psb {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+psb_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define PSB_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) return ::psb::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define PSB_TRY(call)                                                        \
+  do {                                                                       \
+    psb_status s_ = (call);                                                  \
+    if (s_ != PSB_OK) return s_;                                             \
+  } while (0)
+
+#define PSB_ARG(cond, msg)                                                   \
+  do {                                                                       \
+    if (!(cond)) { ::psb::set_error(msg); return PSB_ERR_INVALID_ARGUMENT; } \
+  } while (0)
+
+constexpr uint32_t kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr uint32_t kScanTile = 4096;      // cells per scan workgroup (256 x 16)
+constexpr int kMaxIterSlots = 256;        // PCISPH iteration slots per step
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b = kBlock) {
+  return (uint32_t)((n + b - 1) / b);
+}
+
+// ---------------------------------------------------------------------------
+// Constants the kernels read (derived once on the host, psb_derive)
+// ---------------------------------------------------------------------------
+struct GridDev {
+  float minx, miny, minz, cell;
+  int nx, ny, nz;
+  int R;            // stencil half-width in cells
+};
+
+struct Consts {
+  float h, h2, h2x3, c_spiky, mc, beta, delta, tait_b, inv_rho0, rho0, k, kp;
+  float cv, cs, cn, m, dt, sigma, l, gx, gy, gz;
+  float box_lo[3], box_hi[3], one_plus_e;
+  float tolerance;
+  int eos, mode, box_enabled, min_it, max_it;
+};
+
+psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);
+
+// ---------------------------------------------------------------------------
+// Device helpers shared by all kernels
+// ---------------------------------------------------------------------------
+// Blocks are dispatched round-robin over the 8 XCDs of MI355X; give each XCD
+// a contiguous range of particles so neighbour gathers stay in its L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t X = 8;
+  const uint32_t q = nb / X, r = nb % X;
+  const uint32_t x = b % X, k = b / X;
+  return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ uint32_t particle_index() {
+  return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+  return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
+}
+
+// cell = floor((x - Min) / cell) clamped to the grid (UniformGridKernel.cu:48-50)
+__device__ __forceinline__ int cell_coord(float x, float mn, float cell, int n, uint32_t& bad) {
+  float t = floorf((x - mn) / cell);
+  if (!(t >= 0.0f && t < (float)n)) {
+    bad = 1u;
+    t = (t >= (float)n) ? (float)(n - 1) : 0.0f;
+  }
+  return (int)t;
+}
+
+// BoxCollision_Kernel.cu:7-34 (restitution via one_plus_e)
+__device__ __forceinline__ void box_collide(const Consts& c, float4& x, float4& v) {
+  float px = x.x, py = x.y, pz = x.z, nx = 0.f, ny = 0.f, nz = 0.f;
+  if (x.x >= c.box_hi[0]) { px = c.box_hi[0]; nx = -1.f; }
+  if (x.y >= c.box_hi[1]) { py = c.box_hi[1]; ny = -1.f; }
+  if (x.z >= c.box_hi[2]) { pz = c.box_hi[2]; nz = -1.f; }
+  if (x.x <= c.box_lo[0]) { px = c.box_lo[0]; nx = 1.f; }
+  if (x.y <= c.box_lo[1]) { py = c.box_lo[1]; ny = 1.f; }
+  if (x.z <= c.box_lo[2]) { pz = c.box_lo[2]; nz = 1.f; }
+  float ln2 = dot3(nx, ny, nz, nx, ny, nz);
+  if (ln2 > 0.f) {
+    float ln = sqrtf(ln2);
+    nx = nx / ln; ny = ny / ln; nz = nz / ln;
+    x.x = px; x.y = py; x.z = pz;
+    float vn = c.one_plus_e * dot3(v.x, v.y, v.z, nx, ny, nz);
+    v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
+  }
+}
+
+// EOS (SphKernel.cu:25 linear; Tait); PSB_MODE_CONFIG clamps at zero.
+__device__ __forceinline__ float eos(const Consts& c, float rho) {
+  float pr;
+  if (c.eos == PSB_EOS_TAIT) {
+    float x = rho * c.inv_rho0;
+    float x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x7 = x6 * x;
+    pr = c.tait_b * (x7 - 1.0f);
+  } else {
+    pr = c.k * (rho - c.rho0);
+  }
+  if (c.mode == PSB_MODE_CONFIG && pr < 0.f) pr = 0.f;
+  return pr;
+}
+
+template <int N> __device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = N / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, N));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+}  // namespace psb
+
+struct psb_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // scratch for the standalone wrappers (grown on demand)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  uint32_t* d_flags = nullptr;     // device u32 flags for standalone calls
+  uint32_t* h_pinned = nullptr;    // pinned host staging (4 KiB)
+};
+
+struct psb_grid_s {
+  psb_context ctx = nullptr;
+  psb_grid_desc desc{};
+  uint64_t ncells = 0;
+  uint32_t capacity = 0;
+  uint32_t n = 0;                  // particles of the last store
+  uint32_t* count = nullptr;       // [ncells], zero between builds
+  uint32_t* start = nullptr;       // [ncells + 1]
+  uint32_t* hash = nullptr;        // [cap]
+  uint32_t* slot = nullptr;        // [cap]
+  uint32_t* tmp = nullptr;         // [cap]
+  uint32_t* perm = nullptr;        // [cap]
+  uint32_t* bsum = nullptr;        // [scan blocks]
+  uint32_t* flags = nullptr;       // [1]
+  uint32_t nscan = 0;
+};
+
+namespace psb {
+
+psb_status ensure_scratch(psb_context ctx, size_t bytes);
+GridDev grid_dev(const psb_grid_desc& d, float h);
+
+// Particle state arrays moved by the stable reorder (src -> dst[rank]).
+struct ReorderArrays {
+  const float4* sx; const float4* sv; const float4* svh; const uint32_t* sid; const float* sp;
+  float4* dx; float4* dv; float4* dvh; uint32_t* did; float* dp;
+};
+
+// grid.hip
+psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
+                      hipStream_t s);
+psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
+                            psb_neighbors nb, float* rho_sum_scaled, float mc,
+                            uint32_t* flags, uint32_t* maxcnt, hipStream_t s);
+
+}  // namespace psb

@@ -1,0 +1,548 @@
+// system.hip -- fused SPH / PCISPH system step (SPHSystem::update,
+// PCI_SPHSystem::update) on one device stream.
+//
+// State lives in two sorted buffers:
+//   A = start-of-step state in this step's sorted order (written by the
+//       grid build's reorder); after the step it holds m_dPos[0], m_dVel[0]
+//       and m_dVelInterAv of the reference (old position, old velocity and
+//       start half-step velocity);
+//   B = end-of-step state in the same order (m_dPos[1], m_dVel[1],
+//       m_dVelInterAp).  The next grid build hashes B and reorders B -> A.
+// The reference's 4 device-to-device "restore" copies per PCISPH step
+// (PciSphSystem.cpp:200-203) and its 4-5 device-to-host copies per step
+// (SphSystem.cpp:333,341-344) disappear: predictions always start from A,
+// and the host reads state only when asked.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pcisph.h"
+#include "psb_internal.h"
+
+namespace psb {
+
+enum Phase {
+  PH_GRID, PH_NEIGHBORS, PH_EPILOGUE, PH_FORCES, PH_NONPRESSURE, PH_PCI_DENSITY, PH_PCI_FORCE,
+  PH_COUNT
+};
+// grid_build = hash/count + 3-kernel scan + scatter + stable rank/reorder
+static const char* kPhaseNames[PH_COUNT] = {
+    "grid_build", "neighbors_density", "density_epilogue", "wcsph_forces", "pci_nonpressure",
+    "pci_density", "pci_force"};
+
+psb_status grid_create(psb_context ctx, const psb_grid_desc& d, uint32_t capacity, psb_grid* out);
+
+}  // namespace psb
+
+using namespace psb;
+
+struct psb_system_s {
+  psb_context ctx = nullptr;
+  psb_sph_params prm{};
+  Consts c{};
+  GridDev gd{};
+  uint32_t cap = 0, n = 0;
+  psb_grid grid = nullptr;
+  // A / B state
+  float4 *ax = nullptr, *av = nullptr, *avh = nullptr, *bx = nullptr, *bv = nullptr, *bvh = nullptr;
+  uint32_t *aid = nullptr, *bid = nullptr;
+  float *ap = nullptr, *bp = nullptr;
+  float *rho = nullptr, *pres = nullptr, *rho_pred = nullptr;
+  float4 *fa = nullptr, *fs = nullptr, *anp = nullptr, *xp0 = nullptr, *xp1 = nullptr;
+  uint32_t* ncount = nullptr;
+  uint32_t* nlist = nullptr;
+  uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
+  PciStatus* st = nullptr;
+  int32_t* h_final = nullptr;     // host-mapped final_iter per step slot
+  int32_t* d_final = nullptr;
+  bool has_state = false;
+  bool stepped = false;
+  psb_step_stats stats{};
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  struct Rec { int phase; size_t e0; int step; int iter; };
+  std::vector<Rec> recs;
+  std::vector<int> step_final;    // host-known final iteration per step of this update
+  double t_total[PH_COUNT] = {};
+  int64_t t_count[PH_COUNT] = {};
+  std::vector<hipEvent_t> ev_iter;
+};
+
+namespace psb {
+
+static psb_status ev_get(psb_system sys, size_t* idx) {
+  if (sys->ev_used + 2 > sys->ev.size()) {
+    size_t add = std::max<size_t>(256, sys->ev.size());
+    for (size_t i = 0; i < add; ++i) {
+      hipEvent_t e;
+      PSB_HIP(hipEventCreate(&e));
+      sys->ev.push_back(e);
+    }
+  }
+  *idx = sys->ev_used;
+  sys->ev_used += 2;
+  return PSB_OK;
+}
+
+// Wrap a launch sequence in timing events when timing is on.
+struct Timed {
+  psb_system sys;
+  int phase, step, iter;
+  size_t e0 = 0;
+  bool on;
+  Timed(psb_system s, int ph, int st = 0, int it = -1) : sys(s), phase(ph), step(st), iter(it), on(s->timing) {
+    if (on && ev_get(sys, &e0) == PSB_OK) hipEventRecord(sys->ev[e0], sys->ctx->stream);
+    else on = false;
+  }
+  ~Timed() {
+    if (!on) return;
+    hipEventRecord(sys->ev[e0 + 1], sys->ctx->stream);
+    sys->recs.push_back({phase, e0, step, iter});
+  }
+};
+
+static psb_status collect_timing(psb_system sys) {
+  if (sys->recs.empty()) { sys->ev_used = 0; return PSB_OK; }
+  PSB_HIP(hipStreamSynchronize(sys->ctx->stream));
+  for (const auto& r : sys->recs) {
+    if (r.iter >= 0 && r.step < (int)sys->step_final.size() && r.iter > sys->step_final[r.step])
+      continue;  // an iteration launched ahead and skipped on the device
+    float ms = 0.f;
+    PSB_HIP(hipEventElapsedTime(&ms, sys->ev[r.e0], sys->ev[r.e0 + 1]));
+    sys->t_total[r.phase] += ms;
+    sys->t_count[r.phase] += 1;
+  }
+  sys->recs.clear();
+  sys->ev_used = 0;
+  return PSB_OK;
+}
+
+static psb_status sys_alloc(psb_system s) {
+  const size_t N = s->cap ? s->cap : 1;
+  auto A = [&](void** p, size_t b) -> psb_status { PSB_HIP(hipMalloc(p, b)); return PSB_OK; };
+  PSB_TRY(A((void**)&s->ax, 16 * N)); PSB_TRY(A((void**)&s->av, 16 * N)); PSB_TRY(A((void**)&s->avh, 16 * N));
+  PSB_TRY(A((void**)&s->bx, 16 * N)); PSB_TRY(A((void**)&s->bv, 16 * N)); PSB_TRY(A((void**)&s->bvh, 16 * N));
+  PSB_TRY(A((void**)&s->aid, 4 * N)); PSB_TRY(A((void**)&s->bid, 4 * N));
+  PSB_TRY(A((void**)&s->ap, 4 * N)); PSB_TRY(A((void**)&s->bp, 4 * N));
+  PSB_TRY(A((void**)&s->rho, 4 * N)); PSB_TRY(A((void**)&s->pres, 4 * N)); PSB_TRY(A((void**)&s->rho_pred, 4 * N));
+  if (s->prm.solver == PSB_SOLVER_PCISPH) {
+    if (s->prm.mode == PSB_MODE_REFERENCE) {
+      PSB_TRY(A((void**)&s->fa, 16 * N)); PSB_TRY(A((void**)&s->fs, 16 * N));
+    } else {
+      PSB_TRY(A((void**)&s->anp, 16 * N));
+    }
+    PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
+  }
+  PSB_TRY(A((void**)&s->ncount, 4 * N));
+  PSB_TRY(A((void**)&s->nlist, 4 * N * (size_t)s->prm.max_neighbors));
+  PSB_TRY(A((void**)&s->flags, 64));
+  PSB_TRY(A((void**)&s->st, sizeof(PciStatus)));
+  PSB_HIP(hipHostMalloc((void**)&s->h_final, 64 * sizeof(int32_t),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+  PSB_HIP(hipHostGetDevicePointer((void**)&s->d_final, s->h_final, 0));
+  for (int i = 0; i < 64; ++i) s->h_final[i] = -1;
+  PSB_HIP(hipMemsetAsync(s->flags, 0, 64, s->ctx->stream));
+  PSB_HIP(hipMemsetAsync(s->st, 0, sizeof(PciStatus), s->ctx->stream));
+  for (int k = 0; k < kMaxIterSlots; ++k) {
+    hipEvent_t e;
+    PSB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    s->ev_iter.push_back(e);
+  }
+  return PSB_OK;
+}
+
+__global__ void k_mark_final(PciStatus* st, int32_t* host_slot) {
+  // copy the device-side final iteration of this step into host-mapped memory
+  if (threadIdx.x == 0)
+    __hip_atomic_store(host_slot, st->final_iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+static psb_status grid_phase(psb_system s, int step) {
+  ReorderArrays ra;
+  ra.sx = s->bx; ra.sv = s->bv; ra.svh = s->bvh; ra.sid = s->bid; ra.sp = s->bp;
+  ra.dx = s->ax; ra.dv = s->av; ra.dvh = s->avh; ra.did = s->aid; ra.dp = s->ap;
+  Timed t(s, PH_GRID, step);
+  return grid_build(s->grid, s->bx, s->n, &ra, s->ctx->stream);
+}
+
+static psb_status wcsph_step(psb_system s, int step) {
+  hipStream_t st = s->ctx->stream;
+  PSB_TRY(grid_phase(s, step));
+  psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
+  {
+    Timed t(s, PH_NEIGHBORS, step);
+    PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
+                             s->flags, s->flags + 1, st));
+  }
+  {
+    Timed t(s, PH_EPILOGUE, step);
+    PSB_TRY(launch_density_epilogue(s->n, s->c, 0, 0, s->rho, s->pres, nullptr, s->ax, s->av, st));
+  }
+  {
+    Timed t(s, PH_FORCES, step);
+    PSB_TRY(launch_wcsph_step_forces(s->n, s->c, s->prm.surface_tension != 0.f, s->ncount,
+                                     s->nlist, s->ax, s->av, s->rho, s->pres, s->avh, s->aid, s->bx,
+                                     s->bv, s->bvh, s->bid, s->bp, s->flags, st));
+  }
+  return PSB_OK;
+}
+
+static psb_status pcisph_step(psb_system s, int step, int slot) {
+  hipStream_t st = s->ctx->stream;
+  const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
+  const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  PSB_TRY(launch_pci_reset(s->st, false, st));
+  PSB_TRY(grid_phase(s, step));
+  psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
+  {
+    Timed t(s, PH_NEIGHBORS, step);
+    PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
+                             s->flags, s->flags + 1, st));
+  }
+  {
+    Timed t(s, PH_EPILOGUE, step);
+    PSB_TRY(launch_density_epilogue(s->n, s->c, 1, warm ? 1 : 0, s->rho, s->pres, s->ap, s->ax,
+                                    s->av, st));
+  }
+  PciBuffers b{};
+  b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0;
+  b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
+  PciOut o{};
+  o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
+  const bool surf = ref || s->prm.surface_tension != 0.f;
+  {
+    Timed t(s, PH_NONPRESSURE, step);
+    PSB_TRY(launch_pci_nonpressure(s->n, s->c, surf, warm, false, s->ncount, s->nlist, s->ax, s->av,
+                                   s->rho, s->avh, s->pres, b, o, st));
+  }
+  float4* cur = s->xp0;
+  float4* nxt = s->xp1;
+  if (warm) {
+    Timed t(s, PH_PCI_FORCE, step);
+    PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
+                             b, s->st, -1, 1, o, st));
+    std::swap(cur, nxt);
+  }
+  const int min_it = s->c.min_it, max_it = s->c.max_it;
+  int final_known = -1;
+  if (!ref) s->h_final[slot] = -1;
+  for (int k = 0; k < max_it; ++k) {
+    {
+      Timed t(s, PH_PCI_DENSITY, step, k);
+      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->st,
+                                 k, st));
+    }
+    {
+      Timed t(s, PH_PCI_FORCE, step, k);
+      PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
+                               s->pres, b, s->st, k, 0, o, st));
+    }
+    std::swap(cur, nxt);
+    if (ref) continue;
+    if (k + 1 >= min_it) {
+      hipLaunchKernelGGL(k_mark_final, dim3(1), dim3(64), 0, st, s->st, s->d_final + slot);
+      PSB_HIP(hipEventRecord(s->ev_iter[k], st));
+    }
+    if (k >= min_it) {
+      // iteration k is queued; wait for k-1 and stop queueing once it finished
+      PSB_HIP(hipEventSynchronize(s->ev_iter[k - 1]));
+      int32_t f = __atomic_load_n(&s->h_final[slot], __ATOMIC_ACQUIRE);
+      if (f >= 0) { final_known = f; break; }
+    }
+  }
+  if (ref) final_known = 2;
+  if (final_known < 0) {
+    // finished by the last queued iteration (k == max-1 or the min-th one)
+    PSB_HIP(hipStreamSynchronize(st));
+    final_known = __atomic_load_n(&s->h_final[slot], __ATOMIC_ACQUIRE);
+    if (final_known < 0) final_known = max_it - 1;
+  }
+  if ((int)s->step_final.size() <= step) s->step_final.resize(step + 1, -1);
+  s->step_final[step] = final_known;
+  return PSB_OK;
+}
+
+}  // namespace psb
+
+extern "C" {
+
+psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm, uint32_t capacity,
+                             psb_system* out) {
+  PSB_ARG(ctx && prm && out, "null argument");
+  PSB_ARG(prm->warm_start != PSB_WARM_MLP, "PSB_WARM_MLP is not available in this system");
+  *out = nullptr;
+  psb_system s = new psb_system_s();
+  s->ctx = ctx;
+  s->prm = *prm;
+  s->cap = capacity;
+  psb_status stt = make_consts(*prm, &s->c, &s->gd);
+  if (stt == PSB_OK) {
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
+  }
+  if (stt == PSB_OK) stt = grid_create(ctx, prm->grid, capacity, &s->grid);
+  if (stt == PSB_OK) stt = sys_alloc(s);
+  if (stt == PSB_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) stt = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+  }
+  if (stt != PSB_OK) { psb_system_destroy(s); return stt; }
+  *out = s;
+  return PSB_OK;
+}
+
+psb_status psb_system_destroy(psb_system s) {
+  if (!s) return PSB_OK;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
+                  s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1,
+                  s->ncount, s->nlist, s->flags, s->st};
+  for (void* p : ptrs) if (p) hipFree(p);
+  if (s->h_final) hipHostFree(s->h_final);
+  for (auto e : s->ev) hipEventDestroy(e);
+  for (auto e : s->ev_iter) hipEventDestroy(e);
+  if (s->grid) psb_finalize_grid(s->grid);
+  delete s;
+  return PSB_OK;
+}
+
+psb_status psb_system_get_params(psb_system s, psb_sph_params* out) {
+  PSB_ARG(s && out, "null argument");
+  *out = s->prm;
+  return PSB_OK;
+}
+
+uint32_t psb_system_count(psb_system s) { return s ? s->n : 0; }
+
+static psb_status upload_state(psb_system s, uint32_t n, const std::vector<float>& x,
+                               const std::vector<float>& v, const std::vector<float>& vh,
+                               const std::vector<float>& xo, const std::vector<float>& vo,
+                               const std::vector<float>& vav) {
+  hipStream_t st = s->ctx->stream;
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+  std::vector<float> zeros(n, 0.f);
+  PSB_HIP(hipMemcpyAsync(s->bx, x.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->bv, v.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->bvh, vh.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->ax, xo.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->av, vo.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->avh, vav.data(), 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->bid, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->aid, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->bp, zeros.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->pres, zeros.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->rho, zeros.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->rho_pred, zeros.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  s->n = n;
+  s->has_state = true;
+  s->stepped = false;
+  std::memset(&s->stats, 0, sizeof(s->stats));
+  return PSB_OK;
+}
+
+psb_status psb_system_set_particles(psb_system s, uint32_t n, const double* old_pos,
+                                    const double* new_pos, const double* old_vel,
+                                    const double* new_vel, const double* vel_inter_av,
+                                    const double* vel_inter_ap) {
+  PSB_ARG(s, "null system");
+  if (n > s->cap) { set_error("n exceeds system capacity"); return PSB_ERR_CAPACITY; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  auto pack = [&](const double* src) {
+    std::vector<float> v(4 * (size_t)n, 0.f);
+    if (src)
+      for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) v[4 * (size_t)i + a] = (float)src[3 * (size_t)i + a];
+    return v;
+  };
+  return upload_state(s, n, pack(new_pos), pack(new_vel), pack(vel_inter_ap), pack(old_pos),
+                      pack(old_vel), pack(vel_inter_av));
+}
+
+psb_status psb_system_set_particles_device(psb_system s, uint32_t n, const float* pos4,
+                                           const float* vel4, const float* vh4) {
+  PSB_ARG(s && (pos4 || !n), "null argument");
+  if (n > s->cap) { set_error("n exceeds system capacity"); return PSB_ERR_CAPACITY; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const size_t b4 = 16 * (size_t)n;
+  PSB_HIP(hipMemcpyAsync(s->bx, pos4, b4, hipMemcpyDeviceToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->ax, pos4, b4, hipMemcpyDeviceToDevice, st));
+  if (vel4) {
+    PSB_HIP(hipMemcpyAsync(s->bv, vel4, b4, hipMemcpyDeviceToDevice, st));
+    PSB_HIP(hipMemcpyAsync(s->av, vel4, b4, hipMemcpyDeviceToDevice, st));
+  } else {
+    PSB_HIP(hipMemsetAsync(s->bv, 0, b4, st));
+    PSB_HIP(hipMemsetAsync(s->av, 0, b4, st));
+  }
+  if (vh4) {
+    PSB_HIP(hipMemcpyAsync(s->bvh, vh4, b4, hipMemcpyDeviceToDevice, st));
+    PSB_HIP(hipMemcpyAsync(s->avh, vh4, b4, hipMemcpyDeviceToDevice, st));
+  } else {
+    PSB_HIP(hipMemsetAsync(s->bvh, 0, b4, st));
+    PSB_HIP(hipMemsetAsync(s->avh, 0, b4, st));
+  }
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+  PSB_HIP(hipMemcpyAsync(s->bid, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->aid, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemsetAsync(s->bp, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipMemsetAsync(s->pres, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipMemsetAsync(s->rho, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipMemsetAsync(s->rho_pred, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  s->n = n;
+  s->has_state = true;
+  s->stepped = false;
+  std::memset(&s->stats, 0, sizeof(s->stats));
+  return PSB_OK;
+}
+
+psb_status psb_system_update(psb_system s, int32_t steps) {
+  PSB_ARG(s && steps >= 0, "bad argument");
+  if (!s->has_state) { set_error("psb_system_update before psb_system_set_particles"); return PSB_ERR_STATE; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  PSB_HIP(hipMemsetAsync(s->flags, 0, 8, st));
+  PSB_HIP(hipMemsetAsync(s->grid->flags, 0, 4, st));
+  PSB_TRY(launch_pci_reset(s->st, true, st));
+  s->step_final.assign(steps, -1);
+  for (int k = 0; k < steps && s->n > 0; ++k) {
+    if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k, k & 63));
+    else PSB_TRY(wcsph_step(s, k));
+    if (s->timing && s->recs.size() > 4096) PSB_TRY(collect_timing(s));
+  }
+  // statistics
+  uint32_t fl[2] = {0, 0}, gf = 0;
+  PciStatus hs;
+  PSB_HIP(hipMemcpyAsync(fl, s->flags, 8, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipMemcpyAsync(&gf, s->grid->flags, 4, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipMemcpyAsync(&hs, s->st, 32, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  PSB_TRY(collect_timing(s));
+  s->stats.flags = fl[0] | gf;
+  s->stats.max_neighbors_seen = (int32_t)fl[1];
+  s->stats.steps = steps;
+  if (s->prm.solver == PSB_SOLVER_PCISPH && steps > 0) {
+    s->stats.iterations = hs.last_iters;
+    std::memcpy(&s->stats.max_error, &hs.last_err, 4);
+    s->stats.total_iterations = (int64_t)hs.total_iters;
+    std::memcpy(&s->stats.worst_error, &hs.worst_err, 4);
+  } else {
+    s->stats.iterations = 0;
+    s->stats.max_error = 0.f;
+    s->stats.total_iterations = 0;
+    s->stats.worst_error = 0.f;
+  }
+  if (steps > 0) s->stepped = true;
+  return PSB_OK;
+}
+
+psb_status psb_system_get_stats(psb_system s, psb_step_stats* out) {
+  PSB_ARG(s && out, "null argument");
+  *out = s->stats;
+  return PSB_OK;
+}
+
+psb_status psb_system_get_particles_f32(psb_system s, float* pos4, float* vel4, float* vh4,
+                                        float* density, float* pressure, uint32_t* ids) {
+  PSB_ARG(s, "null system");
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const size_t n = s->n;
+  const float* dens = (s->prm.solver == PSB_SOLVER_PCISPH) ? s->rho_pred : s->rho;
+  if (pos4) PSB_HIP(hipMemcpyAsync(pos4, s->bx, 16 * n, hipMemcpyDeviceToHost, st));
+  if (vel4) PSB_HIP(hipMemcpyAsync(vel4, s->bv, 16 * n, hipMemcpyDeviceToHost, st));
+  if (vh4) PSB_HIP(hipMemcpyAsync(vh4, s->bvh, 16 * n, hipMemcpyDeviceToHost, st));
+  if (density) PSB_HIP(hipMemcpyAsync(density, dens, 4 * n, hipMemcpyDeviceToHost, st));
+  if (pressure) PSB_HIP(hipMemcpyAsync(pressure, s->pres, 4 * n, hipMemcpyDeviceToHost, st));
+  if (ids) PSB_HIP(hipMemcpyAsync(ids, s->bid, 4 * n, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  return PSB_OK;
+}
+
+psb_status psb_system_get_particles(psb_system s, double* old_pos, double* new_pos,
+                                    double* old_vel, double* new_vel, double* vav, double* vap,
+                                    double* density, double* pressure) {
+  PSB_ARG(s, "null system");
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const size_t n = s->n;
+  std::vector<uint32_t> ids(n);
+  PSB_HIP(hipMemcpyAsync(ids.data(), s->bid, 4 * n, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  auto fetch4 = [&](const float4* src, double* dst) -> psb_status {
+    if (!dst) return PSB_OK;
+    std::vector<float> t(4 * n);
+    PSB_HIP(hipMemcpyAsync(t.data(), src, 16 * n, hipMemcpyDeviceToHost, st));
+    PSB_HIP(hipStreamSynchronize(st));
+    for (size_t k = 0; k < n; ++k)
+      for (int a = 0; a < 3; ++a) dst[3 * (size_t)ids[k] + a] = t[4 * k + a];
+    return PSB_OK;
+  };
+  auto fetch1 = [&](const float* src, double* dst) -> psb_status {
+    if (!dst) return PSB_OK;
+    std::vector<float> t(n);
+    PSB_HIP(hipMemcpyAsync(t.data(), src, 4 * n, hipMemcpyDeviceToHost, st));
+    PSB_HIP(hipStreamSynchronize(st));
+    for (size_t k = 0; k < n; ++k) dst[ids[k]] = t[k];
+    return PSB_OK;
+  };
+  PSB_TRY(fetch4(s->bx, new_pos));
+  PSB_TRY(fetch4(s->ax, old_pos));
+  PSB_TRY(fetch4(s->bv, new_vel));
+  PSB_TRY(fetch4(s->av, old_vel));
+  PSB_TRY(fetch4(s->bvh, vap));
+  PSB_TRY(fetch4(s->avh, vav));
+  PSB_TRY(fetch1(s->prm.solver == PSB_SOLVER_PCISPH ? s->rho_pred : s->rho, density));
+  PSB_TRY(fetch1(s->pres, pressure));
+  return PSB_OK;
+}
+
+psb_status psb_system_get_arrays(psb_system s, psb_system_arrays* out) {
+  PSB_ARG(s && out, "null argument");
+  out->pos4 = (const float*)s->bx;
+  out->old_pos4 = (const float*)s->ax;
+  out->vel4 = (const float*)s->bv;
+  out->old_vel4 = (const float*)s->av;
+  out->vel_inter_ap4 = (const float*)s->bvh;
+  out->vel_inter_av4 = (const float*)s->avh;
+  out->density = s->prm.solver == PSB_SOLVER_PCISPH ? s->rho_pred : s->rho;
+  out->pressure = s->pres;
+  out->ids = s->bid;
+  out->hash = s->grid->hash;
+  out->perm = s->grid->perm;
+  out->cell_start = s->grid->start;
+  out->neighbors = psb_neighbors{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
+  out->ncells = s->grid->ncells;
+  return PSB_OK;
+}
+
+psb_status psb_system_set_timing(psb_system s, int32_t enable) {
+  PSB_ARG(s, "null system");
+  s->timing = enable != 0;
+  return PSB_OK;
+}
+
+int32_t psb_system_get_timing(psb_system s, int32_t cap, const char** names, double* total_ms,
+                              int64_t* launches) {
+  if (!s) return 0;
+  for (int i = 0; i < PH_COUNT && i < cap; ++i) {
+    if (names) names[i] = kPhaseNames[i];
+    if (total_ms) total_ms[i] = s->t_total[i];
+    if (launches) launches[i] = s->t_count[i];
+  }
+  return PH_COUNT;
+}
+
+psb_status psb_system_reset_timing(psb_system s) {
+  PSB_ARG(s, "null system");
+  for (int i = 0; i < PH_COUNT; ++i) { s->t_total[i] = 0; s->t_count[i] = 0; }
+  return PSB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""Parameters and seeded synthetic scenes for the SPH/PCISPH workloads.
+
+Scenes are jittered cubic lattices: index order x fastest, then y, then z;
+particle (i, j, k) sits at origin + (i + 1/2, j + 1/2, k + 1/2) s plus a
+uniform jitter in [-j s, +j s] per component (3 draws per particle in index
+order) from numpy's PCG64 seeded with config_id * 1000 + scene_id.
+
+Workloads (sizes as configured):
+  wcsph_dam_break   32^3 = 32,768 particles, s = 0.01 m, block in the corner
+                    of a 0.64^3 m box; rho0 = 1000, m = rho0 s^3, h = 2s,
+                    mu = 0.01, Tait (k = 3, gamma = 7), g = -9.81 y,
+                    dt = 1e-4, box clamp with restitution 0.5; 64^3 cells of s.
+  pcisph_dam_break  64^3 = 262,144 particles, s = 0.005 m, block at the
+                    x-min end of a 1.28 x 0.64 x 0.64 m box; h = 2s, dt = 5e-4,
+                    tolerance 1 %, 3..50 iterations, template delta,
+                    256 x 128 x 128 cells of s (hash < 2^22).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import (EOS_LINEAR, EOS_TAIT, MODE_CONFIG, MODE_REFERENCE, SOLVER_PCISPH,
+                   SOLVER_WCSPH, WARM_NONE, Derived, GridDesc, Params, check, lib)
+
+
+def default_params() -> Params:
+    """Reference defaults (SphParticle.cpp:7-22, SphSystem.cpp:9,59,88,196)."""
+    p = Params()
+    check(lib().psb_default_params(C.byref(p)))
+    return p
+
+
+def derive(p: Params) -> Derived:
+    d = Derived()
+    check(lib().psb_derive(C.byref(p), C.byref(d)))
+    return d
+
+
+def grid_desc(lo, hi, cell: float) -> GridDesc:
+    g = GridDesc()
+    for a in range(3):
+        g.min[a] = float(lo[a])
+        g.dims[a] = int(math.ceil((hi[a] - lo[a]) / cell - 1e-9))
+    g.cell_size = float(cell)
+    return g
+
+
+def lattice(n_side, s: float, origin=(0.0, 0.0, 0.0), jitter: float = 0.01,
+            seed: int = 0) -> np.ndarray:
+    """(N, 4) float32 positions of an nx*ny*nz jittered lattice (w = 0)."""
+    nx, ny, nz = (n_side, n_side, n_side) if np.isscalar(n_side) else n_side
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+    pos = (base + 0.5) * s + np.asarray(origin, dtype=np.float64)
+    if jitter:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.uniform(-jitter * s, jitter * s, size=pos.shape)
+    out = np.zeros((pos.shape[0], 4), dtype=np.float32)
+    out[:, :3] = pos.astype(np.float32)
+    return out
+
+
+def lattice_mass(h: float, s: float, rho0: float) -> float:
+    """Mass that gives exactly rho0 at an interior particle of a perfect cubic
+    lattice of spacing s under the poly6 kernel of support h."""
+    c = 315.0 / (64.0 * math.pi * h ** 9)
+    n = int(math.ceil(h / s))
+    acc = 0.0
+    for a in range(-n, n + 1):
+        for b in range(-n, n + 1):
+            for e in range(-n, n + 1):
+                d2 = (a * a + b * b + e * e) * s * s
+                if d2 <= h * h:
+                    acc += (h * h - d2) ** 3
+    return rho0 / (c * acc)
+
+
+@dataclass
+class Scene:
+    name: str
+    params: Params
+    pos4: np.ndarray
+    vel4: np.ndarray = None
+    vh4: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        n = self.pos4.shape[0]
+        if self.vel4 is None:
+            self.vel4 = np.zeros((n, 4), np.float32)
+        if self.vh4 is None:
+            self.vh4 = np.zeros((n, 4), np.float32)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos4.shape[0])
+
+
+def wcsph_dam_break(n_side: int = 32, s: float = 0.01, mode: int = MODE_CONFIG,
+                    scene_id: int = 0, cell: float | None = None) -> Scene:
+    """Plain WCSPH dam break (32^3 by default)."""
+    p = default_params()
+    p.mode = mode
+    p.solver = SOLVER_WCSPH
+    p.eos = EOS_TAIT if mode == MODE_CONFIG else EOS_LINEAR
+    p.rest_density = 1000.0
+    p.mass = 1000.0 * s ** 3
+    p.support_radius = 2 * s
+    p.gas_stiffness = 3.0
+    p.tait_gamma = 7.0
+    p.viscosity = 0.01
+    p.surface_tension = 0.0 if mode == MODE_CONFIG else 0.0728
+    p.particle_radius = 0.5 * s
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    p.dt = 1e-4
+    L = 2 * n_side * s
+    p.box_enabled = 1
+    for a in range(3):
+        p.box_min[a] = 0.0
+        p.box_max[a] = L
+    p.elast = 0.5
+    p.grid = grid_desc((0.0, 0.0, 0.0), (L, L, L), cell or s)
+    p.max_neighbors = 96
+    p.lattice_spacing = s
+    pos = lattice(n_side, s, seed=1000 + scene_id)
+    return Scene("wcsph_dam_break", p, pos, meta={"s": s, "n_side": n_side})
+
+
+def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
+                     scene_id: int = 0, box=(1.28, 0.64, 0.64), tolerance: float = 0.01,
+                     min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
+                     calibrate_mass: bool = True) -> Scene:
+    """Plain PCISPH dam break (64^3 block at the x-min end of the box)."""
+    p = default_params()
+    p.mode = mode
+    p.solver = SOLVER_PCISPH
+    p.eos = EOS_LINEAR
+    p.warm_start = warm_start
+    h = 2 * s
+    p.rest_density = 1000.0
+    p.support_radius = h
+    p.mass = lattice_mass(h, s, 1000.0) if calibrate_mass else 1000.0 * s ** 3
+    p.gas_stiffness = 3.0
+    p.viscosity = 0.01
+    p.surface_tension = 0.0 if mode == MODE_CONFIG else 0.0728
+    p.particle_radius = 0.5 * s
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    p.dt = 5e-4
+    p.box_enabled = 1
+    for a in range(3):
+        p.box_min[a] = 0.0
+        p.box_max[a] = float(box[a])
+    p.elast = 0.5
+    p.grid = grid_desc((0.0, 0.0, 0.0), box, s)
+    p.max_neighbors = 96
+    p.tolerance = tolerance
+    p.min_iterations = min_it
+    p.max_iterations = max_it
+    p.lattice_spacing = s
+    pos = lattice(n_side, s, seed=2000 + scene_id)
+    return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})

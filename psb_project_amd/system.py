@@ -1,0 +1,126 @@
+"""Python host side of the fused system step (psb_system_* C-ABI).
+
+`FluidSystem` owns one psb_system: it uploads a scene, advances it with
+psb_system_update (SPHSystem::update / PCI_SPHSystem::update of the
+reference) and reads the state back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Params, StepStats, SystemArrays, check, lib
+from .device import Context, DeviceArray
+
+
+class FluidSystem:
+    def __init__(self, ctx: Context, params: Params, capacity: int):
+        self.ctx = ctx
+        self.params = params.copy()
+        self.capacity = int(capacity)
+        h = C.c_void_p()
+        check(lib().psb_system_create(ctx.handle, C.byref(self.params), self.capacity, C.byref(h)))
+        self.handle = h
+
+    # -- state -------------------------------------------------------------
+    def set_particles(self, pos4: np.ndarray, vel4: np.ndarray | None = None,
+                      vh4: np.ndarray | None = None) -> None:
+        """Upload (N,4) float32 positions / velocities / half-step velocities."""
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        n = pos4.shape[0]
+        bufs = [self.ctx.to_device(pos4)]
+        bufs.append(self.ctx.to_device(np.ascontiguousarray(vel4, np.float32)) if vel4 is not None else None)
+        bufs.append(self.ctx.to_device(np.ascontiguousarray(vh4, np.float32)) if vh4 is not None else None)
+        check(lib().psb_system_set_particles_device(
+            self.handle, n, bufs[0].ptr, bufs[1].ptr if bufs[1] else None,
+            bufs[2].ptr if bufs[2] else None))
+        for b in bufs:
+            if b is not None:
+                b.free()
+
+    def set_particles_f64(self, new_pos, old_pos=None, new_vel=None, old_vel=None,
+                          vel_inter_av=None, vel_inter_ap=None) -> None:
+        """Upload host FP64 xyz triples (the reference's m_h* layout)."""
+        arrs = [new_pos, old_pos, new_vel, old_vel, vel_inter_av, vel_inter_ap]
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1, 3) for a in arrs]
+        n = arrs[0].shape[0]
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        check(lib().psb_system_set_particles(self.handle, n, ptr(arrs[1]), ptr(arrs[0]),
+                                             ptr(arrs[3]), ptr(arrs[2]), ptr(arrs[4]), ptr(arrs[5])))
+
+    @property
+    def n(self) -> int:
+        return int(lib().psb_system_count(self.handle))
+
+    def update(self, steps: int = 1) -> None:
+        check(lib().psb_system_update(self.handle, int(steps)))
+
+    def stats(self) -> dict:
+        s = StepStats()
+        check(lib().psb_system_get_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in StepStats._fields_}
+
+    def state(self) -> dict:
+        """Sorted-order FP32 state of the last step plus particle ids."""
+        n = self.n
+        pos = np.empty((n, 4), np.float32)
+        vel = np.empty((n, 4), np.float32)
+        vh = np.empty((n, 4), np.float32)
+        rho = np.empty(n, np.float32)
+        prs = np.empty(n, np.float32)
+        ids = np.empty(n, np.uint32)
+        check(lib().psb_system_get_particles_f32(self.handle, pos.ctypes.data, vel.ctypes.data,
+                                                 vh.ctypes.data, rho.ctypes.data, prs.ctypes.data,
+                                                 ids.ctypes.data))
+        return {"pos": pos, "vel": vel, "vh": vh, "density": rho, "pressure": prs, "ids": ids}
+
+    def state_f64(self) -> dict:
+        """Reference-layout FP64 arrays in the caller's original particle order."""
+        n = self.n
+        out = {k: np.empty((n, 3), np.float64) for k in
+               ("old_pos", "new_pos", "old_vel", "new_vel", "vel_inter_av", "vel_inter_ap")}
+        out["density"] = np.empty(n, np.float64)
+        out["pressure"] = np.empty(n, np.float64)
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        check(lib().psb_system_get_particles(self.handle, P(out["old_pos"]), P(out["new_pos"]),
+                                             P(out["old_vel"]), P(out["new_vel"]),
+                                             P(out["vel_inter_av"]), P(out["vel_inter_ap"]),
+                                             P(out["density"]), P(out["pressure"])))
+        return out
+
+    def arrays(self) -> SystemArrays:
+        a = SystemArrays()
+        check(lib().psb_system_get_arrays(self.handle, C.byref(a)))
+        return a
+
+    def download(self, ptr: int, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        check(lib().psb_copy_array_from_device(self.ctx.handle, out.ctypes.data, ptr, 0, out.nbytes))
+        return out
+
+    # -- timing ------------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        check(lib().psb_system_set_timing(self.handle, 1 if on else 0))
+
+    def reset_timing(self) -> None:
+        check(lib().psb_system_reset_timing(self.handle))
+
+    def timing(self) -> dict:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        tot = (C.c_double * cap)()
+        cnt = (C.c_int64 * cap)()
+        k = lib().psb_system_get_timing(self.handle, cap, names, tot, cnt)
+        return {names[i].decode(): (tot[i], cnt[i]) for i in range(k)}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().psb_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
