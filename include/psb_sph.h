@@ -115,6 +115,12 @@ typedef struct psb_sph_params {
                                full cubic-lattice neighbourhood of spacing
                                lattice_spacing                                */
   float   lattice_spacing;  /* s, used for the template delta                 */
+  int32_t delta_local_cap;  /* PSB_MODE_CONFIG: 1 -> per particle, use
+                               min(delta, 1/(beta (|sum gradW|^2 + sum |gradW|^2)))
+                               over its start-of-step neighbours, so particles
+                               with one-sided neighbourhoods (walls without
+                               boundary particles, free surface) are not
+                               over-corrected; 0 -> template delta only       */
 } psb_sph_params;
 
 /* Fill *p with the reference defaults of SPHParticle / SimulationData_SPHSystem

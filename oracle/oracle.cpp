@@ -348,7 +348,7 @@ template <class R> struct System {
   // A = start-of-step (sorted), B = end-of-step (same order)
   std::vector<V4<R>> Ax, Av, Avh, Bx, Bv, Bvh;
   std::vector<uint32_t> Aid, Bid, hash, perm, start, count, list;
-  std::vector<R> Ap, Bp, rho, pres, rho_pred;
+  std::vector<R> Ap, Bp, rho, pres, rho_pred, dl;
   std::vector<V4<R>> fa, fs4, anp, xp0, xp1;
   uint32_t flags = 0;
   int last_iters = 0;
@@ -370,6 +370,7 @@ template <class R> struct System {
     list.assign((size_t)cap * p.max_neighbors, 0);
     Ap.assign(cap, 0); Bp.assign(cap, 0); rho.assign(cap, 0); pres.assign(cap, 0);
     rho_pred.assign(cap, 0);
+    dl.assign(cap, 0);
   }
 
   // storeParticles + reorder (B -> A by the stable order).
@@ -483,6 +484,7 @@ template <class R> struct System {
       R fS[3] = {0, 0, 0}, nrm[3];
       if (surf) surface_finish(d, s, sigma, fS, nrm);
       const V4<R> x = Ax[p], vh0 = Avh[p];
+      if (!ref) dl[p] = local_delta(p);
       V4<R> xp;
       if (ref) {
         // forcesAccum = (0, -9.81 rho, 0); F = ((Facc + fV) + fP) + fS
@@ -532,6 +534,34 @@ template <class R> struct System {
     iter_hist.push_back(it); err_hist.push_back(err);
   }
 
+  // Pressure scaling of particle p (config mode): the template delta, capped
+  // by the per-particle value 1/(beta (W1.W1 + W2)) of compute_Pressure_PCI
+  // (PciSphSystem_Kernel.cu:44-63) evaluated once at the start positions.
+  R local_delta(uint32_t p) const {
+    if (!prm.delta_local_cap) return d.delta;
+    const V4<R> xi = Ax[p];
+    R W1x = 0, W1y = 0, W1z = 0, W2 = 0;
+    for (uint32_t k = 0; k < count[p]; ++k) {
+      uint32_t q = list[(uint64_t)k * n + p];
+      const V4<R> xj = Ax[q];
+      R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      R d2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (d2 > (R)0 && d2 < d.h2) {
+        R dd = std::sqrt(d2);
+        R invd = (R)1 / dd;
+        R hd = d.h - dd;
+        R gw = d.c_spiky * (hd * hd);
+        R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+        W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+      }
+    }
+    R den = d.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
+    if (!(den > (R)0)) return d.delta;
+    R loc = (R)1 / den;
+    return loc < d.delta ? loc : d.delta;
+  }
+
   // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
   // in reference mode; template-delta pressure update in config mode.
   // Returns max_i max(0, rho_i - rho0)/rho0.
@@ -570,7 +600,7 @@ template <class R> struct System {
         X[p].w = pr / (r * r);
         rho_pred[p] = r;
       } else {
-        pr = pr + d.delta * e;
+        pr = pr + dl[p] * e;
         if (pr < (R)0) pr = (R)0;
         X[p].w = pr;
         rho_pred[p] = r;

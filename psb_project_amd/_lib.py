@@ -77,6 +77,7 @@ class Params(C.Structure):
         ("max_iterations", C.c_int32),
         ("delta", C.c_float),
         ("lattice_spacing", C.c_float),
+        ("delta_local_cap", C.c_int32),
     ]
 
     def copy(self) -> "Params":

@@ -21,6 +21,7 @@ struct PciBuffers {
   float4* fs;    // reference mode: f_surface
   float4* anp;   // config mode: non-pressure acceleration
   float4* xp0;   // first predicted positions
+  float* dl;     // config mode: per-particle pressure scaling delta_i
   const float4* ax;   // start-of-step positions
   const float4* avh;  // start-of-step half-step velocities
   const uint32_t* aid;
@@ -43,7 +44,7 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
                                   hipStream_t s);
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
-                              PciStatus* st, int k, hipStream_t s);
+                              const float* dl, PciStatus* st, int k, hipStream_t s);
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,

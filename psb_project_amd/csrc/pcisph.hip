@@ -43,6 +43,36 @@ __global__ void k_pci_stats_reset(PciStatus* st) {
   }
 }
 
+// Pressure scaling of particle p in config mode: the template delta, capped by
+// the per-particle value 1/(beta (W1.W1 + W2)) of compute_Pressure_PCI
+// (PciSphSystem_Kernel.cu:44-63) evaluated once at the start positions.
+__device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32_t n,
+                                             const uint32_t* __restrict__ cnt,
+                                             const uint32_t* __restrict__ list,
+                                             const float4* __restrict__ xs) {
+  const float4 xi = xs[p];
+  float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
+  const uint32_t m = cnt[p];
+  for (uint32_t k = 0; k < m; ++k) {
+    const float4 xj = xs[list[(uint64_t)k * n + p]];
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (d2 > 0.f && d2 < c.h2) {
+      const float dd = sqrtf(d2);
+      const float invd = 1.0f / dd;
+      const float hd = c.h - dd;
+      const float gw = c.c_spiky * (hd * hd);
+      const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+      W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+      W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+    }
+  }
+  const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
+  if (!(den > 0.f)) return c.delta;
+  const float loc = 1.0f / den;
+  return loc < c.delta ? loc : c.delta;
+}
+
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:117-170)
 // + first prediction (integrateSPH_LeapFrog_Forces with f_P = 0).
 template <bool REF, bool SURF, bool WARM, bool OUT>
@@ -81,6 +111,7 @@ __global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
       if (o.F) o.F[p] = make_float4(0.f, gy * ri, 0.f, 0.f);
     }
   } else {
+    b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs) : c.delta;
     const float inv = 1.0f / ri;
     const float a0 = (fV[0] + fS[0]) * inv + c.gx, a1 = (fV[1] + fS[1]) * inv + c.gy,
                 a2 = (fV[2] + fS[2]) * inv + c.gz;
@@ -112,6 +143,7 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
                                                      const uint32_t* __restrict__ list, float4* X,
                                                      float* __restrict__ pres,
                                                      float* __restrict__ rho_pred,
+                                                     const float* __restrict__ dl,
                                                      PciStatus* __restrict__ st, int k) {
   __shared__ float red[4];
   {
@@ -149,7 +181,7 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
       if (r > c.rho0 && W2 > 0.f) pr = pr + e / (c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
       reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr / (r * r);
     } else {
-      pr = pr + c.delta * e;
+      pr = pr + dl[p] * e;
       if (pr < 0.f) pr = 0.f;
       reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr;
     }
@@ -301,14 +333,14 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
 
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
-                              PciStatus* st, int k, hipStream_t s) {
+                              const float* dl, PciStatus* st, int k, hipStream_t s) {
   if (!n) return PSB_OK;
   if (c.mode == PSB_MODE_REFERENCE)
     hipLaunchKernelGGL(k_pci_density<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt, list,
-                       X, pres, rho_pred, st, k);
+                       X, pres, rho_pred, dl, st, k);
   else
     hipLaunchKernelGGL(k_pci_density<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt,
-                       list, X, pres, rho_pred, st, k);
+                       list, X, pres, rho_pred, dl, st, k);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -355,7 +387,7 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   PSB_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const size_t v4 = 16 * (size_t)n;
-  PSB_TRY(ensure_scratch(ctx, 7 * v4 + sizeof(PciStatus) + 256 + 4 * (size_t)n));
+  PSB_TRY(ensure_scratch(ctx, 7 * v4 + sizeof(PciStatus) + 256 + 8 * (size_t)n));
   char* base = (char*)ctx->scratch;
   float4* xs = (float4*)base;
   float4* vs = (float4*)(base + v4);
@@ -366,7 +398,8 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   b.xp0 = (float4*)(base + 5 * v4);
   float4* xp1 = (float4*)(base + 6 * v4);
   float* tmp_p = (float*)(base + 7 * v4);
-  PciStatus* st = (PciStatus*)(((uintptr_t)(base + 7 * v4 + 4 * (size_t)n) + 255) & ~(uintptr_t)255);
+  b.dl = tmp_p + n;
+  PciStatus* st = (PciStatus*)(((uintptr_t)(base + 7 * v4 + 8 * (size_t)n) + 255) & ~(uintptr_t)255);
   b.ax = reinterpret_cast<const float4*>(pos4);
   b.avh = reinterpret_cast<const float4*>(vel_inter_ap4);
   b.aid = nullptr;
@@ -402,7 +435,7 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   int iters = 0;
   float err = 0.f;
   for (int k = 0; k < c.max_it; ++k) {
-    PSB_TRY(launch_pci_density(n, c, nb.count, nb.list, cur, pressure, density, st, k, s));
+    PSB_TRY(launch_pci_density(n, c, nb.count, nb.list, cur, pressure, density, b.dl, st, k, s));
     PSB_TRY(launch_pci_force(n, c, false, nb.count, nb.list, cur, nxt, density, pressure, b, st, k,
                              0, o, s));
     std::swap(cur, nxt);

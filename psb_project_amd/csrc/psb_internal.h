@@ -67,6 +67,7 @@ struct Consts {
   float box_lo[3], box_hi[3], one_plus_e;
   float tolerance;
   int eos, mode, box_enabled, min_it, max_it;
+  int delta_local_cap;
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);

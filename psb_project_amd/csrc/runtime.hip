@@ -119,6 +119,7 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   c->box_enabled = p.box_enabled;
   c->min_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.min_iterations;
   c->max_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.max_iterations;
+  c->delta_local_cap = p.mode == PSB_MODE_CONFIG && p.delta_local_cap ? 1 : 0;
   if (g) *g = grid_dev(p.grid, p.support_radius);
   return PSB_OK;
 }

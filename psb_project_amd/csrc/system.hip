@@ -49,6 +49,7 @@ struct psb_system_s {
   float *ap = nullptr, *bp = nullptr;
   float *rho = nullptr, *pres = nullptr, *rho_pred = nullptr;
   float4 *fa = nullptr, *fs = nullptr, *anp = nullptr, *xp0 = nullptr, *xp1 = nullptr;
+  float* dl = nullptr;            // config-mode PCISPH per-particle delta
   uint32_t* ncount = nullptr;
   uint32_t* nlist = nullptr;
   uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
@@ -132,6 +133,7 @@ static psb_status sys_alloc(psb_system s) {
       PSB_TRY(A((void**)&s->fa, 16 * N)); PSB_TRY(A((void**)&s->fs, 16 * N));
     } else {
       PSB_TRY(A((void**)&s->anp, 16 * N));
+      PSB_TRY(A((void**)&s->dl, 4 * N));
     }
     PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
   }
@@ -207,7 +209,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
                                     s->av, st));
   }
   PciBuffers b{};
-  b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0;
+  b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
   b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
@@ -231,7 +233,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   for (int k = 0; k < max_it; ++k) {
     {
       Timed t(s, PH_PCI_DENSITY, step, k);
-      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->st,
+      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl, s->st,
                                  k, st));
     }
     {
@@ -298,7 +300,7 @@ psb_status psb_system_destroy(psb_system s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
-                  s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1,
+                  s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
                   s->ncount, s->nlist, s->flags, s->st};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);

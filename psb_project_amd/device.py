@@ -7,6 +7,7 @@ from numpy arrays with psb_copy_array_*.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -25,12 +26,19 @@ class Context:
         check(lib().psb_init(device, C.byref(h)))
         self.handle = h
         self.device = device
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj) -> None:
+        """Register an object whose close() must run before this context ends."""
+        self._children.add(obj)
 
     def sync(self) -> None:
         check(lib().psb_thread_sync(self.handle))
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             lib().psb_finalize(self.handle)
             self.handle = None
 
@@ -63,6 +71,7 @@ class DeviceArray:
         p = C.c_void_p()
         check(lib().psb_allocate_array(ctx.handle, C.byref(p), max(self.nbytes, 4)))
         self.ptr = p
+        ctx.adopt(self)
 
     @property
     def addr(self) -> int:
@@ -86,6 +95,8 @@ class DeviceArray:
         if self.ptr and self.ctx.handle:
             lib().psb_free_array(self.ctx.handle, self.ptr)
         self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:

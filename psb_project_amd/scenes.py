@@ -161,5 +161,6 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
     p.min_iterations = min_it
     p.max_iterations = max_it
     p.lattice_spacing = s
+    p.delta_local_cap = 1
     pos = lattice(n_side, s, seed=2000 + scene_id)
     return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})

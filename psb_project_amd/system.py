@@ -22,6 +22,7 @@ class FluidSystem:
         h = C.c_void_p()
         check(lib().psb_system_create(ctx.handle, C.byref(self.params), self.capacity, C.byref(h)))
         self.handle = h
+        ctx.adopt(self)
 
     # -- state -------------------------------------------------------------
     def set_particles(self, pos4: np.ndarray, vel4: np.ndarray | None = None,
@@ -115,9 +116,10 @@ class FluidSystem:
         return {names[i].decode(): (tot[i], cnt[i]) for i in range(k)}
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        # the context destroys its systems before it is finalised
+        if getattr(self, "handle", None) and self.ctx.handle:
             lib().psb_system_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
