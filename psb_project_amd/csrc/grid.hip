@@ -179,7 +179,11 @@ __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict
 
 // searchNeighbooring (UniformGridKernel.cu:58-95) on sorted positions; the
 // density sum of the same pairs rides along (SphKernel.cu:9-27).
-template <bool DENS>
+// RT > 0: the stencil half-width is a compile-time constant; the (2RT+1)^2
+// row ranges are loaded first (independent table loads) and the candidates of
+// a row are gathered U at a time before being tested in order.  RT = 0: any
+// stencil width, rows visited one after the other.
+template <bool DENS, int RT>
 __global__ __launch_bounds__(256) void k_neighbors(const float4* xs, uint32_t n, GridDev g,
                                                    const uint32_t* __restrict__ start, float r2,
                                                    uint32_t cap, uint32_t* __restrict__ ncount,
@@ -195,29 +199,53 @@ __global__ __launch_bounds__(256) void k_neighbors(const float4* xs, uint32_t n,
     const int cx = cell_coord(xi.x, g.minx, g.cell, g.nx, bad);
     const int cy = cell_coord(xi.y, g.miny, g.cell, g.ny, bad);
     const int cz = cell_coord(xi.z, g.minz, g.cell, g.nz, bad);
-    const int R = g.R;
+    const int R = RT > 0 ? RT : g.R;
     const int x0 = max(cx - R, 0), x1 = min(cx + R, g.nx - 1);
     float S = 0.f;
-    for (int dz = -R; dz <= R; ++dz) {
-      const int z = cz + dz;
-      if (z < 0 || z >= g.nz) continue;
-      for (int dy = -R; dy <= R; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= g.ny) continue;
-        const uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
-        const uint32_t q0 = start[base + x0], q1 = start[base + x1 + 1];
-        for (uint32_t q = q0; q < q1; ++q) {
-          const float4 xj = xs[q];
-          const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
-          const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
-          if (d2 <= r2) {
-            if (cnt < cap) nlist[(uint64_t)cnt * n + p] = q;
-            cnt++;
-            if (DENS) {
-              const float w = r2 - d2;
-              S = __builtin_fmaf(w * w, w, S);
-            }
-          }
+    auto visit = [&](uint32_t q, const float4& xj) {
+      const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
+      const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
+      if (d2 <= r2) {
+        if (cnt < cap) nlist[(uint64_t)cnt * n + p] = q;
+        cnt++;
+        if (DENS) {
+          const float w = r2 - d2;
+          S = __builtin_fmaf(w * w, w, S);
+        }
+      }
+    };
+    auto row = [&](uint32_t q0, uint32_t q1) {
+      uint32_t q = q0;
+      for (; q + kUnroll <= q1; q += kUnroll) {
+        float4 xq[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) xq[u] = xs[q + u];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) visit(q + u, xq[u]);
+      }
+      for (; q < q1; ++q) visit(q, xs[q]);
+    };
+    if (RT > 0) {
+      constexpr int W = 2 * RT + 1;
+      uint32_t q0s[W * W], q1s[W * W];
+#pragma unroll
+      for (int r = 0; r < W * W; ++r) {
+        const int z = cz + r / W - RT, y = cy + r % W - RT;
+        const bool ok = z >= 0 && z < g.nz && y >= 0 && y < g.ny;
+        const uint64_t base = ok ? (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z) : 0;
+        q0s[r] = ok ? start[base + x0] : 0u;
+        q1s[r] = ok ? start[base + x1 + 1] : 0u;
+      }
+      for (int r = 0; r < W * W; ++r) row(q0s[r], q1s[r]);
+    } else {
+      for (int dz = -R; dz <= R; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -R; dy <= R; ++dy) {
+          const int y = cy + dy;
+          if (y < 0 || y >= g.ny) continue;
+          const uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+          row(start[base + x0], start[base + x1 + 1]);
         }
       }
     }
@@ -274,12 +302,16 @@ psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, 
   if (!n) return PSB_OK;
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
-  if (rho)
-    hipLaunchKernelGGL(k_neighbors<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, xs, n, gd,
-                       g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt);
-  else
-    hipLaunchKernelGGL(k_neighbors<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, xs, n, gd,
-                       g->start, r2, nb.capacity, nb.count, nb.list, nullptr, 0.f, flags, maxcnt);
+  dim3 gb(blocks_for(n)), tb(kBlock);
+#define PSB_NB(D_, R_)                                                                      \
+  hipLaunchKernelGGL((k_neighbors<D_, R_>), gb, tb, 0, s, xs, n, gd, g->start, r2, nb.capacity, \
+                     nb.count, nb.list, rho, mc, flags, maxcnt)
+  if (rho) {
+    if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
+  } else {
+    if (R == 1) PSB_NB(false, 1); else if (R == 2) PSB_NB(false, 2); else PSB_NB(false, 0);
+  }
+#undef PSB_NB
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

@@ -52,9 +52,7 @@ __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32
                                              const float4* __restrict__ xs) {
   const float4 xi = xs[p];
   float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
-  const uint32_t m = cnt[p];
-  for (uint32_t k = 0; k < m; ++k) {
-    const float4 xj = xs[list[(uint64_t)k * n + p]];
+  for_neighbors<kUnroll>(p, n, cnt[p], list, xs, [&](uint32_t, const float4& xj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (d2 > 0.f && d2 < c.h2) {
@@ -66,7 +64,7 @@ __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32
       W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
       W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
     }
-  }
+  });
   const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
   if (!(den > 0.f)) return c.delta;
   const float loc = 1.0f / den;
@@ -155,10 +153,7 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
   if (p < n) {
     const float4 xi = X[p];
     float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
-    const uint32_t m = cnt[p];
-    for (uint32_t kk = 0; kk < m; ++kk) {
-      const uint32_t q = list[(uint64_t)kk * n + p];
-      const float4 xj = X[q];
+    for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
       const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const float d2 = dot3(dx, dy, dz, dx, dy, dz);
       float w = c.h2 - d2;
@@ -173,7 +168,7 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
         W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
         W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
       }
-    }
+    });
     const float r = S * c.mc;
     float pr = pres[p];
     const float e = r - c.rho0;
@@ -233,10 +228,7 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
   if (p >= n) return;
   const float4 xi = X[p];
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  const uint32_t m = cnt[p];
-  for (uint32_t kk = 0; kk < m; ++kk) {
-    const uint32_t q = list[(uint64_t)kk * n + p];
-    const float4 xj = X[q];
+  for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (REF) {
@@ -260,7 +252,7 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
       sy = __builtin_fmaf(dy, cc, sy);
       sz = __builtin_fmaf(dz, cc, sz);
     }
-  }
+  });
   const float4 x0 = b.ax[p], vh0 = b.avh[p];
   float4 vh;
   float fP0 = 0.f, fP1 = 0.f, fP2 = 0.f;

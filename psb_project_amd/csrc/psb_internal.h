@@ -135,6 +135,55 @@ __device__ __forceinline__ float eos(const Consts& c, float rho) {
   return pr;
 }
 
+// Visit the ELL neighbour list of particle p in order.  The list loads and
+// gathers of U neighbours are issued before any of them is consumed, so a
+// wave keeps U dependent load chains in flight; f(q, xq) still sees the
+// neighbours in list order, so every sum keeps its sequential association.
+template <int U, class F>
+__device__ __forceinline__ void for_neighbors(uint32_t p, uint32_t n, uint32_t m,
+                                              const uint32_t* __restrict__ list,
+                                              const float4* X, F&& f) {
+  uint32_t k = 0;
+  for (; k + U <= m; k += U) {
+    uint32_t q[U];
+    float4 xq[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + u) * n + p];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) f(q[u], xq[u]);
+  }
+  for (; k < m; ++k) {
+    const uint32_t q = list[(uint64_t)k * n + p];
+    f(q, X[q]);
+  }
+}
+
+// Same with a second per-neighbour vec4 array gathered alongside.
+template <int U, class F>
+__device__ __forceinline__ void for_neighbors2(uint32_t p, uint32_t n, uint32_t m,
+                                               const uint32_t* __restrict__ list,
+                                               const float4* X, const float4* V, F&& f) {
+  uint32_t k = 0;
+  for (; k + U <= m; k += U) {
+    uint32_t q[U];
+    float4 xq[U], vq[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + u) * n + p];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { xq[u] = X[q[u]]; vq[u] = V[q[u]]; }
+#pragma unroll
+    for (int u = 0; u < U; ++u) f(q[u], xq[u], vq[u]);
+  }
+  for (; k < m; ++k) {
+    const uint32_t q = list[(uint64_t)k * n + p];
+    f(q, X[q], V[q]);
+  }
+}
+
+constexpr int kUnroll = 4;
+
 template <int N> __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = N / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, N));

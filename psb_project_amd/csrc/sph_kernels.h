@@ -26,10 +26,8 @@ __device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, u
   s.nn[0] = s.nn[1] = s.nn[2] = 0.f;
   const float4 xi = xs[p], vi = vs[p];
   const float pvi = xi.w;
-  const uint32_t m = cnt[p];
-  for (uint32_t k = 0; k < m; ++k) {
-    const uint32_t q = list[(uint64_t)k * n + p];
-    const float4 xj = xs[q], vj = vs[q];
+  for_neighbors2<kUnroll>(p, n, cnt[p], list, xs, vs, [&](uint32_t, const float4& xj,
+                                                         const float4& vj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (d2 > 0.f) {
@@ -58,7 +56,7 @@ __device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, u
         s.nn[2] = __builtin_fmaf(dz, an, s.nn[2]);
       }
     }
-  }
+  });
   return s;
 }
 
