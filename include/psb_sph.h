@@ -365,6 +365,68 @@ int32_t    psb_system_get_timing(psb_system sys, int32_t cap, const char** names
                                  double* total_ms, int64_t* launches);
 psb_status psb_system_reset_timing(psb_system sys);
 
+/* ------------------------------------------------------------------------ */
+/* Learned PCISPH warm start.  No counterpart in the reference, whose        */
+/* corrector always starts from p = 0 (PciSphSystem_Kernel.cu:131); the      */
+/* unchanged corrector still enforces the tolerance, so a poor prediction    */
+/* only costs iterations.                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_mlp_s* psb_mlp;
+
+/* Per-particle MLP 8 -> 64 -> 64 -> 1 (SiLU), 4,801 FP32 master parameters
+ * (W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3), BF16 MFMA products
+ * with FP32 accumulation.  He-uniform initialisation from `seed`.           */
+psb_status psb_mlp_create(psb_context ctx, uint64_t seed, psb_mlp* out);
+psb_status psb_mlp_destroy(psb_mlp m);
+int32_t    psb_mlp_num_params(void);
+psb_status psb_mlp_set_params(psb_mlp m, const float* host_params, int32_t count);
+psb_status psb_mlp_get_params(psb_mlp m, float* host_params, int32_t count);
+/* Input standardisation x' = (x - mean) / stddev and the output scale:
+ * predicted pressure = pressure_scale * network output.                     */
+psb_status psb_mlp_set_normalization(psb_mlp m, const float mean[8], const float stddev[8],
+                                     float pressure_scale);
+/* out[i] = network output (unscaled) for features[k * n + i] (device SoA). */
+psb_status psb_mlp_forward(psb_mlp m, const float* features, uint32_t n, float* out);
+/* One minibatch of MSE training on (features[k * num_samples + s],
+ * targets[s]) for s = batch_idx[0 .. batch) (device arrays): gradients
+ * accumulate on the device; apply_adam != 0 then takes one Adam step
+ * (lr, beta 0.9 / 0.999, eps 1e-8) and clears them.  *loss_out (host, may be
+ * NULL) receives the minibatch mean squared error before the update.       */
+psb_status psb_mlp_train_step(psb_mlp m, const float* features, const float* targets,
+                              uint64_t num_samples, const uint32_t* batch_idx, uint32_t batch,
+                              float lr, int32_t apply_adam, float* loss_out);
+psb_status psb_mlp_get_grad(psb_mlp m, float* host_grad, int32_t count);
+psb_status psb_mlp_zero_grad(psb_mlp m);
+
+/* Features of the warm start, per particle, computed by the PCISPH passes
+ * (FP32, sorted order, SoA [8][n]):
+ *   0: 100 (rho* - rho0)/rho0 at the first prediction (p = 0)
+ *   1: dt sum_j (m/rho_j)(v_j - v_i).gradW_ij      (velocity divergence)
+ *   2: neighbour count / 32
+ *   3: previous-step pressure / pressure_scale
+ *   4: |v_i| dt / h
+ *   5-7: (h / pressure_scale) sum_j (m/rho_j)(p_j - p_i) gradW_ij  (previous-step pressure)
+ * pressure_scale = delta_template * rho0 / 100 (the pressure correcting a
+ * 1 % compression).                                                        */
+float      psb_system_pressure_scale(psb_system sys);
+/* Attach the network used when params.warm_start == PSB_WARM_MLP. */
+psb_status psb_system_set_mlp(psb_system sys, psb_mlp m);
+/* Device pointer to the feature buffer of the last step ([8][n]); NULL
+ * until features were computed (MLP warm start or recording).             */
+psb_status psb_system_get_features(psb_system sys, const float** features);
+
+/* Training set: feature SoA [8][capacity] + target[capacity] on the device. */
+typedef struct psb_dataset_s* psb_dataset;
+psb_status psb_dataset_create(psb_context ctx, uint64_t capacity, psb_dataset* out);
+psb_status psb_dataset_destroy(psb_dataset d);
+psb_status psb_dataset_reset(psb_dataset d);
+psb_status psb_dataset_info(psb_dataset d, const float** features, const float** targets,
+                            uint64_t* count, uint64_t* capacity);
+/* Record, after every PCISPH step, the features of every `stride`-th particle
+ * (offset rotating with the step) and its converged pressure / pressure_scale
+ * as the target.  Recording stops when the dataset is full.  NULL detaches. */
+psb_status psb_system_set_recorder(psb_system sys, psb_dataset d, uint32_t stride);
+
 #ifdef __cplusplus
 }
 #endif

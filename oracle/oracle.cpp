@@ -59,6 +59,7 @@ template <class R> struct Derived {
   R m, dt, sigma, l, gx, gy, gz;
   R box_lo[3], box_hi[3], one_plus_e;
   int R_stencil;
+  R pscale, inv_ps, h_over_ps, dt_over_h;   // warm-start feature scales
 };
 
 double template_w2(double h, double s) {
@@ -121,6 +122,15 @@ template <class R> Derived<R> derive(const psb_sph_params& p) {
   double rr = std::ceil((double)h / (double)p.grid.cell_size);
   if ((rr - 1.0) * (double)p.grid.cell_size >= (double)h) rr -= 1.0;
   d.R_stencil = (int)rr;
+  {
+    double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
+    const double ps = p.delta > 0.f ? (double)p.delta * rho0 * 0.01
+                                     : rho0 * 0.01 / (beta * template_w2(h, s));
+    d.pscale = (R)(float)ps;
+    d.inv_ps = (R)(float)(1.0 / ps);
+    d.h_over_ps = (R)(float)(h / ps);
+    d.dt_over_h = (R)(float)(dt / h);
+  }
   return d;
 }
 
@@ -338,6 +348,74 @@ inline void surface_finish(const Derived<R>& d, const PairSums<R>& s, R sigma, R
 }
 
 // ---------------------------------------------------------------------------
+// Pressure MLP 8 -> 64 -> 64 -> 1 (SiLU), parameters W1[64][8] b1[64]
+// W2[64][64] b2[64] W3[64] b3 (W[out][in]).  Computed in double; with
+// emulate_bf16 the operands of the GPU's BF16 matrix products are rounded to
+// BF16 (round to nearest even) at the same points: normalised inputs, W1,
+// layer-1 activations, W2, W3, and in the backward pass dZ2, dZ1 and the
+// activations they multiply.  Without it, plain FP64 (for finite differences).
+// ---------------------------------------------------------------------------
+constexpr int kIn = 8, kHid = 64, kNP = 4801;
+constexpr int oW1 = 0, oB1 = 512, oW2 = 576, oB2 = 4672, oW3 = 4736, oB3 = 4800;
+
+inline float bf16_round(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) {
+    u = (u & 0xffff0000u) | ((u & 0xffffu) ? 0x400000u : 0u);
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+  }
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+struct Mlp {
+  const float* w;
+  float mu[kIn], inv_sd[kIn];
+  bool bf;
+  double q(double v) const { return bf ? (double)bf16_round((float)v) : v; }
+  double W1(int j, int i) const { return q(w[oW1 + j * kIn + i]); }
+  double W2(int j, int k) const { return q(w[oW2 + j * kHid + k]); }
+  double W3(int k) const { return q(w[oW3 + k]); }
+  static double sg(double z) { return 1.0 / (1.0 + std::exp(-z)); }
+  // forward of one sample; returns y, fills activations
+  double forward(const double x[kIn], double z1[kHid], double h1[kHid], double z2[kHid],
+                 double h2[kHid]) const {
+    for (int j = 0; j < kHid; ++j) {
+      double s = 0;
+      for (int i = 0; i < kIn; ++i) s += q(x[i]) * W1(j, i);
+      z1[j] = s + w[oB1 + j];
+      h1[j] = z1[j] * sg(z1[j]);
+    }
+    for (int j = 0; j < kHid; ++j) {
+      double s = 0;
+      for (int k = 0; k < kHid; ++k) s += q(h1[k]) * W2(j, k);
+      z2[j] = s + w[oB2 + j];
+      h2[j] = z2[j] * sg(z2[j]);
+    }
+    double y = 0;
+    for (int k = 0; k < kHid; ++k) y += h2[k] * W3(k);
+    return y + w[oB3];
+  }
+  void input(const float* feat, uint64_t stride, uint64_t s, double x[kIn]) const {
+    for (int i = 0; i < kIn; ++i)
+      x[i] = bf ? (double)((feat[i * stride + s] - mu[i]) * inv_sd[i])
+                : ((double)feat[i * stride + s] - mu[i]) * inv_sd[i];
+  }
+};
+
+Mlp make_mlp(const float* w, const float* mu, const float* sd, bool bf) {
+  Mlp m;
+  m.w = w;
+  m.bf = bf;
+  for (int i = 0; i < kIn; ++i) { m.mu[i] = mu[i]; m.inv_sd[i] = 1.0f / sd[i]; }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
 // System: sorted state of one scene, advanced by WCSPH or PCISPH steps.
 // ---------------------------------------------------------------------------
 template <class R> struct System {
@@ -349,6 +427,10 @@ template <class R> struct System {
   std::vector<V4<R>> Ax, Av, Avh, Bx, Bv, Bvh;
   std::vector<uint32_t> Aid, Bid, hash, perm, start, count, list;
   std::vector<R> Ap, Bp, rho, pres, rho_pred, dl;
+  bool feat_on = false;
+  std::vector<R> feat;
+  std::vector<float> mlp_w;
+  float mlp_mu[kIn] = {0}, mlp_sd[kIn] = {1, 1, 1, 1, 1, 1, 1, 1};
   std::vector<V4<R>> fa, fs4, anp, xp0, xp1;
   uint32_t flags = 0;
   int last_iters = 0;
@@ -371,6 +453,7 @@ template <class R> struct System {
     Ap.assign(cap, 0); Bp.assign(cap, 0); rho.assign(cap, 0); pres.assign(cap, 0);
     rho_pred.assign(cap, 0);
     dl.assign(cap, 0);
+    feat.assign(8 * (size_t)cap, 0);
   }
 
   // storeParticles + reorder (B -> A by the stable order).
@@ -466,10 +549,13 @@ template <class R> struct System {
     const R gx = ref ? (R)0 : d.gx, gz = ref ? (R)0 : d.gz;
     const R gy = ref ? (R)-9.81f : d.gy;
     const bool warm = !ref && prm.warm_start == PSB_WARM_PREVIOUS;
+    const bool mlp = !ref && prm.warm_start == PSB_WARM_MLP;
+    const bool feat_now = !ref && (mlp || feat_on);
     // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
       pres[k] = warm ? Ap[k] : (R)0;
+      if (feat_now) Ax[k].w = Ap[k];
       Av[k].w = (R)1 / rho[k];
     }
     // non-pressure forces + first prediction (PciSphSystem_Kernel.cu:117-170,
@@ -484,7 +570,7 @@ template <class R> struct System {
       R fS[3] = {0, 0, 0}, nrm[3];
       if (surf) surface_finish(d, s, sigma, fS, nrm);
       const V4<R> x = Ax[p], vh0 = Avh[p];
-      if (!ref) dl[p] = local_delta(p);
+      if (!ref) dl[p] = feat_now ? delta_and_features(p) : local_delta(p);
       V4<R> xp;
       if (ref) {
         // forcesAccum = (0, -9.81 rho, 0); F = ((Facc + fV) + fP) + fS
@@ -521,7 +607,8 @@ template <class R> struct System {
     int it = 0;
     R err = 0;
     while (true) {
-      err = pci_density(*cur);
+      err = pci_density(*cur, it == 0 && feat_now, it == 0 && mlp);
+      if (it == 0 && mlp) mlp_pressure(*cur);
       it++;
       bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it;
       if (fin && !ref && err > (R)prm.tolerance) flags |= PSB_FLAG_NOT_CONVERGED;
@@ -565,7 +652,63 @@ template <class R> struct System {
   // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
   // in reference mode; template-delta pressure update in config mode.
   // Returns max_i max(0, rho_i - rho0)/rho0.
-  R pci_density(std::vector<V4<R>>& X) {
+  // Warm-start features 1..7 and the pressure scaling of particle p (config
+  // mode), from its start-of-step neighbours; Ax[q].w = previous pressure.
+  R delta_and_features(uint32_t p) {
+    const V4<R> xi = Ax[p], vi = Av[p];
+    R W1x = 0, W1y = 0, W1z = 0, W2 = 0, div = 0, gx = 0, gy = 0, gz = 0;
+    for (uint32_t k = 0; k < count[p]; ++k) {
+      uint32_t q = list[(uint64_t)k * n + p];
+      const V4<R> xj = Ax[q], vj = Av[q];
+      R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      R d2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (d2 > (R)0 && d2 < d.h2) {
+        R dd = std::sqrt(d2);
+        R invd = (R)1 / dd;
+        R hd = d.h - dd;
+        R gw = d.c_spiky * (hd * hd);
+        R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+        W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        R vol = d.m * vj.w;
+        R t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
+        div = fmaR(t, vol, div);
+        R dp = (xj.w - xi.w) * vol;
+        gx = fmaR(ux, dp, gx); gy = fmaR(uy, dp, gy); gz = fmaR(uz, dp, gz);
+      }
+    }
+    feat[1 * (size_t)n + p] = div * d.dt;
+    feat[2 * (size_t)n + p] = (R)(float)count[p] * (R)0.03125f;
+    feat[3 * (size_t)n + p] = xi.w * d.inv_ps;
+    feat[4 * (size_t)n + p] = std::sqrt(dot3(vi.x, vi.y, vi.z, vi.x, vi.y, vi.z)) * d.dt_over_h;
+    feat[5 * (size_t)n + p] = gx * d.h_over_ps;
+    feat[6 * (size_t)n + p] = gy * d.h_over_ps;
+    feat[7 * (size_t)n + p] = gz * d.h_over_ps;
+    if (!prm.delta_local_cap) return d.delta;
+    R den = d.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
+    if (!(den > (R)0)) return d.delta;
+    R loc = (R)1 / den;
+    return loc < d.delta ? loc : d.delta;
+  }
+
+  // MLP warm start: p = max(0, pressure_scale * net(features)), also into X.w
+  void mlp_pressure(std::vector<V4<R>>& X) {
+    std::vector<float> f(8 * (size_t)n);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)feat[i];
+    Mlp m = make_mlp(mlp_w.data(), mlp_mu, mlp_sd, true);
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      double x[kIn], z1[kHid], h1[kHid], z2[kHid], h2[kHid];
+      m.input(f.data(), n, (uint64_t)kk, x);
+      float y = (float)m.forward(x, z1, h1, z2, h2);
+      float v = y * (float)d.pscale;
+      if (v < 0.f) v = 0.f;
+      pres[kk] = (R)v;
+      X[kk].w = (R)v;
+    }
+  }
+
+  R pci_density(std::vector<V4<R>>& X, bool write_feature = false, bool skip_update = false) {
     const bool ref = prm.mode == PSB_MODE_REFERENCE;
     R emax = 0;
 #pragma omp parallel for schedule(static) reduction(max : emax)
@@ -600,8 +743,11 @@ template <class R> struct System {
         X[p].w = pr / (r * r);
         rho_pred[p] = r;
       } else {
-        pr = pr + dl[p] * e;
-        if (pr < (R)0) pr = (R)0;
+        if (write_feature) feat[p] = (e * d.inv_rho0) * (R)100;
+        if (!skip_update) {
+          pr = pr + dl[p] * e;
+          if (pr < (R)0) pr = (R)0;
+        }
         X[p].w = pr;
         rho_pred[p] = r;
       }
@@ -853,6 +999,87 @@ int oracle_system_stats(void* hp, double* out /* 7 */) {
     out[6] = s->max_nbr_seen;
   };
   if (h->prec == 64) fill(h->d); else fill(h->f);
+  return 0;
+}
+
+int oracle_system_enable_features(void* hp, int on) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) h->d->feat_on = on != 0; else h->f->feat_on = on != 0;
+  return 0;
+}
+
+int oracle_system_features(void* hp, float* out) {
+  Handle* h = (Handle*)hp;
+  auto fill = [&](auto* s) { for (size_t i = 0; i < 8 * (size_t)s->n; ++i) out[i] = (float)s->feat[i]; };
+  if (h->prec == 64) fill(h->d); else fill(h->f);
+  return 0;
+}
+
+int oracle_system_set_mlp(void* hp, const float* w, const float* mu, const float* sd) {
+  Handle* h = (Handle*)hp;
+  auto set = [&](auto* s) {
+    s->mlp_w.assign(w, w + kNP);
+    for (int i = 0; i < kIn; ++i) { s->mlp_mu[i] = mu[i]; s->mlp_sd[i] = sd[i]; }
+  };
+  if (h->prec == 64) set(h->d); else set(h->f);
+  return 0;
+}
+
+double oracle_system_pressure_scale(void* hp) {
+  Handle* h = (Handle*)hp;
+  return h->prec == 64 ? (double)h->d->d.pscale : (double)h->f->d.pscale;
+}
+
+// MLP forward over a feature SoA [8][n] (unscaled network output).
+int oracle_mlp_forward(const float* w, const float* mu, const float* sd, const float* feat,
+                       uint64_t n, float* out, int emulate_bf16) {
+  Mlp m = make_mlp(w, mu, sd, emulate_bf16 != 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < (int64_t)n; ++s) {
+    double x[kIn], z1[kHid], h1[kHid], z2[kHid], h2[kHid];
+    m.input(feat, n, (uint64_t)s, x);
+    out[s] = (float)m.forward(x, z1, h1, z2, h2);
+  }
+  return 0;
+}
+
+// Gradient of the minibatch MSE (1/B) sum (y - t)^2 over samples idx[0..nb)
+// of a dataset (feature SoA [8][ns], target[ns]); grad[4801], *loss = mean.
+int oracle_mlp_grad(const float* w, const float* mu, const float* sd, const float* feat,
+                    const float* target, uint64_t ns, const uint32_t* idx, uint32_t nb,
+                    double* grad, double* loss, int emulate_bf16) {
+  Mlp m = make_mlp(w, mu, sd, emulate_bf16 != 0);
+  for (int i = 0; i < kNP; ++i) grad[i] = 0;
+  double L = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint64_t s = idx[b];
+    double x[kIn], z1[kHid], h1[kHid], z2[kHid], h2[kHid];
+    m.input(feat, ns, s, x);
+    const double y = m.forward(x, z1, h1, z2, h2);
+    const double e = y - target[s];
+    L += e * e;
+    const double dy = 2.0 * e / nb;
+    double dz2[kHid], dz1[kHid];
+    for (int k = 0; k < kHid; ++k) {
+      grad[oW3 + k] += h2[k] * dy;
+      const double sg = Mlp::sg(z2[k]);
+      dz2[k] = dy * m.W3(k) * (sg * (1.0 + z2[k] * (1.0 - sg)));
+      grad[oB2 + k] += dz2[k];
+    }
+    grad[oB3] += dy;
+    for (int j = 0; j < kHid; ++j)
+      for (int k = 0; k < kHid; ++k) grad[oW2 + j * kHid + k] += m.q(dz2[j]) * m.q(h1[k]);
+    for (int k = 0; k < kHid; ++k) {
+      double dh = 0;
+      for (int j = 0; j < kHid; ++j) dh += m.q(dz2[j]) * m.W2(j, k);
+      const double sg = Mlp::sg(z1[k]);
+      dz1[k] = dh * (sg * (1.0 + z1[k] * (1.0 - sg)));
+      grad[oB1 + k] += dz1[k];
+    }
+    for (int j = 0; j < kHid; ++j)
+      for (int i = 0; i < kIn; ++i) grad[oW1 + j * kIn + i] += m.q(dz1[j]) * m.q(x[i]);
+  }
+  *loss = L / nb;
   return 0;
 }
 

@@ -48,6 +48,14 @@ def lib():
         L.oracle_system_get.argtypes = [vp] + [vp] * 9
         L.oracle_system_stats.argtypes = [vp, vp]
         L.oracle_system_arrays.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.oracle_system_enable_features.argtypes = [vp, C.c_int]
+        L.oracle_system_features.argtypes = [vp, vp]
+        L.oracle_system_set_mlp.argtypes = [vp, vp, vp, vp]
+        L.oracle_system_pressure_scale.restype = C.c_double
+        L.oracle_system_pressure_scale.argtypes = [vp]
+        L.oracle_mlp_forward.argtypes = [vp, vp, vp, vp, C.c_uint64, vp, C.c_int]
+        L.oracle_mlp_grad.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, C.c_uint32, vp, vp,
+                                      C.c_int]
         _lib = L
     return _lib
 
@@ -106,6 +114,34 @@ def collision_box(params, pos4, vel4):
     return x, v
 
 
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def mlp_forward(w, mean, std, feat, emulate_bf16: bool = True) -> np.ndarray:
+    """Unscaled network output for a feature SoA (8, n)."""
+    feat = _f32(feat)
+    n = feat.shape[1]
+    out = np.empty(n, np.float32)
+    w, mean, std = _f32(w), _f32(mean), _f32(std)
+    lib().oracle_mlp_forward(w.ctypes.data, mean.ctypes.data, std.ctypes.data, feat.ctypes.data,
+                             n, out.ctypes.data, 1 if emulate_bf16 else 0)
+    return out
+
+
+def mlp_grad(w, mean, std, feat, target, idx, emulate_bf16: bool = True):
+    """(grad[4801], loss) of the minibatch MSE over samples idx of (feat (8, ns), target)."""
+    feat, target = _f32(feat), _f32(target)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    w, mean, std = _f32(w), _f32(mean), _f32(std)
+    g = np.zeros(4801, np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().oracle_mlp_grad(w.ctypes.data, mean.ctypes.data, std.ctypes.data, feat.ctypes.data,
+                          target.ctypes.data, feat.shape[1], idx.ctypes.data, idx.size,
+                          g.ctypes.data, loss.ctypes.data, 1 if emulate_bf16 else 0)
+    return g, float(loss[0])
+
+
 class OracleSystem:
     """Oracle counterpart of psb_system (precision 32 = canonical FP32, 64 = FP64)."""
 
@@ -159,6 +195,22 @@ class OracleSystem:
                                    o["list"].ctypes.data)
         o["list"] = o["list"].reshape(cap, n)
         return o
+
+    def enable_features(self, on: bool = True):
+        lib().oracle_system_enable_features(self.h, 1 if on else 0)
+
+    def features(self) -> np.ndarray:
+        out = np.empty(8 * self.n, np.float32)
+        lib().oracle_system_features(self.h, out.ctypes.data)
+        return out.reshape(8, self.n)
+
+    def set_mlp(self, w, mean, std):
+        self._mlp = (_f32(w), _f32(mean), _f32(std))
+        lib().oracle_system_set_mlp(self.h, *(a.ctypes.data for a in self._mlp))
+
+    @property
+    def pressure_scale(self) -> float:
+        return float(lib().oracle_system_pressure_scale(self.h))
 
     def close(self):
         if self.h:

@@ -198,6 +198,25 @@ SIGNATURES = {
     "psb_system_set_timing": (i32, [vp, i32]),
     "psb_system_get_timing": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_int64)]),
     "psb_system_reset_timing": (i32, [vp]),
+    "psb_mlp_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
+    "psb_mlp_destroy": (i32, [vp]),
+    "psb_mlp_num_params": (i32, []),
+    "psb_mlp_set_params": (i32, [vp, vp, i32]),
+    "psb_mlp_get_params": (i32, [vp, vp, i32]),
+    "psb_mlp_set_normalization": (i32, [vp, vp, vp, f32]),
+    "psb_mlp_forward": (i32, [vp, vp, u32, vp]),
+    "psb_mlp_train_step": (i32, [vp, vp, vp, C.c_uint64, vp, u32, f32, i32, C.POINTER(f32)]),
+    "psb_mlp_get_grad": (i32, [vp, vp, i32]),
+    "psb_mlp_zero_grad": (i32, [vp]),
+    "psb_system_pressure_scale": (f32, [vp]),
+    "psb_system_set_mlp": (i32, [vp, vp]),
+    "psb_system_get_features": (i32, [vp, C.POINTER(vp)]),
+    "psb_dataset_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
+    "psb_dataset_destroy": (i32, [vp]),
+    "psb_dataset_reset": (i32, [vp]),
+    "psb_dataset_info": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_uint64)]),
+    "psb_system_set_recorder": (i32, [vp, vp, u32]),
 }
 
 _lib = None

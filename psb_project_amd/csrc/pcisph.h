@@ -22,6 +22,7 @@ struct PciBuffers {
   float4* anp;   // config mode: non-pressure acceleration
   float4* xp0;   // first predicted positions
   float* dl;     // config mode: per-particle pressure scaling delta_i
+  float* feat;   // warm-start features [8][n], or null when not computed
   const float4* ax;   // start-of-step positions
   const float4* avh;  // start-of-step half-step velocities
   const uint32_t* aid;
@@ -42,9 +43,16 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
                                   const float4* vs, const float* rho, const float4* avh,
                                   const float* pres, const PciBuffers& b, const PciOut& o,
                                   hipStream_t s);
+// density-pass flags (config mode)
+constexpr int kDensWriteFeature = 1;   // write feature 0 (first-prediction density error)
+constexpr int kDensSkipUpdate = 2;     // leave the pressure to the MLP
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
-                              const float* dl, PciStatus* st, int k, hipStream_t s);
+                              const float* dl, PciStatus* st, int k, float* feat, int dflags,
+                              hipStream_t s);
+psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
+                         uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
+                         uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s);
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
@@ -55,8 +63,8 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
                                const float4* vel, float* rho, float* pres, float4* xs, float4* vs,
                                hipStream_t s);
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   const float* rho, float* pres, const float* pprev, float4* xs,
-                                   float4* vs, hipStream_t s);
+                                   int xw_prev, const float* rho, float* pres, const float* pprev,
+                                   float4* xs, float4* vs, hipStream_t s);
 psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
                                     const uint32_t* list, const float4* xs, const float4* vs,
                                     const float* rho, const float* pres, const float4* avh,

@@ -71,9 +71,57 @@ __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32
   return loc < c.delta ? loc : c.delta;
 }
 
+// The same neighbour sums plus the warm-start features 1..7 of particle p
+// (velocity divergence, neighbour count, previous pressure, |v|, previous
+// pressure gradient); xs[q].w holds the previous-step pressure, vs[q].w 1/rho.
+__device__ __forceinline__ float delta_and_features(const Consts& c, uint32_t p, uint32_t n,
+                                                    const uint32_t* __restrict__ cnt,
+                                                    const uint32_t* __restrict__ list,
+                                                    const float4* __restrict__ xs,
+                                                    const float4* __restrict__ vs,
+                                                    float* __restrict__ feat) {
+  const float4 xi = xs[p], vi = vs[p];
+  float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
+  float div = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+  const uint32_t m = cnt[p];
+  for_neighbors2<kUnroll>(p, n, m, list, xs, vs, [&](uint32_t, const float4& xj,
+                                                     const float4& vj) {
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (d2 > 0.f && d2 < c.h2) {
+      const float dd = sqrtf(d2);
+      const float invd = 1.0f / dd;
+      const float hd = c.h - dd;
+      const float gw = c.c_spiky * (hd * hd);
+      const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+      W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+      W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+      const float vol = c.m * vj.w;
+      const float t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
+      div = __builtin_fmaf(t, vol, div);
+      const float dp = (xj.w - xi.w) * vol;
+      gx = __builtin_fmaf(ux, dp, gx);
+      gy = __builtin_fmaf(uy, dp, gy);
+      gz = __builtin_fmaf(uz, dp, gz);
+    }
+  });
+  feat[1 * (size_t)n + p] = div * c.dt;
+  feat[2 * (size_t)n + p] = (float)m * 0.03125f;
+  feat[3 * (size_t)n + p] = xi.w * c.inv_ps;
+  feat[4 * (size_t)n + p] = sqrtf(dot3(vi.x, vi.y, vi.z, vi.x, vi.y, vi.z)) * c.dt_over_h;
+  feat[5 * (size_t)n + p] = gx * c.h_over_ps;
+  feat[6 * (size_t)n + p] = gy * c.h_over_ps;
+  feat[7 * (size_t)n + p] = gz * c.h_over_ps;
+  if (!c.delta_local_cap) return c.delta;
+  const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
+  if (!(den > 0.f)) return c.delta;
+  const float loc = 1.0f / den;
+  return loc < c.delta ? loc : c.delta;
+}
+
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:117-170)
 // + first prediction (integrateSPH_LeapFrog_Forces with f_P = 0).
-template <bool REF, bool SURF, bool WARM, bool OUT>
+template <bool REF, bool SURF, bool WARM, bool OUT, bool FEAT>
 __global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
                                                          const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ list,
@@ -109,7 +157,8 @@ __global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
       if (o.F) o.F[p] = make_float4(0.f, gy * ri, 0.f, 0.f);
     }
   } else {
-    b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs) : c.delta;
+    if (FEAT) b.dl[p] = delta_and_features(c, p, n, cnt, list, xs, vs, b.feat);
+    else b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs) : c.delta;
     const float inv = 1.0f / ri;
     const float a0 = (fV[0] + fS[0]) * inv + c.gx, a1 = (fV[1] + fS[1]) * inv + c.gy,
                 a2 = (fV[2] + fS[2]) * inv + c.gz;
@@ -142,7 +191,8 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
                                                      float* __restrict__ pres,
                                                      float* __restrict__ rho_pred,
                                                      const float* __restrict__ dl,
-                                                     PciStatus* __restrict__ st, int k) {
+                                                     PciStatus* __restrict__ st, int k,
+                                                     float* __restrict__ feat, int dflags) {
   __shared__ float red[4];
   {
     const int fi = st->final_iter;
@@ -176,8 +226,11 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
       if (r > c.rho0 && W2 > 0.f) pr = pr + e / (c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
       reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr / (r * r);
     } else {
-      pr = pr + dl[p] * e;
-      if (pr < 0.f) pr = 0.f;
+      if (dflags & kDensWriteFeature) feat[p] = (e * c.inv_rho0) * 100.0f;
+      if (!(dflags & kDensSkipUpdate)) {
+        pr = pr + dl[p] * e;
+        if (pr < 0.f) pr = 0.f;
+      }
       reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr;
     }
     rho_pred[p] = r;
@@ -298,6 +351,24 @@ psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s) {
   return PSB_OK;
 }
 
+template <bool R_, bool S_, bool W_, bool O_, bool F_>
+static void np_launch(dim3 g, dim3 t, hipStream_t s, uint32_t n, const Consts& c,
+                      const uint32_t* cnt, const uint32_t* list, const float4* xs,
+                      const float4* vs, const float* rho, const float4* avh, const float* pres,
+                      const PciBuffers& b, const PciOut& o) {
+  hipLaunchKernelGGL((k_pci_nonpressure<R_, S_, W_, O_, F_>), g, t, 0, s, n, c, cnt, list, xs, vs,
+                     rho, avh, pres, b, o);
+}
+
+template <bool R_, bool S_, bool W_, bool O_>
+static void np_launch_f(bool feat, dim3 g, dim3 t, hipStream_t s, uint32_t n, const Consts& c,
+                        const uint32_t* cnt, const uint32_t* list, const float4* xs,
+                        const float4* vs, const float* rho, const float4* avh, const float* pres,
+                        const PciBuffers& b, const PciOut& o) {
+  if (feat) np_launch<R_, S_, W_, O_, true>(g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
+  else np_launch<R_, S_, W_, O_, false>(g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
+}
+
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,
@@ -305,10 +376,10 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
                                   hipStream_t s) {
   if (!n) return PSB_OK;
   const bool ref = c.mode == PSB_MODE_REFERENCE;
+  const bool feat = !ref && b.feat != nullptr;
   dim3 g(blocks_for(n)), t(kBlock);
-#define PSB_NP(R_, S_, W_, O_)                                                           \
-  hipLaunchKernelGGL((k_pci_nonpressure<R_, S_, W_, O_>), g, t, 0, s, n, c, cnt, list, xs, vs, \
-                     rho, avh, pres, b, o)
+#define PSB_NP(R_, S_, W_, O_) \
+  np_launch_f<R_, S_, W_, O_>(feat, g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o)
   if (ref) {
     if (out) PSB_NP(true, true, false, true); else PSB_NP(true, true, false, false);
   } else if (surf) {
@@ -325,14 +396,43 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
 
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
-                              const float* dl, PciStatus* st, int k, hipStream_t s) {
+                              const float* dl, PciStatus* st, int k, float* feat, int dflags,
+                              hipStream_t s) {
   if (!n) return PSB_OK;
   if (c.mode == PSB_MODE_REFERENCE)
     hipLaunchKernelGGL(k_pci_density<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt, list,
-                       X, pres, rho_pred, dl, st, k);
+                       X, pres, rho_pred, dl, st, k, feat, 0);
   else
     hipLaunchKernelGGL(k_pci_density<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt,
-                       list, X, pres, rho_pred, dl, st, k);
+                       list, X, pres, rho_pred, dl, st, k, feat, dflags);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+// Append every `stride`-th particle (p = offset, offset + stride, ...) of the
+// step: features and target = converged pressure / pressure_scale.
+__global__ __launch_bounds__(256) void k_record(const float* __restrict__ feat,
+                                                const float* __restrict__ pres, uint32_t n,
+                                                uint32_t stride, uint32_t offset, float inv_ps,
+                                                float* __restrict__ dfeat,
+                                                float* __restrict__ dtarget, uint64_t dcap,
+                                                uint64_t base, uint32_t count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const uint64_t p = (uint64_t)offset + (uint64_t)j * stride;
+  const uint64_t d = base + j;
+  if (p >= n || d >= dcap) return;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dfeat[(uint64_t)k * dcap + d] = feat[(uint64_t)k * n + p];
+  dtarget[d] = pres[p] * inv_ps;
+}
+
+psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
+                         uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
+                         uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s) {
+  if (!count) return PSB_OK;
+  hipLaunchKernelGGL(k_record, dim3(blocks_for(count)), dim3(kBlock), 0, s, feat, pres, n, stride,
+                     offset, inv_ps, dfeat, dtarget, dcap, base, count);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -427,7 +527,8 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   int iters = 0;
   float err = 0.f;
   for (int k = 0; k < c.max_it; ++k) {
-    PSB_TRY(launch_pci_density(n, c, nb.count, nb.list, cur, pressure, density, b.dl, st, k, s));
+    PSB_TRY(launch_pci_density(n, c, nb.count, nb.list, cur, pressure, density, b.dl, st, k,
+                               nullptr, 0, s));
     PSB_TRY(launch_pci_force(n, c, false, nb.count, nb.list, cur, nxt, density, pressure, b, st, k,
                              0, o, s));
     std::swap(cur, nxt);

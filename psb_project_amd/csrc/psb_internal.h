@@ -68,6 +68,8 @@ struct Consts {
   float tolerance;
   int eos, mode, box_enabled, min_it, max_it;
   int delta_local_cap;
+  // warm-start features: pressure scale and feature normalisers
+  float pscale, inv_ps, h_over_ps, dt_over_h, e_scale;
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);

@@ -120,6 +120,16 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   c->min_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.min_iterations;
   c->max_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.max_iterations;
   c->delta_local_cap = p.mode == PSB_MODE_CONFIG && p.delta_local_cap ? 1 : 0;
+  {
+    double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
+    const double ps = p.delta > 0.f ? (double)p.delta * rho0 * 0.01
+                                     : rho0 * 0.01 / (beta * template_w2(h, s));
+    c->pscale = (float)ps;
+    c->inv_ps = (float)(1.0 / ps);
+    c->h_over_ps = (float)(h / ps);
+    c->dt_over_h = (float)(dt / h);
+    c->e_scale = (float)(100.0 / rho0);
+  }
   if (g) *g = grid_dev(p.grid, p.support_radius);
   return PSB_OK;
 }

@@ -19,9 +19,11 @@
 namespace psb {
 
 // WCSPH: p = EOS(rho); x.w = p/rho^2; v.w = 1/rho.  PCISPH: p = 0 or the
-// reordered previous pressure; v.w = 1/rho.
+// reordered previous pressure; v.w = 1/rho; with xw_prev, x.w = previous
+// pressure (gathered by the warm-start feature sums).
 __global__ __launch_bounds__(256) void k_density_epilogue(uint32_t n, Consts c, int pcisph,
-                                                          int warm, const float* __restrict__ rho,
+                                                          int warm, int xw_prev,
+                                                          const float* __restrict__ rho,
                                                           float* __restrict__ pres,
                                                           const float* __restrict__ pprev,
                                                           float4* __restrict__ xs,
@@ -35,6 +37,7 @@ __global__ __launch_bounds__(256) void k_density_epilogue(uint32_t n, Consts c, 
     xs[p].w = pr / (r * r);
   } else {
     pres[p] = warm ? pprev[p] : 0.f;
+    if (xw_prev) xs[p].w = pprev[p];
   }
   vs[p].w = 1.0f / r;
 }
@@ -189,11 +192,11 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
 }
 
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   const float* rho, float* pres, const float* pprev, float4* xs,
-                                   float4* vs, hipStream_t s) {
+                                   int xw_prev, const float* rho, float* pres, const float* pprev,
+                                   float4* xs, float4* vs, hipStream_t s) {
   if (!n) return PSB_OK;
   hipLaunchKernelGGL(k_density_epilogue, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, pcisph,
-                     warm, rho, pres, pprev, xs, vs);
+                     warm, xw_prev, rho, pres, pprev, xs, vs);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

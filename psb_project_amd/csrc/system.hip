@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "mlp.h"
 #include "pcisph.h"
 #include "psb_internal.h"
 
@@ -23,18 +24,25 @@ namespace psb {
 
 enum Phase {
   PH_GRID, PH_NEIGHBORS, PH_EPILOGUE, PH_FORCES, PH_NONPRESSURE, PH_PCI_DENSITY, PH_PCI_FORCE,
-  PH_COUNT
+  PH_MLP, PH_RECORD, PH_COUNT
 };
 // grid_build = hash/count + 3-kernel scan + scatter + stable rank/reorder
 static const char* kPhaseNames[PH_COUNT] = {
     "grid_build", "neighbors_density", "density_epilogue", "wcsph_forces", "pci_nonpressure",
-    "pci_density", "pci_force"};
+    "pci_density", "pci_force", "mlp_infer", "record"};
 
 psb_status grid_create(psb_context ctx, const psb_grid_desc& d, uint32_t capacity, psb_grid* out);
 
 }  // namespace psb
 
 using namespace psb;
+
+struct psb_dataset_s {
+  psb_context ctx = nullptr;
+  float* feat = nullptr;    // [8][capacity]
+  float* target = nullptr;  // [capacity]
+  uint64_t capacity = 0, count = 0;
+};
 
 struct psb_system_s {
   psb_context ctx = nullptr;
@@ -50,6 +58,11 @@ struct psb_system_s {
   float *rho = nullptr, *pres = nullptr, *rho_pred = nullptr;
   float4 *fa = nullptr, *fs = nullptr, *anp = nullptr, *xp0 = nullptr, *xp1 = nullptr;
   float* dl = nullptr;            // config-mode PCISPH per-particle delta
+  float* feat = nullptr;          // warm-start features [8][cap] (lazily allocated)
+  psb_mlp mlp = nullptr;          // network of PSB_WARM_MLP
+  psb_dataset rec = nullptr;      // training-set recorder
+  uint32_t rec_stride = 1;
+  int64_t step_counter = 0;
   uint32_t* ncount = nullptr;
   uint32_t* nlist = nullptr;
   uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
@@ -180,7 +193,8 @@ static psb_status wcsph_step(psb_system s, int step) {
   }
   {
     Timed t(s, PH_EPILOGUE, step);
-    PSB_TRY(launch_density_epilogue(s->n, s->c, 0, 0, s->rho, s->pres, nullptr, s->ax, s->av, st));
+    PSB_TRY(launch_density_epilogue(s->n, s->c, 0, 0, 0, s->rho, s->pres, nullptr, s->ax, s->av,
+                                    st));
   }
   {
     Timed t(s, PH_FORCES, step);
@@ -195,6 +209,8 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   hipStream_t st = s->ctx->stream;
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
   const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
+  const bool feat = !ref && (mlp || s->rec != nullptr);
   PSB_TRY(launch_pci_reset(s->st, false, st));
   PSB_TRY(grid_phase(s, step));
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
@@ -205,11 +221,12 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   }
   {
     Timed t(s, PH_EPILOGUE, step);
-    PSB_TRY(launch_density_epilogue(s->n, s->c, 1, warm ? 1 : 0, s->rho, s->pres, s->ap, s->ax,
-                                    s->av, st));
+    PSB_TRY(launch_density_epilogue(s->n, s->c, 1, warm ? 1 : 0, feat ? 1 : 0, s->rho, s->pres,
+                                    s->ap, s->ax, s->av, st));
   }
   PciBuffers b{};
   b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
+  b.feat = feat ? s->feat : nullptr;
   b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
@@ -233,8 +250,15 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   for (int k = 0; k < max_it; ++k) {
     {
       Timed t(s, PH_PCI_DENSITY, step, k);
-      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl, s->st,
-                                 k, st));
+      const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
+      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
+                                 s->st, k, s->feat, dflags, st));
+    }
+    if (k == 0 && mlp) {
+      // the network sets the starting pressure (and the w lane of the
+      // predicted positions the force pass gathers)
+      Timed t(s, PH_MLP, step);
+      PSB_TRY(mlp_infer(s->mlp, s->feat, s->n, s->pres, cur, 1, s->c.pscale, st));
     }
     {
       Timed t(s, PH_PCI_FORCE, step, k);
@@ -263,6 +287,27 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   }
   if ((int)s->step_final.size() <= step) s->step_final.resize(step + 1, -1);
   s->step_final[step] = final_known;
+  if (s->rec && feat) {
+    // features and converged pressure of every rec_stride-th particle
+    psb_dataset d = s->rec;
+    const uint32_t stride = s->rec_stride;
+    const uint32_t offset = (uint32_t)(s->step_counter % stride);
+    uint64_t cnt = offset < s->n ? (uint64_t)(s->n - offset + stride - 1) / stride : 0;
+    if (d->count + cnt > d->capacity) cnt = d->capacity - d->count;
+    if (cnt) {
+      Timed t(s, PH_RECORD, step);
+      PSB_TRY(launch_record(s->feat, s->pres, s->n, stride, offset, s->c.inv_ps, d->feat,
+                            d->target, d->capacity, d->count, (uint32_t)cnt, st));
+      d->count += cnt;
+    }
+  }
+  s->step_counter += 1;
+  return PSB_OK;
+}
+
+static psb_status ensure_features(psb_system s) {
+  if (s->feat) return PSB_OK;
+  PSB_HIP(hipMalloc(&s->feat, 8 * 4 * (size_t)(s->cap ? s->cap : 1)));
   return PSB_OK;
 }
 
@@ -273,7 +318,6 @@ extern "C" {
 psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm, uint32_t capacity,
                              psb_system* out) {
   PSB_ARG(ctx && prm && out, "null argument");
-  PSB_ARG(prm->warm_start != PSB_WARM_MLP, "PSB_WARM_MLP is not available in this system");
   *out = nullptr;
   psb_system s = new psb_system_s();
   s->ctx = ctx;
@@ -301,7 +345,7 @@ psb_status psb_system_destroy(psb_system s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
-                  s->ncount, s->nlist, s->flags, s->st};
+                  s->ncount, s->nlist, s->flags, s->st, s->feat};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (auto e : s->ev) hipEventDestroy(e);
@@ -407,6 +451,12 @@ psb_status psb_system_set_particles_device(psb_system s, uint32_t n, const float
 psb_status psb_system_update(psb_system s, int32_t steps) {
   PSB_ARG(s && steps >= 0, "bad argument");
   if (!s->has_state) { set_error("psb_system_update before psb_system_set_particles"); return PSB_ERR_STATE; }
+  if (s->prm.solver == PSB_SOLVER_PCISPH && s->prm.mode == PSB_MODE_CONFIG &&
+      s->prm.warm_start == PSB_WARM_MLP) {
+    if (!s->mlp) { set_error("PSB_WARM_MLP without psb_system_set_mlp"); return PSB_ERR_STATE; }
+    PSB_ARG(s->c.min_it >= 2, "PSB_WARM_MLP needs min_iterations >= 2 (the first density pass "
+                              "only measures)");
+  }
   PSB_HIP(hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
   PSB_HIP(hipMemsetAsync(s->flags, 0, 8, st));
@@ -544,6 +594,75 @@ int32_t psb_system_get_timing(psb_system s, int32_t cap, const char** names, dou
 psb_status psb_system_reset_timing(psb_system s) {
   PSB_ARG(s, "null system");
   for (int i = 0; i < PH_COUNT; ++i) { s->t_total[i] = 0; s->t_count[i] = 0; }
+  return PSB_OK;
+}
+
+float psb_system_pressure_scale(psb_system s) { return s ? s->c.pscale : 0.f; }
+
+psb_status psb_system_set_mlp(psb_system s, psb_mlp m) {
+  PSB_ARG(s, "null system");
+  PSB_ARG(s->prm.mode == PSB_MODE_CONFIG && s->prm.solver == PSB_SOLVER_PCISPH,
+          "the MLP warm start needs a PSB_MODE_CONFIG PCISPH system");
+  if (m) PSB_TRY(ensure_features(s));
+  s->mlp = m;
+  return PSB_OK;
+}
+
+psb_status psb_system_get_features(psb_system s, const float** features) {
+  PSB_ARG(s && features, "null argument");
+  *features = s->feat;
+  return PSB_OK;
+}
+
+psb_status psb_dataset_create(psb_context ctx, uint64_t capacity, psb_dataset* out) {
+  PSB_ARG(ctx && out && capacity > 0, "bad argument");
+  PSB_HIP(hipSetDevice(ctx->device));
+  psb_dataset d = new psb_dataset_s();
+  d->ctx = ctx;
+  d->capacity = capacity;
+  hipError_t e = hipMalloc(&d->feat, 8 * 4 * capacity);
+  if (e == hipSuccess) e = hipMalloc(&d->target, 4 * capacity);
+  if (e != hipSuccess) { psb_dataset_destroy(d); return hip_fail(e, "dataset alloc", __FILE__, __LINE__); }
+  *out = d;
+  return PSB_OK;
+}
+
+psb_status psb_dataset_destroy(psb_dataset d) {
+  if (!d) return PSB_OK;
+  hipSetDevice(d->ctx->device);
+  hipStreamSynchronize(d->ctx->stream);
+  hipFree(d->feat);
+  hipFree(d->target);
+  delete d;
+  return PSB_OK;
+}
+
+psb_status psb_dataset_reset(psb_dataset d) {
+  PSB_ARG(d, "null dataset");
+  d->count = 0;
+  return PSB_OK;
+}
+
+psb_status psb_dataset_info(psb_dataset d, const float** features, const float** targets,
+                            uint64_t* count, uint64_t* capacity) {
+  PSB_ARG(d, "null dataset");
+  if (features) *features = d->feat;
+  if (targets) *targets = d->target;
+  if (count) *count = d->count;
+  if (capacity) *capacity = d->capacity;
+  return PSB_OK;
+}
+
+psb_status psb_system_set_recorder(psb_system s, psb_dataset d, uint32_t stride) {
+  PSB_ARG(s, "null system");
+  if (d) {
+    PSB_ARG(stride > 0, "stride must be > 0");
+    PSB_ARG(s->prm.mode == PSB_MODE_CONFIG && s->prm.solver == PSB_SOLVER_PCISPH,
+            "recording needs a PSB_MODE_CONFIG PCISPH system");
+    PSB_TRY(ensure_features(s));
+  }
+  s->rec = d;
+  s->rec_stride = stride ? stride : 1;
   return PSB_OK;
 }
 

@@ -100,6 +100,26 @@ class FluidSystem:
         check(lib().psb_copy_array_from_device(self.ctx.handle, out.ctypes.data, ptr, 0, out.nbytes))
         return out
 
+    # -- learned warm start -------------------------------------------------
+    @property
+    def pressure_scale(self) -> float:
+        return float(lib().psb_system_pressure_scale(self.handle))
+
+    def set_mlp(self, model) -> None:
+        check(lib().psb_system_set_mlp(self.handle, model.handle if model is not None else None))
+
+    def set_recorder(self, dataset, stride: int = 16) -> None:
+        check(lib().psb_system_set_recorder(self.handle, dataset.handle if dataset else None,
+                                            int(stride)))
+
+    def features(self) -> np.ndarray:
+        """Warm-start features of the last step, (8, n) in sorted order."""
+        ptr = C.c_void_p()
+        check(lib().psb_system_get_features(self.handle, C.byref(ptr)))
+        if not ptr.value:
+            raise RuntimeError("no features computed (enable the MLP warm start or a recorder)")
+        return self.download(ptr.value, np.float32, (8, self.n))
+
     # -- timing ------------------------------------------------------------
     def set_timing(self, on: bool) -> None:
         check(lib().psb_system_set_timing(self.handle, 1 if on else 0))
