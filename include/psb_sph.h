@@ -121,7 +121,23 @@ typedef struct psb_sph_params {
                                with one-sided neighbourhoods (walls without
                                boundary particles, free surface) are not
                                over-corrected; 0 -> template delta only       */
+  int32_t wall_model;       /* PSB_WALL_*: PCISPH config mode only           */
 } psb_sph_params;
+
+#define PSB_WALL_CLAMP      0  /* box walls act only through the collider (clamp) */
+#define PSB_WALL_LATTICE    1  /* the box walls are also lattice boundary
+                                  particles: implicit particles of mass m at
+                                  box_min + (i + 1/2) lattice_spacing for every
+                                  lattice index i outside the box on some axis
+                                  (ceil(h / spacing) + 1 layers), so a fluid
+                                  lattice at rest keeps rho0 up to the walls,
+                                  edges and corners.  They add to the density
+                                  and to sum gradW of delta_i, and push with
+                                  the particle's own pressure only,
+                                  -m psi_b p_i / rho0^2 gradW with psi_b = m
+                                  (Akinci et al. 2012); they are
+                                  computed in registers, never stored.  The
+                                  collider stays as a safety net              */
 
 /* Fill *p with the reference defaults of SPHParticle / SimulationData_SPHSystem
  * (SphParticle.cpp:7-22): water, h = 0.0457, rho0 = 998.29, k = 3,

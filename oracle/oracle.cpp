@@ -60,6 +60,9 @@ template <class R> struct Derived {
   R box_lo[3], box_hi[3], one_plus_e;
   int R_stencil;
   R pscale, inv_ps, h_over_ps, dt_over_h;   // warm-start feature scales
+  bool wall;                                 // lattice box walls
+  int wall_n[3], wall_layers;
+  R wall_o[3], wall_s, wall_inv_s, wall_r;
 };
 
 double template_w2(double h, double s) {
@@ -131,12 +134,83 @@ template <class R> Derived<R> derive(const psb_sph_params& p) {
     d.h_over_ps = (R)(float)(h / ps);
     d.dt_over_h = (R)(float)(dt / h);
   }
+  d.wall = p.mode == PSB_MODE_CONFIG && p.solver == PSB_SOLVER_PCISPH &&
+           p.wall_model == PSB_WALL_LATTICE && p.box_enabled;
+  {
+    const double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
+    for (int a = 0; a < 3; ++a) {
+      d.wall_o[a] = (R)p.box_min[a];
+      d.wall_n[a] = (int)std::floor(((double)p.box_max[a] - p.box_min[a]) / s + 1e-3);
+    }
+    d.wall_s = (R)(float)s;
+    d.wall_inv_s = (R)(float)(1.0 / s);
+    d.wall_r = (R)(float)(h / s);
+    d.wall_layers = (int)std::ceil(h / s) + 1;
+  }
   return d;
 }
+
 
 template <class R> inline R fmaR(R a, R b, R c) { return std::fma(a, b, c); }
 template <class R> inline R dot3(R ax, R ay, R az, R bx, R by, R bz) {
   return fmaR(ax, bx, fmaR(ay, by, az * bz));
+}
+
+// Lattice box walls (PSB_WALL_LATTICE): implicit boundary particles of mass
+// m at wall_o + (i + 1/2) s for each lattice index outside [0, wall_n) on
+// some axis, within wall_layers of the box.  wall_range gives the candidate
+// index range along one axis (the distance test decides); wall_sums adds the
+// poly6 numerators (h^2 - d^2)^3 and the spiky gradients of the sites within
+// h, visited x-major, then y, then z ascending.
+template <class R>
+inline void wall_range(const Derived<R>& d, int a, R x, int& lo, int& hi) {
+  lo = 0; hi = -1;
+  const R u = (x - d.wall_o[a]) * d.wall_inv_s - (R)0.5;
+  R l = std::ceil(u - d.wall_r), r = std::floor(u + d.wall_r);
+  if (!(l <= r)) return;
+  l = std::max(l, (R)-d.wall_layers);
+  r = std::min(r, (R)(d.wall_n[a] + d.wall_layers - 1));
+  if (l <= r) { lo = (int)l; hi = (int)r; }
+}
+
+template <class R>
+inline void wall_sums(const Derived<R>& d, bool dens, bool grad, R x, R y, R z, R& S, R& gx,
+                      R& gy, R& gz) {
+  S = 0; gx = 0; gy = 0; gz = 0;
+  int lx, hx, ly, hy, lz, hz;
+  wall_range(d, 0, x, lx, hx);
+  wall_range(d, 1, y, ly, hy);
+  wall_range(d, 2, z, lz, hz);
+  const int nx = d.wall_n[0], ny = d.wall_n[1], nz = d.wall_n[2];
+  if (!(lx < 0 || hx >= nx || ly < 0 || hy >= ny || lz < 0 || hz >= nz)) return;
+  for (int ix = lx; ix <= hx; ++ix) {
+    const bool xin = ix >= 0 && ix < nx;
+    const R dx = x - (d.wall_o[0] + ((R)ix + (R)0.5) * d.wall_s);
+    for (int iy = ly; iy <= hy; ++iy) {
+      const bool yin = iy >= 0 && iy < ny;
+      const R dy = y - (d.wall_o[1] + ((R)iy + (R)0.5) * d.wall_s);
+      for (int iz = lz; iz <= hz; ++iz) {
+        if (xin && yin && iz >= 0 && iz < nz) continue;   // inside the box
+        const R dz = z - (d.wall_o[2] + ((R)iz + (R)0.5) * d.wall_s);
+        const R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (d2 < d.h2) {
+          if (dens) {
+            const R w = d.h2 - d2;
+            S = fmaR(w * w, w, S);
+          }
+          if (grad && d2 > (R)0) {
+            const R dd = std::sqrt(d2);
+            const R invd = (R)1 / dd;
+            const R hd = d.h - dd;
+            const R gw = d.c_spiky * (hd * hd);
+            gx = gx + (dx * invd) * gw;
+            gy = gy + (dy * invd) * gw;
+            gz = gz + (dz * invd) * gw;
+          }
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,6 +630,11 @@ template <class R> struct System {
     for (int64_t k = 0; k < (int64_t)n; ++k) {
       pres[k] = warm ? Ap[k] : (R)0;
       if (feat_now) Ax[k].w = Ap[k];
+      if (d.wall) {
+        R Sw, gx_, gy_, gz_;
+        wall_sums(d, true, false, Ax[k].x, Ax[k].y, Ax[k].z, Sw, gx_, gy_, gz_);
+        rho[k] = rho[k] + Sw * d.mc;
+      }
       Av[k].w = (R)1 / rho[k];
     }
     // non-pressure forces + first prediction (PciSphSystem_Kernel.cu:117-170,
@@ -643,6 +722,11 @@ template <class R> struct System {
         W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
       }
     }
+    if (d.wall) {
+      R Sw, wgx, wgy, wgz;
+      wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, wgx, wgy, wgz);
+      W1x = W1x + wgx; W1y = W1y + wgy; W1z = W1z + wgz;
+    }
     R den = d.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
     if (!(den > (R)0)) return d.delta;
     R loc = (R)1 / den;
@@ -685,6 +769,11 @@ template <class R> struct System {
     feat[6 * (size_t)n + p] = gy * d.h_over_ps;
     feat[7 * (size_t)n + p] = gz * d.h_over_ps;
     if (!prm.delta_local_cap) return d.delta;
+    if (d.wall) {
+      R Sw, wgx, wgy, wgz;
+      wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, wgx, wgy, wgz);
+      W1x = W1x + wgx; W1y = W1y + wgy; W1z = W1z + wgz;
+    }
     R den = d.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
     if (!(den > (R)0)) return d.delta;
     R loc = (R)1 / den;
@@ -735,6 +824,11 @@ template <class R> struct System {
         }
       }
       R r = S * d.mc;
+      if (!ref && d.wall) {
+        R Sw, gx_, gy_, gz_;
+        wall_sums(d, true, false, xi.x, xi.y, xi.z, Sw, gx_, gy_, gz_);
+        r = r + Sw * d.mc;
+      }
       R pr = pres[p];
       R e = r - d.rho0;
       if (ref) {
@@ -790,6 +884,11 @@ template <class R> struct System {
           R c = (gw * invd) * (xi.w + xj.w);
           sx = fmaR(dx, c, sx); sy = fmaR(dy, c, sy); sz = fmaR(dz, c, sz);
         }
+      }
+      if (!ref && d.wall) {   // wall particles push with p_i only (p_b = 0)
+        R Sw, gx_, gy_, gz_;
+        wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, gx_, gy_, gz_);
+        sx = fmaR(gx_, xi.w, sx); sy = fmaR(gy_, xi.w, sy); sz = fmaR(gz_, xi.w, sz);
       }
       const V4<R> x0 = Ax[p], vh0 = Avh[p];
       V4<R> vh, x;
@@ -952,6 +1051,17 @@ int oracle_collision_box(const psb_sph_params* prm, uint32_t n, float* pos4, flo
     pos4[4 * i] = x.x; pos4[4 * i + 1] = x.y; pos4[4 * i + 2] = x.z;
     vel4[4 * i] = v.x; vel4[4 * i + 1] = v.y; vel4[4 * i + 2] = v.z;
   }
+  return 0;
+}
+
+// Lattice-wall sums at n points (FP32): out[4 i] = sum (h^2 - d^2)^3,
+// out[4 i + 1 .. 3] = sum of spiky gradients gradW(x_i - x_b).
+int oracle_wall_sums(const psb_sph_params* prm, uint32_t n, const float* xyz, float* out) {
+  Derived<float> d = derive<float>(*prm);
+  if (!d.wall) return 1;
+  for (uint32_t i = 0; i < n; ++i)
+    wall_sums(d, true, true, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], out[4 * i],
+              out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
   return 0;
 }
 

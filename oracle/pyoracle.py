@@ -40,6 +40,7 @@ def lib():
         L.oracle_evaluate_densities_forces.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp,
                                                        vp, vp, vp, vp]
         L.oracle_collision_box.argtypes = [vp, C.c_uint32, vp, vp]
+        L.oracle_wall_sums.argtypes = [vp, C.c_uint32, vp, vp]
         L.oracle_system_create.restype = vp
         L.oracle_system_create.argtypes = [vp, C.c_uint32, C.c_int]
         L.oracle_system_destroy.argtypes = [vp]
@@ -112,6 +113,15 @@ def collision_box(params, pos4, vel4):
     v = np.array(vel4, np.float32, copy=True)
     lib().oracle_collision_box(C.addressof(params), x.shape[0], x.ctypes.data, v.ctypes.data)
     return x, v
+
+
+def wall_sums(params, xyz):
+    """Lattice-wall sums at points xyz [n, 3]: columns S, grad_x, grad_y, grad_z."""
+    x = np.ascontiguousarray(xyz, np.float32)
+    out = np.zeros((x.shape[0], 4), np.float32)
+    rc = lib().oracle_wall_sums(C.addressof(params), x.shape[0], x.ctypes.data, out.ctypes.data)
+    assert rc == 0, "params do not enable lattice walls"
+    return out
 
 
 def _f32(a):

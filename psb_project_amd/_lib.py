@@ -31,6 +31,7 @@ MODE_REFERENCE, MODE_CONFIG = 0, 1
 SOLVER_WCSPH, SOLVER_PCISPH = 0, 1
 EOS_LINEAR, EOS_TAIT = 0, 1
 WARM_NONE, WARM_PREVIOUS, WARM_MLP = 0, 1, 2
+WALL_CLAMP, WALL_LATTICE = 0, 1
 
 
 class PsbError(RuntimeError):
@@ -78,6 +79,7 @@ class Params(C.Structure):
         ("delta", C.c_float),
         ("lattice_spacing", C.c_float),
         ("delta_local_cap", C.c_int32),
+        ("wall_model", C.c_int32),
     ]
 
     def copy(self) -> "Params":

@@ -63,7 +63,7 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
                                const float4* vel, float* rho, float* pres, float4* xs, float4* vs,
                                hipStream_t s);
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   int xw_prev, const float* rho, float* pres, const float* pprev,
+                                   int xw_prev, float* rho, float* pres, const float* pprev,
                                    float4* xs, float4* vs, hipStream_t s);
 psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
                                     const uint32_t* list, const float4* xs, const float4* vs,

@@ -65,6 +65,12 @@ __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32
       W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
     }
   });
+  if (c.wall) {
+    // the static wall particles add to sum gradW, not to sum |gradW|^2
+    float S, gx, gy, gz;
+    wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, gx, gy, gz);
+    W1x = W1x + gx; W1y = W1y + gy; W1z = W1z + gz;
+  }
   const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
   if (!(den > 0.f)) return c.delta;
   const float loc = 1.0f / den;
@@ -113,6 +119,11 @@ __device__ __forceinline__ float delta_and_features(const Consts& c, uint32_t p,
   feat[6 * (size_t)n + p] = gy * c.h_over_ps;
   feat[7 * (size_t)n + p] = gz * c.h_over_ps;
   if (!c.delta_local_cap) return c.delta;
+  if (c.wall) {
+    float S, wgx, wgy, wgz;
+    wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, wgx, wgy, wgz);
+    W1x = W1x + wgx; W1y = W1y + wgy; W1z = W1z + wgz;
+  }
   const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
   if (!(den > 0.f)) return c.delta;
   const float loc = 1.0f / den;
@@ -219,7 +230,12 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
         W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
       }
     });
-    const float r = S * c.mc;
+    float r = S * c.mc;
+    if (!REF && c.wall) {
+      float Sw, gx, gy, gz;
+      wall_sums<true, false>(c, xi.x, xi.y, xi.z, Sw, gx, gy, gz);
+      r = r + Sw * c.mc;
+    }
     float pr = pres[p];
     const float e = r - c.rho0;
     if (REF) {
@@ -306,6 +322,15 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
       sz = __builtin_fmaf(dz, cc, sz);
     }
   });
+  if (!REF && c.wall) {
+    // wall particles push with the particle's own pressure only (p_b = 0:
+    // F = -m psi_b p_i / rho0^2 gradW, psi_b = m on the fluid lattice)
+    float S, gx, gy, gz;
+    wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, gx, gy, gz);
+    sx = __builtin_fmaf(gx, xi.w, sx);
+    sy = __builtin_fmaf(gy, xi.w, sy);
+    sz = __builtin_fmaf(gz, xi.w, sz);
+  }
   const float4 x0 = b.ax[p], vh0 = b.avh[p];
   float4 vh;
   float fP0 = 0.f, fP1 = 0.f, fP2 = 0.f;
@@ -321,7 +346,7 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
   } else {
     const float4 a = b.anp[p];
     fP0 = sx * c.kp; fP1 = sy * c.kp; fP2 = sz * c.kp;
-    const float a0 = a.x + fP0, a1 = a.y + fP1, a2 = a.z + fP2;
+    float a0 = a.x + fP0, a1 = a.y + fP1, a2 = a.z + fP2;
     vh = make_float4(vh0.x + a0 * c.dt, vh0.y + a1 * c.dt, vh0.z + a2 * c.dt, 0.f);
   }
   float4 x = make_float4(x0.x + vh.x * c.dt, x0.y + vh.y * c.dt, x0.z + vh.z * c.dt, 0.f);

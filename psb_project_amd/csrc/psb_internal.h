@@ -70,6 +70,13 @@ struct Consts {
   int delta_local_cap;
   // warm-start features: pressure scale and feature normalisers
   float pscale, inv_ps, h_over_ps, dt_over_h, e_scale;
+  // lattice walls (PSB_WALL_LATTICE): implicit boundary particles at
+  // wall_o + (i + 1/2) s for every lattice index i outside [0, wall_n) on
+  // some axis and inside [-wall_layers, wall_n + wall_layers) on all axes
+  int wall;
+  int wall_n[3], wall_layers;
+  float wall_o[3];
+  float wall_s, wall_inv_s, wall_r;   // spacing, 1/spacing, h/spacing
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);
@@ -120,6 +127,68 @@ __device__ __forceinline__ void box_collide(const Consts& c, float4& x, float4& 
     x.x = px; x.y = py; x.z = pz;
     float vn = c.one_plus_e * dot3(v.x, v.y, v.z, nx, ny, nz);
     v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
+  }
+}
+
+// Index range [lo, hi] along one axis of the lattice sites within h of
+// coordinate x (superset; the distance test decides), clipped to the wall
+// layers [-L, n + L).
+__device__ __forceinline__ void wall_range(const Consts& c, int a, float x, int& lo, int& hi) {
+  lo = 0; hi = -1;
+  const float u = (x - c.wall_o[a]) * c.wall_inv_s - 0.5f;
+  float l = ceilf(u - c.wall_r), r = floorf(u + c.wall_r);
+  if (!(l <= r)) return;  // also NaN
+  l = fmaxf(l, (float)-c.wall_layers);
+  r = fminf(r, (float)(c.wall_n[a] + c.wall_layers - 1));
+  if (l <= r) { lo = (int)l; hi = (int)r; }
+}
+
+// Contributions of the lattice wall particles within h of (x, y, z): the
+// poly6 sum S += (h^2 - d^2)^3 (DENS) and the spiky gradient sum
+// g += gradW(x - x_b) (GRAD), sites visited x-major, then y, then z ascending.
+// A particle whose candidate sites all lie inside the box skips the loop.
+template <bool DENS, bool GRAD>
+__device__ __forceinline__ void wall_sums(const Consts& c, float x, float y, float z, float& S,
+                                          float& gx, float& gy, float& gz) {
+  S = 0.f; gx = 0.f; gy = 0.f; gz = 0.f;
+  int lx, hx, ly, hy, lz, hz;
+  wall_range(c, 0, x, lx, hx);
+  wall_range(c, 1, y, ly, hy);
+  wall_range(c, 2, z, lz, hz);
+  const int nx = c.wall_n[0], ny = c.wall_n[1], nz = c.wall_n[2];
+  if (!(lx < 0 || hx >= nx || ly < 0 || hy >= ny || lz < 0 || hz >= nz)) return;
+  for (int ix = lx; ix <= hx; ++ix) {
+    const bool xin = ix >= 0 && ix < nx;
+    const float dx = x - (c.wall_o[0] + ((float)ix + 0.5f) * c.wall_s);
+    for (int iy = ly; iy <= hy; ++iy) {
+      const bool yin = iy >= 0 && iy < ny;
+      const float dy = y - (c.wall_o[1] + ((float)iy + 0.5f) * c.wall_s);
+      // inside on x and y: only the z layers outside [0, nz) are walls
+      const int za = lz, zb = (xin && yin) ? min(hz, -1) : hz;
+      const int zc = (xin && yin) ? max(lz, nz) : hz + 1, zd = hz;
+      for (int part = 0; part < 2; ++part) {
+        const int z0 = part == 0 ? za : zc, z1 = part == 0 ? zb : zd;
+        for (int iz = z0; iz <= z1; ++iz) {
+          const float dz = z - (c.wall_o[2] + ((float)iz + 0.5f) * c.wall_s);
+          const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+          if (d2 < c.h2) {
+            if (DENS) {
+              const float w = c.h2 - d2;
+              S = __builtin_fmaf(w * w, w, S);
+            }
+            if (GRAD && d2 > 0.f) {
+              const float dd = sqrtf(d2);
+              const float invd = 1.0f / dd;
+              const float hd = c.h - dd;
+              const float gw = c.c_spiky * (hd * hd);
+              gx = gx + (dx * invd) * gw;
+              gy = gy + (dy * invd) * gw;
+              gz = gz + (dz * invd) * gw;
+            }
+          }
+        }
+      }
+    }
   }
 }
 

@@ -76,6 +76,15 @@ static psb_status check_params(const psb_sph_params& p) {
   PSB_ARG(p.min_iterations >= 1 && p.max_iterations >= p.min_iterations &&
               p.max_iterations <= kMaxIterSlots - 1,
           "PCISPH iterations: 1 <= min <= max <= 255");
+  PSB_ARG(p.wall_model == PSB_WALL_CLAMP || p.wall_model == PSB_WALL_LATTICE, "params.wall_model");
+  if (p.wall_model == PSB_WALL_LATTICE && p.box_enabled) {
+    const double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * p.support_radius;
+    PSB_ARG(p.lattice_spacing >= 0.f && p.support_radius <= 4.0 * s,
+            "lattice walls need support_radius <= 4 lattice_spacing");
+    for (int a = 0; a < 3; ++a)
+      PSB_ARG(p.box_max[a] - p.box_min[a] >= s && p.box_max[a] - p.box_min[a] < 1e6 * s,
+              "lattice walls need 1 <= box extent / lattice_spacing < 1e6");
+  }
   return PSB_OK;
 }
 
@@ -129,6 +138,19 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
     c->h_over_ps = (float)(h / ps);
     c->dt_over_h = (float)(dt / h);
     c->e_scale = (float)(100.0 / rho0);
+  }
+  c->wall = (p.mode == PSB_MODE_CONFIG && p.solver == PSB_SOLVER_PCISPH &&
+             p.wall_model == PSB_WALL_LATTICE && p.box_enabled) ? 1 : 0;
+  {
+    const double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
+    for (int a = 0; a < 3; ++a) {
+      c->wall_o[a] = p.box_min[a];
+      c->wall_n[a] = (int)std::floor(((double)p.box_max[a] - p.box_min[a]) / s + 1e-3);
+    }
+    c->wall_s = (float)s;
+    c->wall_inv_s = (float)(1.0 / s);
+    c->wall_r = (float)(h / s);
+    c->wall_layers = (int)std::ceil(h / s) + 1;
   }
   if (g) *g = grid_dev(p.grid, p.support_radius);
   return PSB_OK;

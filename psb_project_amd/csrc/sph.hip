@@ -23,7 +23,7 @@ namespace psb {
 // pressure (gathered by the warm-start feature sums).
 __global__ __launch_bounds__(256) void k_density_epilogue(uint32_t n, Consts c, int pcisph,
                                                           int warm, int xw_prev,
-                                                          const float* __restrict__ rho,
+                                                          float* __restrict__ rho,
                                                           float* __restrict__ pres,
                                                           const float* __restrict__ pprev,
                                                           float4* __restrict__ xs,
@@ -35,11 +35,20 @@ __global__ __launch_bounds__(256) void k_density_epilogue(uint32_t n, Consts c, 
     const float pr = eos(c, r);
     pres[p] = pr;
     xs[p].w = pr / (r * r);
+    vs[p].w = 1.0f / r;
   } else {
     pres[p] = warm ? pprev[p] : 0.f;
     if (xw_prev) xs[p].w = pprev[p];
+    float rr = r;
+    if (c.wall) {
+      const float4 x = xs[p];
+      float S, gx, gy, gz;
+      wall_sums<true, false>(c, x.x, x.y, x.z, S, gx, gy, gz);
+      rr = r + S * c.mc;
+      rho[p] = rr;
+    }
+    vs[p].w = 1.0f / rr;
   }
-  vs[p].w = 1.0f / r;
 }
 
 // Density from an existing list (standalone evaluate_densities_forces);
@@ -192,7 +201,7 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
 }
 
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   int xw_prev, const float* rho, float* pres, const float* pprev,
+                                   int xw_prev, float* rho, float* pres, const float* pprev,
                                    float4* xs, float4* vs, hipStream_t s) {
   if (!n) return PSB_OK;
   hipLaunchKernelGGL(k_density_epilogue, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, pcisph,

@@ -24,7 +24,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import (EOS_LINEAR, EOS_TAIT, MODE_CONFIG, MODE_REFERENCE, SOLVER_PCISPH,
-                   SOLVER_WCSPH, WARM_NONE, Derived, GridDesc, Params, check, lib)
+                   SOLVER_WCSPH, WALL_CLAMP, WALL_LATTICE, WARM_NONE, Derived, GridDesc,
+                   Params, check, lib)
 
 
 def default_params() -> Params:
@@ -133,8 +134,13 @@ def wcsph_dam_break(n_side: int = 32, s: float = 0.01, mode: int = MODE_CONFIG,
 def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
                      scene_id: int = 0, box=(1.28, 0.64, 0.64), tolerance: float = 0.01,
                      min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
-                     calibrate_mass: bool = True) -> Scene:
-    """Plain PCISPH dam break (64^3 block at the x-min end of the box)."""
+                     calibrate_mass: bool = True, lattice_walls: bool = False) -> Scene:
+    """Plain PCISPH dam break (64^3 block at the x-min end of the box).
+
+    lattice_walls: the box walls also act as implicit boundary particles on
+    the fluid lattice continued past them (config mode); otherwise they act
+    only through the box collider.
+    """
     p = default_params()
     p.mode = mode
     p.solver = SOLVER_PCISPH
@@ -162,5 +168,6 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
     p.max_iterations = max_it
     p.lattice_spacing = s
     p.delta_local_cap = 1
+    p.wall_model = WALL_LATTICE if lattice_walls else WALL_CLAMP
     pos = lattice(n_side, s, seed=2000 + scene_id)
     return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})
