@@ -1,6 +1,7 @@
 """Benchmark of the PCISPH timestep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--no-cpu-baseline] [--no-learned] [--timing-every T]
 
 A step is one PCI_SPHSystem::update of the whole scene (grid build, neighbour
 search, non-pressure forces, the tolerance-driven corrector, integration,
@@ -9,10 +10,17 @@ default workload is the 262,144-particle PCISPH dam break (64^3 block,
 s = 5 mm, 1.28 x 0.64 x 0.64 m box, 256 x 128 x 128 cells, 1 % tolerance,
 3..50 iterations).  W untimed steps, then K steps timed between two stream
 synchronisations; per-kernel durations come from HIP events recorded on the
-library's stream over the same timed steps.
+library's stream around every launch of every T-th timed step.
 
-Prints one JSON line with the metric, the roofline of the dominant kernel
-and the CPU baseline (the oracle on this host, a bounded sample).
+The learned warm start is then trained the way it is meant to be used
+(untimed): plain PCISPH records converged pressures on 16 seeded scenes of
+the same shape for 200 steps, the MLP is trained on the GPU, and 4 held-out
+seeds compare plain, previous-pressure and learned starts.  The same K steps
+of the workload are then timed again with the learned start.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel,
+the CPU baseline (the oracle on this host, a bounded sample) and the learned
+warm start's iterations and throughput.
 """
 from __future__ import annotations
 
@@ -30,8 +38,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory) bytes per particle per launch of each kernel:
-# FP32 SoA streams, each array read and written once, neighbour gathers
-# assumed cache resident ("Algorithmic bytes").
+# FP32 streams, each array read and written once, neighbour gathers assumed
+# cache resident (a neighbour's data is reused by its ~40 neighbours).
 ALG_BYTES = {
     "pci_density": 24,        # x* 12 + rho* 4 + p read/write 8
     "pci_force": 68,          # x*,p 16 + x 12 + v_half 12 + a_np 12 + write x* 12 + p 4
@@ -51,8 +59,14 @@ def parse():
     ap.add_argument("--workload", default="pcisph_dam_break",
                     choices=["pcisph_dam_break", "wcsph_dam_break"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    ap.add_argument("--timing-every", type=int, default=16,
+                    help="per-kernel events on every T-th timed step")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start")
+    ap.add_argument("--train-seeds", type=int, default=16)
+    ap.add_argument("--eval-seeds", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -93,11 +107,73 @@ def cpu_baseline(sc, start_state, first_step: int, steps: int):
     }
 
 
+def timed_run(ctx, fs, steps: int, split: bool):
+    """K steps between stream synchronisations; with `split` the state at the
+    midpoint is copied out (untimed) for the CPU baseline."""
+    half = steps // 2 if split else steps
+    total, stats, mid = 0.0, [], None
+    for k, nsteps in enumerate((half, steps - half)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        if nsteps:
+            fs.update(nsteps)
+        ctx.sync()
+        total += time.perf_counter() - t0
+        if nsteps:
+            stats.append(fs.stats())
+        if k == 0 and split:
+            mid = fs.state()
+    out = {"total_iterations": sum(x["total_iterations"] for x in stats),
+           "worst_error": max((x["worst_error"] for x in stats), default=0.0),
+           "flags": 0}
+    for x in stats:
+        out["flags"] |= x["flags"]
+    return total, out, mid
+
+
+def learned_leg(ctx, sc, args):
+    """Train the warm-start MLP (untimed) and time the workload with it."""
+    from psb_project_amd import learned, scenes
+    from psb_project_amd._lib import WARM_MLP
+    from psb_project_amd.system import FluidSystem
+    n_side = sc.meta["n_side"]
+    t0 = time.perf_counter()
+    model, summary = learned.train_and_evaluate(
+        ctx, lambda i: scenes.pcisph_dam_break(n_side=n_side, scene_id=100 + i),
+        range(args.train_seeds), range(args.train_seeds, args.train_seeds + args.eval_seeds),
+        steps=args.train_steps)
+    train_s = time.perf_counter() - t0
+    p = sc.params.copy()
+    p.warm_start = WARM_MLP
+    fs = FluidSystem(ctx, p, sc.n)
+    fs.set_mlp(model)
+    fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
+    if args.warmup:
+        fs.update(args.warmup)
+    elapsed, st, _ = timed_run(ctx, fs, args.steps, split=False)
+    fs.close()
+    model.close()
+    ev = summary["eval"]
+    return {
+        "value": round(sc.n * args.steps / elapsed, 1),
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "iters_per_step": round(st["total_iterations"] / args.steps, 3),
+        "max_density_error": st["worst_error"],
+        "flags": st["flags"],
+        "heldout": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                        for kk, vv in v.items()} for k, v in ev.items()},
+        "train": {"scenes": args.train_seeds, "steps": args.train_steps,
+                  "samples": summary["samples"], "epochs": summary["epochs"],
+                  "mse_per_epoch": [round(x, 5) for x in summary["mse_per_epoch"]],
+                  "seconds": round(train_s, 1)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is not built)"}))
         sys.exit(2)
     from psb_project_amd import scenes
     from psb_project_amd.device import Context
@@ -112,29 +188,12 @@ def main():
         sys_.update(args.warmup)
     ctx.sync()
     sys_.reset_timing()
-    sys_.set_timing(not args.no_timing)
+    sys_.set_timing(not args.no_timing, every=args.timing_every)
     print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
-    # The timed region runs in two halves; between them (untimed) the state is
-    # copied out as the starting point of the CPU baseline sample.
-    half = args.steps // 2
-    parts = []
-    start_state = None
-    for k, nsteps in enumerate((half, args.steps - half)):
-        ctx.sync()
-        t0 = time.perf_counter()
-        if nsteps:
-            sys_.update(nsteps)
-        ctx.sync()
-        parts.append((time.perf_counter() - t0, sys_.stats()))
-        if k == 0 and not args.no_cpu_baseline:
-            start_state = sys_.state()
-    elapsed = parts[0][0] + parts[1][0]
-    s0, s1 = parts[0][1], parts[1][1]
-    stats = {"total_iterations": s0["total_iterations"] + s1["total_iterations"],
-             "worst_error": max(s0["worst_error"], s1["worst_error"]),
-             "flags": s0["flags"] | s1["flags"]}
+    elapsed, stats, start_state = timed_run(ctx, sys_, args.steps, split=not args.no_cpu_baseline)
     timing = sys_.timing()
     sys_.set_timing(False)
+    sys_.close()
 
     value = sc.n * args.steps / elapsed
     iters = stats["total_iterations"] / max(1, args.steps)
@@ -151,11 +210,16 @@ def main():
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": read_traffic(dom, sc.name),
                         "alg_bytes_per_launch": alg,
-                        "avg_launch_us": round(kernels[dom]["avg_us"], 3)}
+                        "avg_launch_us": round(kernels[dom]["avg_us"], 3),
+                        "sampled_launches": kernels[dom]["launches"]}
+    learned = None
+    if not args.no_learned and sc.params.solver == 1:
+        print("bench: learned warm start (train untimed, then time)", file=sys.stderr, flush=True)
+        learned = learned_leg(ctx, sc, args)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sc, start_state, args.warmup + half, args.cpu_steps)
+        cpu = cpu_baseline(sc, start_state, args.warmup + args.steps // 2, args.cpu_steps)
     line = {
         "metric": "particle-steps/s at <=1% density error; PCISPH iters/step, plain vs learned",
         "value": round(value, 1),
@@ -179,11 +243,11 @@ def main():
         "flags": stats["flags"],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned": learned,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
     }
     print(json.dumps(line), flush=True)
-    sys_.close()
     ctx.close()
 
 

@@ -372,10 +372,13 @@ typedef struct psb_system_arrays {
 } psb_system_arrays;
 psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
 
-/* Per-phase timing with HIP events on the system stream (off by default).
+/* Per-phase timing with HIP events on the system stream (off by default):
+ * `every` = 0 disables it, k > 0 brackets every launch of every k-th step
+ * (counted over the system's lifetime) with a pair of events, so k > 1
+ * samples the kernels with 1/k of the event overhead.
  * Phase names: "hash_count", "scan", "scatter", "reorder", "neighbors_density",
  * "forces", "nonpressure", "pci_density", "pci_force", plus "mlp" etc.    */
-psb_status psb_system_set_timing(psb_system sys, int32_t enable);
+psb_status psb_system_set_timing(psb_system sys, int32_t every);
 /* Returns the number of phases; writes up to `cap` entries. */
 int32_t    psb_system_get_timing(psb_system sys, int32_t cap, const char** names,
                                  double* total_ms, int64_t* launches);

@@ -28,8 +28,11 @@
 // (UniformGridKernel.cu:43-55).  Neighbour lists are in ascending sorted index
 // order (rows of the stencil visited z, then y, then x ascending).
 //
-// Parity status of each function (what pins it) is listed in DESIGN.md and in
-// tests/test_oracle_pinning.py.
+// What pins each function (golden values, brute force, closed forms,
+// invariants) is in tests/test_oracle_pinning.py, tests/test_wall_oracle.py
+// and tests/test_mlp_oracle.py.  Config-mode extensions with no reference
+// counterpart: the per-particle delta cap, the lattice walls and the learned
+// warm start (features, MLP).
 
 #include <algorithm>
 #include <cmath>

@@ -35,6 +35,8 @@ struct PciOut {
   float4* pos; float4* old_pos; float4* vav; float4* vap;
   float4* fP; float4* normals; float4* fV; float4* fS; float4* F;
   uint32_t* flags;
+  // optional host-mapped slot that receives the final iteration index
+  int32_t* host_final;
 };
 
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s);

@@ -291,6 +291,9 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
       atomicAdd(&st->total_iters, (unsigned long long)it);
       atomicMax(&st->worst_err, __float_as_uint(err));
       if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
+      // tell the host, which stops queueing speculative iterations
+      if (o.host_final)
+        __hip_atomic_store(o.host_final, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
   const uint32_t p = particle_index();

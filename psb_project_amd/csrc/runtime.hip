@@ -3,6 +3,7 @@
 // Replaces the L0 helpers of src/common.cu (cudaInit :36-55, allocateArray /
 // freeArray :58-66, copyArray* :68-93, computeGridSize :102-107) with
 // status-returning versions on an explicit stream.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -10,6 +11,15 @@
 #include "psb_internal.h"
 
 namespace psb {
+
+// Streaming copy of n 16-byte words (copyArrayDeviceToDevice, common.cu:83):
+// each word read once and written once, grid-stride with coalesced dwordx4.
+__global__ __launch_bounds__(256) void k_copy_stream(uint4* __restrict__ dst,
+                                                     const uint4* __restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
 
 static thread_local std::string g_last_error = "";
 
@@ -300,8 +310,22 @@ psb_status psb_copy_array_from_device(psb_context ctx, void* host, const void* d
 psb_status psb_copy_array_device_to_device(psb_context ctx, void* dst, const void* src,
                                            size_t offset, size_t bytes) {
   PSB_ARG(ctx && dst && src, "null argument");
-  PSB_HIP(hipMemcpyAsync((char*)dst + offset, (const char*)src + offset, bytes,
-                         hipMemcpyDeviceToDevice, ctx->stream));
+  char* d = (char*)dst + offset;
+  const char* sp = (const char*)src + offset;
+  if (bytes == 0) return PSB_OK;
+  const bool aligned = ((uintptr_t)d % 16 == 0) && ((uintptr_t)sp % 16 == 0);
+  const size_t n16 = aligned ? bytes / 16 : 0;
+  if (n16) {
+    // grid-stride over 16-byte words, a few waves per CU of the 256
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n16 + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_copy_stream, dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                       reinterpret_cast<uint4*>(d), reinterpret_cast<const uint4*>(sp), n16);
+    PSB_HIP(hipGetLastError());
+  }
+  const size_t done = n16 * 16;
+  if (done < bytes)
+    PSB_HIP(hipMemcpyAsync(d + done, sp + done, bytes - done, hipMemcpyDeviceToDevice,
+                           ctx->stream));
   return PSB_OK;
 }
 

@@ -73,7 +73,7 @@ struct psb_system_s {
   bool stepped = false;
   psb_step_stats stats{};
   // timing
-  bool timing = false;
+  int32_t timing_every = 0;   // record events on every k-th step (0: off)
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
   struct Rec { int phase; size_t e0; int step; int iter; };
@@ -106,7 +106,9 @@ struct Timed {
   int phase, step, iter;
   size_t e0 = 0;
   bool on;
-  Timed(psb_system s, int ph, int st = 0, int it = -1) : sys(s), phase(ph), step(st), iter(it), on(s->timing) {
+  Timed(psb_system s, int ph, int st = 0, int it = -1)
+      : sys(s), phase(ph), step(st), iter(it),
+        on(s->timing_every > 0 && s->step_counter % s->timing_every == 0) {
     if (on && ev_get(sys, &e0) == PSB_OK) hipEventRecord(sys->ev[e0], sys->ctx->stream);
     else on = false;
   }
@@ -168,12 +170,6 @@ static psb_status sys_alloc(psb_system s) {
   return PSB_OK;
 }
 
-__global__ void k_mark_final(PciStatus* st, int32_t* host_slot) {
-  // copy the device-side final iteration of this step into host-mapped memory
-  if (threadIdx.x == 0)
-    __hip_atomic_store(host_slot, st->final_iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
 static psb_status grid_phase(psb_system s, int step) {
   ReorderArrays ra;
   ra.sx = s->bx; ra.sv = s->bv; ra.svh = s->bvh; ra.sid = s->bid; ra.sp = s->bp;
@@ -230,6 +226,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
+  o.host_final = ref ? nullptr : s->d_final + slot;
   const bool surf = ref || s->prm.surface_tension != 0.f;
   {
     Timed t(s, PH_NONPRESSURE, step);
@@ -267,10 +264,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     }
     std::swap(cur, nxt);
     if (ref) continue;
-    if (k + 1 >= min_it) {
-      hipLaunchKernelGGL(k_mark_final, dim3(1), dim3(64), 0, st, s->st, s->d_final + slot);
-      PSB_HIP(hipEventRecord(s->ev_iter[k], st));
-    }
+    if (k + 1 >= min_it) PSB_HIP(hipEventRecord(s->ev_iter[k], st));
     if (k >= min_it) {
       // iteration k is queued; wait for k-1 and stop queueing once it finished
       PSB_HIP(hipEventSynchronize(s->ev_iter[k - 1]));
@@ -301,7 +295,6 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       d->count += cnt;
     }
   }
-  s->step_counter += 1;
   return PSB_OK;
 }
 
@@ -466,7 +459,8 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   for (int k = 0; k < steps && s->n > 0; ++k) {
     if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k, k & 63));
     else PSB_TRY(wcsph_step(s, k));
-    if (s->timing && s->recs.size() > 4096) PSB_TRY(collect_timing(s));
+    s->step_counter += 1;
+    if (s->recs.size() > 4096) PSB_TRY(collect_timing(s));
   }
   // statistics
   uint32_t fl[2] = {0, 0}, gf = 0;
@@ -574,9 +568,9 @@ psb_status psb_system_get_arrays(psb_system s, psb_system_arrays* out) {
   return PSB_OK;
 }
 
-psb_status psb_system_set_timing(psb_system s, int32_t enable) {
-  PSB_ARG(s, "null system");
-  s->timing = enable != 0;
+psb_status psb_system_set_timing(psb_system s, int32_t every) {
+  PSB_ARG(s && every >= 0, "bad argument");
+  s->timing_every = every;
   return PSB_OK;
 }
 

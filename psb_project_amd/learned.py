@@ -15,7 +15,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import WARM_MLP, WARM_NONE, check, lib
+from ._lib import WARM_MLP, WARM_NONE, WARM_PREVIOUS, check, lib
 from .device import Context
 
 NUM_FEATURES = 8
@@ -72,6 +72,19 @@ class PressureMLP:
 
     def zero_grad(self) -> None:
         check(lib().psb_mlp_zero_grad(self.handle))
+
+    def save(self, path) -> None:
+        """Weights (FP32 master copy), feature normalisation and pressure scale."""
+        np.savez(path, w=self.params(), mean=self.mean, std=self.std,
+                 pressure_scale=np.float64(self.pressure_scale))
+
+    @classmethod
+    def load(cls, ctx: Context, path) -> "PressureMLP":
+        z = np.load(path, allow_pickle=False)
+        m = cls(ctx)
+        m.set_params(z["w"])
+        m.set_normalization(z["mean"], z["std"], float(z["pressure_scale"]))
+        return m
 
     def close(self) -> None:
         if getattr(self, "handle", None) and self.ctx.handle:
@@ -176,3 +189,129 @@ def train(ctx: Context, dataset: Dataset, pressure_scale: float, epochs: int = 3
             log(f"epoch {ep}: mse {losses[-1]:.5f}")
     idx_dev.free()
     return model, losses
+
+
+WARM_NAMES = {WARM_NONE: "plain", WARM_PREVIOUS: "previous_pressure", WARM_MLP: "learned"}
+
+
+def run_scene(ctx: Context, sc, steps: int, warm_start: int, model: PressureMLP | None = None):
+    """Advance one scene `steps` steps with the given warm start; per-step
+    corrector iterations and final density errors, and the wall time."""
+    import time
+    from .system import FluidSystem
+    p = sc.params.copy()
+    p.warm_start = warm_start
+    fs = FluidSystem(ctx, p, sc.n)
+    try:
+        if warm_start == WARM_MLP:
+            if model is None:
+                raise ValueError("the learned warm start needs a trained model")
+            fs.set_mlp(model)
+        fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
+        its = np.zeros(steps, np.int32)
+        err = np.zeros(steps, np.float32)
+        capped = 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            fs.update(1)
+            st = fs.stats()
+            its[k] = st["iterations"]
+            err[k] = st["max_error"]
+            capped += bool(st["iterations"] >= p.max_iterations and st["max_error"] > p.tolerance)
+        wall = time.perf_counter() - t0
+        return {"iterations": its, "max_error": err, "capped": capped, "seconds": wall,
+                "flags": fs.stats()["flags"]}
+    finally:
+        fs.close()
+
+
+def evaluate(ctx: Context, scenes_list, steps: int, model: PressureMLP | None,
+             modes=(WARM_NONE, WARM_PREVIOUS, WARM_MLP)) -> dict:
+    """Mean corrector iterations per step of each warm start over held-out
+    scenes, with the share of steps each one converged within tolerance."""
+    out = {}
+    for mode in modes:
+        if mode == WARM_MLP and model is None:
+            continue
+        runs = [run_scene(ctx, sc, steps, mode, model) for sc in scenes_list]
+        its = np.concatenate([r["iterations"] for r in runs])
+        err = np.concatenate([r["max_error"] for r in runs])
+        tol = scenes_list[0].params.tolerance
+        out[WARM_NAMES[mode]] = {
+            "iters_per_step": float(its.mean()),
+            "iters_p99": float(np.percentile(its, 99)),
+            "steps_within_tolerance": float((err <= tol).mean()),
+            "capped_steps": int(sum(r["capped"] for r in runs)),
+            "max_error": float(err.max()),
+        }
+    if "plain" in out:
+        base = out["plain"]["iters_per_step"]
+        for k, v in out.items():
+            v["iteration_reduction"] = 1.0 - v["iters_per_step"] / base
+    return out
+
+
+def train_and_evaluate(ctx: Context, make_scene, train_ids, eval_ids, steps: int = 200,
+                       stride: int = 16, epochs: int = 4, batch: int = 8192, lr: float = 1e-3,
+                       seed: int = 1, log=None):
+    """Record plain-PCISPH converged pressures on the training scenes, train
+    the network on the GPU, then compare warm starts on the held-out scenes.
+    Returns (model, summary)."""
+    train_scenes = [make_scene(i) for i in train_ids]
+    n_per = sum((sc.n + stride - 1) // stride for sc in train_scenes) * steps
+    ds = Dataset(ctx, n_per)
+    plain_iters = record(ctx, train_scenes, steps, ds, stride=stride)
+    from .system import FluidSystem
+    probe = FluidSystem(ctx, train_scenes[0].params, train_scenes[0].n)
+    pscale = probe.pressure_scale
+    probe.close()
+    model, losses = train(ctx, ds, pscale, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    samples = ds.count
+    ds.close()
+    summary = {"train_scenes": list(train_ids), "eval_scenes": list(eval_ids), "steps": steps,
+               "samples": int(samples), "epochs": epochs, "lr": lr, "batch": batch,
+               "train_plain_iters_per_step": float(np.mean(plain_iters)),
+               "mse_per_epoch": [float(x) for x in losses]}
+    summary["eval"] = evaluate(ctx, [make_scene(i) for i in eval_ids], steps, model)
+    return model, summary
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    from . import scenes
+    ap = argparse.ArgumentParser(description="train and evaluate the learned PCISPH warm start")
+    ap.add_argument("--train-seeds", type=int, default=16)
+    ap.add_argument("--eval-seeds", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--n-side", type=int, default=64)
+    ap.add_argument("--stride", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=None, help="write the trained model (.npz)")
+    ap.add_argument("--json", default=None, help="write the summary (.json)")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    ctx = Context(a.device)
+
+    def make_scene(i):
+        return scenes.pcisph_dam_break(n_side=a.n_side, scene_id=i)
+
+    model, summary = train_and_evaluate(
+        ctx, make_scene, range(a.train_seeds), range(a.train_seeds, a.train_seeds + a.eval_seeds),
+        steps=a.steps, stride=a.stride, epochs=a.epochs, batch=a.batch, lr=a.lr,
+        log=lambda m: print(m, flush=True))
+    if a.out:
+        model.save(a.out)
+    text = json.dumps(summary, indent=1)
+    print(text)
+    if a.json:
+        with open(a.json, "w") as f:
+            f.write(text)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

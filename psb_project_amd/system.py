@@ -121,8 +121,9 @@ class FluidSystem:
         return self.download(ptr.value, np.float32, (8, self.n))
 
     # -- timing ------------------------------------------------------------
-    def set_timing(self, on: bool) -> None:
-        check(lib().psb_system_set_timing(self.handle, 1 if on else 0))
+    def set_timing(self, on, every: int = 1) -> None:
+        """Per-kernel HIP events on every `every`-th step (off when `on` is false)."""
+        check(lib().psb_system_set_timing(self.handle, int(every) if on else 0))
 
     def reset_timing(self) -> None:
         check(lib().psb_system_reset_timing(self.handle))

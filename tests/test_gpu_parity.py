@@ -369,3 +369,18 @@ def test_out_of_grid_and_capacity_errors(ctx, oracle):
     assert e.value.status == 3
     with pytest.raises(PsbError):
         g.update(-1)
+
+
+@pytest.mark.parametrize("offset,nbytes", [(0, 16), (0, 1 << 20), (16, 4096 + 7), (3, 1000),
+                                           (0, (64 << 20) + 12)])
+def test_copy_device_to_device(ctx, offset, nbytes):
+    """copyArrayDeviceToDevice: 16-byte streaming kernel plus unaligned tail."""
+    rng = np.random.default_rng(nbytes)
+    total = offset + nbytes
+    src = rng.integers(0, 256, total, dtype=np.uint8)
+    ds = ctx.to_device(src)
+    dd = ctx.to_device(np.zeros(total, np.uint8))
+    check(lib().psb_copy_array_device_to_device(ctx.handle, dd.ptr, ds.ptr, offset, nbytes))
+    got = dd.download(np.uint8, total)
+    assert np.array_equal(got[offset:], src[offset:])
+    assert not got[:offset].any()
