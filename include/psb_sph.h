@@ -446,6 +446,63 @@ psb_status psb_dataset_info(psb_dataset d, const float** features, const float**
  * as the target.  Recording stops when the dataset is full.  NULL detaches. */
 psb_status psb_system_set_recorder(psb_system sys, psb_dataset d, uint32_t stride);
 
+/* ------------------------------------------------------------------------ */
+/* Batched scenes.  No counterpart in the reference, which simulates one     */
+/* System per process (one UniformGrid, 65,536 particles).  A batch advances */
+/* many independent PCISPH scenes (PSB_MODE_CONFIG) that share one material  */
+/* (h, rho0, m, dt, tolerance, iterations, warm start) and differ in         */
+/* geometry: particle count, grid and collider box.  Every phase of the step */
+/* is one launch over all scenes (workgroups aligned to scene boundaries);   */
+/* each scene keeps its own corrector: it stops at its own tolerance and its */
+/* blocks skip the remaining iterations of slower scenes.  Results equal     */
+/* the same scenes run one by one as psb_system.                             */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_batch_s* psb_batch;
+
+typedef struct psb_scene_desc {
+  uint32_t      count;        /* particles (fixed for the batch's life)      */
+  uint32_t      pad;
+  float         box_min[3];   /* collider box (BoxCollision, per scene)      */
+  float         box_max[3];
+  psb_grid_desc grid;         /* scene grid; cell_size = material grid's     */
+} psb_scene_desc;
+
+/* Scene-batch step phases, for psb_batch_autotune / psb_batch_get_timing.   */
+#define PSB_BATCH_PHASES 6  /* hash_count, neighbors_density, density_epilogue,
+                               pci_nonpressure, pci_density, pci_force       */
+
+psb_status psb_batch_create(psb_context ctx, const psb_sph_params* material, uint32_t num_scenes,
+                            const psb_scene_desc* scenes, psb_batch* out);
+psb_status psb_batch_destroy(psb_batch b);
+uint32_t   psb_batch_num_scenes(psb_batch b);
+/* Host FP32 float4 arrays of scene `scene` (count particles); vel4 / vh4 may
+ * be NULL (zero).                                                          */
+psb_status psb_batch_set_particles(psb_batch b, uint32_t scene, const float* pos4,
+                                   const float* vel4, const float* vh4);
+psb_status psb_batch_update(psb_batch b, int32_t steps);
+/* Per-scene statistics of the last update (flags are batch-wide).          */
+psb_status psb_batch_get_stats(psb_batch b, uint32_t scene, psb_step_stats* out);
+/* State of one scene in its sorted order (host arrays, any may be NULL).   */
+psb_status psb_batch_get_particles(psb_batch b, uint32_t scene, float* pos4, float* vel4,
+                                   float* vel_half4, float* density, float* pressure,
+                                   uint32_t* ids);
+/* Learned warm start (material warm_start = PSB_WARM_MLP).                 */
+psb_status psb_batch_set_mlp(psb_batch b, psb_mlp m);
+/* Time every phase at 64, 128, 256 and 512 threads per workgroup on
+ * `steps` steps from the current state each (state restored afterwards) and
+ * keep the fastest per phase; block_sizes[PSB_BATCH_PHASES] receives the
+ * choice (may be NULL).                                                     */
+psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes);
+/* Replay each step from HIP graphs: the step prologue (grid, neighbours,
+ * non-pressure forces, first min_iterations corrector iterations) and one
+ * graph per later iteration, captured on first use.                        */
+psb_status psb_batch_set_graphs(psb_batch b, int32_t on);
+/* Per-scene step latency (from the step's start to the end of the force
+ * pass that finished that scene), collected over steps while enabled.      */
+psb_status psb_batch_set_latency(psb_batch b, int32_t on);
+psb_status psb_batch_get_latency(psb_batch b, double* p50_ms, double* p99_ms, double* max_ms,
+                                 uint64_t* samples);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,6 +144,19 @@ class SystemArrays(C.Structure):
     ]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("box_min", C.c_float * 3),
+        ("box_max", C.c_float * 3),
+        ("grid", GridDesc),
+    ]
+
+
+BATCH_PHASES = ("hash_count", "neighbors_density", "density_epilogue", "pci_nonpressure",
+                "pci_density", "pci_force")
+
 vp = C.c_void_p
 u32 = C.c_uint32
 i32 = C.c_int32
@@ -219,6 +232,18 @@ SIGNATURES = {
     "psb_dataset_info": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_uint64),
                                C.POINTER(C.c_uint64)]),
     "psb_system_set_recorder": (i32, [vp, vp, u32]),
+    "psb_batch_create": (i32, [vp, C.POINTER(Params), u32, C.POINTER(SceneDesc), C.POINTER(vp)]),
+    "psb_batch_destroy": (i32, [vp]),
+    "psb_batch_num_scenes": (u32, [vp]),
+    "psb_batch_set_particles": (i32, [vp, u32, vp, vp, vp]),
+    "psb_batch_update": (i32, [vp, i32]),
+    "psb_batch_get_stats": (i32, [vp, u32, C.POINTER(StepStats)]),
+    "psb_batch_get_particles": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
+    "psb_batch_set_mlp": (i32, [vp, vp]),
+    "psb_batch_autotune": (i32, [vp, i32, C.POINTER(C.c_int32)]),
+    "psb_batch_set_graphs": (i32, [vp, i32]),
+    "psb_batch_set_latency": (i32, [vp, i32]),
+    "psb_batch_get_latency": (i32, [vp, dp, dp, dp, C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

@@ -1,0 +1,671 @@
+// batch.hip -- many independent PCISPH scenes advanced together.
+//
+// Layout: the scenes' particles are concatenated (scene-major) in every
+// state array, and their grids are concatenated in one global cell space
+// (scene s owns cells [cell_off_s, cell_off_s + nx ny nz)).  One counting
+// sort over the global cells therefore keeps every scene's particles in its
+// own range, sorted by its own cells.  Every per-particle phase is one launch
+// over all scenes with workgroups aligned to scene boundaries (BatchMap): a
+// workgroup serves exactly one scene, so the corrector's error reduction,
+// early exit and convergence decision stay per scene, exactly as in the
+// single-scene step, and results equal the scenes run one by one.
+//
+// Host control per step: the prologue (status reset, grid build, neighbour
+// search + density, epilogue, non-pressure forces and the first
+// min_iterations corrector iterations) is queued at once, from a HIP graph
+// when graphs are on.  Later iterations are queued one ahead: the force pass
+// of the last scene to converge stores the iteration into host-mapped
+// memory, the host stops queueing, and speculative launches of finished
+// scenes return at their first instruction.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mlp.h"
+#include "pcisph.h"
+#include "psb_internal.h"
+
+namespace psb {
+
+psb_status grid_create(psb_context ctx, const psb_grid_desc& d, uint32_t capacity, psb_grid* out);
+
+enum BatchPhase { BP_HASH, BP_NEIGHBORS, BP_EPILOGUE, BP_NONPRESSURE, BP_DENSITY, BP_FORCE,
+                  BP_COUNT };
+static_assert(BP_COUNT == PSB_BATCH_PHASES, "phase count");
+
+}  // namespace psb
+
+using namespace psb;
+
+struct psb_batch_s {
+  psb_context ctx = nullptr;
+  psb_sph_params prm{};
+  Consts c{};
+  GridDev gd{};                       // material grid (stencil width)
+  uint32_t S = 0, n = 0;
+  std::vector<BatchScene> hsc;
+  BatchScene* dsc = nullptr;
+  uint16_t* dblk[kBatchBlockSizes] = {};
+  uint32_t nblk[kBatchBlockSizes] = {};
+  int bsel[BP_COUNT] = {2, 2, 2, 2, 2, 2};   // 256 threads until tuned
+  psb_grid grid = nullptr;
+  float4 *ax = nullptr, *av = nullptr, *avh = nullptr, *bx = nullptr, *bv = nullptr, *bvh = nullptr;
+  uint32_t *aid = nullptr, *bid = nullptr;
+  float *ap = nullptr, *bp = nullptr, *rho = nullptr, *pres = nullptr, *rho_pred = nullptr;
+  float4 *anp = nullptr, *xp0 = nullptr, *xp1 = nullptr;
+  float* dl = nullptr;
+  float* feat = nullptr;
+  uint32_t* ncount = nullptr;
+  uint32_t* nlist = nullptr;
+  uint32_t* flags = nullptr;          // [0] flags, [1] max neighbours
+  PciStatus* st = nullptr;            // [S]
+  uint32_t* done = nullptr;
+  int32_t* h_final = nullptr;         // host-mapped: iteration at which every scene finished
+  int32_t* d_final = nullptr;
+  PciStatus* h_stat = nullptr;        // pinned [2][S] status copies (latency, stats)
+  psb_mlp mlp = nullptr;
+  std::vector<char> has_state;
+  // per-scene statistics of the last update
+  std::vector<psb_step_stats> stats;
+  uint32_t last_flags = 0, last_maxnb = 0;
+  // graphs
+  bool use_graphs = false;
+  hipGraphExec_t g_pro = nullptr;
+  std::vector<hipGraphExec_t> g_iter;
+  // events: step start and per-iteration ends, double-buffered by step parity
+  hipEvent_t ev_start[2] = {};
+  hipEvent_t ev_copy[2] = {};
+  std::vector<hipEvent_t> ev_iter[2];
+  // latency
+  bool lat_on = false;
+  int pending = -1;                   // parity of a step whose latencies are not yet read
+  std::vector<double> lat_ms;
+  // phase timing (autotune)
+  bool tuning = false;
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  struct TRec { int phase; size_t e; };
+  std::vector<TRec> trecs;
+  double t_phase[BP_COUNT] = {};
+  int64_t step_counter = 0;
+};
+
+namespace psb {
+
+static BatchMap bmap(psb_batch b, int phase) {
+  const int s = b->bsel[phase];
+  return BatchMap{b->dsc, b->dblk[s], s};
+}
+
+static Shape bshape(psb_batch b, int phase) {
+  const int s = b->bsel[phase];
+  return Shape{dim3(b->nblk[s]), dim3(64u << s)};
+}
+
+struct BTimed {
+  psb_batch b;
+  int phase;
+  size_t e = 0;
+  bool on;
+  BTimed(psb_batch bb, int ph) : b(bb), phase(ph), on(bb->tuning) {
+    if (!on) return;
+    if (b->tev_used + 2 > b->tev.size()) {
+      for (int i = 0; i < 256; ++i) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) { on = false; return; }
+        b->tev.push_back(ev);
+      }
+    }
+    e = b->tev_used;
+    b->tev_used += 2;
+    hipEventRecord(b->tev[e], b->ctx->stream);
+  }
+  ~BTimed() {
+    if (!on) return;
+    hipEventRecord(b->tev[e + 1], b->ctx->stream);
+    b->trecs.push_back({phase, e});
+  }
+};
+
+static psb_status collect_tuning(psb_batch b) {
+  if (b->trecs.empty()) return PSB_OK;
+  PSB_HIP(hipStreamSynchronize(b->ctx->stream));
+  for (const auto& r : b->trecs) {
+    float ms = 0.f;
+    PSB_HIP(hipEventElapsedTime(&ms, b->tev[r.e], b->tev[r.e + 1]));
+    b->t_phase[r.phase] += ms;
+  }
+  b->trecs.clear();
+  b->tev_used = 0;
+  return PSB_OK;
+}
+
+// predicted-position buffer read by iteration k's density pass
+static float4* pred_buf(psb_batch b, int k) {
+  const int warm = b->prm.warm_start == PSB_WARM_PREVIOUS ? 1 : 0;
+  return ((k + warm) & 1) ? b->xp1 : b->xp0;
+}
+
+static void buffers(psb_batch b, PciBuffers& pb, PciOut& o) {
+  pb = PciBuffers{};
+  pb.anp = b->anp; pb.xp0 = b->xp0; pb.dl = b->dl;
+  pb.feat = b->mlp ? b->feat : nullptr;
+  pb.ax = b->ax; pb.avh = b->avh; pb.aid = b->aid;
+  o = PciOut{};
+  o.bx = b->bx; o.bv = b->bv; o.bvh = b->bvh; o.bid = b->bid; o.bp = b->bp; o.flags = b->flags;
+  o.host_final = b->d_final;
+  o.done = b->done;
+  o.num_scenes = b->S;
+}
+
+static psb_status enqueue_iter(psb_batch b, int k) {
+  hipStream_t st = b->ctx->stream;
+  const bool mlp = b->prm.warm_start == PSB_WARM_MLP;
+  psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
+  PciBuffers pb; PciOut o;
+  buffers(b, pb, o);
+  float4* cur = pred_buf(b, k);
+  float4* nxt = pred_buf(b, k + 1);
+  {
+    BTimed t(b, BP_DENSITY);
+    const int dflags = (k == 0 && mlp) ? (kDensWriteFeature | kDensSkipUpdate) : 0;
+    PSB_TRY(launch_pci_density_batch(bmap(b, BP_DENSITY), bshape(b, BP_DENSITY), b->n, b->c,
+                                     nb.count, nb.list, cur, b->pres, b->rho_pred, b->dl, b->st,
+                                     k, b->feat, dflags, st));
+  }
+  if (k == 0 && mlp) PSB_TRY(mlp_infer(b->mlp, b->feat, b->n, b->pres, cur, 1, b->c.pscale, st));
+  {
+    BTimed t(b, BP_FORCE);
+    PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
+                                   nb.list, cur, nxt, b->rho_pred, b->pres, pb, b->st, k, 0, o,
+                                   st));
+  }
+  return PSB_OK;
+}
+
+static psb_status enqueue_prologue(psb_batch b) {
+  hipStream_t st = b->ctx->stream;
+  const bool warm = b->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = b->prm.warm_start == PSB_WARM_MLP;
+  PSB_TRY(launch_pci_reset(b->st, false, st, b->S, b->done));
+  {
+    BTimed t(b, BP_HASH);
+    ReorderArrays ra;
+    ra.sx = b->bx; ra.sv = b->bv; ra.svh = b->bvh; ra.sid = b->bid; ra.sp = b->bp;
+    ra.dx = b->ax; ra.dv = b->av; ra.dvh = b->avh; ra.did = b->aid; ra.dp = b->ap;
+    const int s = b->bsel[BP_HASH];
+    PSB_TRY(grid_build_batch(b->grid, bmap(b, BP_HASH), b->nblk[s], 64u << s, b->bx, b->n, &ra,
+                             st));
+  }
+  psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
+  {
+    BTimed t(b, BP_NEIGHBORS);
+    const int s = b->bsel[BP_NEIGHBORS];
+    PSB_TRY(launch_neighbors_batch(b->grid, bmap(b, BP_NEIGHBORS), b->nblk[s], 64u << s, b->ax,
+                                   b->n, b->c.h2, b->gd.R, nb, b->rho, b->c.mc, b->flags,
+                                   b->flags + 1, st));
+  }
+  {
+    BTimed t(b, BP_EPILOGUE);
+    const int s = b->bsel[BP_EPILOGUE];
+    PSB_TRY(launch_density_epilogue_batch(bmap(b, BP_EPILOGUE), b->nblk[s], 64u << s, b->n, b->c,
+                                          warm ? 1 : 0, mlp ? 1 : 0, b->rho, b->pres, b->ap,
+                                          b->ax, b->av, st));
+  }
+  PciBuffers pb; PciOut o;
+  buffers(b, pb, o);
+  {
+    BTimed t(b, BP_NONPRESSURE);
+    PSB_TRY(launch_pci_nonpressure_batch(bmap(b, BP_NONPRESSURE), bshape(b, BP_NONPRESSURE), b->n,
+                                         b->c, b->prm.surface_tension != 0.f, warm, nb.count,
+                                         nb.list, b->ax, b->av, b->rho, b->avh, b->pres, pb, o,
+                                         st));
+  }
+  if (warm) {
+    BTimed t(b, BP_FORCE);
+    PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
+                                   nb.list, b->xp0, b->xp1, b->rho_pred, b->pres, pb, b->st, -1, 1,
+                                   o, st));
+  }
+  for (int k = 0; k < b->c.min_it; ++k) PSB_TRY(enqueue_iter(b, k));
+  return PSB_OK;
+}
+
+static void drop_graphs(psb_batch b) {
+  if (b->g_pro) hipGraphExecDestroy(b->g_pro);
+  b->g_pro = nullptr;
+  for (auto g : b->g_iter) if (g) hipGraphExecDestroy(g);
+  b->g_iter.clear();
+}
+
+template <class F>
+static psb_status capture(psb_batch b, hipGraphExec_t* out, F&& enqueue) {
+  hipStream_t st = b->ctx->stream;
+  PSB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  psb_status r = enqueue();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (r != PSB_OK) { if (g) hipGraphDestroy(g); return r; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  return PSB_OK;
+}
+
+static psb_status run_prologue(psb_batch b) {
+  if (!b->use_graphs) return enqueue_prologue(b);
+  if (!b->g_pro) PSB_TRY(capture(b, &b->g_pro, [&] { return enqueue_prologue(b); }));
+  PSB_HIP(hipGraphLaunch(b->g_pro, b->ctx->stream));
+  return PSB_OK;
+}
+
+static psb_status run_iter(psb_batch b, int k) {
+  if (!b->use_graphs) return enqueue_iter(b, k);
+  if ((int)b->g_iter.size() <= k) b->g_iter.resize(k + 1, nullptr);
+  if (!b->g_iter[k]) PSB_TRY(capture(b, &b->g_iter[k], [&] { return enqueue_iter(b, k); }));
+  PSB_HIP(hipGraphLaunch(b->g_iter[k], b->ctx->stream));
+  return PSB_OK;
+}
+
+// latencies of the step whose events carry parity `par`
+static psb_status read_latency(psb_batch b, int par) {
+  PSB_HIP(hipEventSynchronize(b->ev_copy[par]));
+  const PciStatus* hs = b->h_stat + (size_t)par * b->S;
+  for (uint32_t s = 0; s < b->S; ++s) {
+    const int f = hs[s].final_iter;
+    if (f < 0 || f >= (int)b->ev_iter[par].size()) continue;
+    float ms = 0.f;
+    PSB_HIP(hipEventElapsedTime(&ms, b->ev_start[par], b->ev_iter[par][f]));
+    b->lat_ms.push_back(ms);
+  }
+  return PSB_OK;
+}
+
+static psb_status batch_step(psb_batch b, int64_t t) {
+  hipStream_t st = b->ctx->stream;
+  const int par = (int)(t & 1);
+  const int min_it = b->c.min_it, max_it = b->c.max_it;
+  if (b->lat_on) PSB_HIP(hipEventRecord(b->ev_start[par], st));
+  *b->h_final = -1;
+  PSB_TRY(run_prologue(b));
+  PSB_HIP(hipEventRecord(b->ev_iter[par][min_it - 1], st));
+  if (b->lat_on && b->pending >= 0) {
+    PSB_TRY(read_latency(b, b->pending));
+    b->pending = -1;
+  }
+  int final_known = -1;
+  for (int k = min_it; k < max_it; ++k) {
+    PSB_TRY(run_iter(b, k));
+    PSB_HIP(hipEventRecord(b->ev_iter[par][k], st));
+    // iteration k is queued; wait for k-1 and stop queueing once every scene finished
+    PSB_HIP(hipEventSynchronize(b->ev_iter[par][k - 1]));
+    const int32_t f = __atomic_load_n(b->h_final, __ATOMIC_ACQUIRE);
+    if (f >= 0) { final_known = f; break; }
+  }
+  // finished by the last queued iteration: let its host-mapped store land
+  // before the next step resets the flag
+  if (final_known < 0) PSB_HIP(hipStreamSynchronize(st));
+  if (b->lat_on) {
+    PSB_HIP(hipMemcpyAsync(b->h_stat + (size_t)par * b->S, b->st, sizeof(PciStatus) * b->S,
+                           hipMemcpyDeviceToHost, st));
+    PSB_HIP(hipEventRecord(b->ev_copy[par], st));
+    b->pending = par;
+  }
+  if (b->tuning) PSB_TRY(collect_tuning(b));
+  return PSB_OK;
+}
+
+static psb_status batch_alloc(psb_batch b) {
+  const size_t N = b->n;
+  auto A = [&](void** p, size_t bytes) -> psb_status { PSB_HIP(hipMalloc(p, bytes)); return PSB_OK; };
+  PSB_TRY(A((void**)&b->ax, 16 * N)); PSB_TRY(A((void**)&b->av, 16 * N)); PSB_TRY(A((void**)&b->avh, 16 * N));
+  PSB_TRY(A((void**)&b->bx, 16 * N)); PSB_TRY(A((void**)&b->bv, 16 * N)); PSB_TRY(A((void**)&b->bvh, 16 * N));
+  PSB_TRY(A((void**)&b->aid, 4 * N)); PSB_TRY(A((void**)&b->bid, 4 * N));
+  PSB_TRY(A((void**)&b->ap, 4 * N)); PSB_TRY(A((void**)&b->bp, 4 * N));
+  PSB_TRY(A((void**)&b->rho, 4 * N)); PSB_TRY(A((void**)&b->pres, 4 * N));
+  PSB_TRY(A((void**)&b->rho_pred, 4 * N));
+  PSB_TRY(A((void**)&b->anp, 16 * N)); PSB_TRY(A((void**)&b->dl, 4 * N));
+  PSB_TRY(A((void**)&b->xp0, 16 * N)); PSB_TRY(A((void**)&b->xp1, 16 * N));
+  PSB_TRY(A((void**)&b->ncount, 4 * N));
+  PSB_TRY(A((void**)&b->nlist, 4 * N * (size_t)b->prm.max_neighbors));
+  PSB_TRY(A((void**)&b->flags, 64));
+  PSB_TRY(A((void**)&b->st, sizeof(PciStatus) * b->S));
+  PSB_TRY(A((void**)&b->done, 64));
+  PSB_TRY(A((void**)&b->dsc, sizeof(BatchScene) * b->S));
+  PSB_HIP(hipHostMalloc((void**)&b->h_final, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  PSB_HIP(hipHostGetDevicePointer((void**)&b->d_final, b->h_final, 0));
+  *b->h_final = -1;
+  PSB_HIP(hipHostMalloc((void**)&b->h_stat, 2 * sizeof(PciStatus) * b->S, hipHostMallocDefault));
+  hipStream_t st = b->ctx->stream;
+  PSB_HIP(hipMemsetAsync(b->flags, 0, 64, st));
+  PSB_HIP(hipMemsetAsync(b->st, 0, sizeof(PciStatus) * b->S, st));
+  PSB_HIP(hipMemsetAsync(b->done, 0, 64, st));
+  PSB_HIP(hipMemcpyAsync(b->dsc, b->hsc.data(), sizeof(BatchScene) * b->S, hipMemcpyHostToDevice, st));
+  for (int i = 0; i < kBatchBlockSizes; ++i) {
+    const uint32_t bs = 64u << i;
+    std::vector<uint16_t> tab;
+    for (uint32_t s = 0; s < b->S; ++s) {
+      const uint32_t nbk = (b->hsc[s].n + bs - 1) / bs;
+      tab.insert(tab.end(), nbk, (uint16_t)s);
+    }
+    b->nblk[i] = (uint32_t)tab.size();
+    PSB_TRY(A((void**)&b->dblk[i], 2 * tab.size()));
+    PSB_HIP(hipMemcpyAsync(b->dblk[i], tab.data(), 2 * tab.size(), hipMemcpyHostToDevice, st));
+    PSB_HIP(hipStreamSynchronize(st));
+  }
+  for (int p = 0; p < 2; ++p) {
+    PSB_HIP(hipEventCreate(&b->ev_start[p]));
+    PSB_HIP(hipEventCreateWithFlags(&b->ev_copy[p], hipEventDisableTiming));
+    for (int k = 0; k < kMaxIterSlots; ++k) {
+      hipEvent_t e;
+      PSB_HIP(hipEventCreate(&e));
+      b->ev_iter[p].push_back(e);
+    }
+  }
+  PSB_HIP(hipStreamSynchronize(st));
+  return PSB_OK;
+}
+
+static psb_status fetch_stats(psb_batch b) {
+  hipStream_t st = b->ctx->stream;
+  uint32_t fl[2] = {0, 0}, gf = 0;
+  PSB_HIP(hipMemcpyAsync(fl, b->flags, 8, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipMemcpyAsync(&gf, b->grid->flags, 4, hipMemcpyDeviceToHost, st));
+  std::vector<PciStatus> hs(b->S);
+  PSB_HIP(hipMemcpyAsync(hs.data(), b->st, sizeof(PciStatus) * b->S, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  b->last_flags = fl[0] | gf;
+  b->last_maxnb = fl[1];
+  for (uint32_t s = 0; s < b->S; ++s) {
+    psb_step_stats& o = b->stats[s];
+    o.flags = b->last_flags;
+    o.iterations = hs[s].last_iters;
+    std::memcpy(&o.max_error, &hs[s].last_err, 4);
+    o.total_iterations = (int64_t)hs[s].total_iters;
+    std::memcpy(&o.worst_error, &hs[s].worst_err, 4);
+    o.max_neighbors_seen = (int32_t)b->last_maxnb;
+  }
+  return PSB_OK;
+}
+
+}  // namespace psb
+
+extern "C" {
+
+psb_status psb_batch_create(psb_context ctx, const psb_sph_params* material, uint32_t num_scenes,
+                            const psb_scene_desc* scenes, psb_batch* out) {
+  PSB_ARG(ctx && material && scenes && out, "null argument");
+  *out = nullptr;
+  PSB_ARG(num_scenes >= 1 && num_scenes <= 65535, "1 <= num_scenes <= 65535");
+  PSB_ARG(material->mode == PSB_MODE_CONFIG && material->solver == PSB_SOLVER_PCISPH,
+          "batches run PSB_MODE_CONFIG PCISPH");
+  psb_batch b = new psb_batch_s();
+  b->ctx = ctx;
+  b->prm = *material;
+  b->S = num_scenes;
+  psb_status stt = make_consts(*material, &b->c, &b->gd);
+  if (stt != PSB_OK) { delete b; return stt; }
+  const double h = material->support_radius;
+  const double s_lat = material->lattice_spacing > 0.f ? (double)material->lattice_spacing : 0.5 * h;
+  uint64_t cells = 0, parts = 0;
+  for (uint32_t i = 0; i < num_scenes && stt == PSB_OK; ++i) {
+    const psb_scene_desc& d = scenes[i];
+    if (d.count < 1) { set_error("every scene needs at least one particle"); stt = PSB_ERR_INVALID_ARGUMENT; break; }
+    if (d.grid.cell_size != material->grid.cell_size || d.grid.dims[0] <= 0 || d.grid.dims[1] <= 0 ||
+        d.grid.dims[2] <= 0) {
+      set_error("scene grids need positive dims and the material's cell size");
+      stt = PSB_ERR_INVALID_ARGUMENT;
+      break;
+    }
+    BatchScene sc{};
+    sc.grid = grid_dev(d.grid, (float)h);
+    if (sc.grid.R != b->gd.R) { set_error("scene stencil width differs"); stt = PSB_ERR_INVALID_ARGUMENT; break; }
+    for (int a = 0; a < 3; ++a) {
+      sc.box_lo[a] = (float)(d.box_min[a] + material->particle_radius);
+      sc.box_hi[a] = (float)(d.box_max[a] - material->particle_radius);
+      sc.wall_o[a] = d.box_min[a];
+      sc.wall_n[a] = (int)std::floor(((double)d.box_max[a] - d.box_min[a]) / s_lat + 1e-3);
+    }
+    sc.cell_off = (uint32_t)cells;
+    sc.p0 = (uint32_t)parts;
+    sc.n = d.count;
+    cells += (uint64_t)d.grid.dims[0] * d.grid.dims[1] * d.grid.dims[2];
+    parts += d.count;
+    if (cells >= (1ull << 31) || parts >= (1ull << 31)) {
+      set_error("batch exceeds 2^31 cells or particles");
+      stt = PSB_ERR_INVALID_ARGUMENT;
+      break;
+    }
+    b->hsc.push_back(sc);
+  }
+  if (stt != PSB_OK) { delete b; return stt; }
+  for (int i = 0; i < kBatchBlockSizes; ++i) {
+    const uint32_t bs = 64u << i;
+    uint32_t acc = 0;
+    for (auto& sc : b->hsc) { sc.blk0[i] = acc; acc += (sc.n + bs - 1) / bs; }
+  }
+  b->n = (uint32_t)parts;
+  b->has_state.assign(num_scenes, 0);
+  b->stats.assign(num_scenes, psb_step_stats{});
+  {
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
+  }
+  if (stt == PSB_OK) {
+    psb_grid_desc gdesc{};
+    gdesc.cell_size = 1.f;
+    gdesc.dims[0] = (int32_t)cells; gdesc.dims[1] = 1; gdesc.dims[2] = 1;
+    stt = grid_create(ctx, gdesc, b->n, &b->grid);
+  }
+  if (stt == PSB_OK) stt = batch_alloc(b);
+  if (stt != PSB_OK) { psb_batch_destroy(b); return stt; }
+  *out = b;
+  return PSB_OK;
+}
+
+psb_status psb_batch_destroy(psb_batch b) {
+  if (!b) return PSB_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  drop_graphs(b);
+  void* ptrs[] = {b->ax, b->av, b->avh, b->bx, b->bv, b->bvh, b->aid, b->bid, b->ap, b->bp,
+                  b->rho, b->pres, b->rho_pred, b->anp, b->dl, b->xp0, b->xp1, b->feat,
+                  b->ncount, b->nlist, b->flags, b->st, b->done, b->dsc};
+  for (void* p : ptrs) if (p) hipFree(p);
+  for (auto p : b->dblk) if (p) hipFree(p);
+  if (b->h_final) hipHostFree(b->h_final);
+  if (b->h_stat) hipHostFree(b->h_stat);
+  for (int p = 0; p < 2; ++p) {
+    if (b->ev_start[p]) hipEventDestroy(b->ev_start[p]);
+    if (b->ev_copy[p]) hipEventDestroy(b->ev_copy[p]);
+    for (auto e : b->ev_iter[p]) hipEventDestroy(e);
+  }
+  for (auto e : b->tev) hipEventDestroy(e);
+  if (b->grid) psb_finalize_grid(b->grid);
+  delete b;
+  return PSB_OK;
+}
+
+uint32_t psb_batch_num_scenes(psb_batch b) { return b ? b->S : 0; }
+
+psb_status psb_batch_set_particles(psb_batch b, uint32_t scene, const float* pos4,
+                                   const float* vel4, const float* vh4) {
+  PSB_ARG(b && pos4, "null argument");
+  PSB_ARG(scene < b->S, "scene index out of range");
+  PSB_HIP(hipSetDevice(b->ctx->device));
+  const BatchScene& sc = b->hsc[scene];
+  const uint32_t n = sc.n, p0 = sc.p0;
+  hipStream_t st = b->ctx->stream;
+  std::vector<float> zeros4(4 * (size_t)n, 0.f);
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+  const size_t b4 = 16 * (size_t)n;
+  const float* v = vel4 ? vel4 : zeros4.data();
+  const float* vh = vh4 ? vh4 : zeros4.data();
+  PSB_HIP(hipMemcpyAsync(b->bx + p0, pos4, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->ax + p0, pos4, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->bv + p0, v, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->av + p0, v, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->bvh + p0, vh, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->avh + p0, vh, b4, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->bid + p0, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(b->aid + p0, ids.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemsetAsync(b->bp + p0, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipMemsetAsync(b->ap + p0, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipMemsetAsync(b->pres + p0, 0, 4 * (size_t)n, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  b->has_state[scene] = 1;
+  b->stats[scene] = psb_step_stats{};
+  return PSB_OK;
+}
+
+psb_status psb_batch_set_mlp(psb_batch b, psb_mlp m) {
+  PSB_ARG(b, "null batch");
+  if (m && !b->feat) PSB_HIP(hipMalloc(&b->feat, 8 * 4 * (size_t)b->n));
+  b->mlp = m;
+  drop_graphs(b);   // graphs capture the network's arguments
+  return PSB_OK;
+}
+
+psb_status psb_batch_set_graphs(psb_batch b, int32_t on) {
+  PSB_ARG(b, "null batch");
+  drop_graphs(b);
+  b->use_graphs = on != 0;
+  return PSB_OK;
+}
+
+psb_status psb_batch_set_latency(psb_batch b, int32_t on) {
+  PSB_ARG(b, "null batch");
+  b->lat_on = on != 0;
+  b->pending = -1;
+  b->lat_ms.clear();
+  return PSB_OK;
+}
+
+psb_status psb_batch_get_latency(psb_batch b, double* p50_ms, double* p99_ms, double* max_ms,
+                                 uint64_t* samples) {
+  PSB_ARG(b, "null batch");
+  if (b->lat_on && b->pending >= 0) {
+    PSB_TRY(read_latency(b, b->pending));
+    b->pending = -1;
+  }
+  std::vector<double> v = b->lat_ms;
+  std::sort(v.begin(), v.end());
+  auto q = [&](double f) {
+    if (v.empty()) return 0.0;
+    const size_t i = std::min(v.size() - 1, (size_t)std::ceil(f * v.size()) - (f > 0 ? 1 : 0));
+    return v[i];
+  };
+  if (p50_ms) *p50_ms = q(0.5);
+  if (p99_ms) *p99_ms = q(0.99);
+  if (max_ms) *max_ms = v.empty() ? 0.0 : v.back();
+  if (samples) *samples = v.size();
+  return PSB_OK;
+}
+
+psb_status psb_batch_update(psb_batch b, int32_t steps) {
+  PSB_ARG(b && steps >= 0, "bad argument");
+  for (uint32_t s = 0; s < b->S; ++s)
+    if (!b->has_state[s]) { set_error("psb_batch_update before every scene's particles were set"); return PSB_ERR_STATE; }
+  if (b->prm.warm_start == PSB_WARM_MLP) {
+    if (!b->mlp) { set_error("PSB_WARM_MLP without psb_batch_set_mlp"); return PSB_ERR_STATE; }
+    PSB_ARG(b->c.min_it >= 2, "PSB_WARM_MLP needs min_iterations >= 2");
+  }
+  PSB_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  PSB_HIP(hipMemsetAsync(b->flags, 0, 8, st));
+  PSB_HIP(hipMemsetAsync(b->grid->flags, 0, 4, st));
+  PSB_TRY(launch_pci_reset(b->st, true, st, b->S, b->done));
+  for (int k = 0; k < steps; ++k) PSB_TRY(batch_step(b, b->step_counter++));
+  return fetch_stats(b);
+}
+
+psb_status psb_batch_get_stats(psb_batch b, uint32_t scene, psb_step_stats* out) {
+  PSB_ARG(b && out && scene < b->S, "bad argument");
+  *out = b->stats[scene];
+  return PSB_OK;
+}
+
+psb_status psb_batch_get_particles(psb_batch b, uint32_t scene, float* pos4, float* vel4,
+                                   float* vel_half4, float* density, float* pressure,
+                                   uint32_t* ids) {
+  PSB_ARG(b && scene < b->S, "bad argument");
+  PSB_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  const BatchScene& sc = b->hsc[scene];
+  const size_t n = sc.n, p0 = sc.p0;
+  if (pos4) PSB_HIP(hipMemcpyAsync(pos4, b->bx + p0, 16 * n, hipMemcpyDeviceToHost, st));
+  if (vel4) PSB_HIP(hipMemcpyAsync(vel4, b->bv + p0, 16 * n, hipMemcpyDeviceToHost, st));
+  if (vel_half4) PSB_HIP(hipMemcpyAsync(vel_half4, b->bvh + p0, 16 * n, hipMemcpyDeviceToHost, st));
+  if (density) PSB_HIP(hipMemcpyAsync(density, b->rho_pred + p0, 4 * n, hipMemcpyDeviceToHost, st));
+  if (pressure) PSB_HIP(hipMemcpyAsync(pressure, b->bp + p0, 4 * n, hipMemcpyDeviceToHost, st));
+  if (ids) PSB_HIP(hipMemcpyAsync(ids, b->bid + p0, 4 * n, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  return PSB_OK;
+}
+
+psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) {
+  PSB_ARG(b && steps >= 1, "bad argument");
+  for (uint32_t s = 0; s < b->S; ++s)
+    if (!b->has_state[s]) { set_error("psb_batch_autotune before every scene's particles were set"); return PSB_ERR_STATE; }
+  PSB_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  // snapshot of the end-of-step state every step starts from
+  const size_t N = b->n;
+  void* save = nullptr;
+  const size_t bytes = 48 * N + 8 * N;
+  PSB_HIP(hipMalloc(&save, bytes));
+  char* sv = (char*)save;
+  auto snap = [&](bool restore) -> psb_status {
+    struct { void* p; size_t sz; } arr[] = {{b->bx, 16 * N}, {b->bv, 16 * N}, {b->bvh, 16 * N},
+                                            {b->bid, 4 * N}, {b->bp, 4 * N}};
+    size_t off = 0;
+    for (auto& a : arr) {
+      if (restore) PSB_HIP(hipMemcpyAsync(a.p, sv + off, a.sz, hipMemcpyDeviceToDevice, st));
+      else PSB_HIP(hipMemcpyAsync(sv + off, a.p, a.sz, hipMemcpyDeviceToDevice, st));
+      off += a.sz;
+    }
+    return PSB_OK;
+  };
+  psb_status r = snap(false);
+  const bool graphs = b->use_graphs;
+  const bool lat = b->lat_on;
+  b->use_graphs = false;
+  b->lat_on = false;
+  drop_graphs(b);
+  double best[BP_COUNT];
+  int pick[BP_COUNT];
+  for (int p = 0; p < BP_COUNT; ++p) { best[p] = 1e300; pick[p] = 2; }
+  for (int cand = 0; cand < kBatchBlockSizes && r == PSB_OK; ++cand) {
+    for (int p = 0; p < BP_COUNT; ++p) b->bsel[p] = cand;
+    r = snap(true);
+    if (r != PSB_OK) break;
+    std::memset(b->t_phase, 0, sizeof(b->t_phase));
+    b->tuning = true;
+    for (int k = 0; k < steps && r == PSB_OK; ++k) r = batch_step(b, k);
+    b->tuning = false;
+    if (r == PSB_OK) r = collect_tuning(b);
+    for (int p = 0; p < BP_COUNT; ++p)
+      if (b->t_phase[p] < best[p]) { best[p] = b->t_phase[p]; pick[p] = cand; }
+  }
+  b->tuning = false;
+  b->trecs.clear();
+  b->tev_used = 0;
+  for (int p = 0; p < BP_COUNT; ++p) b->bsel[p] = pick[p];
+  if (r == PSB_OK) r = snap(true);
+  if (r == PSB_OK) {
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) r = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+  }
+  hipFree(save);
+  if (r == PSB_OK) r = launch_pci_reset(b->st, true, st, b->S, b->done);
+  b->use_graphs = graphs;
+  b->lat_on = lat;
+  if (block_sizes)
+    for (int p = 0; p < BP_COUNT; ++p) block_sizes[p] = 64 << b->bsel[p];
+  return r;
+}
+
+}  // extern "C"

@@ -24,19 +24,25 @@
 
 namespace psb {
 
-__global__ __launch_bounds__(256) void k_hash_count(const float4* __restrict__ pos, uint32_t n,
-                                                    GridDev g, uint32_t* __restrict__ hash,
-                                                    uint32_t* __restrict__ slot,
-                                                    uint32_t* __restrict__ count,
-                                                    uint32_t* __restrict__ flags) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// M: SingleMap or BatchMap (a batch hashes into its scene's cell range).
+template <class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const float4* __restrict__ pos,
+                                                               uint32_t n, GridDev g0,
+                                                               uint32_t* __restrict__ hash,
+                                                               uint32_t* __restrict__ slot,
+                                                               uint32_t* __restrict__ count,
+                                                               uint32_t* __restrict__ flags) {
+  uint32_t i, sc;
+  bool lead;
+  if (!map.locate(n, i, sc, lead)) return;
+  uint32_t off;
+  const GridDev g = map.grid(g0, sc, off);
   float4 p = pos[i];
   uint32_t bad = 0;
   int cx = cell_coord(p.x, g.minx, g.cell, g.nx, bad);
   int cy = cell_coord(p.y, g.miny, g.cell, g.ny, bad);
   int cz = cell_coord(p.z, g.minz, g.cell, g.nz, bad);
-  uint32_t c = (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
+  uint32_t c = off + (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
   hash[i] = c;
   slot[i] = atomicAdd(&count[c], 1u);
   uint32_t f = 0;
@@ -183,17 +189,24 @@ __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict
 // row ranges are loaded first (independent table loads) and the candidates of
 // a row are gathered U at a time before being tested in order.  RT = 0: any
 // stencil width, rows visited one after the other.
-template <bool DENS, int RT>
-__global__ __launch_bounds__(256) void k_neighbors(const float4* xs, uint32_t n, GridDev g,
-                                                   const uint32_t* __restrict__ start, float r2,
-                                                   uint32_t cap, uint32_t* __restrict__ ncount,
-                                                   uint32_t* __restrict__ nlist,
-                                                   float* __restrict__ rho, float mc,
-                                                   uint32_t* __restrict__ flags,
-                                                   uint32_t* __restrict__ maxcnt) {
-  const uint32_t p = particle_index();
+template <bool DENS, int RT, class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float4* xs, uint32_t n,
+                                                              GridDev g0,
+                                                              const uint32_t* __restrict__ start0,
+                                                              float r2, uint32_t cap,
+                                                              uint32_t* __restrict__ ncount,
+                                                              uint32_t* __restrict__ nlist,
+                                                              float* __restrict__ rho, float mc,
+                                                              uint32_t* __restrict__ flags,
+                                                              uint32_t* __restrict__ maxcnt) {
+  uint32_t p, sc;
+  bool lead;
+  const bool valid = map.locate(n, p, sc, lead);
   uint32_t cnt = 0;
-  if (p < n) {
+  if (valid) {
+    uint32_t off;
+    const GridDev g = map.grid(g0, sc, off);
+    const uint32_t* __restrict__ start = start0 + off;
     const float4 xi = xs[p];
     uint32_t bad = 0;
     const int cx = cell_coord(xi.x, g.minx, g.cell, g.nx, bad);
@@ -268,14 +281,9 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ src, f
   else dst[k] = src[perm[k]];
 }
 
-psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
-                      hipStream_t s) {
-  GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+// counting sort of the hashed particles + cell ranges + reorder
+static psb_status grid_sort(psb_grid g, uint32_t n, const ReorderArrays* ra, hipStream_t s) {
   const uint32_t nb = blocks_for(n);
-  if (n) {
-    hipLaunchKernelGGL(k_hash_count, dim3(nb), dim3(kBlock), 0, s, pos, n, gd, g->hash, g->slot,
-                       g->count, g->flags);
-  }
   hipLaunchKernelGGL(k_scan_reduce, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum);
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, g->bsum, g->nscan, g->start, g->ncells);
   hipLaunchKernelGGL(k_scan_apply, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum,
@@ -296,6 +304,47 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
   return PSB_OK;
 }
 
+psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
+                      hipStream_t s) {
+  GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+  if (n) {
+    hipLaunchKernelGGL(k_hash_count<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags);
+  }
+  return grid_sort(g, n, ra, s);
+}
+
+psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
+                            const float4* pos, uint32_t n, const ReorderArrays* ra,
+                            hipStream_t s) {
+  if (n) {
+    GridDev none{};
+    hipLaunchKernelGGL(k_hash_count<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, pos, n,
+                       none, g->hash, g->slot, g->count, g->flags);
+  }
+  return grid_sort(g, n, ra, s);
+}
+
+psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
+                                  uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
+                                  psb_neighbors nb, float* rho, float mc, uint32_t* flags,
+                                  uint32_t* maxcnt, hipStream_t s) {
+  if (!n) return PSB_OK;
+  GridDev none{};
+  dim3 gb(nblocks), tb(bsize);
+#define PSB_NB(D_, R_)                                                                       \
+  hipLaunchKernelGGL((k_neighbors<D_, R_, BatchMap>), gb, tb, 0, s, map, xs, n, none, g->start, \
+                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt)
+  if (rho) {
+    if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
+  } else {
+    if (R == 1) PSB_NB(false, 1); else if (R == 2) PSB_NB(false, 2); else PSB_NB(false, 0);
+  }
+#undef PSB_NB
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                             uint32_t* maxcnt, hipStream_t s) {
@@ -303,9 +352,9 @@ psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, 
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
   dim3 gb(blocks_for(n)), tb(kBlock);
-#define PSB_NB(D_, R_)                                                                      \
-  hipLaunchKernelGGL((k_neighbors<D_, R_>), gb, tb, 0, s, xs, n, gd, g->start, r2, nb.capacity, \
-                     nb.count, nb.list, rho, mc, flags, maxcnt)
+#define PSB_NB(D_, R_)                                                                       \
+  hipLaunchKernelGGL((k_neighbors<D_, R_, SingleMap>), gb, tb, 0, s, SingleMap{}, xs, n, gd,    \
+                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {

@@ -37,9 +37,19 @@ struct PciOut {
   uint32_t* flags;
   // optional host-mapped slot that receives the final iteration index
   int32_t* host_final;
+  // batches: scenes finished this step; host_final is written by the last
+  uint32_t* done;
+  uint32_t num_scenes;
 };
 
-psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s);
+// Kernel launch shape (grid, workgroup).
+struct Shape {
+  dim3 grid, block;
+};
+
+// one status per scene (scenes = 1 for a single system); done: batch counter
+psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes = 1,
+                            uint32_t* done = nullptr);
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,
@@ -59,6 +69,27 @@ psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
                             PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s);
+
+// batches (scene-aligned blocks; see BatchMap)
+psb_status launch_pci_nonpressure_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                        const Consts& c, bool surf, bool warm,
+                                        const uint32_t* cnt, const uint32_t* list,
+                                        const float4* xs, const float4* vs, const float* rho,
+                                        const float4* avh, const float* pres,
+                                        const PciBuffers& b, const PciOut& o, hipStream_t s);
+psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                    const Consts& c, const uint32_t* cnt, const uint32_t* list,
+                                    float4* X, float* pres, float* rho_pred, const float* dl,
+                                    PciStatus* st, int k, float* feat, int dflags, hipStream_t s);
+psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                  const Consts& c, const uint32_t* cnt, const uint32_t* list,
+                                  const float4* X, float4* Xn, const float* rho_pred,
+                                  const float* pres, const PciBuffers& b, PciStatus* st, int k,
+                                  int warm_pass, const PciOut& o, hipStream_t s);
+psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, uint32_t bsize,
+                                         uint32_t n, const Consts& c, int warm, int xw_prev,
+                                         float* rho, float* pres, const float* pprev, float4* xs,
+                                         float4* vs, hipStream_t s);
 
 // sph.hip
 psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, const float4* pos,

@@ -28,12 +28,16 @@
 
 namespace psb {
 
-__global__ void k_pci_reset(PciStatus* st) {
+// one block per scene; block 0 also clears the batch's finished-scene count
+__global__ void k_pci_reset(PciStatus* st0, uint32_t* done) {
+  PciStatus* st = st0 + blockIdx.x;
   for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
   if (threadIdx.x == 0) st->final_iter = -1;
+  if (done && blockIdx.x == 0 && threadIdx.x == 0) *done = 0u;
 }
 
-__global__ void k_pci_stats_reset(PciStatus* st) {
+__global__ void k_pci_stats_reset(PciStatus* st0) {
+  PciStatus* st = st0 + blockIdx.x;
   if (threadIdx.x == 0) {
     st->total_iters = 0;
     st->worst_err = 0u;
@@ -132,18 +136,16 @@ __device__ __forceinline__ float delta_and_features(const Consts& c, uint32_t p,
 
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:117-170)
 // + first prediction (integrateSPH_LeapFrog_Forces with f_P = 0).
-template <bool REF, bool SURF, bool WARM, bool OUT, bool FEAT>
-__global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
-                                                         const uint32_t* __restrict__ cnt,
-                                                         const uint32_t* __restrict__ list,
-                                                         const float4* __restrict__ xs,
-                                                         const float4* __restrict__ vs,
-                                                         const float* __restrict__ rho,
-                                                         const float4* __restrict__ avh,
-                                                         const float* __restrict__ pres,
-                                                         PciBuffers b, PciOut o) {
-  const uint32_t p = particle_index();
-  if (p >= n) return;
+template <bool REF, bool SURF, bool WARM, bool OUT, bool FEAT, class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
+    M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ list, const float4* __restrict__ xs,
+    const float4* __restrict__ vs, const float* __restrict__ rho, const float4* __restrict__ avh,
+    const float* __restrict__ pres, PciBuffers b, PciOut o) {
+  uint32_t p, sc;
+  bool lead;
+  if (!map.locate(n, p, sc, lead)) return;
+  map.patch(c, sc);
   PairSums s = internal_sums<false, SURF>(c, p, n, cnt, list, xs, vs);
   const float ri = rho[p];
   const float sigma = REF ? 0.0728f : c.sigma;
@@ -195,23 +197,24 @@ __global__ __launch_bounds__(256) void k_pci_nonpressure(uint32_t n, Consts c,
 // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
 // in reference mode; template-delta update p = max(0, p + delta (rho - rho0))
 // in config mode.  X[p].w <- p/rho^2 (reference) or p (config).
-template <bool REF>
-__global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
-                                                     const uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ list, float4* X,
-                                                     float* __restrict__ pres,
-                                                     float* __restrict__ rho_pred,
-                                                     const float* __restrict__ dl,
-                                                     PciStatus* __restrict__ st, int k,
-                                                     float* __restrict__ feat, int dflags) {
-  __shared__ float red[4];
+template <bool REF, class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
+    M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ list, float4* X, float* __restrict__ pres,
+    float* __restrict__ rho_pred, const float* __restrict__ dl, PciStatus* __restrict__ st0, int k,
+    float* __restrict__ feat, int dflags) {
+  __shared__ float red[M::kMaxThreads / 64];
+  PciStatus* __restrict__ st = st0 + map.scene_of_block();   // a block serves one scene
   {
     const int fi = st->final_iter;
     if (fi >= 0 && fi < k) return;
   }
-  const uint32_t p = particle_index();
+  uint32_t p, sc;
+  bool lead;
+  const bool valid = map.locate(n, p, sc, lead);
+  map.patch(c, sc);
   float err = 0.f;
-  if (p < n) {
+  if (valid) {
     const float4 xi = X[p];
     float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
     for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
@@ -257,7 +260,8 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
   if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = err;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float m = red[0];
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
     if (m > 0.f) atomicMax(&st->err[k], __float_as_uint(m));
   }
 }
@@ -266,16 +270,17 @@ __global__ __launch_bounds__(256) void k_pci_density(uint32_t n, Consts c,
 // integrateSPH_LeapFrog_Forces from the start state (SphKernel.cu:95-109,
 // restore_Pos_Vel_PCI :190-202).  warm_pass: the pass before the first
 // density evaluation that turns a warm-start pressure into a force.
-template <bool REF, bool STEP>
-__global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
-                                                   const uint32_t* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ list,
-                                                   const float4* __restrict__ X,
-                                                   float4* __restrict__ Xn,
-                                                   const float* __restrict__ rho_pred,
-                                                   const float* __restrict__ pres, PciBuffers b,
-                                                   PciStatus* __restrict__ st, int k,
-                                                   int warm_pass, PciOut o) {
+template <bool REF, bool STEP, class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
+    M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ list, const float4* __restrict__ X, float4* __restrict__ Xn,
+    const float* __restrict__ rho_pred, const float* __restrict__ pres, PciBuffers b,
+    PciStatus* __restrict__ st0, int k, int warm_pass, PciOut o) {
+  uint32_t p, sc;
+  bool lead;
+  const bool valid = map.locate(n, p, sc, lead);
+  map.patch(c, sc);
+  PciStatus* __restrict__ st = st0 + map.scene_of_block();
   bool fin = false;
   float err = 0.f;
   if (!warm_pass) {
@@ -284,20 +289,22 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
     err = __uint_as_float(st->err[k]);
     const int it = k + 1;
     fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it;
-    if (fin && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (fin && lead) {
       st->final_iter = k;
       st->last_iters = it;
       st->last_err = __float_as_uint(err);
       atomicAdd(&st->total_iters, (unsigned long long)it);
       atomicMax(&st->worst_err, __float_as_uint(err));
       if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
-      // tell the host, which stops queueing speculative iterations
-      if (o.host_final)
+      // tell the host, which stops queueing speculative iterations once
+      // every scene has finished
+      bool all = true;
+      if (o.done) all = atomicAdd(o.done, 1u) + 1u == o.num_scenes;
+      if (all && o.host_final)
         __hip_atomic_store(o.host_final, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
-  const uint32_t p = particle_index();
-  if (p >= n) return;
+  if (!valid) return;
   const float4 xi = X[p];
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
@@ -372,42 +379,43 @@ __global__ __launch_bounds__(256) void k_pci_force(uint32_t n, Consts c,
 }
 
 // ---- launchers ----------------------------------------------------------------
-psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s) {
-  if (stats) hipLaunchKernelGGL(k_pci_stats_reset, dim3(1), dim3(64), 0, s, st);
-  hipLaunchKernelGGL(k_pci_reset, dim3(1), dim3(256), 0, s, st);
+psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes,
+                            uint32_t* done) {
+  if (stats) hipLaunchKernelGGL(k_pci_stats_reset, dim3(scenes), dim3(64), 0, s, st);
+  hipLaunchKernelGGL(k_pci_reset, dim3(scenes), dim3(256), 0, s, st, done);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
 
-template <bool R_, bool S_, bool W_, bool O_, bool F_>
-static void np_launch(dim3 g, dim3 t, hipStream_t s, uint32_t n, const Consts& c,
+template <bool R_, bool S_, bool W_, bool O_, bool F_, class M>
+static void np_launch(const M& map, const Shape& sh, hipStream_t s, uint32_t n, const Consts& c,
                       const uint32_t* cnt, const uint32_t* list, const float4* xs,
                       const float4* vs, const float* rho, const float4* avh, const float* pres,
                       const PciBuffers& b, const PciOut& o) {
-  hipLaunchKernelGGL((k_pci_nonpressure<R_, S_, W_, O_, F_>), g, t, 0, s, n, c, cnt, list, xs, vs,
-                     rho, avh, pres, b, o);
+  hipLaunchKernelGGL((k_pci_nonpressure<R_, S_, W_, O_, F_, M>), sh.grid, sh.block, 0, s, map, n, c,
+                     cnt, list, xs, vs, rho, avh, pres, b, o);
 }
 
-template <bool R_, bool S_, bool W_, bool O_>
-static void np_launch_f(bool feat, dim3 g, dim3 t, hipStream_t s, uint32_t n, const Consts& c,
-                        const uint32_t* cnt, const uint32_t* list, const float4* xs,
-                        const float4* vs, const float* rho, const float4* avh, const float* pres,
-                        const PciBuffers& b, const PciOut& o) {
-  if (feat) np_launch<R_, S_, W_, O_, true>(g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
-  else np_launch<R_, S_, W_, O_, false>(g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
+template <bool R_, bool S_, bool W_, bool O_, class M>
+static void np_launch_f(bool feat, const M& map, const Shape& sh, hipStream_t s, uint32_t n,
+                        const Consts& c, const uint32_t* cnt, const uint32_t* list,
+                        const float4* xs, const float4* vs, const float* rho, const float4* avh,
+                        const float* pres, const PciBuffers& b, const PciOut& o) {
+  if (feat) np_launch<R_, S_, W_, O_, true>(map, sh, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
+  else np_launch<R_, S_, W_, O_, false>(map, sh, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o);
 }
 
-psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
-                                  const uint32_t* cnt, const uint32_t* list, const float4* xs,
-                                  const float4* vs, const float* rho, const float4* avh,
-                                  const float* pres, const PciBuffers& b, const PciOut& o,
-                                  hipStream_t s) {
+template <class M>
+static psb_status nonpressure_impl(const M& map, const Shape& sh, uint32_t n, const Consts& c,
+                                   bool surf, bool warm, bool out, const uint32_t* cnt,
+                                   const uint32_t* list, const float4* xs, const float4* vs,
+                                   const float* rho, const float4* avh, const float* pres,
+                                   const PciBuffers& b, const PciOut& o, hipStream_t s) {
   if (!n) return PSB_OK;
   const bool ref = c.mode == PSB_MODE_REFERENCE;
   const bool feat = !ref && b.feat != nullptr;
-  dim3 g(blocks_for(n)), t(kBlock);
 #define PSB_NP(R_, S_, W_, O_) \
-  np_launch_f<R_, S_, W_, O_>(feat, g, t, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o)
+  np_launch_f<R_, S_, W_, O_>(feat, map, sh, s, n, c, cnt, list, xs, vs, rho, avh, pres, b, o)
   if (ref) {
     if (out) PSB_NP(true, true, false, true); else PSB_NP(true, true, false, false);
   } else if (surf) {
@@ -422,19 +430,92 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
   return PSB_OK;
 }
 
+template <class M>
+static psb_status density_impl(const M& map, const Shape& sh, uint32_t n, const Consts& c,
+                               const uint32_t* cnt, const uint32_t* list, float4* X, float* pres,
+                               float* rho_pred, const float* dl, PciStatus* st, int k,
+                               float* feat, int dflags, hipStream_t s) {
+  if (!n) return PSB_OK;
+  if (c.mode == PSB_MODE_REFERENCE)
+    hipLaunchKernelGGL((k_pci_density<true, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
+                       pres, rho_pred, dl, st, k, feat, 0);
+  else
+    hipLaunchKernelGGL((k_pci_density<false, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
+                       pres, rho_pred, dl, st, k, feat, dflags);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+template <class M>
+static psb_status force_impl(const M& map, const Shape& sh, uint32_t n, const Consts& c, bool step,
+                             const uint32_t* cnt, const uint32_t* list, const float4* X,
+                             float4* Xn, const float* rho_pred, const float* pres,
+                             const PciBuffers& b, PciStatus* st, int k, int warm_pass,
+                             const PciOut& o, hipStream_t s) {
+  if (!n) return PSB_OK;
+  const bool ref = c.mode == PSB_MODE_REFERENCE;
+#define PSB_F(R_, S_)                                                                           \
+  hipLaunchKernelGGL((k_pci_force<R_, S_, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X, \
+                     Xn, rho_pred, pres, b, st, k, warm_pass, o)
+  if (ref) { if (step) PSB_F(true, true); else PSB_F(true, false); }
+  else { if (step) PSB_F(false, true); else PSB_F(false, false); }
+#undef PSB_F
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+static Shape single_shape(uint32_t n) { return Shape{dim3(blocks_for(n)), dim3(kBlock)}; }
+
+psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
+                                  const uint32_t* cnt, const uint32_t* list, const float4* xs,
+                                  const float4* vs, const float* rho, const float4* avh,
+                                  const float* pres, const PciBuffers& b, const PciOut& o,
+                                  hipStream_t s) {
+  return nonpressure_impl(SingleMap{}, single_shape(n), n, c, surf, warm, out, cnt, list, xs, vs,
+                          rho, avh, pres, b, o, s);
+}
+
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
                               hipStream_t s) {
-  if (!n) return PSB_OK;
-  if (c.mode == PSB_MODE_REFERENCE)
-    hipLaunchKernelGGL(k_pci_density<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt, list,
-                       X, pres, rho_pred, dl, st, k, feat, 0);
-  else
-    hipLaunchKernelGGL(k_pci_density<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt,
-                       list, X, pres, rho_pred, dl, st, k, feat, dflags);
-  PSB_HIP(hipGetLastError());
-  return PSB_OK;
+  return density_impl(SingleMap{}, single_shape(n), n, c, cnt, list, X, pres, rho_pred, dl, st, k,
+                      feat, dflags, s);
+}
+
+psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
+                            const uint32_t* list, const float4* X, float4* Xn,
+                            const float* rho_pred, const float* pres, const PciBuffers& b,
+                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s) {
+  return force_impl(SingleMap{}, single_shape(n), n, c, step, cnt, list, X, Xn, rho_pred, pres, b,
+                    st, k, warm_pass, o, s);
+}
+
+psb_status launch_pci_nonpressure_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                        const Consts& c, bool surf, bool warm,
+                                        const uint32_t* cnt, const uint32_t* list,
+                                        const float4* xs, const float4* vs, const float* rho,
+                                        const float4* avh, const float* pres,
+                                        const PciBuffers& b, const PciOut& o, hipStream_t s) {
+  return nonpressure_impl(map, sh, n, c, surf, warm, false, cnt, list, xs, vs, rho, avh, pres, b,
+                          o, s);
+}
+
+psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                    const Consts& c, const uint32_t* cnt, const uint32_t* list,
+                                    float4* X, float* pres, float* rho_pred, const float* dl,
+                                    PciStatus* st, int k, float* feat, int dflags,
+                                    hipStream_t s) {
+  return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags, s);
+}
+
+psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,
+                                  const Consts& c, const uint32_t* cnt, const uint32_t* list,
+                                  const float4* X, float4* Xn, const float* rho_pred,
+                                  const float* pres, const PciBuffers& b, PciStatus* st, int k,
+                                  int warm_pass, const PciOut& o, hipStream_t s) {
+  return force_impl(map, sh, n, c, true, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o,
+                    s);
 }
 
 // Append every `stride`-th particle (p = offset, offset + stride, ...) of the
@@ -461,24 +542,6 @@ psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint3
   if (!count) return PSB_OK;
   hipLaunchKernelGGL(k_record, dim3(blocks_for(count)), dim3(kBlock), 0, s, feat, pres, n, stride,
                      offset, inv_ps, dfeat, dtarget, dcap, base, count);
-  PSB_HIP(hipGetLastError());
-  return PSB_OK;
-}
-
-psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
-                            const uint32_t* list, const float4* X, float4* Xn,
-                            const float* rho_pred, const float* pres, const PciBuffers& b,
-                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s) {
-  if (!n) return PSB_OK;
-  const bool ref = c.mode == PSB_MODE_REFERENCE;
-  dim3 g(blocks_for(n)), t(kBlock);
-  if (ref) {
-    if (step) hipLaunchKernelGGL((k_pci_force<true, true>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
-    else hipLaunchKernelGGL((k_pci_force<true, false>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
-  } else {
-    if (step) hipLaunchKernelGGL((k_pci_force<false, true>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
-    else hipLaunchKernelGGL((k_pci_force<false, false>), g, t, 0, s, n, c, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o);
-  }
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

@@ -97,6 +97,80 @@ __device__ __forceinline__ uint32_t particle_index() {
   return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
+// ---------------------------------------------------------------------------
+// Particle maps: which particle (and scene) a thread works on.
+//
+// SingleMap: one scene, particle = block * blockDim + thread (XCD remapped).
+// BatchMap: many independent scenes in one launch.  Each scene's particles
+// are contiguous (p0 .. p0 + n) and its blocks are contiguous too, padded to
+// a whole number of workgroups, so every workgroup belongs to exactly one
+// scene and block-level logic (error reduction, early exit, the convergence
+// decision) stays per scene.  The scene supplies its grid, cell offset,
+// collider box and lattice-wall origin; the material constants are shared.
+// ---------------------------------------------------------------------------
+constexpr int kBatchBlockSizes = 4;           // 64, 128, 256, 512 threads
+
+struct BatchScene {
+  GridDev grid;                  // scene grid (origin, cell, dims, stencil)
+  float box_lo[3], box_hi[3];    // collider box (radius applied)
+  float wall_o[3];               // lattice-wall origin (box_min)
+  int wall_n[3];                 // lattice sites per axis inside the box
+  uint32_t cell_off;             // first global cell of the scene
+  uint32_t p0, n;                // particle range
+  uint32_t blk0[kBatchBlockSizes];  // first block, per block size
+  uint32_t pad;
+};
+
+struct SingleMap {
+  static constexpr int kMaxThreads = 256;
+  __device__ __forceinline__ bool locate(uint32_t n, uint32_t& p, uint32_t& s, bool& lead) const {
+    p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    s = 0;
+    lead = blockIdx.x == 0 && threadIdx.x == 0;
+    return p < n;
+  }
+  __device__ __forceinline__ void patch(Consts&, uint32_t) const {}
+  __device__ __forceinline__ uint32_t scene_of_block() const { return 0; }
+  __device__ __forceinline__ GridDev grid(const GridDev& g, uint32_t, uint32_t& off) const {
+    off = 0;
+    return g;
+  }
+};
+
+struct BatchMap {
+  static constexpr int kMaxThreads = 512;
+  const BatchScene* __restrict__ sc;
+  const uint16_t* __restrict__ blk_scene;   // scene of each block, for this block size
+  int bsel;                                 // block size = 64 << bsel
+  __device__ __forceinline__ uint32_t scene_of_block() const {
+    return blk_scene[xcd_block(blockIdx.x, gridDim.x)];
+  }
+  __device__ __forceinline__ bool locate(uint32_t, uint32_t& p, uint32_t& s, bool& lead) const {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    s = blk_scene[b];
+    const BatchScene& S = sc[s];
+    const uint32_t lb = b - S.blk0[bsel];
+    const uint32_t local = lb * blockDim.x + threadIdx.x;
+    p = S.p0 + local;
+    lead = lb == 0 && threadIdx.x == 0;
+    return local < S.n;
+  }
+  __device__ __forceinline__ void patch(Consts& c, uint32_t s) const {
+    const BatchScene& S = sc[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      c.box_lo[a] = S.box_lo[a];
+      c.box_hi[a] = S.box_hi[a];
+      c.wall_o[a] = S.wall_o[a];
+      c.wall_n[a] = S.wall_n[a];
+    }
+  }
+  __device__ __forceinline__ GridDev grid(const GridDev&, uint32_t s, uint32_t& off) const {
+    off = sc[s].cell_off;
+    return sc[s].grid;
+  }
+};
+
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
   return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
 }
@@ -315,5 +389,14 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
                             uint32_t* flags, uint32_t* maxcnt, hipStream_t s);
+// batches: g spans the cells of every scene (scene s owns cells
+// [cell_off, cell_off + its grid)); nblocks x bsize threads, scene-aligned
+psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
+                            const float4* pos, uint32_t n, const ReorderArrays* ra,
+                            hipStream_t s);
+psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
+                                  uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
+                                  psb_neighbors nb, float* rho, float mc, uint32_t* flags,
+                                  uint32_t* maxcnt, hipStream_t s);
 
 }  // namespace psb

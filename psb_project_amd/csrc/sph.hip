@@ -21,15 +21,15 @@ namespace psb {
 // WCSPH: p = EOS(rho); x.w = p/rho^2; v.w = 1/rho.  PCISPH: p = 0 or the
 // reordered previous pressure; v.w = 1/rho; with xw_prev, x.w = previous
 // pressure (gathered by the warm-start feature sums).
-__global__ __launch_bounds__(256) void k_density_epilogue(uint32_t n, Consts c, int pcisph,
-                                                          int warm, int xw_prev,
-                                                          float* __restrict__ rho,
-                                                          float* __restrict__ pres,
-                                                          const float* __restrict__ pprev,
-                                                          float4* __restrict__ xs,
-                                                          float4* __restrict__ vs) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+template <class M>
+__global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(
+    M map, uint32_t n, Consts c, int pcisph, int warm, int xw_prev, float* __restrict__ rho,
+    float* __restrict__ pres, const float* __restrict__ pprev, float4* __restrict__ xs,
+    float4* __restrict__ vs) {
+  uint32_t p, sc;
+  bool lead;
+  if (!map.locate(n, p, sc, lead)) return;
+  map.patch(c, sc);
   const float r = rho[p];
   if (!pcisph) {
     const float pr = eos(c, r);
@@ -204,7 +204,18 @@ psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int 
                                    int xw_prev, float* rho, float* pres, const float* pprev,
                                    float4* xs, float4* vs, hipStream_t s) {
   if (!n) return PSB_OK;
-  hipLaunchKernelGGL(k_density_epilogue, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, pcisph,
+  hipLaunchKernelGGL(k_density_epilogue<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                     SingleMap{}, n, c, pcisph, warm, xw_prev, rho, pres, pprev, xs, vs);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, uint32_t bsize,
+                                         uint32_t n, const Consts& c, int warm, int xw_prev,
+                                         float* rho, float* pres, const float* pprev, float4* xs,
+                                         float4* vs, hipStream_t s) {
+  if (!n) return PSB_OK;
+  hipLaunchKernelGGL(k_density_epilogue<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, n, c, 1,
                      warm, xw_prev, rho, pres, pprev, xs, vs);
   PSB_HIP(hipGetLastError());
   return PSB_OK;

@@ -171,3 +171,46 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
     p.wall_model = WALL_LATTICE if lattice_walls else WALL_CLAMP
     pos = lattice(n_side, s, seed=2000 + scene_id)
     return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})
+
+
+def batched_scenes(num_scenes: int = 64, s: float = 0.005, seed: int = 5000,
+                   min_count: int = 4096, max_count: int = 131072,
+                   warm_start: int = WARM_NONE, tolerance: float = 0.01, min_it: int = 3,
+                   max_it: int = 50, lattice_walls: bool = False) -> list:
+    """Independent PCISPH scenes for batched serving.
+
+    Particle counts are drawn log-uniform in [min_count, max_count]
+    (numpy PCG64, `seed`).  Even scenes are dam breaks: a column a x h x a
+    (a = round((N/2)^(1/3)), h = ceil(N / a^2) layers, the last one partly
+    filled) at the x-min end of a 4a x max(2.5a, 1.25h) x a box.  Odd scenes
+    are drops: a cube (a = round(N^(1/3))) released at mid height of a
+    3a x 3a x 3a box.  Counts are exact: the lattice is filled in x-fastest
+    order and cut at N.  Every scene shares the material of the PCISPH dam
+    break (spacing s, h = 2s, calibrated mass, 1% tolerance); its params
+    differ only in box and grid.  Jitter seed of scene i: seed + i.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = np.floor(np.exp(rng.uniform(np.log(min_count), np.log(max_count), num_scenes)))
+    out = []
+    for i, N in enumerate(counts.astype(np.int64)):
+        N = int(N)
+        if i % 2 == 0:
+            a = max(2, int(round((N / 2) ** (1.0 / 3.0))))
+            hgt = -(-N // (a * a))
+            box = (4 * a * s, max(2.5 * a, 1.25 * hgt) * s, a * s)
+            origin = (0.0, 0.0, 0.0)
+            dims = (a, hgt, a)
+            kind = "dam_break"
+        else:
+            a = max(2, int(round(N ** (1.0 / 3.0))))
+            hgt = -(-N // (a * a))
+            box = (3 * a * s, max(3 * a, 2 * hgt) * s, 3 * a * s)
+            origin = (a * s, 0.5 * (box[1] - hgt * s), a * s)
+            dims = (a, hgt, a)
+            kind = "drop"
+        pos = lattice(dims, s, origin=origin, seed=seed + i)[:N]
+        sc = pcisph_dam_break(n_side=1, s=s, box=box, tolerance=tolerance, min_it=min_it,
+                              max_it=max_it, warm_start=warm_start, lattice_walls=lattice_walls)
+        sc.params.grid = grid_desc((0.0, 0.0, 0.0), box, s)
+        out.append(Scene(f"{kind}_{i}", sc.params, pos, meta={"kind": kind, "box": box, "s": s}))
+    return out
