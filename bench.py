@@ -12,7 +12,12 @@ s = 5 mm, 1.28 x 0.64 x 0.64 m box, 256 x 128 x 128 cells, 1 % tolerance,
 synchronisations; per-kernel durations come from HIP events recorded on the
 library's stream around every launch of every T-th timed step.
 
-The learned warm start is then trained the way it is meant to be used
+Then the batched serving workload: 64 independent scenes (log-uniform
+4,096..131,072 particles, dam breaks and drops) advanced by one launch per
+phase, block sizes autotuned, steps replayed from HIP graphs, timed with and
+without the learned start, with per-scene step latency p50/p99.
+
+The learned warm start is trained the way it is meant to be used
 (untimed): plain PCISPH records converged pressures on 16 seeded scenes of
 the same shape for 200 steps, the MLP is trained on the GPU, and 4 held-out
 seeds compare plain, previous-pressure and learned starts.  The same K steps
@@ -67,6 +72,10 @@ def parse():
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--batch-warmup", type=int, default=10)
     return ap.parse_args()
 
 
@@ -152,9 +161,8 @@ def learned_leg(ctx, sc, args):
         fs.update(args.warmup)
     elapsed, st, _ = timed_run(ctx, fs, args.steps, split=False)
     fs.close()
-    model.close()
     ev = summary["eval"]
-    return {
+    return model, {
         "value": round(sc.n * args.steps / elapsed, 1),
         "ms_per_step": round(1e3 * elapsed / args.steps, 4),
         "iters_per_step": round(st["total_iterations"] / args.steps, 3),
@@ -167,6 +175,53 @@ def learned_leg(ctx, sc, args):
                   "mse_per_epoch": [round(x, 5) for x in summary["mse_per_epoch"]],
                   "seconds": round(train_s, 1)},
     }
+
+
+def batched_leg(ctx, args, model):
+    """The 64-scene serving workload: autotuned, graph-replayed batch steps,
+    timed with the learned warm start (when a model is given) and without."""
+    from psb_project_amd import scenes
+    from psb_project_amd._lib import WARM_MLP, WARM_NONE
+    from psb_project_amd.batch import BatchSystem
+    out = {}
+    modes = [("plain", WARM_NONE)] + ([("learned", WARM_MLP)] if model is not None else [])
+    for name, warm in modes:
+        scs = scenes.batched_scenes(num_scenes=args.batch_scenes, warm_start=warm)
+        b = BatchSystem(ctx, scs)
+        if warm == WARM_MLP:
+            b.set_mlp(model)
+        b.set_all()
+        b.update(args.batch_warmup)
+        tuned = b.autotune(steps=2)
+        b.set_graphs(True)
+        b.set_latency(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        b.update(args.batch_steps)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        lat = b.latency()
+        its = [b.stats(i)["total_iterations"] / args.batch_steps for i in range(b.num_scenes)]
+        worst = max(b.stats(i)["worst_error"] for i in range(b.num_scenes))
+        flags = b.stats(0)["flags"]
+        out[name] = {
+            "value": round(b.n * args.batch_steps / dt, 1),
+            "ms_per_step": round(1e3 * dt / args.batch_steps, 4),
+            "iters_per_step_mean": round(float(sum(its) / len(its)), 3),
+            "iters_per_step_max_scene": round(float(max(its)), 3),
+            "latency_p50_ms": round(lat["p50_ms"], 4),
+            "latency_p99_ms": round(lat["p99_ms"], 4),
+            "latency_samples": lat["samples"],
+            "max_density_error": worst,
+            "flags": flags,
+            "block_sizes": tuned,
+        }
+        b.close()
+    out["scenes"] = args.batch_scenes
+    out["particles"] = sum(sc.n for sc in scenes.batched_scenes(num_scenes=args.batch_scenes))
+    out["steps"] = args.batch_steps
+    out["graphs"] = True
+    return out
 
 
 def main():
@@ -213,9 +268,18 @@ def main():
                         "avg_launch_us": round(kernels[dom]["avg_us"], 3),
                         "sampled_launches": kernels[dom]["launches"]}
     learned = None
+    model = None
     if not args.no_learned and sc.params.solver == 1:
         print("bench: learned warm start (train untimed, then time)", file=sys.stderr, flush=True)
-        learned = learned_leg(ctx, sc, args)
+        model, learned = learned_leg(ctx, sc, args)
+    batched = None
+    if not args.no_batch and sc.params.solver == 1:
+        print("bench: batched scenes", file=sys.stderr, flush=True)
+        batched = batched_leg(ctx, args, model)
+        if model is not None and "learned" in batched:
+            batched["learned_vs_plain_single"] = round(batched["learned"]["value"] / value, 3)
+    if model is not None:
+        model.close()
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -244,6 +308,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned": learned,
+        "batched": batched,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
     }

@@ -287,6 +287,24 @@ psb_status psb_collision_box(psb_context ctx, float* new_pos4, const float* old_
                              const float box_min[3], const float box_max[3],
                              int32_t container, float elast);
 
+/* interactionSystem (CudaSystem.cu:6-14 -> collideParticles / collideParticle,
+ * CudaParticle_Kernel.cu:10-55): DEM contact forces over each particle's
+ * neighbour list, accumulated into forces4 (+=, in list order), with the
+ * parameters of particle A (the list's owner).  d = |x_B - x_A|,
+ * n = (x_B - x_A) / d, R = r_A + r_B:
+ *   PSB_MODE_REFERENCE  F += -k (R - d) n for 0 < d < 2R: repulsive in
+ *                       contact, attractive for R < d < 2R; the damping,
+ *                       shear and attraction terms are dropped, as the
+ *                       reference's non-mutating double3 += drops them
+ *                       (double3.h:115-123), and R is rounded to float;
+ *   PSB_MODE_CONFIG     spring-dashpot in contact only (0 < d < R):
+ *                       F += -k (R - d) n + c ((v_B - v_A) . n) n, c = damping.
+ * shear and attraction are accepted and unused in both modes.               */
+psb_status psb_interaction_system(psb_context ctx, int32_t mode, const float* pos4,
+                                  const float* vel4, float* forces4, const float* radius,
+                                  const float* spring, const float* damping, const float* shear,
+                                  const float* attraction, psb_neighbors nb, uint32_t n);
+
 /* pci_SPH_update (PciSphSystem.cu:8-68): one PCISPH predict-correct solve.
  * On entry pos4/vel_inter_ap4 etc. hold the start-of-step state (the
  * reference's restore_* snapshot equals it); on return pos4 holds the final

@@ -1057,6 +1057,55 @@ int oracle_collision_box(const psb_sph_params* prm, uint32_t n, float* pos4, flo
   return 0;
 }
 
+// DEM contact force on A from B (collideParticle, CudaParticle_Kernel.cu:10-38)
+// in FP32: d = |x_B - x_A|, n = (x_B - x_A) * (1/d), R = r_A + r_B.  Reference
+// mode: -k (R - d) n for 0 < d < 2R (damping / shear / attraction are dropped
+// by the reference's non-mutating double3 +=, double3.h:115-123).  Config
+// mode: (-k (R - d) + c (v_B - v_A).n) n in contact only (0 < d < R).
+static void dem_pair(bool ref, const float* xa, const float* xb, const float* vb,
+                     const float* va, float ra, float rb, float k, float c, float* f) {
+  f[0] = 0.f; f[1] = 0.f; f[2] = 0.f;
+  const float dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+  const float dist = std::sqrt(dot3(dx, dy, dz, dx, dy, dz));
+  if (!(dist > 0.f)) return;
+  const float R = ra + rb;
+  if (ref ? !(dist < R * 2.0f) : !(dist < R)) return;
+  const float inv = 1.0f / dist;
+  const float nx = dx * inv, ny = dy * inv, nz = dz * inv;
+  float s = -k * (R - dist);
+  if (!ref) {
+    const float vn = dot3(vb[0] - va[0], vb[1] - va[1], vb[2] - va[2], nx, ny, nz);
+    s = s + c * vn;
+  }
+  f[0] = s * nx; f[1] = s * ny; f[2] = s * nz;
+}
+
+// interactionSystem (CudaSystem.cu:6-14 -> collideParticles,
+// CudaParticle_Kernel.cu:42-55): forces[A] += contact of each listed B, in
+// list order, with A's parameters.
+int oracle_interaction_system(int mode, uint32_t n, const float* pos4, const float* vel4,
+                              float* forces4, const float* radius, const float* spring,
+                              const float* damping, const uint32_t* count, const uint32_t* list) {
+  const bool ref = mode == PSB_MODE_REFERENCE;
+  const float zero[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma omp parallel for schedule(static)
+  for (int64_t aa = 0; aa < (int64_t)n; ++aa) {
+    const uint32_t a = (uint32_t)aa;
+    const float* xa = pos4 + 4 * (size_t)a;
+    const float* va = ref ? zero : vel4 + 4 * (size_t)a;
+    float fx = forces4[4 * (size_t)a], fy = forces4[4 * (size_t)a + 1], fz = forces4[4 * (size_t)a + 2];
+    for (uint32_t j = 0; j < count[a]; ++j) {
+      const uint32_t b = list[(uint64_t)j * n + a];
+      float f[3];
+      dem_pair(ref, xa, pos4 + 4 * (size_t)b, ref ? zero : vel4 + 4 * (size_t)b, va, radius[a],
+               radius[b], spring[a], ref ? 0.f : damping[a], f);
+      fx = fx + f[0]; fy = fy + f[1]; fz = fz + f[2];
+    }
+    forces4[4 * (size_t)a] = fx; forces4[4 * (size_t)a + 1] = fy; forces4[4 * (size_t)a + 2] = fz;
+  }
+  return 0;
+}
+
 // Lattice-wall sums at n points (FP32): out[4 i] = sum (h^2 - d^2)^3,
 // out[4 i + 1 .. 3] = sum of spiky gradients gradW(x_i - x_b).
 int oracle_wall_sums(const psb_sph_params* prm, uint32_t n, const float* xyz, float* out) {

@@ -41,6 +41,8 @@ def lib():
                                                        vp, vp, vp, vp]
         L.oracle_collision_box.argtypes = [vp, C.c_uint32, vp, vp]
         L.oracle_wall_sums.argtypes = [vp, C.c_uint32, vp, vp]
+        L.oracle_interaction_system.argtypes = [C.c_int, C.c_uint32, vp, vp, vp, vp, vp, vp, vp,
+                                                vp]
         L.oracle_system_create.restype = vp
         L.oracle_system_create.argtypes = [vp, C.c_uint32, C.c_int]
         L.oracle_system_destroy.argtypes = [vp]
@@ -122,6 +124,21 @@ def wall_sums(params, xyz):
     rc = lib().oracle_wall_sums(C.addressof(params), x.shape[0], x.ctypes.data, out.ctypes.data)
     assert rc == 0, "params do not enable lattice walls"
     return out
+
+
+def interaction_system(mode, pos4, vel4, forces4, radius, spring, damping, count, lst):
+    """DEM contact forces added to a copy of forces4 (list is [cap][n] or flat)."""
+    f = np.array(forces4, np.float32, copy=True)
+    n = f.shape[0]
+    args = [np.ascontiguousarray(a, t) for a, t in ((pos4, np.float32), (vel4, np.float32),
+                                                     (radius, np.float32), (spring, np.float32),
+                                                     (damping, np.float32), (count, np.uint32),
+                                                     (lst, np.uint32))]
+    p, v, r, k, c, cnt, l = args
+    lib().oracle_interaction_system(int(mode), n, p.ctypes.data, v.ctypes.data, f.ctypes.data,
+                                    r.ctypes.data, k.ctypes.data, c.ctypes.data, cnt.ctypes.data,
+                                    l.ctypes.data)
+    return f
 
 
 def _f32(a):

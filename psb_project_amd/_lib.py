@@ -232,6 +232,7 @@ SIGNATURES = {
     "psb_dataset_info": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_uint64),
                                C.POINTER(C.c_uint64)]),
     "psb_system_set_recorder": (i32, [vp, vp, u32]),
+    "psb_interaction_system": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, Neighbors, u32]),
     "psb_batch_create": (i32, [vp, C.POINTER(Params), u32, C.POINTER(SceneDesc), C.POINTER(vp)]),
     "psb_batch_destroy": (i32, [vp]),
     "psb_batch_num_scenes": (u32, [vp]),

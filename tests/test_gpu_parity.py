@@ -384,3 +384,30 @@ def test_copy_device_to_device(ctx, offset, nbytes):
     got = dd.download(np.uint8, total)
     assert np.array_equal(got[offset:], src[offset:])
     assert not got[:offset].any()
+
+
+@pytest.mark.parametrize("mode", [MODE_REFERENCE, MODE_CONFIG])
+def test_interaction_system_bitwise(ctx, oracle, mode):
+    """interactionSystem: DEM contact forces over grid neighbour lists,
+    accumulated onto existing forces, bitwise against the oracle."""
+    sc = scenes.wcsph_dam_break(n_side=16)
+    p = sc.params
+    n = sc.n
+    r = gpu_grid_and_neighbors(ctx, sc.pos4, p.grid, p.support_radius, p.max_neighbors)
+    rng = np.random.default_rng(12)
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = rng.normal(scale=0.5, size=(n, 3))
+    f0 = np.zeros((n, 4), np.float32)
+    f0[:, :3] = rng.normal(size=(n, 3))
+    rad = rng.uniform(0.003, 0.008, n).astype(np.float32)
+    k = rng.uniform(50, 150, n).astype(np.float32)
+    c = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    d = {name: ctx.to_device(a) for name, a in dict(v=vel, f=f0, r=rad, k=k, c=c).items()}
+    dsorted = r["bufs"][1]
+    check(lib().psb_interaction_system(ctx.handle, mode, dsorted.ptr, d["v"].ptr, d["f"].ptr,
+                                       d["r"].ptr, d["k"].ptr, d["c"].ptr, None, None, r["nb"], n))
+    got = d["f"].download(np.float32, (n, 4))[:, :3]
+    want = oracle.interaction_system(mode, r["xs"], vel, f0, rad, k, c, r["count"],
+                                     r["list"])[:, :3]
+    assert np.array_equal(got, want)
+    assert np.abs(got - f0[:, :3]).max() > 0
