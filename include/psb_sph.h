@@ -465,6 +465,40 @@ psb_status psb_dataset_info(psb_dataset d, const float** features, const float**
 psb_status psb_system_set_recorder(psb_system sys, psb_dataset d, uint32_t stride);
 
 /* ------------------------------------------------------------------------ */
+/* Rigid DEM spheres coupled to a PCISPH system (PSB_MODE_CONFIG).           */
+/* The reference couples particle systems only through interactionSystem   */
+/* (above); here each sphere is also a rigid body of boundary particles:    */
+/* the lattice points (i, j, k) * lattice_spacing strictly inside its       */
+/* radius, mass m each.  They are listed as neighbours of the fluid, add to */
+/* its density and to sum gradW of delta_i, and push it with the fluid      */
+/* particle's own pressure (boundary pressure 0, Akinci et al. 2012).  The  */
+/* sphere receives the opposite force, summed per sphere in boundary-id     */
+/* order; with gravity and the DEM contacts of psb_interaction_system's     */
+/* config law (sphere-sphere, and sphere-wall against the collider box) it  */
+/* moves by symplectic Euler once per step, without rotation (pressure and  */
+/* normal contacts exert no torque on a sphere).  The boundary particles    */
+/* occupy ids n_fluid .. n_fluid + count * per_sphere - 1 after the fluid.  */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_sphere_params {
+  float radius;        /* m                                                  */
+  float density;       /* kg/m^3 (mass = density 4/3 pi radius^3)           */
+  float spring;        /* k_n, N/m                                           */
+  float damping;       /* normal dashpot coefficient c, N s/m                */
+} psb_sphere_params;
+
+/* Set before psb_system_set_particles* (which appends the boundary
+ * particles after the fluid); the capacity must hold fluid + boundary.
+ * centers3 / velocities3: count x 3 floats (velocities may be NULL).      */
+psb_status psb_system_set_spheres(psb_system sys, uint32_t count, const float* centers3,
+                                  const float* velocities3, const psb_sphere_params* params);
+/* Boundary particles per sphere for these params (lattice points inside). */
+uint32_t   psb_sphere_particle_count(const psb_sph_params* prm, const psb_sphere_params* sp);
+/* Sphere state after the last step: centres, velocities and the fluid
+ * force of the last step (count x 3 floats each; any may be NULL).        */
+psb_status psb_system_get_spheres(psb_system sys, float* centers3, float* velocities3,
+                                  float* fluid_forces3);
+
+/* ------------------------------------------------------------------------ */
 /* Batched scenes.  No counterpart in the reference, which simulates one     */
 /* System per process (one UniformGrid, 65,536 particles).  A batch advances */
 /* many independent PCISPH scenes (PSB_MODE_CONFIG) that share one material  */

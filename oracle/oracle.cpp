@@ -518,6 +518,15 @@ template <class R> struct System {
   uint32_t max_nbr_seen = 0;
   std::vector<int> iter_hist;
   std::vector<R> err_hist;
+  // rigid DEM spheres: boundary particles with ids >= n_fluid
+  static constexpr uint32_t kNone = 0xFFFFFFFFu;
+  uint32_t n_fluid = kNone;
+  uint32_t n_spheres = 0, per_sphere = 0;
+  psb_sphere_params sphere{};
+  std::vector<float> sph_off;                       // [per][4]
+  std::vector<V4<R>> sph_c, sph_v, sph_f, fb;       // centres, velocities, fluid forces
+
+  bool bnd(uint32_t p) const { return n_fluid != kNone && Aid[p] >= n_fluid; }
 
   void init(const psb_sph_params& p, uint32_t capacity) {
     prm = p; d = derive<R>(p); g = grid_from(p.grid); cap = capacity;
@@ -631,6 +640,12 @@ template <class R> struct System {
     // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
+      if (!ref && bnd((uint32_t)k)) {      // sphere boundary particle: p = 0, zero volume
+        pres[k] = 0;
+        if (feat_now) Ax[k].w = 0;
+        Av[k].w = 0;
+        continue;
+      }
       pres[k] = warm ? Ap[k] : (R)0;
       if (feat_now) Ax[k].w = Ap[k];
       if (d.wall) {
@@ -645,6 +660,14 @@ template <class R> struct System {
 #pragma omp parallel for schedule(static)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
+      if (!ref && bnd(p)) {               // static during the corrector
+        anp[p] = {0, 0, 0, 0};
+        dl[p] = 0;
+        xp0[p] = {Ax[p].x, Ax[p].y, Ax[p].z, 0};
+        if (feat_now)
+          for (int k = 1; k < 8; ++k) feat[(size_t)k * n + p] = 0;
+        continue;
+      }
       PairSums<R> s = surf ? internal_sums<R, false, true>(d, p, n, count.data(), list.data(), Ax.data(), Av.data())
                            : internal_sums<R, false, false>(d, p, n, count.data(), list.data(), Ax.data(), Av.data());
       const R ri = rho[p];
@@ -701,6 +724,101 @@ template <class R> struct System {
     last_iters = it; last_err = err;
     total_iters += it; worst_err = std::max(worst_err, err);
     iter_hist.push_back(it); err_hist.push_back(err);
+    if (!ref && n_spheres) bodies_step(*cur);
+  }
+
+  // Rigid spheres after the corrector, X = the final iteration's predicted
+  // positions (w = final pressure).  (1) the opposite of the pressure force
+  // each boundary particle exerted on its fluid neighbours; (2) per sphere:
+  // fluid force in boundary-id order, gravity, config-law DEM contacts with
+  // every other sphere in index order and the six walls (signed distance,
+  // inward normal), symplectic Euler; (3) boundary particles follow.
+  void bodies_step(const std::vector<V4<R>>& X) {
+    const uint32_t nb = n_spheres * per_sphere;
+    std::vector<V4<R>> fbb(nb);
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      if (!bnd(p)) continue;
+      const V4<R> xb = X[p];
+      R sx = 0, sy = 0, sz = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        const V4<R> xq = X[list[(uint64_t)k * n + p]];
+        R dx = xq.x - xb.x, dy = xq.y - xb.y, dz = xq.z - xb.z;
+        R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (d2 > (R)0 && d2 < d.h2) {
+          R dd = std::sqrt(d2);
+          R invd = (R)1 / dd;
+          R hd = d.h - dd;
+          R gw = d.c_spiky * (hd * hd);
+          R cc = (gw * invd) * xq.w;
+          sx = fmaR(dx, cc, sx); sy = fmaR(dy, cc, sy); sz = fmaR(dz, cc, sz);
+        }
+      }
+      const R sc = -(d.m * d.kp);
+      fbb[Aid[p] - n_fluid] = {sx * sc, sy * sc, sz * sc, 0};
+    }
+    const R Rr = (R)sphere.radius;
+    const R mass = (R)(float)(sphere.density * 4.0 / 3.0 * M_PI * std::pow((double)sphere.radius, 3));
+    const R k = (R)sphere.spring, c = (R)sphere.damping;
+    std::vector<V4<R>> nc(n_spheres), nv(n_spheres);
+    for (uint32_t j = 0; j < n_spheres; ++j) {
+      R fx = 0, fy = 0, fz = 0;
+      for (uint32_t l = 0; l < per_sphere; ++l) {
+        const V4<R>& f = fbb[(size_t)j * per_sphere + l];
+        fx = fx + f.x; fy = fy + f.y; fz = fz + f.z;
+      }
+      sph_f[j] = {fx, fy, fz, 0};
+      fx = fx + mass * d.gx; fy = fy + mass * d.gy; fz = fz + mass * d.gz;
+      const V4<R> cj = sph_c[j], vj = sph_v[j];
+      for (uint32_t i = 0; i < n_spheres; ++i) {
+        if (i == j) continue;
+        // config-law DEM pair, the same expressions as dem_pair
+        R dx = sph_c[i].x - cj.x, dy = sph_c[i].y - cj.y, dz = sph_c[i].z - cj.z;
+        R dist = std::sqrt(dot3(dx, dy, dz, dx, dy, dz));
+        if (!(dist > (R)0)) continue;
+        R RR = Rr + Rr;
+        if (!(dist < RR)) continue;
+        R inv = (R)1 / dist;
+        R nx = dx * inv, ny = dy * inv, nz = dz * inv;
+        R sdem = -k * (RR - dist);
+        R vn = dot3(sph_v[i].x - vj.x, sph_v[i].y - vj.y, sph_v[i].z - vj.z, nx, ny, nz);
+        sdem = sdem + c * vn;
+        fx = fx + sdem * nx; fy = fy + sdem * ny; fz = fz + sdem * nz;
+      }
+      const R pc[3] = {cj.x, cj.y, cj.z}, pv[3] = {vj.x, vj.y, vj.z};
+      R fw[3] = {0, 0, 0};
+      for (int ax = 0; ax < 3; ++ax)
+        for (int side = 0; side < 2; ++side) {
+          const R dd = side == 0 ? pc[ax] - (R)prm.box_min[ax] : (R)prm.box_max[ax] - pc[ax];
+          if (dd < Rr) {
+            const R nin = side == 0 ? (R)1 : (R)-1;
+            const R sw = k * (Rr - dd) - c * (pv[ax] * nin);
+            fw[ax] = fw[ax] + sw * nin;
+          }
+        }
+      fx = fx + fw[0]; fy = fy + fw[1]; fz = fz + fw[2];
+      const R inv_m = (R)1 / mass;
+      const R vx = vj.x + (fx * inv_m) * d.dt, vy = vj.y + (fy * inv_m) * d.dt,
+              vz = vj.z + (fz * inv_m) * d.dt;
+      nv[j] = {vx, vy, vz, 0};
+      nc[j] = {cj.x + vx * d.dt, cj.y + vy * d.dt, cj.z + vz * d.dt, 0};
+    }
+    sph_c = nc;
+    sph_v = nv;
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      if (!bnd(p)) continue;
+      const uint32_t kb = Aid[p] - n_fluid;
+      const uint32_t j = kb / per_sphere, l = kb - j * per_sphere;
+      Bx[p] = {sph_c[j].x + (R)sph_off[4 * l], sph_c[j].y + (R)sph_off[4 * l + 1],
+               sph_c[j].z + (R)sph_off[4 * l + 2], 0};
+      Bv[p] = sph_v[j];
+      Bvh[p] = sph_v[j];
+      Bid[p] = Aid[p];
+      Bp[p] = 0;
+    }
   }
 
   // Pressure scaling of particle p (config mode): the template delta, capped
@@ -722,7 +840,8 @@ template <class R> struct System {
         R gw = d.c_spiky * (hd * hd);
         R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
         W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        // sphere boundary particles (v.w = 0) do not move: W1 only
+        if (n_fluid == kNone || Av[q].w > (R)0) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
       }
     }
     if (d.wall) {
@@ -756,7 +875,7 @@ template <class R> struct System {
         R gw = d.c_spiky * (hd * hd);
         R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
         W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        if (vj.w > (R)0) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);   // boundary: W1 only
         R vol = d.m * vj.w;
         R t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
         div = fmaR(t, vol, div);
@@ -795,6 +914,7 @@ template <class R> struct System {
       float y = (float)m.forward(x, z1, h1, z2, h2);
       float v = y * (float)d.pscale;
       if (v < 0.f) v = 0.f;
+      if (bnd((uint32_t)kk)) v = 0.f;
       pres[kk] = (R)v;
       X[kk].w = (R)v;
     }
@@ -806,6 +926,13 @@ template <class R> struct System {
 #pragma omp parallel for schedule(static) reduction(max : emax)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
+      if (!ref && bnd(p)) {               // sphere boundary particle: no pressure, no error
+        X[p].w = 0;
+        pres[p] = 0;
+        rho_pred[p] = d.rho0;
+        if (write_feature) feat[p] = 0;
+        continue;
+      }
       const V4<R> xi = X[p];
       R S = 0, W1x = 0, W1y = 0, W1z = 0, W2 = 0;
       for (uint32_t k = 0; k < count[p]; ++k) {
@@ -864,6 +991,10 @@ template <class R> struct System {
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       const V4<R> xi = X[p];
+      if (!ref && bnd(p)) {               // static; the sphere update writes its end state
+        if (!fin) Xn[p] = {xi.x, xi.y, xi.z, 0};
+        continue;
+      }
       R sx = 0, sy = 0, sz = 0;
       for (uint32_t k = 0; k < count[p]; ++k) {
         uint32_t q = list[(uint64_t)k * n + p];
@@ -934,7 +1065,9 @@ struct Handle {
 };
 
 template <class R>
-void set_particles(System<R>* s, uint32_t n, const float* pos4, const float* vel4, const float* vh4) {
+int set_particles(System<R>* s, uint32_t n, const float* pos4, const float* vel4, const float* vh4) {
+  const uint32_t nb = s->n_spheres * s->per_sphere;
+  if ((uint64_t)n + nb > s->cap) return 3;
   s->n = n;
   for (uint32_t i = 0; i < n; ++i) {
     s->Bx[i] = {(R)pos4[4 * i], (R)pos4[4 * i + 1], (R)pos4[4 * i + 2], 0};
@@ -943,7 +1076,52 @@ void set_particles(System<R>* s, uint32_t n, const float* pos4, const float* vel
     s->Bid[i] = i;
     s->Bp[i] = 0;
   }
+  // sphere boundary particles after the fluid, at rest relative to their sphere
+  s->n_fluid = s->n_spheres ? n : System<R>::kNone;
+  for (uint32_t j = 0; j < s->n_spheres; ++j)
+    for (uint32_t l = 0; l < s->per_sphere; ++l) {
+      const uint32_t k = n + j * s->per_sphere + l;
+      s->Bx[k] = {s->sph_c[j].x + (R)s->sph_off[4 * l], s->sph_c[j].y + (R)s->sph_off[4 * l + 1],
+                  s->sph_c[j].z + (R)s->sph_off[4 * l + 2], 0};
+      s->Bv[k] = s->sph_v[j];
+      s->Bvh[k] = s->sph_v[j];
+      s->Bid[k] = k;
+      s->Bp[k] = 0;
+    }
+  s->n = n + nb;
   s->Ax = s->Bx; s->Av = s->Bv; s->Avh = s->Bvh; s->Aid = s->Bid;
+  return 0;
+}
+
+template <class R>
+void set_spheres(System<R>* s, uint32_t count, const float* c3, const float* v3,
+                 const psb_sphere_params* sp) {
+  s->n_spheres = count;
+  s->per_sphere = 0;
+  s->sph_off.clear();
+  if (!count) return;
+  s->sphere = *sp;
+  const double lat = s->prm.lattice_spacing > 0.f ? (double)s->prm.lattice_spacing
+                                                  : 0.5 * (double)s->prm.support_radius;
+  const double r = sp->radius;
+  const int m = (int)std::ceil(r / lat);
+  for (int k = -m; k <= m; ++k)
+    for (int j = -m; j <= m; ++j)
+      for (int i = -m; i <= m; ++i) {
+        const double d2 = ((double)i * i + (double)j * j + (double)k * k) * lat * lat;
+        if (d2 < r * r) {
+          s->sph_off.push_back((float)(i * lat)); s->sph_off.push_back((float)(j * lat));
+          s->sph_off.push_back((float)(k * lat)); s->sph_off.push_back(0.f);
+        }
+      }
+  s->per_sphere = (uint32_t)(s->sph_off.size() / 4);
+  s->sph_c.assign(count, {});
+  s->sph_v.assign(count, {});
+  s->sph_f.assign(count, {});
+  for (uint32_t j = 0; j < count; ++j) {
+    s->sph_c[j] = {(R)c3[3 * j], (R)c3[3 * j + 1], (R)c3[3 * j + 2], 0};
+    if (v3) s->sph_v[j] = {(R)v3[3 * j], (R)v3[3 * j + 1], (R)v3[3 * j + 2], 0};
+  }
 }
 
 template <class R>
@@ -1133,9 +1311,40 @@ void oracle_system_destroy(void* hp) {
 int oracle_system_set_particles(void* hp, uint32_t n, const float* pos4, const float* vel4,
                                 const float* vh4) {
   Handle* h = (Handle*)hp;
-  if (h->prec == 64) { if (n > h->d->cap) return 3; set_particles(h->d, n, pos4, vel4, vh4); }
-  else { if (n > h->f->cap) return 3; set_particles(h->f, n, pos4, vel4, vh4); }
+  if (h->prec == 64) return set_particles(h->d, n, pos4, vel4, vh4);
+  return set_particles(h->f, n, pos4, vel4, vh4);
+}
+
+uint32_t oracle_system_count(void* hp) {
+  Handle* h = (Handle*)hp;
+  return h->prec == 64 ? h->d->n : h->f->n;
+}
+
+// rigid DEM spheres (before set_particles, which appends their boundary particles)
+int oracle_system_set_spheres(void* hp, uint32_t count, const float* centers3,
+                              const float* velocities3, const psb_sphere_params* sp) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) set_spheres(h->d, count, centers3, velocities3, sp);
+  else set_spheres(h->f, count, centers3, velocities3, sp);
   return 0;
+}
+
+int oracle_system_get_spheres(void* hp, float* c3, float* v3, float* f3) {
+  Handle* h = (Handle*)hp;
+  auto get = [&](auto* s) {
+    for (uint32_t j = 0; j < s->n_spheres; ++j) {
+      const float cc[3] = {(float)s->sph_c[j].x, (float)s->sph_c[j].y, (float)s->sph_c[j].z};
+      const float vv[3] = {(float)s->sph_v[j].x, (float)s->sph_v[j].y, (float)s->sph_v[j].z};
+      const float ff[3] = {(float)s->sph_f[j].x, (float)s->sph_f[j].y, (float)s->sph_f[j].z};
+      for (int a = 0; a < 3; ++a) {
+        if (c3) c3[3 * j + a] = cc[a];
+        if (v3) v3[3 * j + a] = vv[a];
+        if (f3) f3[3 * j + a] = ff[a];
+      }
+    }
+    return s->n_spheres;
+  };
+  return (int)(h->prec == 64 ? get(h->d) : get(h->f));
 }
 
 int oracle_system_update(void* hp, int steps) {

@@ -154,6 +154,15 @@ class SceneDesc(C.Structure):
     ]
 
 
+class SphereParams(C.Structure):
+    _fields_ = [
+        ("radius", C.c_float),
+        ("density", C.c_float),
+        ("spring", C.c_float),
+        ("damping", C.c_float),
+    ]
+
+
 BATCH_PHASES = ("hash_count", "neighbors_density", "density_epilogue", "pci_nonpressure",
                 "pci_density", "pci_force")
 
@@ -233,6 +242,9 @@ SIGNATURES = {
                                C.POINTER(C.c_uint64)]),
     "psb_system_set_recorder": (i32, [vp, vp, u32]),
     "psb_interaction_system": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, Neighbors, u32]),
+    "psb_system_set_spheres": (i32, [vp, u32, vp, vp, C.POINTER(SphereParams)]),
+    "psb_system_get_spheres": (i32, [vp, vp, vp, vp]),
+    "psb_sphere_particle_count": (u32, [C.POINTER(Params), C.POINTER(SphereParams)]),
     "psb_batch_create": (i32, [vp, C.POINTER(Params), u32, C.POINTER(SceneDesc), C.POINTER(vp)]),
     "psb_batch_destroy": (i32, [vp]),
     "psb_batch_num_scenes": (u32, [vp]),

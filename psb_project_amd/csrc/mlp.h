@@ -27,7 +27,8 @@ struct MlpNorm {
 };
 
 psb_status mlp_infer(psb_mlp m, const float* feat, uint32_t n, float* out, float4* xw, int relu,
-                     float scale, hipStream_t s);
+                     float scale, hipStream_t s, const uint32_t* ids = nullptr,
+                     uint32_t n_fluid = kNoBodies);
 float mlp_pscale(psb_mlp m);
 
 }  // namespace psb

@@ -148,11 +148,14 @@ __device__ __forceinline__ void tile_forward(const MlpFrags& f, const float* __r
 }
 
 // Inference over n samples of a feature SoA [8][n]; y = max(0, out * pscale)
-// written to pres[] and, when xw != nullptr, to the w lane of xw[].
+// written to pres[] and, when xw != nullptr, to the w lane of xw[].  With
+// ids, samples whose id >= n_fluid (rigid-sphere boundary particles) get 0.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, uint32_t n,
                                                    const float* __restrict__ w, MlpFrags f,
                                                    MlpNorm nm, float* __restrict__ out,
-                                                   float4* xw, float pscale, int clamp_relu) {
+                                                   float4* xw, float pscale, int clamp_relu,
+                                                   const uint32_t* __restrict__ ids,
+                                                   uint32_t n_fluid) {
   __shared__ uint16_t lds_all[4 * 16 * kLdsRow];
   const int wave = threadIdx.x >> 6;
   uint16_t* lds = lds_all + wave * 16 * kLdsRow;
@@ -174,6 +177,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
       if (p < n) {
         float v = t.y[j] * pscale;
         if (clamp_relu && v < 0.f) v = 0.f;
+        if (ids && ids[p] >= n_fluid) v = 0.f;
         out[p] = v;
         if (xw) reinterpret_cast<float*>(xw)[4 * p + 3] = v;
       }
@@ -374,11 +378,11 @@ psb_status mlp_pack(psb_mlp m) {
 }
 
 psb_status mlp_infer(psb_mlp m, const float* feat, uint32_t n, float* out, float4* xw, int relu,
-                     float scale, hipStream_t s) {
+                     float scale, hipStream_t s, const uint32_t* ids, uint32_t n_fluid) {
   if (!n) return PSB_OK;
   const uint32_t tiles = (n + 15) / 16;
   hipLaunchKernelGGL(k_mlp_infer, dim3((tiles + 3) / 4), dim3(256), 0, s, feat, n, m->w, m->frags,
-                     m->norm, out, xw, scale, relu);
+                     m->norm, out, xw, scale, relu, ids, n_fluid);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

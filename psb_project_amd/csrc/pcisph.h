@@ -61,7 +61,7 @@ constexpr int kDensSkipUpdate = 2;     // leave the pressure to the MLP
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
-                              hipStream_t s);
+                              hipStream_t s, const uint32_t* ids = nullptr);
 psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
                          uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
                          uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s);
@@ -97,7 +97,8 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
                                hipStream_t s);
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
                                    int xw_prev, float* rho, float* pres, const float* pprev,
-                                   float4* xs, float4* vs, hipStream_t s);
+                                   float4* xs, float4* vs, hipStream_t s,
+                                   const uint32_t* ids = nullptr);
 psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
                                     const uint32_t* list, const float4* xs, const float4* vs,
                                     const float* rho, const float* pres, const float4* avh,

@@ -50,13 +50,15 @@ __global__ void k_pci_stats_reset(PciStatus* st0) {
 // Pressure scaling of particle p in config mode: the template delta, capped by
 // the per-particle value 1/(beta (W1.W1 + W2)) of compute_Pressure_PCI
 // (PciSphSystem_Kernel.cu:44-63) evaluated once at the start positions.
+// With rigid-sphere boundary particles (static, v.w = 0) they add to W1 only.
 __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32_t n,
                                              const uint32_t* __restrict__ cnt,
                                              const uint32_t* __restrict__ list,
-                                             const float4* __restrict__ xs) {
+                                             const float4* __restrict__ xs,
+                                             const float4* __restrict__ vs) {
   const float4 xi = xs[p];
   float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
-  for_neighbors<kUnroll>(p, n, cnt[p], list, xs, [&](uint32_t, const float4& xj) {
+  auto pair = [&](const float4& xj, bool moves) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (d2 > 0.f && d2 < c.h2) {
@@ -66,9 +68,17 @@ __device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32
       const float gw = c.c_spiky * (hd * hd);
       const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
       W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-      W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+      if (moves) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
     }
-  });
+  };
+  if (c.n_fluid == kNoBodies)
+    for_neighbors<kUnroll>(p, n, cnt[p], list, xs,
+                           [&](uint32_t, const float4& xj) { pair(xj, true); });
+  else
+    for_neighbors2<kUnroll>(p, n, cnt[p], list, xs, vs,
+                            [&](uint32_t, const float4& xj, const float4& vj) {
+                              pair(xj, vj.w > 0.f);
+                            });
   if (c.wall) {
     // the static wall particles add to sum gradW, not to sum |gradW|^2
     float S, gx, gy, gz;
@@ -105,7 +115,7 @@ __device__ __forceinline__ float delta_and_features(const Consts& c, uint32_t p,
       const float gw = c.c_spiky * (hd * hd);
       const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
       W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-      W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+      if (vj.w > 0.f) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);   // boundary: W1 only
       const float vol = c.m * vj.w;
       const float t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
       div = __builtin_fmaf(t, vol, div);
@@ -146,6 +156,16 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
   bool lead;
   if (!map.locate(n, p, sc, lead)) return;
   map.patch(c, sc);
+  if (!REF && is_boundary(c, b.aid, p)) {
+    // rigid-sphere boundary particle: static during the corrector
+    const float4 x = xs[p];
+    b.anp[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    b.dl[p] = 0.f;
+    b.xp0[p] = make_float4(x.x, x.y, x.z, 0.f);
+    if (FEAT)
+      for (int k = 1; k < 8; ++k) b.feat[(size_t)k * n + p] = 0.f;
+    return;
+  }
   PairSums s = internal_sums<false, SURF>(c, p, n, cnt, list, xs, vs);
   const float ri = rho[p];
   const float sigma = REF ? 0.0728f : c.sigma;
@@ -171,7 +191,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
     }
   } else {
     if (FEAT) b.dl[p] = delta_and_features(c, p, n, cnt, list, xs, vs, b.feat);
-    else b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs) : c.delta;
+    else b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs, vs) : c.delta;
     const float inv = 1.0f / ri;
     const float a0 = (fV[0] + fS[0]) * inv + c.gx, a1 = (fV[1] + fS[1]) * inv + c.gy,
                 a2 = (fV[2] + fS[2]) * inv + c.gz;
@@ -202,7 +222,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ list, float4* X, float* __restrict__ pres,
     float* __restrict__ rho_pred, const float* __restrict__ dl, PciStatus* __restrict__ st0, int k,
-    float* __restrict__ feat, int dflags) {
+    float* __restrict__ feat, int dflags, const uint32_t* __restrict__ ids) {
   __shared__ float red[M::kMaxThreads / 64];
   PciStatus* __restrict__ st = st0 + map.scene_of_block();   // a block serves one scene
   {
@@ -214,7 +234,15 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
   const bool valid = map.locate(n, p, sc, lead);
   map.patch(c, sc);
   float err = 0.f;
-  if (valid) {
+  const bool bnd = !REF && valid && is_boundary(c, ids, p);
+  if (bnd) {
+    // rigid-sphere boundary particle: no pressure, no error
+    reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = 0.f;
+    pres[p] = 0.f;
+    rho_pred[p] = c.rho0;
+    if (dflags & kDensWriteFeature) feat[p] = 0.f;
+  }
+  if (valid && !bnd) {
     const float4 xi = X[p];
     float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
     for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
@@ -306,6 +334,12 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   }
   if (!valid) return;
   const float4 xi = X[p];
+  if (!REF && is_boundary(c, b.aid, p)) {
+    // rigid-sphere boundary particle: static during the corrector; its
+    // end-of-step state is written by the sphere update
+    if (!fin) Xn[p] = make_float4(xi.x, xi.y, xi.z, 0.f);
+    return;
+  }
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -434,14 +468,14 @@ template <class M>
 static psb_status density_impl(const M& map, const Shape& sh, uint32_t n, const Consts& c,
                                const uint32_t* cnt, const uint32_t* list, float4* X, float* pres,
                                float* rho_pred, const float* dl, PciStatus* st, int k,
-                               float* feat, int dflags, hipStream_t s) {
+                               float* feat, int dflags, const uint32_t* ids, hipStream_t s) {
   if (!n) return PSB_OK;
   if (c.mode == PSB_MODE_REFERENCE)
     hipLaunchKernelGGL((k_pci_density<true, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, 0);
+                       pres, rho_pred, dl, st, k, feat, 0, ids);
   else
     hipLaunchKernelGGL((k_pci_density<false, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, dflags);
+                       pres, rho_pred, dl, st, k, feat, dflags, ids);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -478,9 +512,9 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
-                              hipStream_t s) {
+                              hipStream_t s, const uint32_t* ids) {
   return density_impl(SingleMap{}, single_shape(n), n, c, cnt, list, X, pres, rho_pred, dl, st, k,
-                      feat, dflags, s);
+                      feat, dflags, ids, s);
 }
 
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
@@ -506,7 +540,8 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags,
                                     hipStream_t s) {
-  return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags, s);
+  return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags,
+                      nullptr, s);
 }
 
 psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,

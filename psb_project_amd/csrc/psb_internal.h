@@ -77,9 +77,13 @@ struct Consts {
   int wall_n[3], wall_layers;
   float wall_o[3];
   float wall_s, wall_inv_s, wall_r;   // spacing, 1/spacing, h/spacing
+  // rigid-sphere boundary particles: ids >= n_fluid (kNoBodies: none)
+  uint32_t n_fluid;
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);
+
+constexpr uint32_t kNoBodies = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------------------
 // Device helpers shared by all kernels
@@ -173,6 +177,12 @@ struct BatchMap {
 
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
   return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
+}
+
+// sorted particle p is a rigid-sphere boundary particle (ids: start-of-step ids)
+__device__ __forceinline__ bool is_boundary(const Consts& c, const uint32_t* __restrict__ ids,
+                                            uint32_t p) {
+  return c.n_fluid != kNoBodies && ids[p] >= c.n_fluid;
 }
 
 // cell = floor((x - Min) / cell) clamped to the grid (UniformGridKernel.cu:48-50)

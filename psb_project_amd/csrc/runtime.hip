@@ -149,6 +149,7 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
     c->dt_over_h = (float)(dt / h);
     c->e_scale = (float)(100.0 / rho0);
   }
+  c->n_fluid = kNoBodies;
   c->wall = (p.mode == PSB_MODE_CONFIG && p.solver == PSB_SOLVER_PCISPH &&
              p.wall_model == PSB_WALL_LATTICE && p.box_enabled) ? 1 : 0;
   {

@@ -20,12 +20,14 @@ namespace psb {
 
 // WCSPH: p = EOS(rho); x.w = p/rho^2; v.w = 1/rho.  PCISPH: p = 0 or the
 // reordered previous pressure; v.w = 1/rho; with xw_prev, x.w = previous
-// pressure (gathered by the warm-start feature sums).
+// pressure (gathered by the warm-start feature sums).  Rigid-sphere boundary
+// particles get p = 0 and v.w = 0: zero volume in every neighbour sum that
+// weights by 1/rho_j (viscosity, surface normals, warm-start features).
 template <class M>
 __global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(
     M map, uint32_t n, Consts c, int pcisph, int warm, int xw_prev, float* __restrict__ rho,
     float* __restrict__ pres, const float* __restrict__ pprev, float4* __restrict__ xs,
-    float4* __restrict__ vs) {
+    float4* __restrict__ vs, const uint32_t* __restrict__ ids) {
   uint32_t p, sc;
   bool lead;
   if (!map.locate(n, p, sc, lead)) return;
@@ -36,6 +38,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(
     pres[p] = pr;
     xs[p].w = pr / (r * r);
     vs[p].w = 1.0f / r;
+  } else if (is_boundary(c, ids, p)) {
+    pres[p] = 0.f;
+    if (xw_prev) xs[p].w = 0.f;
+    vs[p].w = 0.f;
   } else {
     pres[p] = warm ? pprev[p] : 0.f;
     if (xw_prev) xs[p].w = pprev[p];
@@ -202,10 +208,10 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
 
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
                                    int xw_prev, float* rho, float* pres, const float* pprev,
-                                   float4* xs, float4* vs, hipStream_t s) {
+                                   float4* xs, float4* vs, hipStream_t s, const uint32_t* ids) {
   if (!n) return PSB_OK;
   hipLaunchKernelGGL(k_density_epilogue<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                     SingleMap{}, n, c, pcisph, warm, xw_prev, rho, pres, pprev, xs, vs);
+                     SingleMap{}, n, c, pcisph, warm, xw_prev, rho, pres, pprev, xs, vs, ids);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -216,7 +222,7 @@ psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, 
                                          float4* vs, hipStream_t s) {
   if (!n) return PSB_OK;
   hipLaunchKernelGGL(k_density_epilogue<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, n, c, 1,
-                     warm, xw_prev, rho, pres, pprev, xs, vs);
+                     warm, xw_prev, rho, pres, pprev, xs, vs, (const uint32_t*)nullptr);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

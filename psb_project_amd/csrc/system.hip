@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "bodies.h"
 #include "mlp.h"
 #include "pcisph.h"
 #include "psb_internal.h"
@@ -82,6 +83,17 @@ struct psb_system_s {
   double t_total[PH_COUNT] = {};
   int64_t t_count[PH_COUNT] = {};
   std::vector<hipEvent_t> ev_iter;
+  // rigid DEM spheres (boundary particles after the fluid)
+  uint32_t n_fluid = 0;
+  uint32_t n_spheres = 0, per_sphere = 0;
+  psb_sphere_params sphere{};
+  std::vector<float> sphere_c, sphere_v, sphere_off;   // host: centres, velocities, offsets
+  float4* d_sc[2] = {nullptr, nullptr};
+  float4* d_sv[2] = {nullptr, nullptr};
+  float4* d_sf = nullptr;
+  float4* d_soff = nullptr;
+  float4* d_fb = nullptr;
+  int sphere_cur = 0;
 };
 
 namespace psb {
@@ -218,7 +230,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   {
     Timed t(s, PH_EPILOGUE, step);
     PSB_TRY(launch_density_epilogue(s->n, s->c, 1, warm ? 1 : 0, feat ? 1 : 0, s->rho, s->pres,
-                                    s->ap, s->ax, s->av, st));
+                                    s->ap, s->ax, s->av, st, s->aid));
   }
   PciBuffers b{};
   b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
@@ -249,13 +261,14 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st));
+                                 s->st, k, s->feat, dflags, st, s->aid));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
       // predicted positions the force pass gathers)
       Timed t(s, PH_MLP, step);
-      PSB_TRY(mlp_infer(s->mlp, s->feat, s->n, s->pres, cur, 1, s->c.pscale, st));
+      PSB_TRY(mlp_infer(s->mlp, s->feat, s->n, s->pres, cur, 1, s->c.pscale, st,
+                        s->n_spheres ? s->aid : nullptr, s->c.n_fluid));
     }
     {
       Timed t(s, PH_PCI_FORCE, step, k);
@@ -281,6 +294,25 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   }
   if ((int)s->step_final.size() <= step) s->step_final.resize(step + 1, -1);
   s->step_final[step] = final_known;
+  if (s->n_spheres) {
+    // the pressure force of the final iteration used the positions its
+    // density pass read
+    float4* xfin = ((final_known + (warm ? 1 : 0)) & 1) ? s->xp1 : s->xp0;
+    SphereArgs a{};
+    a.c_in = s->d_sc[s->sphere_cur]; a.v_in = s->d_sv[s->sphere_cur];
+    a.c_out = s->d_sc[s->sphere_cur ^ 1]; a.v_out = s->d_sv[s->sphere_cur ^ 1];
+    a.f_fluid = s->d_sf; a.fb = s->d_fb; a.offsets = s->d_soff;
+    a.count = s->n_spheres; a.per = s->per_sphere;
+    a.radius = s->sphere.radius;
+    a.mass = (float)(s->sphere.density * 4.0 / 3.0 * M_PI * std::pow((double)s->sphere.radius, 3));
+    a.spring = s->sphere.spring;
+    a.damping = s->sphere.damping;
+    a.wall_damping = s->sphere.damping;
+    for (int ax = 0; ax < 3; ++ax) { a.box_min[ax] = s->prm.box_min[ax]; a.box_max[ax] = s->prm.box_max[ax]; }
+    PSB_TRY(launch_bodies(s->n, s->c, s->ncount, s->nlist, xfin, s->aid, a, s->bx, s->bv, s->bvh,
+                          s->bid, s->bp, st));
+    s->sphere_cur ^= 1;
+  }
   if (s->rec && feat) {
     // features and converged pressure of every rec_stride-th particle
     psb_dataset d = s->rec;
@@ -341,6 +373,9 @@ psb_status psb_system_destroy(psb_system s) {
                   s->ncount, s->nlist, s->flags, s->st, s->feat};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
+  for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
+                  (void*)s->d_sf, (void*)s->d_soff, (void*)s->d_fb})
+    if (p) hipFree(p);
   for (auto e : s->ev) hipEventDestroy(e);
   for (auto e : s->ev_iter) hipEventDestroy(e);
   if (s->grid) psb_finalize_grid(s->grid);
@@ -355,6 +390,55 @@ psb_status psb_system_get_params(psb_system s, psb_sph_params* out) {
 }
 
 uint32_t psb_system_count(psb_system s) { return s ? s->n : 0; }
+
+// Boundary particles of the rigid spheres after the n_fluid fluid particles
+// (ids n_fluid ..), at rest relative to their sphere; sphere state uploaded.
+static psb_status append_spheres(psb_system s, uint32_t n_fluid) {
+  s->n_fluid = n_fluid;
+  s->c.n_fluid = kNoBodies;
+  if (!s->n_spheres) return PSB_OK;
+  const uint32_t nb = s->n_spheres * s->per_sphere;
+  if ((uint64_t)n_fluid + nb > s->cap) {
+    set_error("capacity must hold the fluid and the sphere boundary particles");
+    return PSB_ERR_CAPACITY;
+  }
+  std::vector<float> x(4 * (size_t)nb), v(4 * (size_t)nb, 0.f), zeros(nb, 0.f);
+  std::vector<uint32_t> ids(nb);
+  std::vector<float> c4(4 * (size_t)s->n_spheres, 0.f), v4(4 * (size_t)s->n_spheres, 0.f);
+  for (uint32_t j = 0; j < s->n_spheres; ++j)
+    for (int a = 0; a < 3; ++a) {
+      c4[4 * j + a] = s->sphere_c[3 * j + a];
+      v4[4 * j + a] = s->sphere_v[3 * j + a];
+    }
+  for (uint32_t j = 0; j < s->n_spheres; ++j)
+    for (uint32_t l = 0; l < s->per_sphere; ++l) {
+      const size_t k = (size_t)j * s->per_sphere + l;
+      for (int a = 0; a < 3; ++a) {
+        x[4 * k + a] = c4[4 * j + a] + s->sphere_off[4 * l + a];
+        v[4 * k + a] = v4[4 * j + a];
+      }
+      x[4 * k + 3] = 0.f;
+      ids[k] = n_fluid + (uint32_t)k;
+    }
+  hipStream_t st = s->ctx->stream;
+  const size_t b4 = 16 * (size_t)nb, b1 = 4 * (size_t)nb;
+  for (float4* dst : {s->bx, s->ax})
+    PSB_HIP(hipMemcpyAsync(dst + n_fluid, x.data(), b4, hipMemcpyHostToDevice, st));
+  for (float4* dst : {s->bv, s->av, s->bvh, s->avh})
+    PSB_HIP(hipMemcpyAsync(dst + n_fluid, v.data(), b4, hipMemcpyHostToDevice, st));
+  for (uint32_t* dst : {s->bid, s->aid})
+    PSB_HIP(hipMemcpyAsync(dst + n_fluid, ids.data(), b1, hipMemcpyHostToDevice, st));
+  for (float* dst : {s->bp, s->pres, s->rho, s->rho_pred})
+    PSB_HIP(hipMemcpyAsync(dst + n_fluid, zeros.data(), b1, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->d_sc[0], c4.data(), 16 * (size_t)s->n_spheres, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemcpyAsync(s->d_sv[0], v4.data(), 16 * (size_t)s->n_spheres, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipMemsetAsync(s->d_sf, 0, 16 * (size_t)s->n_spheres, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  s->sphere_cur = 0;
+  s->n = n_fluid + nb;
+  s->c.n_fluid = n_fluid;
+  return PSB_OK;
+}
 
 static psb_status upload_state(psb_system s, uint32_t n, const std::vector<float>& x,
                                const std::vector<float>& v, const std::vector<float>& vh,
@@ -378,6 +462,7 @@ static psb_status upload_state(psb_system s, uint32_t n, const std::vector<float
   PSB_HIP(hipMemcpyAsync(s->rho_pred, zeros.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
   PSB_HIP(hipStreamSynchronize(st));
   s->n = n;
+  PSB_TRY(append_spheres(s, n));
   s->has_state = true;
   s->stepped = false;
   std::memset(&s->stats, 0, sizeof(s->stats));
@@ -435,6 +520,7 @@ psb_status psb_system_set_particles_device(psb_system s, uint32_t n, const float
   PSB_HIP(hipMemsetAsync(s->rho_pred, 0, 4 * (size_t)n, st));
   PSB_HIP(hipStreamSynchronize(st));
   s->n = n;
+  PSB_TRY(append_spheres(s, n));
   s->has_state = true;
   s->stepped = false;
   std::memset(&s->stats, 0, sizeof(s->stats));
@@ -592,6 +678,72 @@ psb_status psb_system_reset_timing(psb_system s) {
 }
 
 float psb_system_pressure_scale(psb_system s) { return s ? s->c.pscale : 0.f; }
+
+uint32_t psb_sphere_particle_count(const psb_sph_params* prm, const psb_sphere_params* sp) {
+  if (!prm || !sp || !(sp->radius > 0.f)) return 0;
+  const double s = prm->lattice_spacing > 0.f ? (double)prm->lattice_spacing
+                                              : 0.5 * (double)prm->support_radius;
+  return (uint32_t)(sphere_offsets(sp->radius, s).size() / 4);
+}
+
+psb_status psb_system_set_spheres(psb_system s, uint32_t count, const float* centers3,
+                                  const float* velocities3, const psb_sphere_params* sp) {
+  PSB_ARG(s, "null system");
+  PSB_ARG(s->prm.mode == PSB_MODE_CONFIG && s->prm.solver == PSB_SOLVER_PCISPH,
+          "spheres need a PSB_MODE_CONFIG PCISPH system");
+  PSB_ARG(count == 0 || (centers3 && sp), "null argument");
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
+                  (void*)s->d_sf, (void*)s->d_soff, (void*)s->d_fb})
+    if (p) hipFree(p);
+  s->d_sc[0] = s->d_sc[1] = s->d_sv[0] = s->d_sv[1] = s->d_sf = s->d_soff = s->d_fb = nullptr;
+  s->n_spheres = 0;
+  s->per_sphere = 0;
+  s->has_state = false;   // set the particles again: the boundary particles follow the fluid
+  if (!count) return PSB_OK;
+  PSB_ARG(sp->radius > 0.f && sp->density > 0.f && sp->spring >= 0.f && sp->damping >= 0.f,
+          "sphere radius and density must be > 0, spring and damping >= 0");
+  const double lat = s->prm.lattice_spacing > 0.f ? (double)s->prm.lattice_spacing
+                                                  : 0.5 * (double)s->prm.support_radius;
+  s->sphere_off = sphere_offsets(sp->radius, lat);
+  PSB_ARG(!s->sphere_off.empty(), "sphere radius below the lattice spacing");
+  s->sphere = *sp;
+  s->per_sphere = (uint32_t)(s->sphere_off.size() / 4);
+  s->sphere_c.assign(centers3, centers3 + 3 * (size_t)count);
+  s->sphere_v.assign(3 * (size_t)count, 0.f);
+  if (velocities3) s->sphere_v.assign(velocities3, velocities3 + 3 * (size_t)count);
+  const size_t b4 = 16 * (size_t)count;
+  PSB_HIP(hipMalloc(&s->d_sc[0], b4)); PSB_HIP(hipMalloc(&s->d_sc[1], b4));
+  PSB_HIP(hipMalloc(&s->d_sv[0], b4)); PSB_HIP(hipMalloc(&s->d_sv[1], b4));
+  PSB_HIP(hipMalloc(&s->d_sf, b4));
+  PSB_HIP(hipMalloc(&s->d_soff, 16 * (size_t)s->per_sphere));
+  PSB_HIP(hipMalloc(&s->d_fb, 16 * (size_t)s->per_sphere * count));
+  PSB_HIP(hipMemcpy(s->d_soff, s->sphere_off.data(), 16 * (size_t)s->per_sphere,
+                    hipMemcpyHostToDevice));
+  s->n_spheres = count;
+  return PSB_OK;
+}
+
+psb_status psb_system_get_spheres(psb_system s, float* centers3, float* velocities3,
+                                  float* fluid_forces3) {
+  PSB_ARG(s, "null system");
+  if (!s->n_spheres) return PSB_OK;
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  const uint32_t S = s->n_spheres;
+  std::vector<float> c(4 * (size_t)S), v(4 * (size_t)S), f(4 * (size_t)S);
+  hipStream_t st = s->ctx->stream;
+  PSB_HIP(hipMemcpyAsync(c.data(), s->d_sc[s->sphere_cur], 16 * (size_t)S, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipMemcpyAsync(v.data(), s->d_sv[s->sphere_cur], 16 * (size_t)S, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipMemcpyAsync(f.data(), s->d_sf, 16 * (size_t)S, hipMemcpyDeviceToHost, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  for (uint32_t j = 0; j < S; ++j)
+    for (int a = 0; a < 3; ++a) {
+      if (centers3) centers3[3 * j + a] = c[4 * j + a];
+      if (velocities3) velocities3[3 * j + a] = v[4 * j + a];
+      if (fluid_forces3) fluid_forces3[3 * j + a] = f[4 * j + a];
+    }
+  return PSB_OK;
+}
 
 psb_status psb_system_set_mlp(psb_system s, psb_mlp m) {
   PSB_ARG(s, "null system");

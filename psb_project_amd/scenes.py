@@ -214,3 +214,51 @@ def batched_scenes(num_scenes: int = 64, s: float = 0.005, seed: int = 5000,
         sc.params.grid = grid_desc((0.0, 0.0, 0.0), box, s)
         out.append(Scene(f"{kind}_{i}", sc.params, pos, meta={"kind": kind, "box": box, "s": s}))
     return out
+
+
+def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28, 1.28),
+                         num_spheres: int = 512, sphere_radius_cells: float = 3.0,
+                         sphere_density: float = 500.0, spring: float = 1e4,
+                         damping_ratio: float = 0.3, seed: int = 3000, tolerance: float = 0.01,
+                         min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
+                         lattice_walls: bool = True) -> Scene:
+    """PCISPH double dam break with rigid DEM spheres.
+
+    Two fluid blocks of block = (nx, ny, nz) lattice particles at the x-min
+    and x-max ends of the box (2 x 128 x 128 x 64 = 2,097,152 particles at
+    full size), walls as lattice boundary particles, and `num_spheres`
+    spheres of radius sphere_radius_cells * s and the given density placed
+    uniformly at random (PCG64 `seed`) in the free space between the blocks,
+    at least 2 radii + s apart and one radius + s from the walls.  Contact
+    spring k_n = spring; the normal dashpot coefficient is
+    c = 2 damping_ratio sqrt(k_n m / 2) (m / 2: reduced mass of two spheres).
+    meta holds the sphere centres and parameters.
+    """
+    nx, ny, nz = block
+    sc = pcisph_dam_break(n_side=1, s=s, box=box, tolerance=tolerance, min_it=min_it,
+                          max_it=max_it, warm_start=warm_start, lattice_walls=lattice_walls)
+    p = sc.params
+    p.grid = grid_desc((0.0, 0.0, 0.0), box, s)
+    left = lattice((nx, ny, nz), s, seed=seed)
+    right = lattice((nx, ny, nz), s, origin=(box[0] - nx * s, 0.0, 0.0), seed=seed + 1)
+    pos = np.concatenate([left, right])
+    R = sphere_radius_cells * s
+    mass = sphere_density * 4.0 / 3.0 * math.pi * R ** 3
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    lo = np.array([nx * s + R + s, R + s, R + s])
+    hi = np.array([box[0] - nx * s - R - s, box[1] - R - s, box[2] - R - s])
+    centers = []
+    tries = 0
+    while len(centers) < num_spheres:
+        tries += 1
+        if tries > 1000 * max(1, num_spheres):
+            raise ValueError("spheres do not fit")
+        c = rng.uniform(lo, hi)
+        if all(np.sum((c - q) ** 2) >= (2 * R + s) ** 2 for q in centers):
+            centers.append(c)
+    centers = np.array(centers, np.float32).reshape(-1, 3)
+    meta = {"s": s, "block": block, "sphere_centers": centers, "sphere_radius": R,
+            "sphere_density": sphere_density, "spring": spring,
+            "damping": 2.0 * damping_ratio * math.sqrt(spring * mass / 2.0),
+            "sphere_mass": mass}
+    return Scene("double_dam_break_dem", p, pos, meta=meta)

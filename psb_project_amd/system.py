@@ -105,6 +105,27 @@ class FluidSystem:
     def pressure_scale(self) -> float:
         return float(lib().psb_system_pressure_scale(self.handle))
 
+    def set_spheres(self, centers, velocities=None, radius: float = 0.015,
+                    density: float = 500.0, spring: float = 1e4, damping: float = 0.0) -> None:
+        """Rigid DEM spheres; call before set_particles (their boundary
+        particles follow the fluid's)."""
+        from ._lib import SphereParams
+        c = np.ascontiguousarray(centers, np.float32).reshape(-1, 3)
+        v = None if velocities is None else np.ascontiguousarray(velocities, np.float32).reshape(-1, 3)
+        sp = SphereParams(radius, density, spring, damping)
+        check(lib().psb_system_set_spheres(self.handle, c.shape[0], c.ctypes.data,
+                                           None if v is None else v.ctypes.data, C.byref(sp)))
+        self.num_spheres = c.shape[0]
+
+    def spheres(self) -> dict:
+        m = getattr(self, "num_spheres", 0)
+        o = {k: np.zeros((m, 3), np.float32) for k in ("centers", "velocities", "fluid_force")}
+        if m:
+            check(lib().psb_system_get_spheres(self.handle, o["centers"].ctypes.data,
+                                               o["velocities"].ctypes.data,
+                                               o["fluid_force"].ctypes.data))
+        return o
+
     def set_mlp(self, model) -> None:
         check(lib().psb_system_set_mlp(self.handle, model.handle if model is not None else None))
 
