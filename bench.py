@@ -37,6 +37,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -76,6 +78,9 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--no-dem", action="store_true", help="skip the DEM double dam break")
+    ap.add_argument("--dem-steps", type=int, default=1000)
+    ap.add_argument("--dem-warmup", type=int, default=5)
     return ap.parse_args()
 
 
@@ -224,6 +229,38 @@ def batched_leg(ctx, args, model):
     return out
 
 
+def dem_leg(ctx, args):
+    """Double dam break with 512 rigid DEM spheres (2 x 128 x 128 x 64 fluid
+    particles, lattice walls), timed over --dem-steps steps."""
+    from psb_project_amd import scenes
+    from psb_project_amd.system import FluidSystem, sphere_particle_count
+    sc = scenes.double_dam_break_dem()
+    m = sc.meta
+    per = sphere_particle_count(sc.params, m["sphere_radius"])
+    fs = FluidSystem(ctx, sc.params, sc.n + per * len(m["sphere_centers"]))
+    fs.set_spheres(m["sphere_centers"], None, m["sphere_radius"], m["sphere_density"],
+                   m["spring"], m["damping"])
+    fs.set_particles(sc.pos4)
+    fs.update(args.dem_warmup)
+    elapsed, st, _ = timed_run(ctx, fs, args.dem_steps, split=False)
+    sp = fs.spheres()
+    out = {
+        "value": round(sc.n * args.dem_steps / elapsed, 1),
+        "unit": "fluid particle-steps/s",
+        "ms_per_step": round(1e3 * elapsed / args.dem_steps, 4),
+        "fluid_particles": sc.n,
+        "spheres": int(len(m["sphere_centers"])),
+        "boundary_particles": fs.n - sc.n,
+        "steps": args.dem_steps,
+        "iters_per_step": round(st["total_iterations"] / args.dem_steps, 3),
+        "max_density_error": st["worst_error"],
+        "flags": st["flags"],
+        "sphere_speed_max": round(float(np.linalg.norm(sp["velocities"], axis=1).max()), 4),
+    }
+    fs.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -280,6 +317,10 @@ def main():
             batched["learned_vs_plain_single"] = round(batched["learned"]["value"] / value, 3)
     if model is not None:
         model.close()
+    dem = None
+    if not args.no_dem and sc.params.solver == 1:
+        print("bench: double dam break with DEM spheres", file=sys.stderr, flush=True)
+        dem = dem_leg(ctx, args)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -309,6 +350,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "batched": batched,
+        "dem": dem,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
     }

@@ -14,6 +14,13 @@ from ._lib import Params, StepStats, SystemArrays, check, lib
 from .device import Context, DeviceArray
 
 
+def sphere_particle_count(params, radius: float) -> int:
+    """Boundary particles per rigid sphere of this radius (lattice points inside)."""
+    from ._lib import SphereParams
+    sp = SphereParams(radius, 1.0, 0.0, 0.0)
+    return int(lib().psb_sphere_particle_count(C.byref(params), C.byref(sp)))
+
+
 class FluidSystem:
     def __init__(self, ctx: Context, params: Params, capacity: int):
         self.ctx = ctx
