@@ -32,13 +32,18 @@ def _rows(path):
         yield from csv.DictReader(f)
 
 
-def counter_per_dispatch(path, kernel_regex: str, counter: str):
-    """Mean value of `counter` per dispatch of kernels matching the regex."""
+def counter_per_dispatch(path, kernel_regex: str, counter: str, min_us: float = 0.0):
+    """Mean value of `counter` per dispatch of kernels matching the regex,
+    over dispatches that ran at least min_us (drops the corrector's
+    early-exit launches)."""
     rx = re.compile(kernel_regex)
     per = defaultdict(float)
     for r in _rows(path):
         if r.get("Counter_Name") != counter or not rx.search(r.get("Kernel_Name", "")):
             continue
+        if min_us > 0 and "Start_Timestamp" in r:
+            if (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 < min_us:
+                continue
         per[r.get("Dispatch_Id") or r.get("Correlation_Id")] += float(r["Counter_Value"])
     if not per:
         raise SystemExit(f"{path}: no {counter} rows for kernels matching {kernel_regex!r}")
@@ -112,8 +117,8 @@ def cmd_kernel_stats(a) -> int:
 
 
 def cmd_traffic(a) -> int:
-    fk, nd = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE")
-    wk, _ = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE")
+    fk, nd = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE", a.full_min_us)
+    wk, _ = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE", a.full_min_us)
     cf, _ = counter_per_dispatch(a.copy_fetch, "k_copy_stream", "FETCH_SIZE")
     cw, _ = counter_per_dispatch(a.copy_write, "k_copy_stream", "WRITE_SIZE")
     ff = a.copy_bytes / (cf * 1024.0)
@@ -124,7 +129,7 @@ def cmd_traffic(a) -> int:
     lines = [
         f"# HBM traffic per launch: {a.title}", "",
         "commands:", *[f"  {c}" for c in a.commands.split(";;")], "",
-        f"kernel regex: {a.kernel}   dispatches averaged: {nd}",
+        f"kernel regex: {a.kernel}   dispatches averaged: {nd} (those >= {a.full_min_us} us)",
         f"FETCH_SIZE per dispatch: {fk:.1f} KiB (raw)",
         f"WRITE_SIZE per dispatch: {wk:.1f} KiB (raw)",
         f"copy calibration (k_copy_stream, {a.copy_bytes} B read and written per dispatch):",
@@ -166,6 +171,7 @@ def main(argv=None) -> int:
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--alg-bytes", type=float, required=True)
     t.add_argument("--title", default="")
+    t.add_argument("--full-min-us", type=float, default=5.0)
     a = ap.parse_args(argv)
     return {"copy": cmd_copy, "kernel-stats": cmd_kernel_stats, "traffic": cmd_traffic}[a.cmd](a)
 

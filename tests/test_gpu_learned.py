@@ -130,17 +130,18 @@ def test_mlp_warm_start_step_matches_oracle(ctx, oracle):
 
 
 def test_learned_warm_start_cuts_iterations(ctx):
-    """Train on plain-PCISPH converged pressures of 4 seeded 64^3 dam breaks
-    (100 steps), then run a held-out seed: the learned start needs fewer
-    corrector iterations than the zero start, every step still within the
-    1% tolerance (the corrector is unchanged)."""
+    """The warm-start protocol at half size: train on plain-PCISPH converged
+    pressures of 8 seeded 64^3 dam breaks (200 steps each, every 16th
+    particle, 4 epochs), then run 2 held-out seeds: the learned start needs at
+    least 25 % fewer corrector iterations than the zero start, every step
+    still within the 1 % tolerance (the corrector is unchanged)."""
     def make(i):
         return scenes.pcisph_dam_break(scene_id=200 + i)
-    model, summary = learned.train_and_evaluate(ctx, make, range(4), range(4, 5), steps=100,
-                                                stride=32, epochs=3)
+    model, summary = learned.train_and_evaluate(ctx, make, range(8), range(8, 10), steps=200,
+                                                stride=16, epochs=4)
     ev = summary["eval"]
     plain, lrn = ev["plain"], ev["learned"]
     assert plain["steps_within_tolerance"] == 1.0 and lrn["steps_within_tolerance"] == 1.0
     assert plain["iters_per_step"] > 5
-    assert lrn["iters_per_step"] <= 0.8 * plain["iters_per_step"], ev
+    assert lrn["iters_per_step"] <= 0.75 * plain["iters_per_step"], ev
     model.close()
