@@ -122,6 +122,13 @@ typedef struct psb_sph_params {
                                boundary particles, free surface) are not
                                over-corrected; 0 -> template delta only       */
   int32_t wall_model;       /* PSB_WALL_*: PCISPH config mode only           */
+  float   relaxation;       /* PCISPH config mode: delta = relaxation * template
+                               delta (<= 0 means 1).  Below 1 under-relaxes
+                               the corrector: a deep column on a rigid floor
+                               (boundary-particle walls, ~128 layers) makes the
+                               full template over-correct and diverge; 0.6
+                               holds there.  The warm-start feature scale
+                               keeps the unrelaxed delta.                     */
 } psb_sph_params;
 
 #define PSB_WALL_CLAMP      0  /* box walls act only through the collider (clamp) */

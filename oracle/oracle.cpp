@@ -104,6 +104,9 @@ template <class R> Derived<R> derive(const psb_sph_params& p) {
     double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
     d.delta = (R)(1.0 / (beta * template_w2(h, s)));
   }
+  // under-relaxed corrector (config mode); the feature scale keeps the template
+  if (p.mode == PSB_MODE_CONFIG && p.relaxation > 0.f && p.relaxation != 1.0f)
+    d.delta = d.delta * (R)p.relaxation;
   const double gamma = p.tait_gamma > 0.f ? p.tait_gamma : 7.0;
   d.tait_b = (R)(p.tait_b > 0.f ? (double)p.tait_b : (double)p.gas_stiffness * rho0 / gamma);
   d.inv_rho0 = (R)(1.0 / rho0);

@@ -80,6 +80,7 @@ class Params(C.Structure):
         ("lattice_spacing", C.c_float),
         ("delta_local_cap", C.c_int32),
         ("wall_model", C.c_int32),
+        ("relaxation", C.c_float),
     ]
 
     def copy(self) -> "Params":

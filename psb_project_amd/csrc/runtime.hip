@@ -113,6 +113,8 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
     double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
     c->delta = (float)(1.0 / (beta * template_w2(h, s)));
   }
+  if (p.mode == PSB_MODE_CONFIG && p.relaxation > 0.f && p.relaxation != 1.0f)
+    c->delta = c->delta * p.relaxation;
   const double gamma = p.tait_gamma > 0.f ? p.tait_gamma : 7.0;
   c->tait_b = (float)(p.tait_b > 0.f ? (double)p.tait_b : (double)p.gas_stiffness * rho0 / gamma);
   c->inv_rho0 = (float)(1.0 / rho0);

@@ -134,7 +134,8 @@ def wcsph_dam_break(n_side: int = 32, s: float = 0.01, mode: int = MODE_CONFIG,
 def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
                      scene_id: int = 0, box=(1.28, 0.64, 0.64), tolerance: float = 0.01,
                      min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
-                     calibrate_mass: bool = True, lattice_walls: bool = False) -> Scene:
+                     calibrate_mass: bool = True, lattice_walls: bool = False,
+                     relaxation: float = 1.0) -> Scene:
     """Plain PCISPH dam break (64^3 block at the x-min end of the box).
 
     lattice_walls: the box walls also act as implicit boundary particles on
@@ -169,6 +170,7 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
     p.lattice_spacing = s
     p.delta_local_cap = 1
     p.wall_model = WALL_LATTICE if lattice_walls else WALL_CLAMP
+    p.relaxation = relaxation
     pos = lattice(n_side, s, seed=2000 + scene_id)
     return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})
 
@@ -221,7 +223,7 @@ def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28
                          sphere_density: float = 500.0, spring: float = 1e4,
                          damping_ratio: float = 0.3, seed: int = 3000, tolerance: float = 0.01,
                          min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
-                         lattice_walls: bool = True) -> Scene:
+                         lattice_walls: bool = True, relaxation: float = 0.6) -> Scene:
     """PCISPH double dam break with rigid DEM spheres.
 
     Two fluid blocks of block = (nx, ny, nz) lattice particles at the x-min
@@ -232,11 +234,15 @@ def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28
     at least 2 radii + s apart and one radius + s from the walls.  Contact
     spring k_n = spring; the normal dashpot coefficient is
     c = 2 damping_ratio sqrt(k_n m / 2) (m / 2: reduced mass of two spheres).
+    The corrector is under-relaxed (delta = relaxation * template): with
+    boundary-particle walls under 128-layer columns the full template
+    over-corrects and diverges within 30 steps; 0.6 holds the collapse.
     meta holds the sphere centres and parameters.
     """
     nx, ny, nz = block
     sc = pcisph_dam_break(n_side=1, s=s, box=box, tolerance=tolerance, min_it=min_it,
-                          max_it=max_it, warm_start=warm_start, lattice_walls=lattice_walls)
+                          max_it=max_it, warm_start=warm_start, lattice_walls=lattice_walls,
+                          relaxation=relaxation)
     p = sc.params
     p.grid = grid_desc((0.0, 0.0, 0.0), box, s)
     left = lattice((nx, ny, nz), s, seed=seed)
