@@ -79,6 +79,24 @@ def build_library(verbose: bool = False, jobs: int | None = None) -> Path:
     return LIB
 
 
+HOST = PKG / "host"
+EXAMPLE = HOST / "example_dam_break"
+
+
+def build_host_example(verbose: bool = False) -> Path:
+    """Compile the C++ host-API example against libpsb_sph.so (rpath to it)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", f"-I{HOST}",
+           str(HOST / "example_dam_break.cpp"), f"-L{PKG}", "-lpsb_sph",
+           "-Wl,-rpath,$ORIGIN/..", "-o", str(EXAMPLE)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host example build failed:\n{r.stdout}\n{r.stderr}")
+    return EXAMPLE
+
+
 if __name__ == "__main__":
     path = build_library(verbose="-v" in sys.argv)
+    build_host_example(verbose="-v" in sys.argv)
     print(path)

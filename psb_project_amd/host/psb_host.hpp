@@ -1,0 +1,403 @@
+// psb_host.hpp -- C++ host API over the C-ABI (include/psb_sph.h) that keeps
+// the reference simulator's class interface for the SPH / PCISPH path:
+//
+//   SimulationData_SPHSystem / _PCI_SPHSystem  materials, with the
+//        Fluid_Material XML load / save (SimulationData_SPHSystem.cpp:115-160)
+//   SPHParticle / PCI_SPHParticle             particles (SphParticle.h:8-67)
+//   BoxCollision, ForceExt_Constante          collider and constant force
+//   EmitterBox                                lattice emitter (EmitterBox.cpp:29-47)
+//   System -> SPHSystem -> PCI_SPHSystem       init / update / collide /
+//        emitParticles / addForce / addObjectCollision / setDt (System.h:16-89)
+//
+// Differences that are deliberate: no GL (System() needs none), errors throw
+// std::runtime_error with psb_last_error() instead of being ignored, the host
+// particle mirror is refreshed on demand (getParticles / syncParticles)
+// instead of after every step, and the solver mode is selectable
+// (PSB_MODE_REFERENCE, the default, reproduces the reference's semantics).
+// Header only; link libpsb_sph.so.
+#pragma once
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "psb_sph.h"
+
+namespace psbhost {
+
+inline void check(psb_status s) {
+  if (s != PSB_OK) throw std::runtime_error(psb_last_error());
+}
+
+struct Vector3 {
+  float x = 0.f, y = 0.f, z = 0.f;
+  Vector3() = default;
+  Vector3(float a, float b, float c) : x(a), y(b), z(c) {}
+  Vector3 operator+(const Vector3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+  Vector3 operator-(const Vector3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+  Vector3 operator*(float s) const { return {x * s, y * s, z * s}; }
+};
+
+// ---------------------------------------------------------------------------
+// Materials
+// ---------------------------------------------------------------------------
+// Attributes of the first element named `tag` in a small XML file (the
+// Fluid_Material format: one element, its settings as attributes).
+inline bool read_xml_attributes(const char* filename, const std::string& tag,
+                                std::map<std::string, std::string>& out) {
+  std::ifstream f(filename);
+  if (!f) return false;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string s = ss.str();
+  const size_t b = s.find("<" + tag);
+  if (b == std::string::npos) return false;
+  const size_t e = s.find('>', b);
+  if (e == std::string::npos) return false;
+  size_t i = b + 1 + tag.size();
+  while (i < e) {
+    while (i < e && std::isspace((unsigned char)s[i])) ++i;
+    const size_t eq = s.find('=', i);
+    if (eq == std::string::npos || eq > e) break;
+    std::string key = s.substr(i, eq - i);
+    while (!key.empty() && std::isspace((unsigned char)key.back())) key.pop_back();
+    const size_t q0 = s.find_first_of("\"'", eq);
+    if (q0 == std::string::npos || q0 > e) break;
+    const size_t q1 = s.find(s[q0], q0 + 1);
+    if (q1 == std::string::npos) break;
+    out[key] = s.substr(q0 + 1, q1 - q0 - 1);
+    i = q1 + 1;
+  }
+  return true;
+}
+
+// SimulationData_SPHSystem (SimulationData_SPHSystem.h:10-62): water by default
+// (SimulationData_SPHSystem.cpp:5-16).
+class SimulationData_SPHSystem {
+ public:
+  SimulationData_SPHSystem() = default;
+  // derived quantities as SimulationData_SPHSystem.cpp:19-31
+  SimulationData_SPHSystem(float particleRadius, float mass, Vector3 color, float restDensity,
+                           float viscosity, float surfaceTension, float gasStiffness,
+                           float kernelParticles)
+      : particleMass(mass), color(color), restDensity(restDensity), viscosity(viscosity),
+        surfaceTension(surfaceTension), gasStiffness(gasStiffness),
+        kernelParticles(kernelParticles) {
+    (void)particleRadius;
+    threshold = std::sqrt(restDensity / kernelParticles);
+    supportRadius = std::pow((3 * mass * kernelParticles) / (4 * (float)M_PI * restDensity), 0.333f);
+    this->particleRadius = std::pow(3 * mass / (4 * (float)M_PI * restDensity), 0.333f);
+  }
+  virtual ~SimulationData_SPHSystem() = default;
+
+  bool loadConfiguration(const char* filename) {
+    std::map<std::string, std::string> a;
+    if (!read_xml_attributes(filename, "Fluid_Material", a)) return false;
+    auto get = [&](const char* k, float& v) {
+      auto it = a.find(k);
+      if (it != a.end()) v = std::stof(it->second);
+    };
+    get("visualizationRadius", particleRadius);
+    get("particle_mass", particleMass);
+    get("restDensity", restDensity);
+    get("viscosity", viscosity);
+    get("surfaceTension", surfaceTension);
+    get("threshold", threshold);
+    get("gasStiffness", gasStiffness);
+    get("kernelParticles", kernelParticles);
+    get("supportRadius", supportRadius);
+    return true;
+  }
+  bool saveConfiguration(const char* filename) const {
+    std::ofstream f(filename);
+    if (!f) return false;
+    f.precision(9);
+    f << "<?xml version=\"1.0\" ?>\n<Fluid_Material name=\"Fluid_Material\""
+      << " visualizationRadius=\"" << particleRadius << "\" particle_mass=\"" << particleMass
+      << "\" restDensity=\"" << restDensity << "\" viscosity=\"" << viscosity
+      << "\" surfaceTension=\"" << surfaceTension << "\" threshold=\"" << threshold
+      << "\" gasStiffness=\"" << gasStiffness << "\" kernelParticles=\"" << kernelParticles
+      << "\" supportRadius=\"" << supportRadius << "\" />\n";
+    return (bool)f;
+  }
+
+  float getParticleRadius() const { return particleRadius; }
+  float getParticleMass() const { return particleMass; }
+  float getRestDensity() const { return restDensity; }
+  float getViscosity() const { return viscosity; }
+  float getSurfaceTension() const { return surfaceTension; }
+  float getThreshold() const { return threshold; }
+  float getGasStiffness() const { return gasStiffness; }
+  float getKernelParticles() const { return kernelParticles; }
+  float getSupportRadius() const { return supportRadius; }
+  void setParticleRadius(float v) { particleRadius = v; }
+  void setParticleMass(float v) { particleMass = v; }
+  void setRestDensity(float v) { restDensity = v; }
+  void setViscosity(float v) { viscosity = v; }
+  void setSurfaceTension(float v) { surfaceTension = v; }
+  void setThreshold(float v) { threshold = v; }
+  void setGasStiffness(float v) { gasStiffness = v; }
+  void setKernelParticles(float v) { kernelParticles = v; }
+  void setSupportRadius(float v) { supportRadius = v; }
+
+ protected:
+  float particleRadius = 0.0457f / 2, particleMass = 0.02f;
+  Vector3 color{0, 0, 1};
+  float restDensity = 998.29f, viscosity = 3.5f, surfaceTension = 0.0728f, threshold = 7.065f;
+  float gasStiffness = 3.f, kernelParticles = 20.f, supportRadius = 0.0457f;
+};
+
+// SimulationData_PCI_SPHSystem (SimulationData_PCI_SPHSystem.h:7-23): same data
+class SimulationData_PCI_SPHSystem : public SimulationData_SPHSystem {
+ public:
+  using SimulationData_SPHSystem::SimulationData_SPHSystem;
+};
+
+// ---------------------------------------------------------------------------
+// Particles, collider, force, emitter
+// ---------------------------------------------------------------------------
+class SPHParticle {
+ public:
+  SPHParticle(Vector3 pos, Vector3 vel, const SimulationData_SPHSystem* data)
+      : pos(pos), vel(vel), data(data) {}
+  virtual ~SPHParticle() = default;
+  Vector3 getPos() const { return pos; }
+  Vector3 getVel() const { return vel; }
+  Vector3 getVelInterAv() const { return velInterAv; }
+  Vector3 getVelInterAp() const { return velInterAp; }
+  float getDensity() const { return density; }
+  float getPressure() const { return pressure; }
+  float getMass() const { return data->getParticleMass(); }
+  float getInteractionRadius() const { return data->getSupportRadius(); }
+  float getRestDensity() const { return data->getRestDensity(); }
+  const SimulationData_SPHSystem* getData() const { return data; }
+  void setPos(Vector3 v) { pos = v; }
+  void setVel(Vector3 v) { vel = v; }
+  void setVelInterAv(Vector3 v) { velInterAv = v; }
+  void setVelInterAp(Vector3 v) { velInterAp = v; }
+  void setDensity(float v) { density = v; }
+  void setPressure(float v) { pressure = v; }
+
+ private:
+  Vector3 pos, vel, velInterAv, velInterAp;
+  float density = 0.f, pressure = 0.f;
+  const SimulationData_SPHSystem* data;
+};
+
+class PCI_SPHParticle : public SPHParticle {
+ public:
+  using SPHParticle::SPHParticle;
+};
+
+// BoxCollision::create(origin, length, width, depth, restitution, container)
+// (BoxCollision.cpp:22-30): the box centred at origin.
+class BoxCollision {
+ public:
+  BoxCollision(Vector3 origin, float length, float width, float depth, float elast,
+               bool container = true)
+      : min(origin - Vector3(length, width, depth) * 0.5f),
+        max(origin + Vector3(length, width, depth) * 0.5f), elast(elast), container(container) {}
+  static BoxCollision* fromBounds(Vector3 lo, Vector3 hi, float elast) {
+    return new BoxCollision((lo + hi) * 0.5f, hi.x - lo.x, hi.y - lo.y, hi.z - lo.z, elast);
+  }
+  Vector3 getMin() const { return min; }
+  Vector3 getMax() const { return max; }
+  float getElast() const { return elast; }
+  bool getContainer() const { return container; }
+
+ private:
+  Vector3 min, max;
+  float elast;
+  bool container;
+};
+
+// ForceExt_Constante(direction, amplitude) (ForceExt_Constante.cu:15-24)
+class ForceExt_Constante {
+ public:
+  ForceExt_Constante(Vector3 direction, float amplitude) : direction(direction), amplitude(amplitude) {}
+  Vector3 getDirection() const { return direction; }
+  float getAmplitude() const { return amplitude; }
+
+ private:
+  Vector3 direction;
+  float amplitude;
+};
+
+// EmitterBox::emitParticles (EmitterBox.cpp:29-47): a lattice of spacing
+// 2 particleRadius from center - size/2 while <= center + size/2, z outer,
+// x inner, the same float stepping as the reference.
+class EmitterBox {
+ public:
+  EmitterBox(Vector3 center, float sizeX, float sizeY, float sizeZ,
+             const SimulationData_SPHSystem* data, Vector3 worldPosition = {})
+      : center(center), sizeX(sizeX), sizeY(sizeY), sizeZ(sizeZ), data(data),
+        worldPosition(worldPosition) {}
+  std::vector<SPHParticle*> emitParticles() const {
+    std::vector<SPHParticle*> out;
+    const float step = data->getParticleRadius() * 2;
+    for (float z = center.z - sizeZ / 2; z <= center.z + sizeZ / 2; z += step)
+      for (float y = center.y - sizeY / 2; y <= center.y + sizeY / 2; y += step)
+        for (float x = center.x - sizeX / 2; x <= center.x + sizeX / 2; x += step)
+          out.push_back(new SPHParticle(Vector3(x, y, z) + worldPosition, Vector3(), data));
+    return out;
+  }
+
+ private:
+  Vector3 center;
+  float sizeX, sizeY, sizeZ;
+  const SimulationData_SPHSystem* data;
+  Vector3 worldPosition;
+};
+
+// ---------------------------------------------------------------------------
+// Systems
+// ---------------------------------------------------------------------------
+class System {
+ public:
+  virtual ~System() = default;
+  virtual void init(std::vector<SPHParticle*> particles) = 0;
+  virtual void update() = 0;
+  virtual void emitParticles() {}
+  virtual void collide() {}   // fused into update()
+  std::vector<SPHParticle*> getParticles() { syncParticles(); return particles; }
+  void addObjectCollision(BoxCollision* b) { box.reset(b); }
+  void addForce(ForceExt_Constante* f) { forces.emplace_back(f); }
+  int getNumParticles() const { return (int)particles.size(); }
+  double getDt() const { return dt; }
+  void setDt(double v) { dt = v; }
+  virtual void syncParticles() {}
+
+ protected:
+  std::vector<SPHParticle*> particles;
+  std::unique_ptr<BoxCollision> box;
+  std::vector<std::unique_ptr<ForceExt_Constante>> forces;
+  double dt = 0.01;   // SphSystem.cpp:9
+};
+
+// SPHSystem (SphSystem.h): WCSPH on the GPU through psb_system.
+class SPHSystem : public System {
+ public:
+  explicit SPHSystem(int device = 0) {
+    check(psb_init(device, &ctx));
+    check(psb_default_params(&prm));
+  }
+  ~SPHSystem() override {
+    if (sys) psb_system_destroy(sys);
+    if (ctx) psb_finalize(ctx);
+    for (auto* p : particles) delete p;
+  }
+  SPHSystem(const SPHSystem&) = delete;
+  SPHSystem& operator=(const SPHSystem&) = delete;
+
+  // Solver settings beyond the reference's fixed ones (mode, tolerance ...):
+  // edit before init().
+  psb_sph_params& params() { return prm; }
+
+  void init(std::vector<SPHParticle*> ps) override {
+    particles = std::move(ps);
+    if (particles.empty()) throw std::runtime_error("SPHSystem::init without particles");
+    const SimulationData_SPHSystem* d = particles[0]->getData();
+    prm.solver = solver();
+    prm.mass = d->getParticleMass();
+    prm.support_radius = d->getSupportRadius();
+    prm.rest_density = d->getRestDensity();
+    prm.gas_stiffness = d->getGasStiffness();
+    prm.viscosity = d->getViscosity();
+    prm.surface_tension = d->getSurfaceTension();
+    prm.threshold = d->getThreshold();
+    prm.particle_radius = d->getParticleRadius();
+    prm.dt = (float)dt;
+    Vector3 g;
+    for (auto& f : forces) g = g + f->getDirection() * f->getAmplitude();
+    prm.gravity[0] = g.x; prm.gravity[1] = g.y; prm.gravity[2] = g.z;
+    if (box) {
+      prm.box_enabled = 1;
+      const Vector3 lo = box->getMin(), hi = box->getMax();
+      prm.box_min[0] = lo.x; prm.box_min[1] = lo.y; prm.box_min[2] = lo.z;
+      prm.box_max[0] = hi.x; prm.box_max[1] = hi.y; prm.box_max[2] = hi.z;
+      prm.elast = box->getElast();
+      // grid over the box, cell = lattice spacing when set, else h
+      const float cell = prm.lattice_spacing > 0.f ? prm.lattice_spacing : prm.support_radius;
+      check(psb_create_grid_desc_box(lo, hi, cell));
+    }
+    const uint32_t n = (uint32_t)particles.size();
+    if (sys) { psb_system_destroy(sys); sys = nullptr; }
+    check(psb_system_create(ctx, &prm, n, &sys));
+    std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+      const Vector3 p = particles[i]->getPos(), u = particles[i]->getVel();
+      x[3 * i] = p.x; x[3 * i + 1] = p.y; x[3 * i + 2] = p.z;
+      v[3 * i] = u.x; v[3 * i + 1] = u.y; v[3 * i + 2] = u.z;
+    }
+    check(psb_system_set_particles(sys, n, x.data(), x.data(), v.data(), v.data(), v.data(),
+                                   v.data()));
+    dirty = false;
+  }
+
+  void update() override {
+    if (!sys) throw std::runtime_error("SPHSystem::update before init");
+    check(psb_system_update(sys, 1));
+    dirty = true;
+  }
+
+  // host mirror of positions, velocities, density and pressure, by particle id
+  void syncParticles() override {
+    if (!dirty || !sys) return;
+    const uint32_t n = psb_system_count(sys);
+    std::vector<float> x(4 * (size_t)n), v(4 * (size_t)n), vh(4 * (size_t)n), rho(n), p(n);
+    std::vector<uint32_t> ids(n);
+    check(psb_system_get_particles_f32(sys, x.data(), v.data(), vh.data(), rho.data(), p.data(),
+                                       ids.data()));
+    for (uint32_t k = 0; k < n; ++k) {
+      if (ids[k] >= particles.size()) continue;
+      SPHParticle* q = particles[ids[k]];
+      q->setPos(Vector3(x[4 * k], x[4 * k + 1], x[4 * k + 2]));
+      q->setVel(Vector3(v[4 * k], v[4 * k + 1], v[4 * k + 2]));
+      q->setVelInterAp(Vector3(vh[4 * k], vh[4 * k + 1], vh[4 * k + 2]));
+      q->setDensity(rho[k]);
+      q->setPressure(p[k]);
+    }
+    dirty = false;
+  }
+
+  psb_step_stats stats() const {
+    psb_step_stats s{};
+    check(psb_system_get_stats(sys, &s));
+    return s;
+  }
+  psb_system handle() const { return sys; }
+
+ protected:
+  virtual int32_t solver() const { return PSB_SOLVER_WCSPH; }
+  psb_status psb_create_grid_desc_box(Vector3 lo, Vector3 hi, float cell) {
+    prm.grid.min[0] = lo.x; prm.grid.min[1] = lo.y; prm.grid.min[2] = lo.z;
+    prm.grid.cell_size = cell;
+    prm.grid.dims[0] = (int32_t)std::ceil((hi.x - lo.x) / cell - 1e-6f);
+    prm.grid.dims[1] = (int32_t)std::ceil((hi.y - lo.y) / cell - 1e-6f);
+    prm.grid.dims[2] = (int32_t)std::ceil((hi.z - lo.z) / cell - 1e-6f);
+    return PSB_OK;
+  }
+
+  psb_context ctx = nullptr;
+  psb_system sys = nullptr;
+  psb_sph_params prm{};
+  bool dirty = false;
+};
+
+// PCI_SPHSystem (PciSphSystem.h): PCISPH on the GPU through psb_system.
+class PCI_SPHSystem : public SPHSystem {
+ public:
+  using SPHSystem::SPHSystem;
+
+ protected:
+  int32_t solver() const override { return PSB_SOLVER_PCISPH; }
+};
+
+}  // namespace psbhost
