@@ -397,6 +397,12 @@ typedef struct psb_system_arrays {
 } psb_system_arrays;
 psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
 
+/* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
+ * sums split into even and odd list positions, one lane each), or 0 (the
+ * default, 1).  Results do not depend on it: the sums always run in the
+ * even + odd order.                                                        */
+psb_status psb_system_set_corrector_lanes(psb_system sys, int32_t lanes);
+
 /* Per-phase timing with HIP events on the system stream (off by default):
  * `every` = 0 disables it, k > 0 brackets every launch of every k-th step
  * (counted over the system's lifetime) with a pair of events, so k > 1
@@ -547,11 +553,18 @@ psb_status psb_batch_get_particles(psb_batch b, uint32_t scene, float* pos4, flo
                                    uint32_t* ids);
 /* Learned warm start (material warm_start = PSB_WARM_MLP).                 */
 psb_status psb_batch_set_mlp(psb_batch b, psb_mlp m);
-/* Time every phase at 64, 128, 256 and 512 threads per workgroup on
- * `steps` steps from the current state each (state restored afterwards) and
- * keep the fastest per phase; block_sizes[PSB_BATCH_PHASES] receives the
- * choice (may be NULL).                                                     */
+/* Time every phase at 64, 128, 256 and 512 threads per workgroup, and the
+ * corrector passes (pci_density, pci_force) also at 128, 256 and 512 threads
+ * with two threads per particle, on `steps` steps from the current state each
+ * (state restored afterwards) and keep the fastest per phase;
+ * block_sizes[PSB_BATCH_PHASES] receives the choice (may be NULL): threads
+ * per workgroup, negated for two threads per particle.  Results do not depend
+ * on the choice.                                                            */
 psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes);
+/* Set the launch shapes directly, in the form psb_batch_autotune returns
+ * (e.g. a stored tuning): 64, 128, 256 or 512 per phase; -128, -256 or -512
+ * for two threads per particle in pci_density and pci_force.               */
+psb_status psb_batch_set_block_sizes(psb_batch b, const int32_t* block_sizes);
 /* Replay each step from HIP graphs: the step prologue (grid, neighbours,
  * non-pressure forces, first min_iterations corrector iterations) and one
  * graph per later iteration, captured on first use.                        */

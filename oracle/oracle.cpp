@@ -937,25 +937,31 @@ template <class R> struct System {
         continue;
       }
       const V4<R> xi = X[p];
-      R S = 0, W1x = 0, W1y = 0, W1z = 0, W2 = 0;
+      // two partial sums, over even and over odd list positions, added as
+      // even + odd (the canonical order of the corrector passes)
+      R S2[2] = {0, 0}, W1x2[2] = {0, 0}, W1y2[2] = {0, 0}, W1z2[2] = {0, 0}, W22[2] = {0, 0};
       for (uint32_t k = 0; k < count[p]; ++k) {
+        const int a = (int)(k & 1u);
         uint32_t q = list[(uint64_t)k * n + p];
         const V4<R> xj = X[q];
         R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         R d2 = dot3(dx, dy, dz, dx, dy, dz);
         R w = d.h2 - d2;
         if (!ref) w = std::max(w, (R)0);
-        S = fmaR(w * w, w, S);
+        S2[a] = fmaR(w * w, w, S2[a]);
         if (ref && d2 > (R)0) {
           R dd = std::sqrt(d2);
           R invd = (R)1 / dd;
           R hd = d.h - dd;
           R gw = d.c_spiky * (hd * hd);
           R ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
-          W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-          W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+          W1x2[a] = W1x2[a] + ux; W1y2[a] = W1y2[a] + uy; W1z2[a] = W1z2[a] + uz;
+          W22[a] = W22[a] + dot3(ux, uy, uz, ux, uy, uz);
         }
       }
+      const R S = S2[0] + S2[1];
+      const R W1x = W1x2[0] + W1x2[1], W1y = W1y2[0] + W1y2[1], W1z = W1z2[0] + W1z2[1];
+      const R W2 = W22[0] + W22[1];
       R r = S * d.mc;
       if (!ref && d.wall) {
         R Sw, gx_, gy_, gz_;
@@ -998,8 +1004,9 @@ template <class R> struct System {
         if (!fin) Xn[p] = {xi.x, xi.y, xi.z, 0};
         continue;
       }
-      R sx = 0, sy = 0, sz = 0;
+      R sx2[2] = {0, 0}, sy2[2] = {0, 0}, sz2[2] = {0, 0};   // even / odd list positions
       for (uint32_t k = 0; k < count[p]; ++k) {
+        const int a = (int)(k & 1u);
         uint32_t q = list[(uint64_t)k * n + p];
         const V4<R> xj = X[q];
         R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1011,7 +1018,8 @@ template <class R> struct System {
             R hd = d.h - dd;
             R gw = d.c_spiky * (hd * hd);
             R c = gw * (xi.w + xj.w);
-            sx = fmaR(dx * invd, c, sx); sy = fmaR(dy * invd, c, sy); sz = fmaR(dz * invd, c, sz);
+            sx2[a] = fmaR(dx * invd, c, sx2[a]); sy2[a] = fmaR(dy * invd, c, sy2[a]);
+            sz2[a] = fmaR(dz * invd, c, sz2[a]);
           }
         } else if (d2 > (R)0 && d2 < d.h2) {
           R dd = std::sqrt(d2);
@@ -1019,9 +1027,10 @@ template <class R> struct System {
           R hd = d.h - dd;
           R gw = d.c_spiky * (hd * hd);
           R c = (gw * invd) * (xi.w + xj.w);
-          sx = fmaR(dx, c, sx); sy = fmaR(dy, c, sy); sz = fmaR(dz, c, sz);
+          sx2[a] = fmaR(dx, c, sx2[a]); sy2[a] = fmaR(dy, c, sy2[a]); sz2[a] = fmaR(dz, c, sz2[a]);
         }
       }
+      R sx = sx2[0] + sx2[1], sy = sy2[0] + sy2[1], sz = sz2[0] + sz2[1];
       if (!ref && d.wall) {   // wall particles push with p_i only (p_b = 0)
         R Sw, gx_, gy_, gz_;
         wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, gx_, gy_, gz_);

@@ -221,6 +221,7 @@ SIGNATURES = {
     "psb_system_get_particles_f32": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "psb_system_get_arrays": (i32, [vp, C.POINTER(SystemArrays)]),
     "psb_system_set_timing": (i32, [vp, i32]),
+    "psb_system_set_corrector_lanes": (i32, [vp, i32]),
     "psb_system_get_timing": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_int64)]),
     "psb_system_reset_timing": (i32, [vp]),
     "psb_mlp_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
@@ -255,6 +256,7 @@ SIGNATURES = {
     "psb_batch_get_particles": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
     "psb_batch_set_mlp": (i32, [vp, vp]),
     "psb_batch_autotune": (i32, [vp, i32, C.POINTER(C.c_int32)]),
+    "psb_batch_set_block_sizes": (i32, [vp, C.POINTER(C.c_int32)]),
     "psb_batch_set_graphs": (i32, [vp, i32]),
     "psb_batch_set_latency": (i32, [vp, i32]),
     "psb_batch_get_latency": (i32, [vp, dp, dp, dp, C.POINTER(C.c_uint64)]),

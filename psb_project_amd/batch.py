@@ -77,6 +77,12 @@ class BatchSystem:
         check(lib().psb_batch_autotune(self.handle, int(steps), out))
         return dict(zip(BATCH_PHASES, [int(x) for x in out]))
 
+    def set_block_sizes(self, sizes: dict) -> None:
+        """Launch shapes per phase, as autotune returns them (negative: two
+        threads per particle in pci_density / pci_force)."""
+        arr = (C.c_int32 * len(BATCH_PHASES))(*[int(sizes[p]) for p in BATCH_PHASES])
+        check(lib().psb_batch_set_block_sizes(self.handle, arr))
+
     def set_graphs(self, on: bool) -> None:
         check(lib().psb_batch_set_graphs(self.handle, 1 if on else 0))
 

@@ -49,6 +49,7 @@ struct psb_batch_s {
   uint16_t* dblk[kBatchBlockSizes] = {};
   uint32_t nblk[kBatchBlockSizes] = {};
   int bsel[BP_COUNT] = {2, 2, 2, 2, 2, 2};   // 256 threads until tuned
+  int lanes[BP_COUNT] = {1, 1, 1, 1, 1, 1};  // threads per particle (D and F: 1 or 2)
   psb_grid grid = nullptr;
   float4 *ax = nullptr, *av = nullptr, *avh = nullptr, *bx = nullptr, *bv = nullptr, *bvh = nullptr;
   uint32_t *aid = nullptr, *bid = nullptr;
@@ -93,14 +94,17 @@ struct psb_batch_s {
 
 namespace psb {
 
+// block size 64 << bsel threads; particles per block (64 << bsel) / lanes,
+// whose block table has index bsel - (lanes - 1)
+static int btable(psb_batch b, int phase) { return b->bsel[phase] - (b->lanes[phase] - 1); }
+
 static BatchMap bmap(psb_batch b, int phase) {
-  const int s = b->bsel[phase];
-  return BatchMap{b->dsc, b->dblk[s], s};
+  const int t = btable(b, phase);
+  return BatchMap{b->dsc, b->dblk[t], t};
 }
 
 static Shape bshape(psb_batch b, int phase) {
-  const int s = b->bsel[phase];
-  return Shape{dim3(b->nblk[s]), dim3(64u << s)};
+  return Shape{dim3(b->nblk[btable(b, phase)]), dim3(64u << b->bsel[phase])};
 }
 
 struct BTimed {
@@ -172,14 +176,14 @@ static psb_status enqueue_iter(psb_batch b, int k) {
     const int dflags = (k == 0 && mlp) ? (kDensWriteFeature | kDensSkipUpdate) : 0;
     PSB_TRY(launch_pci_density_batch(bmap(b, BP_DENSITY), bshape(b, BP_DENSITY), b->n, b->c,
                                      nb.count, nb.list, cur, b->pres, b->rho_pred, b->dl, b->st,
-                                     k, b->feat, dflags, st));
+                                     k, b->feat, dflags, st, b->lanes[BP_DENSITY]));
   }
   if (k == 0 && mlp) PSB_TRY(mlp_infer(b->mlp, b->feat, b->n, b->pres, cur, 1, b->c.pscale, st));
   {
     BTimed t(b, BP_FORCE);
     PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
                                    nb.list, cur, nxt, b->rho_pred, b->pres, pb, b->st, k, 0, o,
-                                   st));
+                                   st, b->lanes[BP_FORCE]));
   }
   return PSB_OK;
 }
@@ -226,7 +230,7 @@ static psb_status enqueue_prologue(psb_batch b) {
     BTimed t(b, BP_FORCE);
     PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
                                    nb.list, b->xp0, b->xp1, b->rho_pred, b->pres, pb, b->st, -1, 1,
-                                   o, st));
+                                   o, st, b->lanes[BP_FORCE]));
   }
   for (int k = 0; k < b->c.min_it; ++k) PSB_TRY(enqueue_iter(b, k));
   return PSB_OK;
@@ -606,6 +610,25 @@ psb_status psb_batch_get_particles(psb_batch b, uint32_t scene, float* pos4, flo
   return PSB_OK;
 }
 
+psb_status psb_batch_set_block_sizes(psb_batch b, const int32_t* block_sizes) {
+  PSB_ARG(b && block_sizes, "null argument");
+  int sel[BP_COUNT], ln[BP_COUNT];
+  for (int p = 0; p < BP_COUNT; ++p) {
+    const int32_t v = block_sizes[p];
+    ln[p] = v < 0 ? 2 : 1;
+    const int32_t t = v < 0 ? -v : v;
+    sel[p] = -1;
+    for (int k = 0; k < kBatchBlockSizes; ++k)
+      if (t == (64 << k)) sel[p] = k;
+    PSB_ARG(sel[p] >= 0, "block sizes must be 64, 128, 256 or 512");
+    PSB_ARG(ln[p] == 1 || ((p == BP_DENSITY || p == BP_FORCE) && sel[p] >= 1),
+            "two threads per particle only for pci_density / pci_force at 128..512 threads");
+  }
+  for (int p = 0; p < BP_COUNT; ++p) { b->bsel[p] = sel[p]; b->lanes[p] = ln[p]; }
+  drop_graphs(b);
+  return PSB_OK;
+}
+
 psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) {
   PSB_ARG(b && steps >= 1, "bad argument");
   for (uint32_t s = 0; s < b->S; ++s)
@@ -635,11 +658,19 @@ psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) 
   b->use_graphs = false;
   b->lat_on = false;
   drop_graphs(b);
+  // candidates (threads per particle, block size index): every block size
+  // with one thread per particle, and 128..512 threads with two for the
+  // corrector passes (other phases run the one-thread variant again)
+  const int kCand = kBatchBlockSizes + kBatchBlockSizes - 1;
+  auto cand_lanes = [&](int cand, int p) {
+    return (cand >= kBatchBlockSizes && (p == BP_DENSITY || p == BP_FORCE)) ? 2 : 1;
+  };
+  auto cand_bsel = [&](int cand) { return cand < kBatchBlockSizes ? cand : cand - kBatchBlockSizes + 1; };
   double best[BP_COUNT];
   int pick[BP_COUNT];
   for (int p = 0; p < BP_COUNT; ++p) { best[p] = 1e300; pick[p] = 2; }
-  for (int cand = 0; cand < kBatchBlockSizes && r == PSB_OK; ++cand) {
-    for (int p = 0; p < BP_COUNT; ++p) b->bsel[p] = cand;
+  for (int cand = 0; cand < kCand && r == PSB_OK; ++cand) {
+    for (int p = 0; p < BP_COUNT; ++p) { b->bsel[p] = cand_bsel(cand); b->lanes[p] = cand_lanes(cand, p); }
     r = snap(true);
     if (r != PSB_OK) break;
     std::memset(b->t_phase, 0, sizeof(b->t_phase));
@@ -653,7 +684,7 @@ psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) 
   b->tuning = false;
   b->trecs.clear();
   b->tev_used = 0;
-  for (int p = 0; p < BP_COUNT; ++p) b->bsel[p] = pick[p];
+  for (int p = 0; p < BP_COUNT; ++p) { b->bsel[p] = cand_bsel(pick[p]); b->lanes[p] = cand_lanes(pick[p], p); }
   if (r == PSB_OK) r = snap(true);
   if (r == PSB_OK) {
     hipError_t e = hipStreamSynchronize(st);
@@ -663,8 +694,9 @@ psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) 
   if (r == PSB_OK) r = launch_pci_reset(b->st, true, st, b->S, b->done);
   b->use_graphs = graphs;
   b->lat_on = lat;
-  if (block_sizes)
-    for (int p = 0; p < BP_COUNT; ++p) block_sizes[p] = 64 << b->bsel[p];
+  if (block_sizes)   // threads per workgroup; negative: two threads per particle
+    for (int p = 0; p < BP_COUNT; ++p)
+      block_sizes[p] = (b->lanes[p] == 2 ? -1 : 1) * (64 << b->bsel[p]);
   return r;
 }
 

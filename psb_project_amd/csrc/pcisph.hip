@@ -234,8 +234,11 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
   const bool valid = map.locate(n, p, sc, lead);
   map.patch(c, sc);
   float err = 0.f;
+  constexpr int L = M::kLanes;
+  const uint32_t lane = L == 1 ? 0u : (threadIdx.x & 1u);
+  const bool writer = lane == 0;   // the pair's other lane only sums
   const bool bnd = !REF && valid && is_boundary(c, ids, p);
-  if (bnd) {
+  if (bnd && writer) {
     // rigid-sphere boundary particle: no pressure, no error
     reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = 0.f;
     pres[p] = 0.f;
@@ -244,23 +247,30 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
   }
   if (valid && !bnd) {
     const float4 xi = X[p];
-    float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
-    for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
+    float S2[2] = {0.f, 0.f}, W1x2[2] = {0.f, 0.f}, W1y2[2] = {0.f, 0.f}, W1z2[2] = {0.f, 0.f};
+    float W22[2] = {0.f, 0.f};
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, list, X, [&](int a, const float4& xj) {
       const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const float d2 = dot3(dx, dy, dz, dx, dy, dz);
       float w = c.h2 - d2;
       if (!REF) w = fmaxf(w, 0.f);
-      S = __builtin_fmaf(w * w, w, S);
+      S2[a] = __builtin_fmaf(w * w, w, S2[a]);
       if (REF && d2 > 0.f) {
         const float dd = sqrtf(d2);
         const float invd = 1.0f / dd;
         const float hd = c.h - dd;
         const float gw = c.c_spiky * (hd * hd);
         const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
-        W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-        W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        W1x2[a] = W1x2[a] + ux; W1y2[a] = W1y2[a] + uy; W1z2[a] = W1z2[a] + uz;
+        W22[a] = W22[a] + dot3(ux, uy, uz, ux, uy, uz);
       }
     });
+    const float S = split_total<L>(S2, lane);
+    float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
+    if (REF) {
+      W1x = split_total<L>(W1x2, lane); W1y = split_total<L>(W1y2, lane);
+      W1z = split_total<L>(W1z2, lane); W2 = split_total<L>(W22, lane);
+    }
     float r = S * c.mc;
     if (!REF && c.wall) {
       float Sw, gx, gy, gz;
@@ -271,17 +281,19 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     const float e = r - c.rho0;
     if (REF) {
       if (r > c.rho0 && W2 > 0.f) pr = pr + e / (c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
-      reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr / (r * r);
+      if (writer) reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr / (r * r);
     } else {
-      if (dflags & kDensWriteFeature) feat[p] = (e * c.inv_rho0) * 100.0f;
+      if ((dflags & kDensWriteFeature) && writer) feat[p] = (e * c.inv_rho0) * 100.0f;
       if (!(dflags & kDensSkipUpdate)) {
         pr = pr + dl[p] * e;
         if (pr < 0.f) pr = 0.f;
       }
-      reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr;
+      if (writer) reinterpret_cast<float*>(X)[4 * (size_t)p + 3] = pr;
     }
-    rho_pred[p] = r;
-    pres[p] = pr;
+    if (writer) {
+      rho_pred[p] = r;
+      pres[p] = pr;
+    }
     err = (e > 0.f ? e : 0.f) * c.inv_rho0;
   }
   err = wave_max<64>(err);
@@ -334,14 +346,16 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   }
   if (!valid) return;
   const float4 xi = X[p];
+  constexpr int L = M::kLanes;
+  const uint32_t lane = L == 1 ? 0u : (threadIdx.x & 1u);
   if (!REF && is_boundary(c, b.aid, p)) {
     // rigid-sphere boundary particle: static during the corrector; its
     // end-of-step state is written by the sphere update
-    if (!fin) Xn[p] = make_float4(xi.x, xi.y, xi.z, 0.f);
+    if (!fin && lane == 0) Xn[p] = make_float4(xi.x, xi.y, xi.z, 0.f);
     return;
   }
-  float sx = 0.f, sy = 0.f, sz = 0.f;
-  for_neighbors<kUnroll>(p, n, cnt[p], list, X, [&](uint32_t, const float4& xj) {
+  float sx2[2] = {0.f, 0.f}, sy2[2] = {0.f, 0.f}, sz2[2] = {0.f, 0.f};
+  for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, list, X, [&](int a, const float4& xj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (REF) {
@@ -351,9 +365,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
         const float hd = c.h - dd;
         const float gw = c.c_spiky * (hd * hd);
         const float cc = gw * (xi.w + xj.w);
-        sx = __builtin_fmaf(dx * invd, cc, sx);
-        sy = __builtin_fmaf(dy * invd, cc, sy);
-        sz = __builtin_fmaf(dz * invd, cc, sz);
+        sx2[a] = __builtin_fmaf(dx * invd, cc, sx2[a]);
+        sy2[a] = __builtin_fmaf(dy * invd, cc, sy2[a]);
+        sz2[a] = __builtin_fmaf(dz * invd, cc, sz2[a]);
       }
     } else if (d2 > 0.f && d2 < c.h2) {
       const float dd = sqrtf(d2);
@@ -361,11 +375,14 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       const float hd = c.h - dd;
       const float gw = c.c_spiky * (hd * hd);
       const float cc = (gw * invd) * (xi.w + xj.w);
-      sx = __builtin_fmaf(dx, cc, sx);
-      sy = __builtin_fmaf(dy, cc, sy);
-      sz = __builtin_fmaf(dz, cc, sz);
+      sx2[a] = __builtin_fmaf(dx, cc, sx2[a]);
+      sy2[a] = __builtin_fmaf(dy, cc, sy2[a]);
+      sz2[a] = __builtin_fmaf(dz, cc, sz2[a]);
     }
   });
+  float sx = split_total<L>(sx2, lane), sy = split_total<L>(sy2, lane),
+        sz = split_total<L>(sz2, lane);
+  if (lane != 0) return;   // the pair's second lane only summed
   if (!REF && c.wall) {
     // wall particles push with the particle's own pressure only (p_b = 0:
     // F = -m psi_b p_i / rho0^2 gradW, psi_b = m on the fluid lattice)
@@ -498,7 +515,9 @@ static psb_status force_impl(const M& map, const Shape& sh, uint32_t n, const Co
   return PSB_OK;
 }
 
-static Shape single_shape(uint32_t n) { return Shape{dim3(blocks_for(n)), dim3(kBlock)}; }
+static Shape single_shape(uint32_t n, int lanes = 1) {
+  return Shape{dim3(blocks_for((uint64_t)n * lanes)), dim3(kBlock)};
+}
 
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
@@ -512,7 +531,10 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
-                              hipStream_t s, const uint32_t* ids) {
+                              hipStream_t s, const uint32_t* ids, int lanes) {
+  if (lanes == 2)
+    return density_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, cnt, list, X, pres, rho_pred,
+                        dl, st, k, feat, dflags, ids, s);
   return density_impl(SingleMap{}, single_shape(n), n, c, cnt, list, X, pres, rho_pred, dl, st, k,
                       feat, dflags, ids, s);
 }
@@ -520,7 +542,11 @@ psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
-                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s) {
+                            PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s,
+                            int lanes) {
+  if (lanes == 2)
+    return force_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, step, cnt, list, X, Xn, rho_pred,
+                      pres, b, st, k, warm_pass, o, s);
   return force_impl(SingleMap{}, single_shape(n), n, c, step, cnt, list, X, Xn, rho_pred, pres, b,
                     st, k, warm_pass, o, s);
 }
@@ -539,7 +565,10 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags,
-                                    hipStream_t s) {
+                                    hipStream_t s, int lanes) {
+  if (lanes == 2)
+    return density_impl(BatchMapT<2>{map.sc, map.blk_scene, map.bsel}, sh, n, c, cnt, list, X,
+                        pres, rho_pred, dl, st, k, feat, dflags, nullptr, s);
   return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags,
                       nullptr, s);
 }
@@ -548,7 +577,10 @@ psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t
                                   const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                   const float4* X, float4* Xn, const float* rho_pred,
                                   const float* pres, const PciBuffers& b, PciStatus* st, int k,
-                                  int warm_pass, const PciOut& o, hipStream_t s) {
+                                  int warm_pass, const PciOut& o, hipStream_t s, int lanes) {
+  if (lanes == 2)
+    return force_impl(BatchMapT<2>{map.sc, map.blk_scene, map.bsel}, sh, n, c, true, cnt, list, X,
+                      Xn, rho_pred, pres, b, st, k, warm_pass, o, s);
   return force_impl(map, sh, n, c, true, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o,
                     s);
 }

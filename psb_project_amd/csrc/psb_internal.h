@@ -125,10 +125,14 @@ struct BatchScene {
   uint32_t pad;
 };
 
-struct SingleMap {
+// LANES: threads per particle (1, or 2 for the corrector passes, whose
+// neighbour sums split into even and odd neighbours).
+template <int LANES = 1>
+struct SingleMapT {
   static constexpr int kMaxThreads = 256;
+  static constexpr int kLanes = LANES;
   __device__ __forceinline__ bool locate(uint32_t n, uint32_t& p, uint32_t& s, bool& lead) const {
-    p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    p = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LANES;
     s = 0;
     lead = blockIdx.x == 0 && threadIdx.x == 0;
     return p < n;
@@ -141,11 +145,15 @@ struct SingleMap {
   }
 };
 
-struct BatchMap {
+using SingleMap = SingleMapT<1>;
+
+template <int LANES = 1>
+struct BatchMapT {
   static constexpr int kMaxThreads = 512;
+  static constexpr int kLanes = LANES;
   const BatchScene* __restrict__ sc;
   const uint16_t* __restrict__ blk_scene;   // scene of each block, for this block size
-  int bsel;                                 // block size = 64 << bsel
+  int bsel;                                 // particles per block = 64 << bsel
   __device__ __forceinline__ uint32_t scene_of_block() const {
     return blk_scene[xcd_block(blockIdx.x, gridDim.x)];
   }
@@ -154,7 +162,7 @@ struct BatchMap {
     s = blk_scene[b];
     const BatchScene& S = sc[s];
     const uint32_t lb = b - S.blk0[bsel];
-    const uint32_t local = lb * blockDim.x + threadIdx.x;
+    const uint32_t local = (lb * blockDim.x + threadIdx.x) / LANES;
     p = S.p0 + local;
     lead = lb == 0 && threadIdx.x == 0;
     return local < S.n;
@@ -174,6 +182,8 @@ struct BatchMap {
     return sc[s].grid;
   }
 };
+
+using BatchMap = BatchMapT<1>;
 
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
   return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
@@ -338,6 +348,55 @@ __device__ __forceinline__ void for_neighbors2(uint32_t p, uint32_t n, uint32_t 
 }
 
 constexpr int kUnroll = 4;
+
+// Neighbour sums of the corrector passes in their canonical order: two
+// partial sums, over the even and over the odd list positions k, each in
+// increasing k, added as even + odd.  LANES = 1: one thread keeps both
+// (f(acc = k & 1, ...)); LANES = 2: lane l of the particle's pair keeps the
+// sum over k = l (mod 2) (f(acc = 0, ...)) and split_total adds the pair.
+template <int LANES, int U, class F>
+__device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint32_t m,
+                                                    uint32_t lane,
+                                                    const uint32_t* __restrict__ list,
+                                                    const float4* X, F&& f) {
+  static_assert(U % 2 == 0, "U even keeps k & 1 static");
+  if (LANES == 1) {
+    uint32_t k = 0;
+    for (; k + U <= m; k += U) {
+      uint32_t q[U];
+      float4 xq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + u) * n + p];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) f(u & 1, xq[u]);
+    }
+    for (; k < m; ++k) f((int)(k & 1u), X[list[(uint64_t)k * n + p]]);
+  } else {
+    uint32_t k = lane;
+    for (; k + 2 * (U - 1) < m; k += 2 * U) {
+      uint32_t q[U];
+      float4 xq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + 2 * u) * n + p];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) f(0, xq[u]);
+    }
+    for (; k < m; k += 2) f(0, X[list[(uint64_t)k * n + p]]);
+  }
+}
+
+// even + odd partial sums (see for_neighbors_split); both lanes of a pair
+// get the same value
+template <int LANES>
+__device__ __forceinline__ float split_total(const float (&a)[2], uint32_t lane) {
+  if (LANES == 1) return a[0] + a[1];
+  const float other = __shfl_xor(a[0], 1, 64);
+  return lane == 0 ? a[0] + other : other + a[0];
+}
 
 template <int N> __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll

@@ -94,6 +94,7 @@ struct psb_system_s {
   float4* d_soff = nullptr;
   float4* d_fb = nullptr;
   int sphere_cur = 0;
+  int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: by size)
 };
 
 namespace psb {
@@ -213,8 +214,17 @@ static psb_status wcsph_step(psb_system s, int step) {
   return PSB_OK;
 }
 
+// Threads per particle in the corrector passes.  One unless set: two lanes
+// double the waves (8 per SIMD at n = 262,144 on 256 CUs) but measured no
+// faster on the 262,144-particle dam break (pci_density 19.9 us against
+// 17.0 us, pci_force 18.7 us against 18.9 us).
+static int corrector_lanes(psb_system s) {
+  return s->corrector_lanes == 2 ? 2 : 1;
+}
+
 static psb_status pcisph_step(psb_system s, int step, int slot) {
   hipStream_t st = s->ctx->stream;
+  const int lanes = corrector_lanes(s);
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
   const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
@@ -250,7 +260,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   if (warm) {
     Timed t(s, PH_PCI_FORCE, step);
     PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
-                             b, s->st, -1, 1, o, st));
+                             b, s->st, -1, 1, o, st, lanes));
     std::swap(cur, nxt);
   }
   const int min_it = s->c.min_it, max_it = s->c.max_it;
@@ -261,7 +271,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st, s->aid));
+                                 s->st, k, s->feat, dflags, st, s->aid, lanes));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
@@ -273,7 +283,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     {
       Timed t(s, PH_PCI_FORCE, step, k);
       PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
-                               s->pres, b, s->st, k, 0, o, st));
+                               s->pres, b, s->st, k, 0, o, st, lanes));
     }
     std::swap(cur, nxt);
     if (ref) continue;
@@ -742,6 +752,12 @@ psb_status psb_system_get_spheres(psb_system s, float* centers3, float* velociti
       if (velocities3) velocities3[3 * j + a] = v[4 * j + a];
       if (fluid_forces3) fluid_forces3[3 * j + a] = f[4 * j + a];
     }
+  return PSB_OK;
+}
+
+psb_status psb_system_set_corrector_lanes(psb_system s, int32_t lanes) {
+  PSB_ARG(s && lanes >= 0 && lanes <= 2, "lanes must be 0 (by size), 1 or 2");
+  s->corrector_lanes = lanes;
   return PSB_OK;
 }
 

@@ -148,6 +148,10 @@ class FluidSystem:
             raise RuntimeError("no features computed (enable the MLP warm start or a recorder)")
         return self.download(ptr.value, np.float32, (8, self.n))
 
+    def set_corrector_lanes(self, lanes: int) -> None:
+        """Threads per particle in the corrector passes: 1, 2, or 0 (default, 1)."""
+        check(lib().psb_system_set_corrector_lanes(self.handle, int(lanes)))
+
     # -- timing ------------------------------------------------------------
     def set_timing(self, on, every: int = 1) -> None:
         """Per-kernel HIP events on every `every`-th step (off when `on` is false)."""

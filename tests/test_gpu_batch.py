@@ -83,11 +83,38 @@ def test_autotune_keeps_results(ctx):
     choice = b.autotune(steps=1)
     assert set(choice) == {"hash_count", "neighbors_density", "density_epilogue",
                            "pci_nonpressure", "pci_density", "pci_force"}
-    assert all(v in (64, 128, 256, 512) for v in choice.values())
+    assert all(abs(v) in (64, 128, 256, 512) for v in choice.values())
+    assert all(v > 0 for k, v in choice.items() if k not in ("pci_density", "pci_force"))
     a.update(8)
     b.update(8)
     for i in range(len(scs)):
         assert_same(b.state(i), a.state(i), f"scene {i}")
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dens,force", [(-128, -512), (-512, -256), (64, -128)])
+def test_two_threads_per_particle_keeps_results(ctx, dens, force):
+    """The corrector passes with two threads per particle (even and odd
+    neighbour positions, added in that order) equal the one-thread runs."""
+    scs = small_batch(WARM_NONE, False, n=4, seed=81)
+    a = BatchSystem(ctx, scs)
+    b = BatchSystem(ctx, scs)
+    a.set_all()
+    b.set_all()
+    sizes = dict.fromkeys(("hash_count", "neighbors_density", "density_epilogue",
+                           "pci_nonpressure"), 128)
+    a.set_block_sizes(dict(sizes, pci_density=256, pci_force=256))
+    b.set_block_sizes(dict(sizes, pci_density=dens, pci_force=force))
+    a.update(6)
+    b.update(6)
+    for i in range(len(scs)):
+        assert_same(b.state(i), a.state(i), f"scene {i}")
+        assert b.stats(i)["total_iterations"] == a.stats(i)["total_iterations"]
+    with pytest.raises(PsbError):
+        b.set_block_sizes(dict(sizes, pci_density=-64, pci_force=256))
+    with pytest.raises(PsbError):
+        b.set_block_sizes(dict(sizes, hash_count=-128, pci_density=256, pci_force=256))
     a.close()
     b.close()
 
