@@ -397,6 +397,13 @@ typedef struct psb_system_arrays {
 } psb_system_arrays;
 psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
 
+/* Compact neighbour list for the PCISPH corrector passes (on by default):
+ * the neighbour search also writes 16-bit offsets q - p, which the
+ * per-iteration density and pressure-force passes read instead of the
+ * 32-bit indices; a step where some offset exceeds 32767 falls back to the
+ * 32-bit list by itself.  Results do not depend on it.                     */
+psb_status psb_system_set_compact_list(psb_system sys, int32_t on);
+
 /* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
  * sums split into even and odd list positions, one lane each), or 0 (the
  * default, 1).  Results do not depend on it: the sums always run in the

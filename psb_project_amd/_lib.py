@@ -222,6 +222,7 @@ SIGNATURES = {
     "psb_system_get_arrays": (i32, [vp, C.POINTER(SystemArrays)]),
     "psb_system_set_timing": (i32, [vp, i32]),
     "psb_system_set_corrector_lanes": (i32, [vp, i32]),
+    "psb_system_set_compact_list": (i32, [vp, i32]),
     "psb_system_get_timing": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_int64)]),
     "psb_system_reset_timing": (i32, [vp]),
     "psb_mlp_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),

@@ -59,6 +59,7 @@ struct psb_batch_s {
   float* feat = nullptr;
   uint32_t* ncount = nullptr;
   uint32_t* nlist = nullptr;
+  uint16_t* nlist16 = nullptr;    // compact twin of nlist for the corrector passes
   uint32_t* flags = nullptr;          // [0] flags, [1] max neighbours
   PciStatus* st = nullptr;            // [S]
   uint32_t* done = nullptr;
@@ -176,14 +177,14 @@ static psb_status enqueue_iter(psb_batch b, int k) {
     const int dflags = (k == 0 && mlp) ? (kDensWriteFeature | kDensSkipUpdate) : 0;
     PSB_TRY(launch_pci_density_batch(bmap(b, BP_DENSITY), bshape(b, BP_DENSITY), b->n, b->c,
                                      nb.count, nb.list, cur, b->pres, b->rho_pred, b->dl, b->st,
-                                     k, b->feat, dflags, st, b->lanes[BP_DENSITY]));
+                                     k, b->feat, dflags, st, b->lanes[BP_DENSITY], b->nlist16));
   }
   if (k == 0 && mlp) PSB_TRY(mlp_infer(b->mlp, b->feat, b->n, b->pres, cur, 1, b->c.pscale, st));
   {
     BTimed t(b, BP_FORCE);
     PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
                                    nb.list, cur, nxt, b->rho_pred, b->pres, pb, b->st, k, 0, o,
-                                   st, b->lanes[BP_FORCE]));
+                                   st, b->lanes[BP_FORCE], b->nlist16));
   }
   return PSB_OK;
 }
@@ -206,9 +207,13 @@ static psb_status enqueue_prologue(psb_batch b) {
   {
     BTimed t(b, BP_NEIGHBORS);
     const int s = b->bsel[BP_NEIGHBORS];
+    Compact16 c16;
+    c16.list16 = b->nlist16;
+    c16.wide = &b->st[0].wide;
+    c16.wide_stride = sizeof(PciStatus) / sizeof(uint32_t);
     PSB_TRY(launch_neighbors_batch(b->grid, bmap(b, BP_NEIGHBORS), b->nblk[s], 64u << s, b->ax,
                                    b->n, b->c.h2, b->gd.R, nb, b->rho, b->c.mc, b->flags,
-                                   b->flags + 1, st));
+                                   b->flags + 1, st, c16));
   }
   {
     BTimed t(b, BP_EPILOGUE);
@@ -230,7 +235,7 @@ static psb_status enqueue_prologue(psb_batch b) {
     BTimed t(b, BP_FORCE);
     PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, b->c, nb.count,
                                    nb.list, b->xp0, b->xp1, b->rho_pred, b->pres, pb, b->st, -1, 1,
-                                   o, st, b->lanes[BP_FORCE]));
+                                   o, st, b->lanes[BP_FORCE], b->nlist16));
   }
   for (int k = 0; k < b->c.min_it; ++k) PSB_TRY(enqueue_iter(b, k));
   return PSB_OK;
@@ -334,6 +339,7 @@ static psb_status batch_alloc(psb_batch b) {
   PSB_TRY(A((void**)&b->xp0, 16 * N)); PSB_TRY(A((void**)&b->xp1, 16 * N));
   PSB_TRY(A((void**)&b->ncount, 4 * N));
   PSB_TRY(A((void**)&b->nlist, 4 * N * (size_t)b->prm.max_neighbors));
+  PSB_TRY(A((void**)&b->nlist16, 2 * N * (size_t)b->prm.max_neighbors));
   PSB_TRY(A((void**)&b->flags, 64));
   PSB_TRY(A((void**)&b->st, sizeof(PciStatus) * b->S));
   PSB_TRY(A((void**)&b->done, 64));
@@ -476,7 +482,7 @@ psb_status psb_batch_destroy(psb_batch b) {
   drop_graphs(b);
   void* ptrs[] = {b->ax, b->av, b->avh, b->bx, b->bv, b->bvh, b->aid, b->bid, b->ap, b->bp,
                   b->rho, b->pres, b->rho_pred, b->anp, b->dl, b->xp0, b->xp1, b->feat,
-                  b->ncount, b->nlist, b->flags, b->st, b->done, b->dsc};
+                  b->ncount, b->nlist, b->nlist16, b->flags, b->st, b->done, b->dsc};
   for (void* p : ptrs) if (p) hipFree(p);
   for (auto p : b->dblk) if (p) hipFree(p);
   if (b->h_final) hipHostFree(b->h_final);

@@ -198,7 +198,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
                                                               uint32_t* __restrict__ nlist,
                                                               float* __restrict__ rho, float mc,
                                                               uint32_t* __restrict__ flags,
-                                                              uint32_t* __restrict__ maxcnt) {
+                                                              uint32_t* __restrict__ maxcnt,
+                                                              Compact16 c16) {
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -215,11 +216,21 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     const int R = RT > 0 ? RT : g.R;
     const int x0 = max(cx - R, 0), x1 = min(cx + R, g.nx - 1);
     float S = 0.f;
+    bool wide = false;
     auto visit = [&](uint32_t q, const float4& xj) {
       const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
       const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
       if (d2 <= r2) {
-        if (cnt < cap) nlist[(uint64_t)cnt * n + p] = q;
+        if (cnt < cap) {
+          nlist[(uint64_t)cnt * n + p] = q;
+          if (c16.list16) {
+            const int d = (int)(q - p);
+            if (d >= -kCompactReach && d <= kCompactReach)
+              c16.list16[(uint64_t)cnt * n + p] = (uint16_t)(d + 32768);
+            else
+              wide = true;
+          }
+        }
         cnt++;
         if (DENS) {
           const float w = r2 - d2;
@@ -265,6 +276,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     ncount[p] = cnt < cap ? cnt : cap;
     if (DENS) rho[p] = S * mc;
     if (cnt > cap) atomicOr(flags, PSB_FLAG_NEIGHBOR_OVERFLOW);
+    if (wide) atomicOr(c16.wide + (size_t)sc * c16.wide_stride, 1u);
   }
   if (maxcnt) {
     uint32_t m = wave_max_u(cnt);
@@ -328,13 +340,13 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
 psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
-                                  uint32_t* maxcnt, hipStream_t s) {
+                                  uint32_t* maxcnt, hipStream_t s, const Compact16& c16) {
   if (!n) return PSB_OK;
   GridDev none{};
   dim3 gb(nblocks), tb(bsize);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, BatchMap>), gb, tb, 0, s, map, xs, n, none, g->start, \
-                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt)
+                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {
@@ -347,14 +359,14 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
 
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho, float mc, uint32_t* flags,
-                            uint32_t* maxcnt, hipStream_t s) {
+                            uint32_t* maxcnt, hipStream_t s, const Compact16& c16) {
   if (!n) return PSB_OK;
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
   dim3 gb(blocks_for(n)), tb(kBlock);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, SingleMap>), gb, tb, 0, s, SingleMap{}, xs, n, gd,    \
-                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt)
+                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {

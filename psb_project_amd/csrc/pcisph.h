@@ -12,7 +12,8 @@ struct PciStatus {
   uint32_t last_err;               // float bits: max compression error, last step
   uint32_t worst_err;              // float bits: largest final error since reset
   unsigned long long total_iters;  // iterations since reset
-  uint32_t pad[2];
+  uint32_t wide;                   // 1: some compact neighbour offset overflowed this step
+  uint32_t pad;
   uint32_t err[kMaxIterSlots];     // float bits: max error of D(k)
 };
 
@@ -61,7 +62,8 @@ constexpr int kDensSkipUpdate = 2;     // leave the pressure to the MLP
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
-                              hipStream_t s, const uint32_t* ids = nullptr, int lanes = 1);
+                              hipStream_t s, const uint32_t* ids = nullptr, int lanes = 1,
+                              const uint16_t* list16 = nullptr);
 psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
                          uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
                          uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s);
@@ -69,7 +71,7 @@ psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
                             PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s,
-                            int lanes = 1);
+                            int lanes = 1, const uint16_t* list16 = nullptr);
 
 // batches (scene-aligned blocks; see BatchMap)
 psb_status launch_pci_nonpressure_batch(const BatchMap& map, const Shape& sh, uint32_t n,
@@ -82,12 +84,13 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags, hipStream_t s,
-                                    int lanes = 1);
+                                    int lanes = 1, const uint16_t* list16 = nullptr);
 psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,
                                   const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                   const float4* X, float4* Xn, const float* rho_pred,
                                   const float* pres, const PciBuffers& b, PciStatus* st, int k,
-                                  int warm_pass, const PciOut& o, hipStream_t s, int lanes = 1);
+                                  int warm_pass, const PciOut& o, hipStream_t s, int lanes = 1,
+                                  const uint16_t* list16 = nullptr);
 psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, uint32_t bsize,
                                          uint32_t n, const Consts& c, int warm, int xw_prev,
                                          float* rho, float* pres, const float* pprev, float4* xs,

@@ -32,7 +32,7 @@ namespace psb {
 __global__ void k_pci_reset(PciStatus* st0, uint32_t* done) {
   PciStatus* st = st0 + blockIdx.x;
   for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
-  if (threadIdx.x == 0) st->final_iter = -1;
+  if (threadIdx.x == 0) { st->final_iter = -1; st->wide = 0u; }
   if (done && blockIdx.x == 0 && threadIdx.x == 0) *done = 0u;
 }
 
@@ -222,7 +222,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ list, float4* X, float* __restrict__ pres,
     float* __restrict__ rho_pred, const float* __restrict__ dl, PciStatus* __restrict__ st0, int k,
-    float* __restrict__ feat, int dflags, const uint32_t* __restrict__ ids) {
+    float* __restrict__ feat, int dflags, const uint32_t* __restrict__ ids,
+    const uint16_t* __restrict__ list16) {
   __shared__ float red[M::kMaxThreads / 64];
   PciStatus* __restrict__ st = st0 + map.scene_of_block();   // a block serves one scene
   {
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     const float4 xi = X[p];
     float S2[2] = {0.f, 0.f}, W1x2[2] = {0.f, 0.f}, W1y2[2] = {0.f, 0.f}, W1z2[2] = {0.f, 0.f};
     float W22[2] = {0.f, 0.f};
-    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, list, X, [&](int a, const float4& xj) {
+    auto pair = [&](int a, const float4& xj) {
       const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const float d2 = dot3(dx, dy, dz, dx, dy, dz);
       float w = c.h2 - d2;
@@ -264,7 +265,11 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
         W1x2[a] = W1x2[a] + ux; W1y2[a] = W1y2[a] + uy; W1z2[a] = W1z2[a] + uz;
         W22[a] = W22[a] + dot3(ux, uy, uz, ux, uy, uz);
       }
-    });
+    };
+    if (list16 && st->wide == 0u)
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
+    else
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
     const float S = split_total<L>(S2, lane);
     float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
     if (REF) {
@@ -315,7 +320,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
     M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
     const uint32_t* __restrict__ list, const float4* __restrict__ X, float4* __restrict__ Xn,
     const float* __restrict__ rho_pred, const float* __restrict__ pres, PciBuffers b,
-    PciStatus* __restrict__ st0, int k, int warm_pass, PciOut o) {
+    PciStatus* __restrict__ st0, int k, int warm_pass, PciOut o,
+    const uint16_t* __restrict__ list16) {
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -355,31 +361,38 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
     return;
   }
   float sx2[2] = {0.f, 0.f}, sy2[2] = {0.f, 0.f}, sz2[2] = {0.f, 0.f};
-  for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, list, X, [&](int a, const float4& xj) {
+  auto pair = [&](int a, const float4& xj) {
+    // the pressure term wj is read with the position (one 16-byte gather)
+    float wj = xj.w;
+    asm volatile("" : "+v"(wj));
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (REF) {
       if (d2 > 0.f) {
-        const float dd = sqrtf(d2);
-        const float invd = 1.0f / dd;
+        float dd, invd;
+        dist_and_inverse(d2, dd, invd);
         const float hd = c.h - dd;
         const float gw = c.c_spiky * (hd * hd);
-        const float cc = gw * (xi.w + xj.w);
+        const float cc = gw * (xi.w + wj);
         sx2[a] = __builtin_fmaf(dx * invd, cc, sx2[a]);
         sy2[a] = __builtin_fmaf(dy * invd, cc, sy2[a]);
         sz2[a] = __builtin_fmaf(dz * invd, cc, sz2[a]);
       }
     } else if (d2 > 0.f && d2 < c.h2) {
-      const float dd = sqrtf(d2);
-      const float invd = 1.0f / dd;
+      float dd, invd;
+      dist_and_inverse(d2, dd, invd);
       const float hd = c.h - dd;
       const float gw = c.c_spiky * (hd * hd);
-      const float cc = (gw * invd) * (xi.w + xj.w);
+      const float cc = (gw * invd) * (xi.w + wj);
       sx2[a] = __builtin_fmaf(dx, cc, sx2[a]);
       sy2[a] = __builtin_fmaf(dy, cc, sy2[a]);
       sz2[a] = __builtin_fmaf(dz, cc, sz2[a]);
     }
-  });
+  };
+  if (list16 && st->wide == 0u)
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
+  else
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
   float sx = split_total<L>(sx2, lane), sy = split_total<L>(sy2, lane),
         sz = split_total<L>(sz2, lane);
   if (lane != 0) return;   // the pair's second lane only summed
@@ -485,14 +498,15 @@ template <class M>
 static psb_status density_impl(const M& map, const Shape& sh, uint32_t n, const Consts& c,
                                const uint32_t* cnt, const uint32_t* list, float4* X, float* pres,
                                float* rho_pred, const float* dl, PciStatus* st, int k,
-                               float* feat, int dflags, const uint32_t* ids, hipStream_t s) {
+                               float* feat, int dflags, const uint32_t* ids, hipStream_t s,
+                               const uint16_t* l16) {
   if (!n) return PSB_OK;
   if (c.mode == PSB_MODE_REFERENCE)
     hipLaunchKernelGGL((k_pci_density<true, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, 0, ids);
+                       pres, rho_pred, dl, st, k, feat, 0, ids, l16);
   else
     hipLaunchKernelGGL((k_pci_density<false, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, dflags, ids);
+                       pres, rho_pred, dl, st, k, feat, dflags, ids, l16);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -502,12 +516,12 @@ static psb_status force_impl(const M& map, const Shape& sh, uint32_t n, const Co
                              const uint32_t* cnt, const uint32_t* list, const float4* X,
                              float4* Xn, const float* rho_pred, const float* pres,
                              const PciBuffers& b, PciStatus* st, int k, int warm_pass,
-                             const PciOut& o, hipStream_t s) {
+                             const PciOut& o, hipStream_t s, const uint16_t* l16) {
   if (!n) return PSB_OK;
   const bool ref = c.mode == PSB_MODE_REFERENCE;
 #define PSB_F(R_, S_)                                                                           \
   hipLaunchKernelGGL((k_pci_force<R_, S_, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X, \
-                     Xn, rho_pred, pres, b, st, k, warm_pass, o)
+                     Xn, rho_pred, pres, b, st, k, warm_pass, o, l16)
   if (ref) { if (step) PSB_F(true, true); else PSB_F(true, false); }
   else { if (step) PSB_F(false, true); else PSB_F(false, false); }
 #undef PSB_F
@@ -531,24 +545,25 @@ psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool w
 psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
-                              hipStream_t s, const uint32_t* ids, int lanes) {
+                              hipStream_t s, const uint32_t* ids, int lanes,
+                              const uint16_t* list16) {
   if (lanes == 2)
     return density_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, cnt, list, X, pres, rho_pred,
-                        dl, st, k, feat, dflags, ids, s);
+                        dl, st, k, feat, dflags, ids, s, list16);
   return density_impl(SingleMap{}, single_shape(n), n, c, cnt, list, X, pres, rho_pred, dl, st, k,
-                      feat, dflags, ids, s);
+                      feat, dflags, ids, s, list16);
 }
 
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
                             PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s,
-                            int lanes) {
+                            int lanes, const uint16_t* list16) {
   if (lanes == 2)
     return force_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, step, cnt, list, X, Xn, rho_pred,
-                      pres, b, st, k, warm_pass, o, s);
+                      pres, b, st, k, warm_pass, o, s, list16);
   return force_impl(SingleMap{}, single_shape(n), n, c, step, cnt, list, X, Xn, rho_pred, pres, b,
-                    st, k, warm_pass, o, s);
+                    st, k, warm_pass, o, s, list16);
 }
 
 psb_status launch_pci_nonpressure_batch(const BatchMap& map, const Shape& sh, uint32_t n,
@@ -565,24 +580,25 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags,
-                                    hipStream_t s, int lanes) {
+                                    hipStream_t s, int lanes, const uint16_t* list16) {
   if (lanes == 2)
     return density_impl(BatchMapT<2>{map.sc, map.blk_scene, map.bsel}, sh, n, c, cnt, list, X,
-                        pres, rho_pred, dl, st, k, feat, dflags, nullptr, s);
+                        pres, rho_pred, dl, st, k, feat, dflags, nullptr, s, list16);
   return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags,
-                      nullptr, s);
+                      nullptr, s, list16);
 }
 
 psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,
                                   const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                   const float4* X, float4* Xn, const float* rho_pred,
                                   const float* pres, const PciBuffers& b, PciStatus* st, int k,
-                                  int warm_pass, const PciOut& o, hipStream_t s, int lanes) {
+                                  int warm_pass, const PciOut& o, hipStream_t s, int lanes,
+                                  const uint16_t* list16) {
   if (lanes == 2)
     return force_impl(BatchMapT<2>{map.sc, map.blk_scene, map.bsel}, sh, n, c, true, cnt, list, X,
-                      Xn, rho_pred, pres, b, st, k, warm_pass, o, s);
+                      Xn, rho_pred, pres, b, st, k, warm_pass, o, s, list16);
   return force_impl(map, sh, n, c, true, cnt, list, X, Xn, rho_pred, pres, b, st, k, warm_pass, o,
-                    s);
+                    s, list16);
 }
 
 // Append every `stride`-th particle (p = offset, offset + stride, ...) of the

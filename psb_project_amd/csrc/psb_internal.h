@@ -349,15 +349,74 @@ __device__ __forceinline__ void for_neighbors2(uint32_t p, uint32_t n, uint32_t 
 
 constexpr int kUnroll = 4;
 
+// Correctly rounded sqrt(x) and 1/d on the fast path of the IEEE
+// expansions: the hardware approximations (within 1 ulp) and the same
+// correction steps, without the range scaling and special-value handling
+// that x >= 2^-96 (so d = sqrt(x) >= 2^-48, a normal quotient 1/d) never
+// needs.  sqrt: the candidates s - ulp, s, s + ulp are decided by the
+// residuals fma(-(s -/+ ulp), s, x); 1/d: one Newton step on the
+// reciprocal and two quotient corrections (Markstein's scheme).  The
+// results equal sqrtf(x) and 1.0f / d bit for bit on that range.
+constexpr float kFastSqrtMin = 0x1p-96f;
+__device__ __forceinline__ float sqrt_rn_fast(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sdn = __int_as_float(__float_as_int(s) - 1);
+  const float sup = __int_as_float(__float_as_int(s) + 1);
+  const float rdn = __builtin_fmaf(-sdn, s, x);
+  const float rup = __builtin_fmaf(-sup, s, x);
+  const float t = rdn <= 0.f ? sdn : s;
+  return rup > 0.f ? sup : t;
+}
+__device__ __forceinline__ float rcp_rn_fast(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  r = __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+  float q = r;
+  q = __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), r, q);
+  return __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), r, q);
+}
+// dd = sqrtf(d2), invd = 1.0f / dd for d2 > 0
+__device__ __forceinline__ void dist_and_inverse(float d2, float& dd, float& invd) {
+  if (__builtin_expect(d2 >= kFastSqrtMin, 1)) {
+    dd = sqrt_rn_fast(d2);
+    invd = rcp_rn_fast(dd);
+  } else {
+    dd = sqrtf(d2);
+    invd = 1.0f / dd;
+  }
+}
+
+// Neighbour list readers: the ELL list of 32-bit indices, and its compact
+// twin of 16-bit offsets (q - p + 32768, |q - p| <= 32767) that halves the
+// list bytes the corrector passes stream every iteration.
+struct List32 {
+  const uint32_t* __restrict__ l;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t) const { return l[i]; }
+};
+struct List16 {
+  const uint16_t* __restrict__ l;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t p) const {
+    return p + (uint32_t)l[i] - 32768u;
+  }
+};
+constexpr int kCompactReach = 32767;
+
+// Compact list output of the neighbour search: list16 (same ELL shape as
+// the 32-bit list) and a flag per scene, wide[scene * wide_stride], set when
+// some offset does not fit (the readers then take the 32-bit list).
+struct Compact16 {
+  uint16_t* list16 = nullptr;
+  uint32_t* wide = nullptr;
+  uint32_t wide_stride = 0;
+};
+
 // Neighbour sums of the corrector passes in their canonical order: two
 // partial sums, over the even and over the odd list positions k, each in
 // increasing k, added as even + odd.  LANES = 1: one thread keeps both
 // (f(acc = k & 1, ...)); LANES = 2: lane l of the particle's pair keeps the
 // sum over k = l (mod 2) (f(acc = 0, ...)) and split_total adds the pair.
-template <int LANES, int U, class F>
+template <int LANES, int U, class LI, class F>
 __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint32_t m,
-                                                    uint32_t lane,
-                                                    const uint32_t* __restrict__ list,
+                                                    uint32_t lane, const LI& list,
                                                     const float4* X, F&& f) {
   static_assert(U % 2 == 0, "U even keeps k & 1 static");
   if (LANES == 1) {
@@ -366,26 +425,26 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
       uint32_t q[U];
       float4 xq[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + u) * n + p];
+      for (int u = 0; u < U; ++u) q[u] = list((uint64_t)(k + u) * n + p, p);
 #pragma unroll
       for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
 #pragma unroll
       for (int u = 0; u < U; ++u) f(u & 1, xq[u]);
     }
-    for (; k < m; ++k) f((int)(k & 1u), X[list[(uint64_t)k * n + p]]);
+    for (; k < m; ++k) f((int)(k & 1u), X[list((uint64_t)k * n + p, p)]);
   } else {
     uint32_t k = lane;
     for (; k + 2 * (U - 1) < m; k += 2 * U) {
       uint32_t q[U];
       float4 xq[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = list[(uint64_t)(k + 2 * u) * n + p];
+      for (int u = 0; u < U; ++u) q[u] = list((uint64_t)(k + 2 * u) * n + p, p);
 #pragma unroll
       for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
 #pragma unroll
       for (int u = 0; u < U; ++u) f(0, xq[u]);
     }
-    for (; k < m; k += 2) f(0, X[list[(uint64_t)k * n + p]]);
+    for (; k < m; k += 2) f(0, X[list((uint64_t)k * n + p, p)]);
   }
 }
 
@@ -457,7 +516,8 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
                       hipStream_t s);
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
-                            uint32_t* flags, uint32_t* maxcnt, hipStream_t s);
+                            uint32_t* flags, uint32_t* maxcnt, hipStream_t s,
+                            const Compact16& c16 = Compact16{});
 // batches: g spans the cells of every scene (scene s owns cells
 // [cell_off, cell_off + its grid)); nblocks x bsize threads, scene-aligned
 psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
@@ -466,6 +526,7 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
 psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
-                                  uint32_t* maxcnt, hipStream_t s);
+                                  uint32_t* maxcnt, hipStream_t s,
+                                  const Compact16& c16 = Compact16{});
 
 }  // namespace psb

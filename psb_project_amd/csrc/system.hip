@@ -66,6 +66,8 @@ struct psb_system_s {
   int64_t step_counter = 0;
   uint32_t* ncount = nullptr;
   uint32_t* nlist = nullptr;
+  uint16_t* nlist16 = nullptr;     // compact twin of nlist for the corrector passes (PCISPH)
+  bool compact = true;
   uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
   PciStatus* st = nullptr;
   int32_t* h_final = nullptr;     // host-mapped final_iter per step slot
@@ -164,6 +166,7 @@ static psb_status sys_alloc(psb_system s) {
       PSB_TRY(A((void**)&s->dl, 4 * N));
     }
     PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
+    PSB_TRY(A((void**)&s->nlist16, 2 * N * (size_t)s->prm.max_neighbors));
   }
   PSB_TRY(A((void**)&s->ncount, 4 * N));
   PSB_TRY(A((void**)&s->nlist, 4 * N * (size_t)s->prm.max_neighbors));
@@ -232,10 +235,16 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   PSB_TRY(launch_pci_reset(s->st, false, st));
   PSB_TRY(grid_phase(s, step));
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
+  Compact16 c16;
+  if (s->compact && s->nlist16) {
+    c16.list16 = s->nlist16;
+    c16.wide = &s->st->wide;
+  }
+  const uint16_t* l16 = c16.list16;
   {
     Timed t(s, PH_NEIGHBORS, step);
     PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
-                             s->flags, s->flags + 1, st));
+                             s->flags, s->flags + 1, st, c16));
   }
   {
     Timed t(s, PH_EPILOGUE, step);
@@ -260,7 +269,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   if (warm) {
     Timed t(s, PH_PCI_FORCE, step);
     PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
-                             b, s->st, -1, 1, o, st, lanes));
+                             b, s->st, -1, 1, o, st, lanes, l16));
     std::swap(cur, nxt);
   }
   const int min_it = s->c.min_it, max_it = s->c.max_it;
@@ -271,7 +280,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st, s->aid, lanes));
+                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
@@ -283,7 +292,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     {
       Timed t(s, PH_PCI_FORCE, step, k);
       PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
-                               s->pres, b, s->st, k, 0, o, st, lanes));
+                               s->pres, b, s->st, k, 0, o, st, lanes, l16));
     }
     std::swap(cur, nxt);
     if (ref) continue;
@@ -380,7 +389,7 @@ psb_status psb_system_destroy(psb_system s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
-                  s->ncount, s->nlist, s->flags, s->st, s->feat};
+                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
@@ -752,6 +761,12 @@ psb_status psb_system_get_spheres(psb_system s, float* centers3, float* velociti
       if (velocities3) velocities3[3 * j + a] = v[4 * j + a];
       if (fluid_forces3) fluid_forces3[3 * j + a] = f[4 * j + a];
     }
+  return PSB_OK;
+}
+
+psb_status psb_system_set_compact_list(psb_system s, int32_t on) {
+  PSB_ARG(s, "null system");
+  s->compact = on != 0;
   return PSB_OK;
 }
 

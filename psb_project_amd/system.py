@@ -148,6 +148,10 @@ class FluidSystem:
             raise RuntimeError("no features computed (enable the MLP warm start or a recorder)")
         return self.download(ptr.value, np.float32, (8, self.n))
 
+    def set_compact_list(self, on: bool) -> None:
+        """16-bit neighbour offsets in the corrector passes (default on)."""
+        check(lib().psb_system_set_compact_list(self.handle, 1 if on else 0))
+
     def set_corrector_lanes(self, lanes: int) -> None:
         """Threads per particle in the corrector passes: 1, 2, or 0 (default, 1)."""
         check(lib().psb_system_set_corrector_lanes(self.handle, int(lanes)))

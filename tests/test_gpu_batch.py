@@ -119,6 +119,28 @@ def test_two_threads_per_particle_keeps_results(ctx, dens, force):
     b.close()
 
 
+def test_batch_wide_offsets_match_single(ctx):
+    """One scene whose neighbour offsets overflow the 16-bit list (a thin
+    180 x 100 x 4 slab) beside one where they fit: each equals its single
+    run with the 32-bit list."""
+    slab = scenes.pcisph_dam_break(n_side=4)
+    slab = scenes.Scene("slab", slab.params, scenes.lattice((180, 100, 4), 0.005, seed=11))
+    small = scenes.pcisph_dam_break(n_side=12)
+    scs = [slab, small]
+    b = BatchSystem(ctx, scs)
+    b.set_all()
+    b.update(3)
+    for i, sc in enumerate(scs):
+        fs = FluidSystem(ctx, sc.params, sc.n)
+        fs.set_compact_list(False)
+        fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
+        fs.update(3)
+        assert_same(b.state(i), fs.state(), f"scene {i}")
+        assert b.stats(i)["total_iterations"] == fs.stats()["total_iterations"]
+        fs.close()
+    b.close()
+
+
 def test_batch_learned_warm_start_matches_single(ctx):
     scs = small_batch(WARM_MLP, False, n=3, seed=80)
     rng = np.random.default_rng(3)

@@ -177,10 +177,11 @@ def test_pointwise_kernels_bitwise(ctx, oracle):
 
 
 # ------------------------------------------------------------------ systems
-def run_pair(ctx, oracle, sc, steps, chunks=1, lanes=0):
+def run_pair(ctx, oracle, sc, steps, chunks=1, lanes=0, compact=True):
     from psb_project_amd.system import FluidSystem
     g = FluidSystem(ctx, sc.params, sc.n)
     g.set_corrector_lanes(lanes)
+    g.set_compact_list(compact)
     o = oracle.OracleSystem(sc.params, sc.n)
     g.set_particles(sc.pos4, sc.vel4, sc.vh4)
     o.set_particles(sc.pos4, sc.vel4, sc.vh4)
@@ -209,12 +210,13 @@ def test_wcsph_system_bitwise(ctx, oracle, mode, steps):
     assert gi[0]["flags"] == oi[0]["flags"]
 
 
-@pytest.mark.parametrize("lanes", [1, 2])
+@pytest.mark.parametrize("lanes,compact", [(1, True), (2, True), (1, False)])
 @pytest.mark.parametrize("mode", [MODE_REFERENCE, MODE_CONFIG])
-def test_pcisph_system_bitwise(ctx, oracle, mode, lanes):
-    """One and two threads per particle in the corrector passes."""
+def test_pcisph_system_bitwise(ctx, oracle, mode, lanes, compact):
+    """One and two threads per particle in the corrector passes; 16-bit
+    and 32-bit neighbour lists."""
     sc = scenes.pcisph_dam_break(n_side=24, mode=mode)
-    g, o, gi, oi = run_pair(ctx, oracle, sc, 1, chunks=6, lanes=lanes)
+    g, o, gi, oi = run_pair(ctx, oracle, sc, 1, chunks=6, lanes=lanes, compact=compact)
     for a, b in zip(gi, oi):
         assert a["iterations"] == b["iterations"]
         assert a["max_error"] == np.float32(b["max_error"])
@@ -244,6 +246,34 @@ def test_pcisph_lattice_walls_through_impact_bitwise(ctx, oracle, lanes):
         assert a["max_error"] == np.float32(b["max_error"])
     assert gi[-1]["flags"] == oi[-1]["flags"]
     assert_state_equal(g.state(), o.state())
+
+
+def test_compact_list_falls_back_on_wide_offsets(ctx):
+    """A thin 180 x 100 x 4 slab on 5 mm cells: neighbours one or two cell
+    layers apart in z sit more than 32767 places away in the sorted order,
+    so the 16-bit list cannot hold them and the corrector passes read the
+    32-bit list; the run equals the one without the compact list."""
+    from psb_project_amd.system import FluidSystem
+    sc = scenes.pcisph_dam_break(n_side=4)
+    pos = scenes.lattice((180, 100, 4), 0.005, seed=11)
+    runs = []
+    for compact in (True, False):
+        g = FluidSystem(ctx, sc.params, pos.shape[0])
+        g.set_compact_list(compact)
+        g.set_particles(pos)
+        g.update(3)
+        runs.append(g)
+    nb = runs[0].arrays().neighbors
+    n = runs[0].n
+    cnt = runs[0].download(nb.count, np.uint32, n)
+    lst = runs[0].download(nb.list, np.uint32, (nb.capacity, n)).astype(np.int64)
+    k = np.arange(nb.capacity)[:, None]
+    off = np.where(k < cnt[None, :], lst - np.arange(n)[None, :], 0)
+    assert np.abs(off).max() > 32767
+    assert_state_equal(runs[0].state(), runs[1].state())
+    assert runs[0].stats()["total_iterations"] == runs[1].stats()["total_iterations"]
+    for g in runs:
+        g.close()
 
 
 def test_pcisph_warm_start_previous_bitwise(ctx, oracle):
