@@ -46,6 +46,8 @@ typedef int32_t psb_status;
 #define PSB_ERR_NO_DEVICE         4
 #define PSB_ERR_OUT_OF_MEMORY     5
 #define PSB_ERR_STATE             6  /* call order (e.g. step before set)   */
+#define PSB_ERR_IO                7  /* a file could not be read / written,
+                                        or is not a valid checkpoint        */
 
 /* Bits of psb_step_stats.flags, raised by the kernels on the device.       */
 #define PSB_FLAG_NEIGHBOR_OVERFLOW  0x1u  /* > max_neighbors within h; list truncated */
@@ -363,6 +365,8 @@ psb_status psb_system_set_particles_device(psb_system sys, uint32_t n,
                                            const float* pos4, const float* vel4,
                                            const float* vel_inter_ap4);
 uint32_t   psb_system_count(psb_system sys);
+/* Number of rigid spheres (psb_system_set_spheres or a loaded checkpoint).  */
+uint32_t psb_system_num_spheres(psb_system sys);
 /* SPHSystem::update / PCI_SPHSystem::update, `steps` times. */
 psb_status psb_system_update(psb_system sys, int32_t steps);
 psb_status psb_system_get_stats(psb_system sys, psb_step_stats* out);
@@ -377,6 +381,33 @@ psb_status psb_system_get_particles(psb_system sys, double* old_pos, double* new
 psb_status psb_system_get_particles_f32(psb_system sys, float* pos4, float* vel4,
                                         float* vel_inter_ap4, float* density,
                                         float* pressure, uint32_t* ids);
+
+/* State in a given storage order (the inverse of psb_system_get_particles_f32):
+ * pos4 / vel4 / vel_inter_ap4 as float4 (w ignored), pressure (NULL: zero; the
+ * PSB_WARM_PREVIOUS start), ids a permutation of 0..n-1 naming each particle.
+ * Spheres set with psb_system_set_spheres are appended after the n fluid
+ * particles, as by psb_system_set_particles.  No reference counterpart: it
+ * restores a state written by ParticleExporter_XML-style exporters in the
+ * storage order the next step sorts from, so that the run continues bit for
+ * bit.                                                                    */
+psb_status psb_system_set_state(psb_system sys, uint32_t n, const float* pos4,
+                                const float* vel4, const float* vel_inter_ap4,
+                                const float* pressure, const uint32_t* ids);
+
+/* Checkpoint / resume.  The file holds the parameters, the sphere setup and
+ * statistics, and every per-particle array that carries from one step to the
+ * next, in storage order, with a 64-bit hash; a system loaded from it
+ * continues exactly as the one that saved it (same results bit for bit).
+ * Load needs a system created with the same psb_sph_params (byte for byte)
+ * and the capacity for psb_checkpoint_info's count; MLP and recorder
+ * settings are not part of the file.  Counterpart of the reference's
+ * ParticleExporter_XML::_export_SPHParticles / ParticleLoader_XML::
+ * loadSPHSystem (ParticleExporter_XML.cpp:131-190, ParticleLoader_XML.cpp:
+ * 121-180), which keep positions and velocities only.                      */
+psb_status psb_system_save_checkpoint(psb_system sys, const char* path);
+psb_status psb_system_load_checkpoint(psb_system sys, const char* path);
+psb_status psb_checkpoint_info(const char* path, psb_sph_params* params, uint32_t* count,
+                               uint32_t* fluid_count, uint32_t* num_spheres);
 
 /* Device pointers of the system's sorted state (valid until the next update). */
 typedef struct psb_system_arrays {

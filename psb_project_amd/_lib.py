@@ -19,6 +19,7 @@ STATUS_NAMES = {
     4: "PSB_ERR_NO_DEVICE",
     5: "PSB_ERR_OUT_OF_MEMORY",
     6: "PSB_ERR_STATE",
+    7: "PSB_ERR_IO",
 }
 
 FLAG_NEIGHBOR_OVERFLOW = 0x1
@@ -215,6 +216,7 @@ SIGNATURES = {
     "psb_system_set_particles": (i32, [vp, u32, dp, dp, dp, dp, dp, dp]),
     "psb_system_set_particles_device": (i32, [vp, u32, vp, vp, vp]),
     "psb_system_count": (u32, [vp]),
+    "psb_system_num_spheres": (u32, [vp]),
     "psb_system_update": (i32, [vp, i32]),
     "psb_system_get_stats": (i32, [vp, C.POINTER(StepStats)]),
     "psb_system_get_particles": (i32, [vp, dp, dp, dp, dp, dp, dp, dp, dp]),
@@ -223,6 +225,10 @@ SIGNATURES = {
     "psb_system_set_timing": (i32, [vp, i32]),
     "psb_system_set_corrector_lanes": (i32, [vp, i32]),
     "psb_system_set_compact_list": (i32, [vp, i32]),
+    "psb_system_set_state": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+    "psb_system_save_checkpoint": (i32, [vp, C.c_char_p]),
+    "psb_system_load_checkpoint": (i32, [vp, C.c_char_p]),
+    "psb_checkpoint_info": (i32, [C.c_char_p, vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
     "psb_system_get_timing": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_int64)]),
     "psb_system_reset_timing": (i32, [vp]),
     "psb_mlp_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),

@@ -14,6 +14,7 @@
 // and the host reads state only when asked.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "bodies.h"
@@ -96,7 +97,7 @@ struct psb_system_s {
   float4* d_soff = nullptr;
   float4* d_fb = nullptr;
   int sphere_cur = 0;
-  int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: by size)
+  int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: one)
 };
 
 namespace psb {
@@ -409,6 +410,7 @@ psb_status psb_system_get_params(psb_system s, psb_sph_params* out) {
 }
 
 uint32_t psb_system_count(psb_system s) { return s ? s->n : 0; }
+uint32_t psb_system_num_spheres(psb_system s) { return s ? s->n_spheres : 0; }
 
 // Boundary particles of the rigid spheres after the n_fluid fluid particles
 // (ids n_fluid ..), at rest relative to their sphere; sphere state uploaded.
@@ -771,7 +773,7 @@ psb_status psb_system_set_compact_list(psb_system s, int32_t on) {
 }
 
 psb_status psb_system_set_corrector_lanes(psb_system s, int32_t lanes) {
-  PSB_ARG(s && lanes >= 0 && lanes <= 2, "lanes must be 0 (by size), 1 or 2");
+  PSB_ARG(s && lanes >= 0 && lanes <= 2, "lanes must be 0 (default), 1 or 2");
   s->corrector_lanes = lanes;
   return PSB_OK;
 }
@@ -827,6 +829,237 @@ psb_status psb_dataset_info(psb_dataset d, const float** features, const float**
   if (targets) *targets = d->target;
   if (count) *count = d->count;
   if (capacity) *capacity = d->capacity;
+  return PSB_OK;
+}
+
+psb_status psb_system_set_state(psb_system s, uint32_t n, const float* pos4, const float* vel4,
+                                const float* vel_half4, const float* pressure,
+                                const uint32_t* ids) {
+  PSB_ARG(s && (n == 0 || (pos4 && ids)), "null argument");
+  PSB_ARG(n <= s->cap, "more particles than the capacity");
+  std::vector<uint8_t> seen(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    PSB_ARG(ids[i] < n && !seen[ids[i]], "ids must be a permutation of 0..n-1");
+    seen[ids[i]] = 1;
+  }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  std::vector<float> x(pos4, pos4 + 4 * (size_t)n);
+  std::vector<float> v(4 * (size_t)n, 0.f), vh(4 * (size_t)n, 0.f);
+  if (vel4) v.assign(vel4, vel4 + 4 * (size_t)n);
+  if (vel_half4) vh.assign(vel_half4, vel_half4 + 4 * (size_t)n);
+  PSB_TRY(upload_state(s, n, x, v, vh, x, v, vh));
+  hipStream_t st = s->ctx->stream;
+  if (n) {
+    PSB_HIP(hipMemcpyAsync(s->bid, ids, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    PSB_HIP(hipMemcpyAsync(s->aid, ids, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    if (pressure) {
+      PSB_HIP(hipMemcpyAsync(s->bp, pressure, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+      PSB_HIP(hipMemcpyAsync(s->pres, pressure, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    }
+    PSB_HIP(hipStreamSynchronize(st));
+  }
+  return PSB_OK;
+}
+
+// ---- checkpoints ---------------------------------------------------------------
+// File layout (little-endian): CkptHeader, psb_sph_params, psb_sphere_params,
+// psb_step_stats; the state in storage order: bx bv bvh ax av avh (16 n B
+// each), bid aid (4 n B each), bp ap rho rho_pred pres (4 n B each); the
+// spheres' centres, velocities and fluid forces (16 B each); an FNV-1a
+// 64-bit hash of every byte before it.
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, params_bytes, sphere_bytes, stats_bytes;
+  uint32_t n, n_fluid, n_spheres, per_sphere;
+  uint32_t stepped, solver;
+  int64_t step_counter;
+};
+constexpr char kCkptMagic[8] = {'P', 'S', 'B', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void* p, size_t nb) {
+    const uint8_t* b = (const uint8_t*)p;
+    for (size_t i = 0; i < nb; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+  }
+};
+
+struct CkptArrays {
+  std::vector<uint8_t> buf;
+  // (device pointer, bytes) in file order
+  static std::vector<std::pair<void*, size_t>> layout(psb_system s, uint32_t n) {
+    const size_t v4 = 16 * (size_t)n, v1 = 4 * (size_t)n;
+    return {{s->bx, v4},  {s->bv, v4},  {s->bvh, v4}, {s->ax, v4},       {s->av, v4},
+            {s->avh, v4}, {s->bid, v1}, {s->aid, v1}, {s->bp, v1},       {s->ap, v1},
+            {s->rho, v1}, {s->rho_pred, v1}, {s->pres, v1}};
+  }
+};
+
+bool read_header(FILE* f, CkptHeader& h, psb_sph_params& prm, psb_sphere_params& sp,
+                 psb_step_stats& stats, Fnv& fnv) {
+  if (fread(&h, sizeof h, 1, f) != 1) return false;
+  if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion ||
+      h.params_bytes != sizeof(psb_sph_params) || h.sphere_bytes != sizeof(psb_sphere_params) ||
+      h.stats_bytes != sizeof(psb_step_stats))
+    return false;
+  if (fread(&prm, sizeof prm, 1, f) != 1 || fread(&sp, sizeof sp, 1, f) != 1 ||
+      fread(&stats, sizeof stats, 1, f) != 1)
+    return false;
+  fnv.add(&h, sizeof h); fnv.add(&prm, sizeof prm); fnv.add(&sp, sizeof sp);
+  fnv.add(&stats, sizeof stats);
+  return true;
+}
+}  // namespace
+
+psb_status psb_system_save_checkpoint(psb_system s, const char* path) {
+  PSB_ARG(s && path, "null argument");
+  if (!s->has_state) { set_error("psb_system_save_checkpoint before psb_system_set_particles"); return PSB_ERR_STATE; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const uint32_t n = s->n, S = s->n_spheres;
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = kCkptVersion;
+  h.params_bytes = sizeof(psb_sph_params);
+  h.sphere_bytes = sizeof(psb_sphere_params);
+  h.stats_bytes = sizeof(psb_step_stats);
+  h.n = n; h.n_fluid = S ? s->n_fluid : n; h.n_spheres = S; h.per_sphere = s->per_sphere;
+  h.stepped = s->stepped ? 1u : 0u; h.solver = (uint32_t)s->prm.solver;
+  h.step_counter = s->step_counter;
+  auto lay = CkptArrays::layout(s, n);
+  if (S) {
+    lay.push_back({s->d_sc[s->sphere_cur], 16 * (size_t)S});
+    lay.push_back({s->d_sv[s->sphere_cur], 16 * (size_t)S});
+    lay.push_back({s->d_sf, 16 * (size_t)S});
+  }
+  size_t total = 0;
+  for (auto& a : lay) total += a.first ? a.second : 0;
+  std::vector<uint8_t> buf(total, 0);
+  size_t off = 0;
+  for (auto& a : lay) {
+    // arrays a solver does not allocate (WCSPH: rho_pred) are stored as zeros
+    if (a.first && a.second) PSB_HIP(hipMemcpyAsync(buf.data() + off, a.first, a.second, hipMemcpyDeviceToHost, st));
+    off += a.first ? a.second : 0;
+  }
+  PSB_HIP(hipStreamSynchronize(st));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) { set_error(std::string("cannot write ") + path); return PSB_ERR_IO; }
+  Fnv fnv;
+  auto put = [&](const void* p, size_t nb) { fnv.add(p, nb); return std::fwrite(p, 1, nb, f) == nb; };
+  bool ok = put(&h, sizeof h) && put(&s->prm, sizeof s->prm) && put(&s->sphere, sizeof s->sphere) &&
+            put(&s->stats, sizeof s->stats);
+  // a presence mask records which arrays follow
+  uint32_t mask = 0;
+  for (size_t i = 0; i < lay.size(); ++i) if (lay[i].first) mask |= 1u << i;
+  ok = ok && put(&mask, sizeof mask) && (total == 0 || put(buf.data(), total));
+  const uint64_t hash = fnv.h;
+  ok = ok && std::fwrite(&hash, sizeof hash, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) { set_error(std::string("write failed: ") + path); return PSB_ERR_IO; }
+  return PSB_OK;
+}
+
+psb_status psb_checkpoint_info(const char* path, psb_sph_params* params, uint32_t* count,
+                               uint32_t* fluid_count, uint32_t* num_spheres) {
+  PSB_ARG(path, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("cannot read ") + path); return PSB_ERR_IO; }
+  CkptHeader h;
+  psb_sph_params prm;
+  psb_sphere_params sp;
+  psb_step_stats stats;
+  Fnv fnv;
+  const bool ok = read_header(f, h, prm, sp, stats, fnv);
+  std::fclose(f);
+  if (!ok) { set_error(std::string("not a checkpoint of this library version: ") + path); return PSB_ERR_IO; }
+  if (params) *params = prm;
+  if (count) *count = h.n;
+  if (fluid_count) *fluid_count = h.n_fluid;
+  if (num_spheres) *num_spheres = h.n_spheres;
+  return PSB_OK;
+}
+
+psb_status psb_system_load_checkpoint(psb_system s, const char* path) {
+  PSB_ARG(s && path, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("cannot read ") + path); return PSB_ERR_IO; }
+  CkptHeader h;
+  psb_sph_params prm;
+  psb_sphere_params sp;
+  psb_step_stats stats;
+  Fnv fnv;
+  uint32_t mask = 0;
+  if (!read_header(f, h, prm, sp, stats, fnv) || std::fread(&mask, sizeof mask, 1, f) != 1) {
+    std::fclose(f);
+    set_error(std::string("not a checkpoint of this library version: ") + path);
+    return PSB_ERR_IO;
+  }
+  fnv.add(&mask, sizeof mask);
+  if (std::memcmp(&prm, &s->prm, sizeof prm) != 0) {
+    std::fclose(f);
+    set_error("checkpoint parameters differ from the system's (psb_checkpoint_info returns them)");
+    return PSB_ERR_INVALID_ARGUMENT;
+  }
+  if (h.n > s->cap) { std::fclose(f); set_error("checkpoint holds more particles than the capacity"); return PSB_ERR_CAPACITY; }
+  auto lay = CkptArrays::layout(s, h.n);
+  const uint32_t S = h.n_spheres;
+  if (S) {
+    lay.push_back({nullptr, 16 * (size_t)S});
+    lay.push_back({nullptr, 16 * (size_t)S});
+    lay.push_back({nullptr, 16 * (size_t)S});
+  }
+  size_t total = 0;
+  for (size_t i = 0; i < lay.size(); ++i) if (mask >> i & 1u) total += lay[i].second;
+  std::vector<uint8_t> buf(total);
+  uint64_t hash = 0;
+  const bool rd = (total == 0 || std::fread(buf.data(), 1, total, f) == total) &&
+                  std::fread(&hash, sizeof hash, 1, f) == 1;
+  std::fclose(f);
+  if (!rd) { set_error(std::string("truncated checkpoint: ") + path); return PSB_ERR_IO; }
+  fnv.add(buf.data(), total);
+  if (fnv.h != hash) { set_error(std::string("checkpoint hash mismatch: ") + path); return PSB_ERR_IO; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  // spheres: same parameters and count, state from the file
+  if (S) {
+    std::vector<float> c3(3 * (size_t)S, 0.f);
+    PSB_TRY(psb_system_set_spheres(s, S, c3.data(), nullptr, &sp));
+    if (s->per_sphere != h.per_sphere) { set_error("checkpoint sphere lattice differs"); return PSB_ERR_INVALID_ARGUMENT; }
+  } else if (s->n_spheres) {
+    PSB_TRY(psb_system_set_spheres(s, 0, nullptr, nullptr, nullptr));
+  }
+  hipStream_t st = s->ctx->stream;
+  size_t off = 0;
+  for (size_t i = 0; i < lay.size(); ++i) {
+    if (!(mask >> i & 1u)) continue;
+    void* dst = lay[i].first;
+    if (i == lay.size() - 3 && S) dst = s->d_sc[0];
+    if (i == lay.size() - 2 && S) dst = s->d_sv[0];
+    if (i == lay.size() - 1 && S) dst = s->d_sf;
+    if (dst && lay[i].second) PSB_HIP(hipMemcpyAsync(dst, buf.data() + off, lay[i].second, hipMemcpyHostToDevice, st));
+    off += lay[i].second;
+  }
+  PSB_HIP(hipStreamSynchronize(st));
+  if (S) {
+    // host copies of the current centres / velocities (a later set_particles
+    // re-appends the spheres there)
+    const float* c4 = (const float*)(buf.data() + total - 48 * (size_t)S);
+    const float* v4 = c4 + 4 * (size_t)S;
+    for (uint32_t j = 0; j < S; ++j)
+      for (int a = 0; a < 3; ++a) {
+        s->sphere_c[3 * j + a] = c4[4 * j + a];
+        s->sphere_v[3 * j + a] = v4[4 * j + a];
+      }
+    s->sphere_cur = 0;
+  }
+  s->n = h.n;
+  s->n_fluid = h.n_fluid;
+  s->c.n_fluid = S ? h.n_fluid : kNoBodies;
+  s->step_counter = h.step_counter;
+  s->stepped = h.stepped != 0;
+  s->stats = stats;
+  s->has_state = true;
   return PSB_OK;
 }
 

@@ -5,11 +5,19 @@
 //
 //   example_dam_break --material FILE --print-material      (no GPU needed)
 //   example_dam_break --save-material FILE                  (no GPU needed)
+//   example_dam_break --print-xml FILE | --print-txt FILE   (no GPU needed)
+//       particle files read by ParticleLoader_XML::load / ParticleLoader_Txt,
+//       one JSON line per particle
 //   example_dam_break --steps K --side N --out FILE [--config]
 //       runs K steps of an N^3 block (spacing 5 mm) and writes the binary
 //       record: params (psb_sph_params), n, initial float4 positions, final
 //       float4 positions / velocities (by particle id), iterations of the
 //       last step.
+//   state files, written after the K steps / read before them:
+//       --export-xml FILE / --export-txt FILE   ParticleExporter_XML / _Txt
+//       --checkpoint FILE                       binary checkpoint
+//       --resume-xml FILE                       ParticleLoader_XML::restore
+//       --resume FILE                           binary checkpoint
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +38,8 @@ static void print_material(const SimulationData_SPHSystem& m) {
 }
 
 int main(int argc, char** argv) {
-  std::string material, save, out;
+  std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
+  std::string print_xml, print_txt;
   bool print = false, config = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
@@ -43,6 +52,13 @@ int main(int argc, char** argv) {
     else if (a == "--side") side = std::stoi(next());
     else if (a == "--out") out = next();
     else if (a == "--config") config = true;
+    else if (a == "--export-xml") export_xml = next();
+    else if (a == "--export-txt") export_txt = next();
+    else if (a == "--checkpoint") checkpoint = next();
+    else if (a == "--resume-xml") resume_xml = next();
+    else if (a == "--resume") resume = next();
+    else if (a == "--print-xml") print_xml = next();
+    else if (a == "--print-txt") print_txt = next();
     else { std::fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
   }
   SimulationData_PCI_SPHSystem water;
@@ -51,6 +67,35 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (print) { print_material(water); return 0; }
+  if (!print_xml.empty()) {
+    ParticleLoader_XML loader;
+    std::vector<ParticleLoader_XML::Record> rs;
+    double dt = 0;
+    if (!loader.read(print_xml.c_str(), rs, &dt)) { std::fprintf(stderr, "not an SPH System file\n"); return 1; }
+    std::vector<SPHParticle*> ps = loader.load(print_xml.c_str());
+    std::printf("{\"count\": %zu, \"dt\": %.9g}\n", ps.size(), dt);
+    for (size_t i = 0; i < ps.size(); ++i) {
+      const SPHParticle* q = ps[i];
+      const Vector3 x = q->getPos(), v = q->getVel(), a = q->getVelInterAv(), b = q->getVelInterAp();
+      std::printf("{\"index\": %u, \"pos\": [%.9g, %.9g, %.9g], \"vel\": [%.9g, %.9g, %.9g], "
+                  "\"vav\": [%.9g, %.9g, %.9g], \"vap\": [%.9g, %.9g, %.9g], \"mass\": %.9g, "
+                  "\"support_radius\": %.9g, \"rest_density\": %.9g, \"viscosity\": %.9g}\n",
+                  rs[i].index, x.x, x.y, x.z, v.x, v.y, v.z, a.x, a.y, a.z, b.x, b.y, b.z,
+                  q->getMass(), q->getInteractionRadius(), q->getRestDensity(),
+                  q->getData()->getViscosity());
+      delete ps[i];
+    }
+    return 0;
+  }
+  if (!print_txt.empty()) {
+    std::vector<ParticleLoader_Txt::Record> rs;
+    if (!ParticleLoader_Txt().read(print_txt.c_str(), rs)) { std::fprintf(stderr, "not an SPH text file\n"); return 1; }
+    std::printf("{\"count\": %zu}\n", rs.size());
+    for (const auto& r : rs)
+      std::printf("{\"pos\": [%.9g, %.9g, %.9g], \"mass\": %.9g, \"density\": %.9g}\n", r.pos.x,
+                  r.pos.y, r.pos.z, r.mass, r.density);
+    return 0;
+  }
   if (!save.empty()) return water.saveConfiguration(save.c_str()) ? 0 : 1;
 
   try {
@@ -96,7 +141,15 @@ int main(int argc, char** argv) {
       x0[4 * i] = q.x; x0[4 * i + 1] = q.y; x0[4 * i + 2] = q.z;
     }
     sys.init(ps);
+    if (!resume_xml.empty() && !ParticleLoader_XML().restore(&sys, resume_xml.c_str())) {
+      std::fprintf(stderr, "cannot restore %s\n", resume_xml.c_str());
+      return 1;
+    }
+    if (!resume.empty()) sys.loadCheckpoint(resume.c_str());
     for (int k = 0; k < steps; ++k) sys.update();
+    if (!export_xml.empty() && !ParticleExporter_XML()._export(export_xml.c_str(), &sys)) return 1;
+    if (!export_txt.empty() && !ParticleExporter_Txt()._export(export_txt.c_str(), &sys)) return 1;
+    if (!checkpoint.empty()) sys.saveCheckpoint(checkpoint.c_str());
     std::vector<SPHParticle*> res = sys.getParticles();
     const psb_step_stats st = sys.stats();
     std::printf("particles %u steps %d iterations %d max_error %.6g\n", n, steps, st.iterations,

@@ -8,6 +8,11 @@
 //   EmitterBox                                lattice emitter (EmitterBox.cpp:29-47)
 //   System -> SPHSystem -> PCI_SPHSystem       init / update / collide /
 //        emitParticles / addForce / addObjectCollision / setDt (System.h:16-89)
+//   ParticleExporter_XML / ParticleLoader_XML  particle files
+//        (ParticleExporter_XML.cpp:131-190, ParticleLoader_XML.cpp:121-180)
+//   ParticleExporter_Txt / ParticleLoader_Txt  "SPH n" text files
+//        (ParticleExporter_Txt.cpp:26-34)
+//   SPHSystem::saveCheckpoint / loadCheckpoint  bit-exact resume (C-ABI)
 //
 // Differences that are deliberate: no GL (System() needs none), errors throw
 // std::runtime_error with psb_last_error() instead of being ignored, the host
@@ -20,6 +25,8 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -373,6 +380,38 @@ class SPHSystem : public System {
     return s;
   }
   psb_system handle() const { return sys; }
+  psb_context context() const { return ctx; }
+  Vector3 gravity() const {
+    Vector3 g;
+    for (auto& f : forces) g = g + f->getDirection() * f->getAmplitude();
+    return g;
+  }
+
+  // Binary checkpoint of the whole system state (psb_system_save_checkpoint);
+  // loadCheckpoint needs init() with the same setup first and continues the
+  // run bit for bit.
+  void saveCheckpoint(const char* filename) const {
+    if (!sys) throw std::runtime_error("saveCheckpoint before init");
+    check(psb_system_save_checkpoint(sys, filename));
+  }
+  void loadCheckpoint(const char* filename) {
+    if (!sys) throw std::runtime_error("loadCheckpoint before init");
+    check(psb_system_load_checkpoint(sys, filename));
+    dirty = true;
+  }
+
+  // State in a given storage order (ids: particle index of each slot), e.g.
+  // from ParticleLoader_XML::restore; marks the host mirror stale.
+  void setState(const std::vector<float>& pos4, const std::vector<float>& vel4,
+                const std::vector<float>& vh4, const std::vector<float>& pressure,
+                const std::vector<uint32_t>& ids) {
+    if (!sys) throw std::runtime_error("setState before init");
+    const uint32_t n = (uint32_t)ids.size();
+    if (n != particles.size()) throw std::runtime_error("setState: particle count differs from init()");
+    check(psb_system_set_state(sys, n, pos4.data(), vel4.data(), vh4.data(), pressure.data(),
+                               ids.data()));
+    dirty = true;
+  }
 
  protected:
   virtual int32_t solver() const { return PSB_SOLVER_WCSPH; }
@@ -398,6 +437,315 @@ class PCI_SPHSystem : public SPHSystem {
 
  protected:
   int32_t solver() const override { return PSB_SOLVER_PCISPH; }
+};
+
+// ---------------------------------------------------------------------------
+// Particle files
+// ---------------------------------------------------------------------------
+// Storage-order snapshot of a system: float4 arrays, scalars, ids.
+struct SystemSnapshot {
+  uint32_t n = 0;
+  std::vector<float> pos4, vel4, vh4, vav4, density, pressure;
+  std::vector<uint32_t> ids;
+};
+
+inline SystemSnapshot snapshot(const SPHSystem& S) {
+  SystemSnapshot o;
+  psb_system sys = S.handle();
+  if (!sys) throw std::runtime_error("snapshot before init");
+  o.n = psb_system_count(sys);
+  const size_t n = o.n;
+  o.pos4.resize(4 * n); o.vel4.resize(4 * n); o.vh4.resize(4 * n); o.vav4.resize(4 * n);
+  o.density.resize(n); o.pressure.resize(n); o.ids.resize(n);
+  check(psb_system_get_particles_f32(sys, o.pos4.data(), o.vel4.data(), o.vh4.data(),
+                                     o.density.data(), o.pressure.data(), o.ids.data()));
+  psb_system_arrays a{};
+  check(psb_system_get_arrays(sys, &a));
+  if (n) check(psb_copy_array_from_device(S.context(), o.vav4.data(), a.vel_inter_av4, 0, 16 * n));
+  return o;
+}
+
+// ParticleExporter_XML (ParticleExporter_XML.cpp:15-37, 131-190):
+//   <Particles name="SPH System" dt count> <Gravity X Y Z Amp/>
+//   <Particle index mass radius kernelParticles interactionRadius restDensity
+//    gasStiffness threshold surfaceTension viscosity>
+//     <position/> <velocity/> <velocityInterAv/> <velocityInterAp/> <color/>
+// Particles are written in the system's storage order with index = particle
+// id.  Differences: numbers carry 9 significant digits (an FP32 round trip;
+// TinyXML writes %f) and each Particle also carries its pressure, so that
+// ParticleLoader_XML::restore continues the run bit for bit.  Files it
+// writes load with the reference's loader.
+class ParticleExporter_XML {
+ public:
+  bool _export(const char* filename, SPHSystem* S) const {
+    const SystemSnapshot o = snapshot(*S);
+    FILE* f = std::fopen(filename, "w");
+    if (!f) return false;
+    const std::vector<SPHParticle*>& ps = S->getParticles();
+    if (ps.size() < o.n) { std::fclose(f); throw std::runtime_error("export: system holds more particles than init()"); }
+    const Vector3 g = S->gravity();
+    const float amp = std::sqrt(g.x * g.x + g.y * g.y + g.z * g.z);
+    const Vector3 dir = amp > 0.f ? g * (1.0f / amp) : Vector3();
+    std::fprintf(f, "<?xml version=\"1.0\" ?>\n<Particles name=\"SPH System\" dt=\"%.9g\" count=\"%u\">\n",
+                 S->getDt(), o.n);
+    std::fprintf(f, "    <Gravity X=\"%.9g\" Y=\"%.9g\" Z=\"%.9g\" Amp=\"%.9g\" />\n", dir.x, dir.y,
+                 dir.z, amp);
+    auto vec = [&](const char* tag, const float* v) {
+      std::fprintf(f, "        <%s X=\"%.9g\" Y=\"%.9g\" Z=\"%.9g\" />\n", tag, v[0], v[1], v[2]);
+    };
+    for (uint32_t k = 0; k < o.n; ++k) {
+      if (o.ids[k] >= ps.size()) { std::fclose(f); throw std::runtime_error("export: particle id beyond init()"); }
+      const SimulationData_SPHSystem* d = ps[o.ids[k]]->getData();
+      std::fprintf(f,
+                   "    <Particle index=\"%u\" mass=\"%.9g\" radius=\"%.9g\" kernelParticles=\"%.9g\" "
+                   "interactionRadius=\"%.9g\" restDensity=\"%.9g\" gasStiffness=\"%.9g\" "
+                   "threshold=\"%.9g\" surfaceTension=\"%.9g\" viscosity=\"%.9g\" "
+                   "pressure=\"%.9g\" density=\"%.9g\">\n",
+                   o.ids[k], d->getParticleMass(), d->getParticleRadius(), d->getKernelParticles(),
+                   d->getSupportRadius(), d->getRestDensity(), d->getGasStiffness(),
+                   d->getThreshold(), d->getSurfaceTension(), d->getViscosity(), o.pressure[k],
+                   o.density[k]);
+      vec("position", &o.pos4[4 * k]);
+      vec("velocity", &o.vel4[4 * k]);
+      vec("velocityInterAv", &o.vav4[4 * k]);
+      vec("velocityInterAp", &o.vh4[4 * k]);
+      const float col[3] = {0.f, 0.f, 1.f};
+      vec("color", col);
+      std::fprintf(f, "    </Particle>\n");
+    }
+    std::fprintf(f, "</Particles>\n");
+    return std::fclose(f) == 0;
+  }
+};
+
+// Streaming reader of the Particles / Particle / child-element structure.
+namespace detail {
+struct XmlElem {
+  std::string name;
+  std::map<std::string, std::string> attr;
+  bool closing = false, self_closing = false;
+};
+// Next element tag at or after pos (skips text, comments, declarations).
+inline bool next_tag(const std::string& s, size_t& pos, XmlElem& e) {
+  while (true) {
+    const size_t lt = s.find('<', pos);
+    if (lt == std::string::npos) return false;
+    if (s.compare(lt, 4, "<!--") == 0) {
+      const size_t end = s.find("-->", lt);
+      if (end == std::string::npos) return false;
+      pos = end + 3;
+      continue;
+    }
+    if (s.compare(lt, 2, "<?") == 0 || s.compare(lt, 2, "<!") == 0) {
+      const size_t end = s.find('>', lt);
+      if (end == std::string::npos) return false;
+      pos = end + 1;
+      continue;
+    }
+    const size_t gt = s.find('>', lt);
+    if (gt == std::string::npos) return false;
+    e = XmlElem{};
+    size_t i = lt + 1;
+    if (s[i] == '/') { e.closing = true; ++i; }
+    size_t j = i;
+    while (j < gt && !std::isspace((unsigned char)s[j]) && s[j] != '/') ++j;
+    e.name = s.substr(i, j - i);
+    e.self_closing = s[gt - 1] == '/';
+    i = j;
+    while (i < gt) {
+      while (i < gt && (std::isspace((unsigned char)s[i]) || s[i] == '/')) ++i;
+      const size_t eq = s.find('=', i);
+      if (eq == std::string::npos || eq > gt) break;
+      std::string key = s.substr(i, eq - i);
+      while (!key.empty() && std::isspace((unsigned char)key.back())) key.pop_back();
+      const size_t q0 = s.find_first_of("\"'", eq);
+      if (q0 == std::string::npos || q0 > gt) break;
+      const size_t q1 = s.find(s[q0], q0 + 1);
+      if (q1 == std::string::npos) break;
+      e.attr[key] = s.substr(q0 + 1, q1 - q0 - 1);
+      i = q1 + 1;
+    }
+    pos = gt + 1;
+    return true;
+  }
+}
+inline double attr_d(const XmlElem& e, const char* k, double dflt = 0.0) {
+  auto it = e.attr.find(k);
+  return it == e.attr.end() ? dflt : std::strtod(it->second.c_str(), nullptr);
+}
+inline bool read_file(const char* filename, std::string& out) {
+  std::ifstream f(filename, std::ios::binary);
+  if (!f) return false;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  out = ss.str();
+  return true;
+}
+}  // namespace detail
+
+// ParticleLoader_XML (ParticleLoader_XML.cpp:18-37, 121-180).
+//   load:    SPHParticles in file order, each with a material made of its
+//            per-particle attributes (the loader owns the materials);
+//   restore: the state of a file written by ParticleExporter_XML into a
+//            system init()ed with the same particles: storage order = file
+//            order, ids = index, with pressure -- the run continues bit for
+//            bit.  Spheres are not part of the file (use the checkpoint).
+class ParticleLoader_XML {
+ public:
+  struct Record {
+    uint32_t index = 0;
+    Vector3 pos, vel, vav, vap, color;
+    float pressure = 0.f, density = 0.f;
+    bool has_pressure = false;
+    float mass = 0, radius = 0, kernelParticles = 0, interactionRadius = 0, restDensity = 0,
+          gasStiffness = 0, threshold = 0, surfaceTension = 0, viscosity = 0;
+  };
+
+  // every Particle of an "SPH System" file; false when the file or its
+  // root is not one
+  bool read(const char* filename, std::vector<Record>& out, double* dt = nullptr) const {
+    std::string s;
+    if (!detail::read_file(filename, s)) return false;
+    size_t pos = 0;
+    detail::XmlElem e;
+    bool root = false;
+    while (detail::next_tag(s, pos, e)) {
+      if (e.closing) continue;
+      if (e.name == "Particles") {
+        auto it = e.attr.find("name");
+        if (it == e.attr.end() || it->second != "SPH System") return false;
+        if (dt) *dt = detail::attr_d(e, "dt", 0.0);
+        root = true;
+        continue;
+      }
+      if (!root || e.name != "Particle") continue;
+      Record r;
+      r.index = (uint32_t)detail::attr_d(e, "index", (double)out.size());
+      r.mass = (float)detail::attr_d(e, "mass");
+      r.radius = (float)detail::attr_d(e, "radius");
+      r.kernelParticles = (float)detail::attr_d(e, "kernelParticles");
+      r.interactionRadius = (float)detail::attr_d(e, "interactionRadius");
+      r.restDensity = (float)detail::attr_d(e, "restDensity");
+      r.gasStiffness = (float)detail::attr_d(e, "gasStiffness");
+      r.threshold = (float)detail::attr_d(e, "threshold");
+      r.surfaceTension = (float)detail::attr_d(e, "surfaceTension");
+      r.viscosity = (float)detail::attr_d(e, "viscosity");
+      r.has_pressure = e.attr.count("pressure") != 0;
+      r.pressure = (float)detail::attr_d(e, "pressure");
+      r.density = (float)detail::attr_d(e, "density");
+      if (!e.self_closing) {
+        detail::XmlElem c;
+        while (detail::next_tag(s, pos, c)) {
+          if (c.closing && c.name == "Particle") break;
+          const Vector3 v((float)detail::attr_d(c, "X"), (float)detail::attr_d(c, "Y"),
+                          (float)detail::attr_d(c, "Z"));
+          if (c.name == "position") r.pos = v;
+          else if (c.name == "velocity") r.vel = v;
+          else if (c.name == "velocityInterAv") r.vav = v;
+          else if (c.name == "velocityInterAp") r.vap = v;
+          else if (c.name == "color") r.color = v;
+        }
+      }
+      out.push_back(r);
+    }
+    return root;
+  }
+
+  std::vector<SPHParticle*> load(const char* filename) {
+    std::vector<Record> rs;
+    std::vector<SPHParticle*> out;
+    if (!read(filename, rs)) return out;
+    for (const Record& r : rs) {
+      auto m = std::make_unique<SimulationData_SPHSystem>();
+      m->setParticleMass(r.mass);
+      m->setParticleRadius(r.radius);
+      m->setKernelParticles(r.kernelParticles);
+      m->setSupportRadius(r.interactionRadius);
+      m->setRestDensity(r.restDensity);
+      m->setGasStiffness(r.gasStiffness);
+      m->setThreshold(r.threshold);
+      m->setSurfaceTension(r.surfaceTension);
+      m->setViscosity(r.viscosity);
+      SPHParticle* p = new SPHParticle(r.pos, r.vel, m.get());
+      p->setVelInterAv(r.vav);
+      p->setVelInterAp(r.vap);
+      p->setPressure(r.pressure);
+      p->setDensity(r.density);
+      materials.push_back(std::move(m));
+      out.push_back(p);
+    }
+    return out;
+  }
+
+  bool restore(SPHSystem* S, const char* filename) const {
+    std::vector<Record> rs;
+    if (!read(filename, rs)) return false;
+    const size_t n = rs.size();
+    std::vector<float> x(4 * n, 0.f), v(4 * n, 0.f), vh(4 * n, 0.f), p(n, 0.f);
+    std::vector<uint32_t> ids(n);
+    for (size_t k = 0; k < n; ++k) {
+      const Record& r = rs[k];
+      x[4 * k] = r.pos.x; x[4 * k + 1] = r.pos.y; x[4 * k + 2] = r.pos.z;
+      v[4 * k] = r.vel.x; v[4 * k + 1] = r.vel.y; v[4 * k + 2] = r.vel.z;
+      vh[4 * k] = r.vap.x; vh[4 * k + 1] = r.vap.y; vh[4 * k + 2] = r.vap.z;
+      p[k] = r.pressure;
+      ids[k] = r.index;
+    }
+    S->setState(x, v, vh, p, ids);
+    return true;
+  }
+
+ private:
+  std::vector<std::unique_ptr<SimulationData_SPHSystem>> materials;
+};
+
+// ParticleExporter_Txt, SPH branch (ParticleExporter_Txt.cpp:26-34):
+//   "SPH <count>" then "-p x y z -m mass -d density" per particle, by id
+//   (the reference prints %lf; 9 significant digits here).
+class ParticleExporter_Txt {
+ public:
+  bool _export(const char* filename, SPHSystem* S) const {
+    const SystemSnapshot o = snapshot(*S);
+    const std::vector<SPHParticle*>& ps = S->getParticles();
+    if (ps.size() != o.n) throw std::runtime_error("export: particle count differs from init()");
+    std::vector<uint32_t> slot(o.n);
+    for (uint32_t k = 0; k < o.n; ++k) slot[o.ids[k]] = k;
+    FILE* f = std::fopen(filename, "w");
+    if (!f) return false;
+    std::fprintf(f, "SPH %u\n", o.n);
+    for (uint32_t i = 0; i < o.n; ++i) {
+      const uint32_t k = slot[i];
+      std::fprintf(f, "-p %.9g %.9g %.9g -m %.9g -d %.9g\n", o.pos4[4 * k], o.pos4[4 * k + 1],
+                   o.pos4[4 * k + 2], ps[i]->getMass(), o.density[k]);
+    }
+    return std::fclose(f) == 0;
+  }
+};
+
+// Reader of the text format above (the reference reads text files of its
+// simple particle system only, ParticleLoader_Txt.cpp:17-43): positions,
+// masses and densities by particle id, velocities zero.
+class ParticleLoader_Txt {
+ public:
+  struct Record { Vector3 pos; float mass = 0.f, density = 0.f; };
+  bool read(const char* filename, std::vector<Record>& out) const {
+    FILE* f = std::fopen(filename, "r");
+    if (!f) return false;
+    unsigned count = 0;
+    if (std::fscanf(f, "SPH %u", &count) != 1) { std::fclose(f); return false; }
+    out.resize(count);
+    for (unsigned i = 0; i < count; ++i) {
+      Record& r = out[i];
+      if (std::fscanf(f, " -p %f %f %f -m %f -d %f", &r.pos.x, &r.pos.y, &r.pos.z, &r.mass,
+                      &r.density) != 5) {
+        std::fclose(f);
+        out.resize(i);
+        return false;
+      }
+    }
+    std::fclose(f);
+    return true;
+  }
 };
 
 }  // namespace psbhost

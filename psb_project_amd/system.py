@@ -122,10 +122,13 @@ class FluidSystem:
         sp = SphereParams(radius, density, spring, damping)
         check(lib().psb_system_set_spheres(self.handle, c.shape[0], c.ctypes.data,
                                            None if v is None else v.ctypes.data, C.byref(sp)))
-        self.num_spheres = c.shape[0]
+
+    @property
+    def num_spheres(self) -> int:
+        return int(lib().psb_system_num_spheres(self.handle))
 
     def spheres(self) -> dict:
-        m = getattr(self, "num_spheres", 0)
+        m = self.num_spheres
         o = {k: np.zeros((m, 3), np.float32) for k in ("centers", "velocities", "fluid_force")}
         if m:
             check(lib().psb_system_get_spheres(self.handle, o["centers"].ctypes.data,
@@ -147,6 +150,31 @@ class FluidSystem:
         if not ptr.value:
             raise RuntimeError("no features computed (enable the MLP warm start or a recorder)")
         return self.download(ptr.value, np.float32, (8, self.n))
+
+    def set_state(self, pos4, vel4=None, vh4=None, pressure=None, ids=None) -> None:
+        """Particles in a given storage order with their ids and pressure."""
+        p = np.ascontiguousarray(pos4, np.float32)
+        n = p.shape[0]
+        ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32)
+                for a in (vel4, vh4, pressure)]
+        check(lib().psb_system_set_state(self.handle, n, p.ctypes.data,
+                                         *[None if a is None else a.ctypes.data for a in arrs],
+                                         ids.ctypes.data))
+
+    def save_checkpoint(self, path) -> None:
+        check(lib().psb_system_save_checkpoint(self.handle, str(path).encode()))
+
+    def load_checkpoint(self, path) -> None:
+        check(lib().psb_system_load_checkpoint(self.handle, str(path).encode()))
+
+    @staticmethod
+    def checkpoint_info(path) -> dict:
+        prm = Params()
+        n, nf, ns = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().psb_checkpoint_info(str(path).encode(), C.byref(prm), C.byref(n), C.byref(nf),
+                                        C.byref(ns)))
+        return {"params": prm, "count": n.value, "fluid_count": nf.value, "num_spheres": ns.value}
 
     def set_compact_list(self, on: bool) -> None:
         """16-bit neighbour offsets in the corrector passes (default on)."""
