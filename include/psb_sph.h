@@ -296,6 +296,47 @@ psb_status psb_collision_box(psb_context ctx, float* new_pos4, const float* old_
                              const float box_min[3], const float box_max[3],
                              int32_t container, float elast);
 
+/* Static colliders besides the box, on the same integrate -> collide seam.
+ *   PSB_COLLIDER_SPHERE  SphereCollision (SphereCollision_Kernel.cu:5-80):
+ *       container: particles beyond `radius` from `origin` go back to the
+ *       surface; obstacle (container 0): particles inside go out to it.
+ *   PSB_COLLIDER_PLANE   PlaneCollision (PlaneCollision_Kernel.cu:6-80): an
+ *       axis-aligned quadrilateral `corners` A B C D with unit `normal`;
+ *       particles on or behind it inside the corners' bounds go to where
+ *       their step crossed it.
+ * Both then set v <- v - (1 + r)(v.n) n, r = elast |x - x_hit| / (dt |v|).
+ * Differences from the reference: FP32; the wrapper's swapped arguments
+ * (SphereCollision.cu:15-16 passes dt as `container`) are not reproduced;
+ * the obstacle branch, dead there (:28-45), pushes particles out along
+ * their direction from the centre; the plane's bounds test uses 1 - |N|
+ * (1 - N there misses every particle of a plane facing -axis); a
+ * zero-length or plane-parallel step projects onto the plane instead of
+ * dividing by zero.                                                       */
+#define PSB_COLLIDER_SPHERE 1
+#define PSB_COLLIDER_PLANE  2
+#define PSB_MAX_COLLIDERS   16
+typedef struct psb_collider {
+  int32_t type;
+  int32_t container;       /* sphere: 1 keeps particles inside            */
+  float   elast;
+  float   radius;          /* sphere                                       */
+  float   origin[3];       /* sphere centre                                */
+  float   corners[4][3];   /* plane: A B C D (Quadrilateral::create order) */
+  float   normal[3];       /* plane: axis-aligned unit normal               */
+} psb_collider;
+
+/* collisionSystem_Sphere_CUDA (SphereCollision.cu:8-17) and collisionPlan_CUDA
+ * (PlaneCollision.cu:8-22) on float4 arrays; old_vel4 and radius_particle
+ * are accepted and unused, as in the reference.                             */
+psb_status psb_collision_sphere(psb_context ctx, const float* old_pos4, float* new_pos4,
+                                const float* old_vel4, float* new_vel4, float radius_particle,
+                                float dt, uint32_t n, const float origin[3], float radius,
+                                float elast, int32_t container);
+psb_status psb_collision_plane(psb_context ctx, const float* old_pos4, float* new_pos4,
+                               const float* old_vel4, float* new_vel4, const float a[3],
+                               const float b[3], const float c[3], const float d[3],
+                               const float normal[3], float elast, float dt, uint32_t n);
+
 /* interactionSystem (CudaSystem.cu:6-14 -> collideParticles / collideParticle,
  * CudaParticle_Kernel.cu:10-55): DEM contact forces over each particle's
  * neighbour list, accumulated into forces4 (+=, in list order), with the
@@ -365,6 +406,12 @@ psb_status psb_system_set_particles_device(psb_system sys, uint32_t n,
                                            const float* pos4, const float* vel4,
                                            const float* vel_inter_ap4);
 uint32_t   psb_system_count(psb_system sys);
+/* System::addObjectCollision (System.h:38) for sphere and plane colliders:
+ * applied after the box each step, in the order added (the box, from
+ * psb_sph_params, comes first).  Not part of checkpoints.                  */
+psb_status psb_system_add_collider(psb_system sys, const psb_collider* collider);
+psb_status psb_system_clear_colliders(psb_system sys);
+
 /* Number of rigid spheres (psb_system_set_spheres or a loaded checkpoint).  */
 uint32_t psb_system_num_spheres(psb_system sys);
 /* SPHSystem::update / PCI_SPHSystem::update, `steps` times. */

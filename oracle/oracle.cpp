@@ -361,6 +361,112 @@ template <class R> inline void box_collide(const Derived<R>& d, V4<R>& x, V4<R>&
   }
 }
 
+// Colliders after the box (psb_collider).  SphereCollision_Kernel.cu:5-80 and
+// PlaneCollision_Kernel.cu:6-80 restated with the documented differences:
+// the obstacle sphere pushes particles out along their direction from the
+// centre (the reference's branch is dead, :28-45), the plane's bounds test
+// ignores the normal's axis through 1 - |N| (1 - N in the reference, which
+// misses every particle of a plane facing -axis), a zero-length or
+// plane-parallel step projects onto the plane (the reference divides by
+// zero), and each contact normal is normalised once.
+struct OColl {
+  int type = 0, container = 0;
+  float elast = 0, radius = 0, o[3] = {0, 0, 0}, a[3] = {0, 0, 0};
+  float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+};
+
+inline OColl make_ocoll(const psb_collider& in) {
+  OColl k;
+  k.type = in.type;
+  k.container = in.container ? 1 : 0;
+  k.elast = in.elast;
+  k.radius = in.radius;
+  for (int a = 0; a < 3; ++a) {
+    k.o[a] = in.origin[a];
+    // corner bounds as detectionPlan forms them (PlaneCollision_Kernel.cu:9-19)
+    float mn = 100000.f, mx = -100000.f;
+    for (int v = 0; v < 4; ++v) {
+      if (mn >= in.corners[v][a]) mn = in.corners[v][a];
+      if (mx <= in.corners[v][a]) mx = in.corners[v][a];
+    }
+    k.mn[a] = mn; k.mx[a] = mx;
+    k.a[a] = in.corners[0][a];
+    k.nrm[a] = in.normal[a];
+  }
+  return k;
+}
+
+// r = elast |x - x_int| / (dt |v|) (|v| > 0, elast > 0); v -= ((1 + r) v.n) n
+template <class R> inline void collider_response(R elast, R dt, V4<R>& x, V4<R>& v, R ix, R iy,
+                                                 R iz, R nx, R ny, R nz) {
+  const R vl = std::sqrt(dot3(v.x, v.y, v.z, v.x, v.y, v.z));
+  R r = 0;
+  if (vl > (R)0 && elast > (R)0) {
+    const R ex = x.x - ix, ey = x.y - iy, ez = x.z - iz;
+    r = (elast * std::sqrt(dot3(ex, ey, ez, ex, ey, ez))) / (dt * vl);
+  }
+  const R nl = std::sqrt(dot3(nx, ny, nz, nx, ny, nz));
+  nx = nx / nl; ny = ny / nl; nz = nz / nl;
+  const R vn = ((R)1 + r) * dot3(v.x, v.y, v.z, nx, ny, nz);
+  v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
+  x.x = ix; x.y = iy; x.z = iz;
+}
+
+template <class R> inline bool sphere_hit(const OColl& k, const V4<R>& x, R& ix, R& iy, R& iz,
+                                          R& nx, R& ny, R& nz) {
+  const R rad = (R)k.radius;
+  const R ox = x.x - (R)k.o[0], oy = x.y - (R)k.o[1], oz = x.z - (R)k.o[2];
+  const R d = std::sqrt(dot3(ox, oy, oz, ox, oy, oz));
+  if (k.container ? !(d >= rad) : !(d <= rad)) return false;
+  if (d == rad) {
+    ix = x.x; iy = x.y; iz = x.z;
+    nx = -ox / d; ny = -oy / d; nz = -oz / d;
+    return true;
+  }
+  R ux = 0, uy = 1, uz = 0;
+  if (d > (R)0) { ux = ox / d; uy = oy / d; uz = oz / d; }
+  ix = (R)k.o[0] + ux * rad; iy = (R)k.o[1] + uy * rad; iz = (R)k.o[2] + uz * rad;
+  if (k.container) { nx = (R)k.o[0] - ix; ny = (R)k.o[1] - iy; nz = (R)k.o[2] - iz; }
+  else { nx = ix - (R)k.o[0]; ny = iy - (R)k.o[1]; nz = iz - (R)k.o[2]; }
+  return true;
+}
+
+template <class R> inline bool plane_hit(const OColl& k, const V4<R>& x0, const V4<R>& x, R& ix,
+                                         R& iy, R& iz, R& nx, R& ny, R& nz) {
+  const R N0 = (R)k.nrm[0], N1 = (R)k.nrm[1], N2 = (R)k.nrm[2];
+  const R fx = (R)1 - std::fabs(N0), fy = (R)1 - std::fabs(N1), fz = (R)1 - std::fabs(N2);
+  const R px = x.x * fx, py = x.y * fy, pz = x.z * fz;
+  if (!(px <= (R)k.mx[0] * fx && px >= (R)k.mn[0] * fx && py <= (R)k.mx[1] * fy &&
+        py >= (R)k.mn[1] * fy && pz <= (R)k.mx[2] * fz && pz >= (R)k.mn[2] * fz))
+    return false;
+  const R d1 = dot3(x.x - (R)k.a[0], x.y - (R)k.a[1], x.z - (R)k.a[2], N0, N1, N2);
+  if (!(d1 <= (R)0)) return false;
+  const R aon = dot3((R)k.a[0] - x0.x, (R)k.a[1] - x0.y, (R)k.a[2] - x0.z, N0, N1, N2);
+  R dx = x.x - x0.x, dy = x.y - x0.y, dz = x.z - x0.z;
+  const R dl = std::sqrt(dot3(dx, dy, dz, dx, dy, dz));
+  R dn = 0;
+  if (dl > (R)0) {
+    dx = dx / dl; dy = dy / dl; dz = dz / dl;
+    dn = dot3(dx, dy, dz, N0, N1, N2);
+  }
+  if (dn != (R)0) {
+    const R t = aon / dn;
+    ix = x0.x + t * dx; iy = x0.y + t * dy; iz = x0.z + t * dz;
+  } else {
+    ix = x.x - d1 * N0; iy = x.y - d1 * N1; iz = x.z - d1 * N2;
+  }
+  nx = N0; ny = N1; nz = N2;
+  return true;
+}
+
+template <class R> inline void collide_one(const OColl& k, R dt, const V4<R>& x0, V4<R>& x,
+                                           V4<R>& v) {
+  R ix, iy, iz, nx, ny, nz;
+  const bool hit = k.type == PSB_COLLIDER_SPHERE ? sphere_hit(k, x, ix, iy, iz, nx, ny, nz)
+                                                 : plane_hit(k, x0, x, ix, iy, iz, nx, ny, nz);
+  if (hit) collider_response((R)k.elast, dt, x, v, ix, iy, iz, nx, ny, nz);
+}
+
 // Pair sums of internalForces (SphKernel.cu:47-78) and of
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:134-158)
 // for particle p; xs[q].w = p_q / rho_q^2 (pressure term, only when PRESS),
@@ -528,6 +634,7 @@ template <class R> struct System {
   psb_sphere_params sphere{};
   std::vector<float> sph_off;                       // [per][4]
   std::vector<V4<R>> sph_c, sph_v, sph_f, fb;       // centres, velocities, fluid forces
+  std::vector<OColl> colls;                         // colliders after the box
 
   bool bnd(uint32_t p) const { return n_fluid != kNone && Aid[p] >= n_fluid; }
 
@@ -573,6 +680,7 @@ template <class R> struct System {
   void finish_particle(uint32_t p, V4<R> x, V4<R> vh) {
     // collide (SphSystem.cpp:323-328) then interpolate (SphKernel.cu:128-135)
     if (prm.box_enabled) box_collide(d, x, vh);
+    for (const OColl& k : colls) collide_one(k, (R)d.dt, Ax[p], x, vh);
     Bx[p] = {x.x, x.y, x.z, 0};
     Bvh[p] = {vh.x, vh.y, vh.z, 0};
     const V4<R> va = Avh[p];
@@ -1333,6 +1441,27 @@ uint32_t oracle_system_count(void* hp) {
 }
 
 // rigid DEM spheres (before set_particles, which appends their boundary particles)
+int oracle_collision_collider(const psb_collider* k, float dt, uint32_t n, const float* old_pos4,
+                              float* new_pos4, float* new_vel4) {
+  const OColl c = make_ocoll(*k);
+  for (uint32_t i = 0; i < n; ++i) {
+    const V4<float> x0 = {old_pos4[4 * i], old_pos4[4 * i + 1], old_pos4[4 * i + 2], 0.f};
+    V4<float> x = {new_pos4[4 * i], new_pos4[4 * i + 1], new_pos4[4 * i + 2], new_pos4[4 * i + 3]};
+    V4<float> v = {new_vel4[4 * i], new_vel4[4 * i + 1], new_vel4[4 * i + 2], new_vel4[4 * i + 3]};
+    collide_one(c, dt, x0, x, v);
+    new_pos4[4 * i] = x.x; new_pos4[4 * i + 1] = x.y; new_pos4[4 * i + 2] = x.z;
+    new_vel4[4 * i] = v.x; new_vel4[4 * i + 1] = v.y; new_vel4[4 * i + 2] = v.z;
+  }
+  return 0;
+}
+
+int oracle_system_add_collider(void* hp, const psb_collider* k) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) h->d->colls.push_back(make_ocoll(*k));
+  else h->f->colls.push_back(make_ocoll(*k));
+  return 0;
+}
+
 int oracle_system_set_spheres(void* hp, uint32_t count, const float* centers3,
                               const float* velocities3, const psb_sphere_params* sp) {
   Handle* h = (Handle*)hp;

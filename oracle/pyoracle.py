@@ -42,6 +42,8 @@ def lib():
         L.oracle_collision_box.argtypes = [vp, C.c_uint32, vp, vp]
         L.oracle_wall_sums.argtypes = [vp, C.c_uint32, vp, vp]
         L.oracle_system_set_spheres.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.oracle_collision_collider.argtypes = [vp, C.c_float, C.c_uint32, vp, vp, vp]
+        L.oracle_system_add_collider.argtypes = [vp, vp]
         L.oracle_system_count.restype = C.c_uint32
         L.oracle_system_count.argtypes = [vp]
         L.oracle_system_get_spheres.argtypes = [vp, vp, vp, vp]
@@ -118,6 +120,17 @@ def collision_box(params, pos4, vel4):
     x = np.array(pos4, np.float32, copy=True)
     v = np.array(vel4, np.float32, copy=True)
     lib().oracle_collision_box(C.addressof(params), x.shape[0], x.ctypes.data, v.ctypes.data)
+    return x, v
+
+
+def collision_collider(collider, dt, old_pos4, pos4, vel4):
+    """One sphere / plane collider (psb_collider) on new positions and
+    half-step velocities, old_pos4 the start-of-step positions."""
+    x0 = np.ascontiguousarray(old_pos4, np.float32)
+    x = np.array(pos4, np.float32, copy=True)
+    v = np.array(vel4, np.float32, copy=True)
+    lib().oracle_collision_collider(C.addressof(collider), dt, x.shape[0], x0.ctypes.data,
+                                    x.ctypes.data, v.ctypes.data)
     return x, v
 
 
@@ -238,6 +251,9 @@ class OracleSystem:
         lib().oracle_system_set_spheres(self.h, c.shape[0], c.ctypes.data,
                                         None if v is None else v.ctypes.data,
                                         C.addressof(self._sp))
+
+    def add_collider(self, collider):
+        lib().oracle_system_add_collider(self.h, C.addressof(collider))
 
     def spheres(self) -> dict:
         m = getattr(self, "num_spheres", 0)

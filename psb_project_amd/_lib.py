@@ -165,6 +165,66 @@ class SphereParams(C.Structure):
     ]
 
 
+COLLIDER_SPHERE = 1
+COLLIDER_PLANE = 2
+MAX_COLLIDERS = 16
+
+
+class Collider(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("container", C.c_int32),
+        ("elast", C.c_float),
+        ("radius", C.c_float),
+        ("origin", C.c_float * 3),
+        ("corners", (C.c_float * 3) * 4),
+        ("normal", C.c_float * 3),
+    ]
+
+    @classmethod
+    def sphere(cls, origin, radius: float, elast: float = 0.0, container: bool = True):
+        k = cls()
+        k.type = COLLIDER_SPHERE
+        k.container = 1 if container else 0
+        k.elast = elast
+        k.radius = radius
+        for a in range(3):
+            k.origin[a] = float(origin[a])
+        return k
+
+    @classmethod
+    def plane(cls, corners, normal, elast: float = 0.0):
+        k = cls()
+        k.type = COLLIDER_PLANE
+        k.elast = elast
+        for v in range(4):
+            for a in range(3):
+                k.corners[v][a] = float(corners[v][a])
+        for a in range(3):
+            k.normal[a] = float(normal[a])
+        return k
+
+    @classmethod
+    def quadrilateral(cls, origin, length: float, width: float, orientation, elast: float = 0.0):
+        """PlaneCollision::create -> Quadrilateral::create (Quadrilateral.cpp:37-77):
+        corners of a length x width rectangle centred at origin, normal the
+        axis of `orientation` (+-1 on one axis)."""
+        import numpy as np
+        o = np.asarray(origin, np.float32)
+        L, W = np.float32(length) / np.float32(2), np.float32(width) / np.float32(2)
+        ax = int(np.flatnonzero(np.asarray(orientation) != 0)[0])
+        u, w = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[ax]
+        corners = []
+        for du, dw in ((-L, -W), (L, -W), (L, W), (-L, W)):
+            c = o.copy()
+            c[u] = o[u] + du
+            c[w] = o[w] + dw
+            corners.append(c)
+        n = [0.0, 0.0, 0.0]
+        n[ax] = float(orientation[ax])
+        return cls.plane(corners, n, elast)
+
+
 BATCH_PHASES = ("hash_count", "neighbors_density", "density_epilogue", "pci_nonpressure",
                 "pci_density", "pci_force")
 
@@ -217,6 +277,12 @@ SIGNATURES = {
     "psb_system_set_particles_device": (i32, [vp, u32, vp, vp, vp]),
     "psb_system_count": (u32, [vp]),
     "psb_system_num_spheres": (u32, [vp]),
+    "psb_system_add_collider": (i32, [vp, vp]),
+    "psb_system_clear_colliders": (i32, [vp]),
+    "psb_collision_sphere": (i32, [vp, vp, vp, vp, vp, C.c_float, C.c_float, u32, vp, C.c_float,
+                                   C.c_float, i32]),
+    "psb_collision_plane": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float,
+                                  u32]),
     "psb_system_update": (i32, [vp, i32]),
     "psb_system_get_stats": (i32, [vp, C.POINTER(StepStats)]),
     "psb_system_get_particles": (i32, [vp, dp, dp, dp, dp, dp, dp, dp, dp]),

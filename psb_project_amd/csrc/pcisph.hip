@@ -429,7 +429,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
     return;
   }
   if (STEP) {
-    finish_particle(c, p, x, vh, vh0, o.bx, o.bv, o.bvh);
+    finish_particle(c, p, x, vh, vh0, x0, o.bx, o.bv, o.bvh);
     o.bid[p] = b.aid[p];
     o.bp[p] = pres[p];
   } else {

@@ -61,6 +61,16 @@ struct GridDev {
   int R;            // stencil half-width in cells
 };
 
+// Static collider after the box (psb_collider on the device)
+struct ColliderDev {
+  int type, container;
+  float elast, radius;
+  float o[3];            // sphere centre
+  float a[3];            // plane corner A
+  float mn[3], mx[3];    // plane corner bounds as the reference forms them
+  float nrm[3];          // plane normal
+};
+
 struct Consts {
   float h, h2, h2x3, c_spiky, mc, beta, delta, tait_b, inv_rho0, rho0, k, kp;
   float cv, cs, cn, m, dt, sigma, l, gx, gy, gz;
@@ -79,9 +89,14 @@ struct Consts {
   float wall_s, wall_inv_s, wall_r;   // spacing, 1/spacing, h/spacing
   // rigid-sphere boundary particles: ids >= n_fluid (kNoBodies: none)
   uint32_t n_fluid;
+  // colliders applied after the box, in order (device array)
+  const ColliderDev* coll;
+  int ncoll;
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);
+// validated device form of a psb_collider
+psb_status make_collider(const psb_collider& in, ColliderDev* out);
 
 constexpr uint32_t kNoBodies = 0xFFFFFFFFu;
 
@@ -222,6 +237,96 @@ __device__ __forceinline__ void box_collide(const Consts& c, float4& x, float4& 
     float vn = c.one_plus_e * dot3(v.x, v.y, v.z, nx, ny, nz);
     v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
   }
+}
+
+// Velocity response and placement shared by the sphere and plane colliders
+// (SphereCollision_Kernel.cu:62-76, PlaneCollision_Kernel.cu:62-76):
+// r = elast |x - x_int| / (dt |v|) when |v| > 0 and elast > 0; with n the
+// normalised contact normal v <- v - ((1 + r)(v.n)) n; x <- x_int.
+__device__ __forceinline__ void collider_response(float elast, float dt, float4& x, float4& v,
+                                                  float ix, float iy, float iz, float nx,
+                                                  float ny, float nz) {
+  const float vl = sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z));
+  float r = 0.f;
+  if (vl > 0.f && elast > 0.f) {
+    const float ex = x.x - ix, ey = x.y - iy, ez = x.z - iz;
+    r = (elast * sqrtf(dot3(ex, ey, ez, ex, ey, ez))) / (dt * vl);
+  }
+  const float nl = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+  nx = nx / nl; ny = ny / nl; nz = nz / nl;
+  const float vn = (1.0f + r) * dot3(v.x, v.y, v.z, nx, ny, nz);
+  v.x = v.x - vn * nx; v.y = v.y - vn * ny; v.z = v.z - vn * nz;
+  x.x = ix; x.y = iy; x.z = iz;
+}
+
+// detection_collisionSphere (SphereCollision_Kernel.cu:5-47) in FP32.
+// Container: a particle at distance d >= R from the centre goes to the
+// surface point along its direction, normal towards the centre.  Obstacle
+// (d <= R): the surface point along its direction, normal outwards -- the
+// reference's obstacle branch could only fire at d == R (:28-45) and would
+// have used the opposite surface point.  d == R: stays, normal -OP/d.  A
+// particle exactly at the centre of an obstacle leaves along +y.
+__device__ __forceinline__ bool sphere_hit(const ColliderDev& k, const float4& x, float& ix,
+                                           float& iy, float& iz, float& nx, float& ny,
+                                           float& nz) {
+  const float ox = x.x - k.o[0], oy = x.y - k.o[1], oz = x.z - k.o[2];
+  const float d = sqrtf(dot3(ox, oy, oz, ox, oy, oz));
+  if (k.container ? !(d >= k.radius) : !(d <= k.radius)) return false;
+  if (d == k.radius) {
+    ix = x.x; iy = x.y; iz = x.z;
+    nx = -ox / d; ny = -oy / d; nz = -oz / d;
+    return true;
+  }
+  float ux = 0.f, uy = 1.f, uz = 0.f;
+  if (d > 0.f) { ux = ox / d; uy = oy / d; uz = oz / d; }
+  ix = k.o[0] + ux * k.radius; iy = k.o[1] + uy * k.radius; iz = k.o[2] + uz * k.radius;
+  if (k.container) { nx = k.o[0] - ix; ny = k.o[1] - iy; nz = k.o[2] - iz; }
+  else { nx = ix - k.o[0]; ny = iy - k.o[1]; nz = iz - k.o[2]; }
+  return true;
+}
+
+// detectionPlan (PlaneCollision_Kernel.cu:6-46) in FP32: inside the corner
+// bounds (the normal's axis ignored through the factors 1 - |N|) and on or
+// behind the plane, the particle goes to where its step x0 -> x crossed the
+// plane.  The reference's factors 1 - N compare the normal's axis at twice
+// its coordinate when N = -1, which no particle off the plane passes; a step
+// of zero length or parallel to the plane (the reference divides by zero)
+// projects x onto the plane instead.
+__device__ __forceinline__ bool plane_hit(const ColliderDev& k, const float4& x0, const float4& x,
+                                          float& ix, float& iy, float& iz, float& nx, float& ny,
+                                          float& nz) {
+  const float fx = 1.0f - fabsf(k.nrm[0]), fy = 1.0f - fabsf(k.nrm[1]), fz = 1.0f - fabsf(k.nrm[2]);
+  const float px = x.x * fx, py = x.y * fy, pz = x.z * fz;
+  if (!(px <= k.mx[0] * fx && px >= k.mn[0] * fx && py <= k.mx[1] * fy && py >= k.mn[1] * fy &&
+        pz <= k.mx[2] * fz && pz >= k.mn[2] * fz))
+    return false;
+  const float d1 = dot3(x.x - k.a[0], x.y - k.a[1], x.z - k.a[2], k.nrm[0], k.nrm[1], k.nrm[2]);
+  if (!(d1 <= 0.f)) return false;
+  const float aon = dot3(k.a[0] - x0.x, k.a[1] - x0.y, k.a[2] - x0.z, k.nrm[0], k.nrm[1], k.nrm[2]);
+  float dx = x.x - x0.x, dy = x.y - x0.y, dz = x.z - x0.z;
+  const float dl = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+  float dn = 0.f;
+  if (dl > 0.f) {
+    dx = dx / dl; dy = dy / dl; dz = dz / dl;
+    dn = dot3(dx, dy, dz, k.nrm[0], k.nrm[1], k.nrm[2]);
+  }
+  if (dn != 0.f) {
+    const float t = aon / dn;
+    ix = x0.x + t * dx; iy = x0.y + t * dy; iz = x0.z + t * dz;
+  } else {
+    ix = x.x - d1 * k.nrm[0]; iy = x.y - d1 * k.nrm[1]; iz = x.z - d1 * k.nrm[2];
+  }
+  nx = k.nrm[0]; ny = k.nrm[1]; nz = k.nrm[2];
+  return true;
+}
+
+// one collider on (x, v) with start-of-step position x0
+__device__ __forceinline__ void collide_one(const ColliderDev& k, float dt, const float4& x0,
+                                            float4& x, float4& v) {
+  float ix, iy, iz, nx, ny, nz;
+  const bool hit = k.type == PSB_COLLIDER_SPHERE ? sphere_hit(k, x, ix, iy, iz, nx, ny, nz)
+                                                 : plane_hit(k, x0, x, ix, iy, iz, nx, ny, nz);
+  if (hit) collider_response(k.elast, dt, x, v, ix, iy, iz, nx, ny, nz);
 }
 
 // Index range [lo, hi] along one axis of the lattice sites within h of

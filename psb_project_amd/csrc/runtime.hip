@@ -98,6 +98,46 @@ static psb_status check_params(const psb_sph_params& p) {
   return PSB_OK;
 }
 
+psb_status make_collider(const psb_collider& in, ColliderDev* out) {
+  ColliderDev k{};
+  k.type = in.type;
+  k.container = in.container ? 1 : 0;
+  k.elast = in.elast;
+  PSB_ARG(std::isfinite(in.elast) && in.elast >= 0.f, "collider elast must be finite and >= 0");
+  if (in.type == PSB_COLLIDER_SPHERE) {
+    PSB_ARG(std::isfinite(in.radius) && in.radius > 0.f, "sphere collider radius must be > 0");
+    for (int a = 0; a < 3; ++a) {
+      PSB_ARG(std::isfinite(in.origin[a]), "sphere collider origin must be finite");
+      k.o[a] = in.origin[a];
+    }
+    k.radius = in.radius;
+  } else if (in.type == PSB_COLLIDER_PLANE) {
+    int axis = -1;
+    for (int a = 0; a < 3; ++a) {
+      const float v = in.normal[a];
+      PSB_ARG(v == 0.f || v == 1.f || v == -1.f, "plane normal must be an axis direction");
+      if (v != 0.f) { PSB_ARG(axis < 0, "plane normal must be an axis direction"); axis = a; }
+    }
+    PSB_ARG(axis >= 0, "plane normal must be an axis direction");
+    // corner bounds as detectionPlan forms them (PlaneCollision_Kernel.cu:9-19)
+    for (int a = 0; a < 3; ++a) {
+      float mn = 100000.f, mx = -100000.f;
+      for (int v = 0; v < 4; ++v) {
+        PSB_ARG(std::isfinite(in.corners[v][a]), "plane corners must be finite");
+        if (mn >= in.corners[v][a]) mn = in.corners[v][a];
+        if (mx <= in.corners[v][a]) mx = in.corners[v][a];
+      }
+      k.mn[a] = mn; k.mx[a] = mx;
+      k.a[a] = in.corners[0][a];
+      k.nrm[a] = in.normal[a];
+    }
+  } else {
+    PSB_ARG(false, "collider type must be PSB_COLLIDER_SPHERE or PSB_COLLIDER_PLANE");
+  }
+  *out = k;
+  return PSB_OK;
+}
+
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   PSB_TRY(check_params(p));
   const double h = p.support_radius, m = p.mass, rho0 = p.rest_density, dt = p.dt, pi = M_PI;

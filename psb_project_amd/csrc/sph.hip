@@ -136,9 +136,10 @@ __global__ __launch_bounds__(256) void k_wcsph_forces(uint32_t n, Consts c,
   vh.x = vh.x + (F0 * inv) * c.dt;
   vh.y = vh.y + (F1 * inv) * c.dt;
   vh.z = vh.z + (F2 * inv) * c.dt;
-  float4 x = xs[p];
+  const float4 x0 = xs[p];
+  float4 x = x0;
   x.x = x.x + vh.x * c.dt; x.y = x.y + vh.y * c.dt; x.z = x.z + vh.z * c.dt;
-  finish_particle(c, p, x, vh, va, o.bx, o.bv, o.bvh);
+  finish_particle(c, p, x, vh, va, x0, o.bx, o.bv, o.bvh);
   o.bid[p] = o.aid[p];
   o.bp[p] = o.pres[p];
 }
@@ -191,6 +192,17 @@ __global__ __launch_bounds__(256) void k_box(uint32_t n, Consts c, float4* __res
   if (i >= n) return;
   float4 xi = x[i], vi = v[i];
   box_collide(c, xi, vi);
+  x[i] = xi;
+  v[i] = vi;
+}
+
+__global__ __launch_bounds__(256) void k_collider(uint32_t n, ColliderDev k, float dt,
+                                                  const float4* __restrict__ x0,
+                                                  float4* __restrict__ x, float4* __restrict__ v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 xi = x[i], vi = v[i];
+  collide_one(k, dt, x0[i], xi, vi);
   x[i] = xi;
   v[i] = vi;
 }
@@ -341,6 +353,51 @@ psb_status psb_collision_box(psb_context ctx, float* new_pos4, const float* old_
                      reinterpret_cast<float4*>(new_pos4), reinterpret_cast<float4*>(new_vel4));
   PSB_HIP(hipGetLastError());
   return PSB_OK;
+}
+
+static psb_status run_collider(psb_context ctx, const psb_collider& in, float dt, uint32_t n,
+                               const float* old_pos4, float* new_pos4, float* new_vel4) {
+  ColliderDev k;
+  PSB_TRY(make_collider(in, &k));
+  PSB_ARG(dt > 0.f, "dt must be > 0");
+  if (!n) return PSB_OK;
+  hipLaunchKernelGGL(k_collider, dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, n, k, dt,
+                     reinterpret_cast<const float4*>(old_pos4),
+                     reinterpret_cast<float4*>(new_pos4), reinterpret_cast<float4*>(new_vel4));
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
+psb_status psb_collision_sphere(psb_context ctx, const float* old_pos4, float* new_pos4,
+                                const float* old_vel4, float* new_vel4, float radius_particle,
+                                float dt, uint32_t n, const float origin[3], float radius,
+                                float elast, int32_t container) {
+  (void)old_vel4; (void)radius_particle;
+  PSB_ARG(ctx && origin && (n == 0 || (old_pos4 && new_pos4 && new_vel4)), "null argument");
+  psb_collider k{};
+  k.type = PSB_COLLIDER_SPHERE;
+  k.container = container;
+  k.elast = elast;
+  k.radius = radius;
+  for (int a = 0; a < 3; ++a) k.origin[a] = origin[a];
+  return run_collider(ctx, k, dt, n, old_pos4, new_pos4, new_vel4);
+}
+
+psb_status psb_collision_plane(psb_context ctx, const float* old_pos4, float* new_pos4,
+                               const float* old_vel4, float* new_vel4, const float a[3],
+                               const float b[3], const float c[3], const float d[3],
+                               const float normal[3], float elast, float dt, uint32_t n) {
+  (void)old_vel4;
+  PSB_ARG(ctx && a && b && c && d && normal && (n == 0 || (old_pos4 && new_pos4 && new_vel4)),
+          "null argument");
+  psb_collider k{};
+  k.type = PSB_COLLIDER_PLANE;
+  k.elast = elast;
+  for (int i = 0; i < 3; ++i) {
+    k.corners[0][i] = a[i]; k.corners[1][i] = b[i]; k.corners[2][i] = c[i]; k.corners[3][i] = d[i];
+    k.normal[i] = normal[i];
+  }
+  return run_collider(ctx, k, dt, n, old_pos4, new_pos4, new_vel4);
 }
 
 }  // extern "C"

@@ -77,13 +77,15 @@ __device__ __forceinline__ void surface_finish(const Consts& c, const PairSums& 
   }
 }
 
-// collide (box on position and half-step velocity, SphSystem.cpp:323-328)
-// then interpolateSPH_velocities (SphKernel.cu:128-135); writes the B state.
+// collide (box, then the added colliders in order, on position and half-step
+// velocity, SphSystem.cpp:323-328) then interpolateSPH_velocities
+// (SphKernel.cu:128-135); writes the B state.  x0: start-of-step position.
 __device__ __forceinline__ void finish_particle(const Consts& c, uint32_t p, float4 x, float4 vh,
-                                                float4 va, float4* __restrict__ bx,
+                                                float4 va, float4 x0, float4* __restrict__ bx,
                                                 float4* __restrict__ bv,
                                                 float4* __restrict__ bvh) {
   if (c.box_enabled) box_collide(c, x, vh);
+  for (int k = 0; k < c.ncoll; ++k) collide_one(c.coll[k], c.dt, x0, x, vh);
   bx[p] = make_float4(x.x, x.y, x.z, 0.f);
   bvh[p] = make_float4(vh.x, vh.y, vh.z, 0.f);
   bv[p] = make_float4((va.x + vh.x) * 0.5f, (va.y + vh.y) * 0.5f, (va.z + vh.z) * 0.5f, 0.f);

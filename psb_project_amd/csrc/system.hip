@@ -98,6 +98,8 @@ struct psb_system_s {
   float4* d_fb = nullptr;
   int sphere_cur = 0;
   int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: one)
+  std::vector<ColliderDev> colliders;   // after the box, in order
+  ColliderDev* d_coll = nullptr;
 };
 
 namespace psb {
@@ -394,7 +396,7 @@ psb_status psb_system_destroy(psb_system s) {
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
-                  (void*)s->d_sf, (void*)s->d_soff, (void*)s->d_fb})
+                  (void*)s->d_sf, (void*)s->d_soff, (void*)s->d_fb, (void*)s->d_coll})
     if (p) hipFree(p);
   for (auto e : s->ev) hipEventDestroy(e);
   for (auto e : s->ev_iter) hipEventDestroy(e);
@@ -764,6 +766,35 @@ psb_status psb_system_get_spheres(psb_system s, float* centers3, float* velociti
       if (fluid_forces3) fluid_forces3[3 * j + a] = f[4 * j + a];
     }
   return PSB_OK;
+}
+
+static psb_status upload_colliders(psb_system s) {
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  PSB_HIP(hipStreamSynchronize(s->ctx->stream));   // no step still reads the old array
+  if (s->d_coll) { PSB_HIP(hipFree(s->d_coll)); s->d_coll = nullptr; }
+  if (!s->colliders.empty()) {
+    const size_t nb = sizeof(ColliderDev) * s->colliders.size();
+    PSB_HIP(hipMalloc(&s->d_coll, nb));
+    PSB_HIP(hipMemcpy(s->d_coll, s->colliders.data(), nb, hipMemcpyHostToDevice));
+  }
+  s->c.coll = s->d_coll;
+  s->c.ncoll = (int)s->colliders.size();
+  return PSB_OK;
+}
+
+psb_status psb_system_add_collider(psb_system s, const psb_collider* k) {
+  PSB_ARG(s && k, "null argument");
+  PSB_ARG(s->colliders.size() < PSB_MAX_COLLIDERS, "at most PSB_MAX_COLLIDERS colliders");
+  ColliderDev d;
+  PSB_TRY(make_collider(*k, &d));
+  s->colliders.push_back(d);
+  return upload_colliders(s);
+}
+
+psb_status psb_system_clear_colliders(psb_system s) {
+  PSB_ARG(s, "null system");
+  s->colliders.clear();
+  return upload_colliders(s);
 }
 
 psb_status psb_system_set_compact_list(psb_system s, int32_t on) {

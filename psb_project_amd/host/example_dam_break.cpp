@@ -13,6 +13,8 @@
 //       record: params (psb_sph_params), n, initial float4 positions, final
 //       float4 positions / velocities (by particle id), iterations of the
 //       last step.
+//       --colliders adds a SphereCollision obstacle at the block's front and
+//       a PlaneCollision plate under it (printed as JSON lines "collider")
 //   state files, written after the K steps / read before them:
 //       --export-xml FILE / --export-txt FILE   ParticleExporter_XML / _Txt
 //       --checkpoint FILE                       binary checkpoint
@@ -40,7 +42,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
   std::string print_xml, print_txt;
-  bool print = false, config = false;
+  bool print = false, config = false, colliders = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -52,6 +54,7 @@ int main(int argc, char** argv) {
     else if (a == "--side") side = std::stoi(next());
     else if (a == "--out") out = next();
     else if (a == "--config") config = true;
+    else if (a == "--colliders") colliders = true;
     else if (a == "--export-xml") export_xml = next();
     else if (a == "--export-txt") export_txt = next();
     else if (a == "--checkpoint") checkpoint = next();
@@ -132,6 +135,24 @@ int main(int argc, char** argv) {
     sys.addForce(new ForceExt_Constante(Vector3(0, -1, 0), 9.81f));
     const float L = side * s;
     sys.addObjectCollision(BoxCollision::fromBounds(Vector3(0, 0, 0), Vector3(4 * L, 2 * L, 2 * L), 0.5f));
+    if (colliders) {
+      SphereCollision* ob = new SphereCollision(Vector3(1.2f * L, 0.2f * L, 0.5f * L), 0.25f * L,
+                                                0.f, 0.f, false);
+      PlaneCollision* plate = new PlaneCollision();
+      plate->create(Vector3(0.4f * L, 0.08f * L, 0.5f * L), 0.8f * L, L, 0.f, 0.f, false,
+                    Vector3(0, 1, 0));
+      for (const psb_collider& k : {ob->desc(), plate->desc()}) {
+        std::printf("{\"collider\": %d, \"container\": %d, \"elast\": %.9g, \"radius\": %.9g, "
+                    "\"origin\": [%.9g, %.9g, %.9g], \"corners\": [", k.type, k.container, k.elast,
+                    k.radius, k.origin[0], k.origin[1], k.origin[2]);
+        for (int v = 0; v < 4; ++v)
+          std::printf("%s[%.9g, %.9g, %.9g]", v ? ", " : "", k.corners[v][0], k.corners[v][1],
+                      k.corners[v][2]);
+        std::printf("], \"normal\": [%.9g, %.9g, %.9g]}\n", k.normal[0], k.normal[1], k.normal[2]);
+      }
+      sys.addObjectCollision(ob);
+      sys.addObjectCollision(plate);
+    }
     EmitterBox emitter(Vector3(0.5f * L, 0.5f * L, 0.5f * L), L - s, L - s, L - s, &water);
     std::vector<SPHParticle*> ps = emitter.emitParticles();
     const uint32_t n = (uint32_t)ps.size();

@@ -225,6 +225,86 @@ class BoxCollision {
   bool container;
 };
 
+// SphereCollision(origin, radius, elast, friction, is_container)
+// (SphereCollision.cpp:20-22): a static sphere collider; friction is kept
+// and unused, as in the reference.
+class SphereCollision {
+ public:
+  SphereCollision(Vector3 origin, float radius, float elast, float friction = 0.f,
+                  bool is_container = true)
+      : origin(origin), radius(radius), elast(elast), friction(friction), container(is_container) {}
+  psb_collider desc() const {
+    psb_collider k{};
+    k.type = PSB_COLLIDER_SPHERE;
+    k.container = container ? 1 : 0;
+    k.elast = elast;
+    k.radius = radius;
+    k.origin[0] = origin.x; k.origin[1] = origin.y; k.origin[2] = origin.z;
+    return k;
+  }
+  Vector3 getCenter() const { return origin; }
+  float getRadius() const { return radius; }
+
+ private:
+  Vector3 origin;
+  float radius, elast, friction;
+  bool container;
+};
+
+// PlaneCollision::create(origin, length, width, elast, friction, container,
+// orientation) (PlaneCollision.cpp:29-36 -> Quadrilateral::create,
+// Quadrilateral.cpp:37-77): an axis-aligned length x width rectangle
+// centred at origin, corners in the reference's order, normal along the
+// orientation's axis.
+class PlaneCollision {
+ public:
+  void create(Vector3 origin, float length, float width, float elast, float friction,
+              bool is_container, Vector3 orientation) {
+    (void)friction; (void)is_container;
+    this->elast = elast;
+    const float L = length / 2, W = width / 2;
+    Vector3 v[4];
+    if (orientation.x == -1 || orientation.x == 1) {
+      v[0] = Vector3(origin.x, origin.y - L, origin.z - W);
+      v[1] = Vector3(origin.x, origin.y + L, origin.z - W);
+      v[2] = Vector3(origin.x, origin.y + L, origin.z + W);
+      v[3] = Vector3(origin.x, origin.y - L, origin.z + W);
+      normal = Vector3(orientation.x, 0, 0);
+    } else if (orientation.y == -1 || orientation.y == 1) {
+      v[0] = Vector3(origin.x - L, origin.y, origin.z - W);
+      v[1] = Vector3(origin.x + L, origin.y, origin.z - W);
+      v[2] = Vector3(origin.x + L, origin.y, origin.z + W);
+      v[3] = Vector3(origin.x - L, origin.y, origin.z + W);
+      normal = Vector3(0, orientation.y, 0);
+    } else if (orientation.z == -1 || orientation.z == 1) {
+      v[0] = Vector3(origin.x - L, origin.y - W, origin.z);
+      v[1] = Vector3(origin.x + L, origin.y - W, origin.z);
+      v[2] = Vector3(origin.x + L, origin.y + W, origin.z);
+      v[3] = Vector3(origin.x - L, origin.y + W, origin.z);
+      normal = Vector3(0, 0, orientation.z);
+    } else {
+      throw std::runtime_error("PlaneCollision::create: orientation must be an axis direction");
+    }
+    for (int i = 0; i < 4; ++i) vertex[i] = v[i];
+  }
+  psb_collider desc() const {
+    psb_collider k{};
+    k.type = PSB_COLLIDER_PLANE;
+    k.elast = elast;
+    for (int i = 0; i < 4; ++i) {
+      k.corners[i][0] = vertex[i].x; k.corners[i][1] = vertex[i].y; k.corners[i][2] = vertex[i].z;
+    }
+    k.normal[0] = normal.x; k.normal[1] = normal.y; k.normal[2] = normal.z;
+    return k;
+  }
+  Vector3 getVertex(int i) const { return vertex[i]; }
+  Vector3 getNormale() const { return normal; }
+
+ private:
+  Vector3 vertex[4], normal;
+  float elast = 0.f;
+};
+
 // ForceExt_Constante(direction, amplitude) (ForceExt_Constante.cu:15-24)
 class ForceExt_Constante {
  public:
@@ -275,6 +355,9 @@ class System {
   virtual void collide() {}   // fused into update()
   std::vector<SPHParticle*> getParticles() { syncParticles(); return particles; }
   void addObjectCollision(BoxCollision* b) { box.reset(b); }
+  // colliders after the box, in the order added
+  void addObjectCollision(SphereCollision* c) { colliders.push_back(c->desc()); delete c; }
+  void addObjectCollision(PlaneCollision* c) { colliders.push_back(c->desc()); delete c; }
   void addForce(ForceExt_Constante* f) { forces.emplace_back(f); }
   int getNumParticles() const { return (int)particles.size(); }
   double getDt() const { return dt; }
@@ -284,6 +367,7 @@ class System {
  protected:
   std::vector<SPHParticle*> particles;
   std::unique_ptr<BoxCollision> box;
+  std::vector<psb_collider> colliders;
   std::vector<std::unique_ptr<ForceExt_Constante>> forces;
   double dt = 0.01;   // SphSystem.cpp:9
 };
@@ -337,6 +421,7 @@ class SPHSystem : public System {
     const uint32_t n = (uint32_t)particles.size();
     if (sys) { psb_system_destroy(sys); sys = nullptr; }
     check(psb_system_create(ctx, &prm, n, &sys));
+    for (const psb_collider& k : colliders) check(psb_system_add_collider(sys, &k));
     std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
       const Vector3 p = particles[i]->getPos(), u = particles[i]->getVel();

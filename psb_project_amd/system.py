@@ -123,6 +123,13 @@ class FluidSystem:
         check(lib().psb_system_set_spheres(self.handle, c.shape[0], c.ctypes.data,
                                            None if v is None else v.ctypes.data, C.byref(sp)))
 
+    def add_collider(self, collider) -> None:
+        """A _lib.Collider (sphere or plane) applied after the box each step."""
+        check(lib().psb_system_add_collider(self.handle, C.addressof(collider)))
+
+    def clear_colliders(self) -> None:
+        check(lib().psb_system_clear_colliders(self.handle))
+
     @property
     def num_spheres(self) -> int:
         return int(lib().psb_system_num_spheres(self.handle))
