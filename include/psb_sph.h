@@ -456,6 +456,55 @@ psb_status psb_system_load_checkpoint(psb_system sys, const char* path);
 psb_status psb_checkpoint_info(const char* path, psb_sph_params* params, uint32_t* count,
                                uint32_t* fluid_count, uint32_t* num_spheres);
 
+/* ------------------------------------------------------------------------ */
+/* Surface extraction: SurfaceSPH::extract (SurfaceSPH.cpp:29-190) --        */
+/* evaluateIso_Kernel (UniformGridKernel.cu:98-138) on a node grid over the  */
+/* particles' bounds plus a margin, marching cubes (MarchingCube_Kernel.cu:  */
+/* 14-158), vertex normals.  Differences from the reference: FP32; the field */
+/* sums every particle within r = h / scale (the reference scans floor(1/    */
+/* scale) cells of size h, which misses part of that radius); one vertex per */
+/* crossed grid edge shared by the triangles that use it (the reference      */
+/* merges vertices by float equality on the host); normals are the direction */
+/* of -grad of the extracted field (the reference sums an SPH colour         */
+/* gradient six times over); the case table is generated from the contours  */
+/* (same crossed edges and polygons as the reference's table in all 256      */
+/* cases, fan triangulation).                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct psb_surface_s* psb_surface;
+
+typedef struct psb_surface_params {
+  float cell_size;   /* node spacing (SurfaceSPH.cpp:31: 0.03)                  */
+  float scale;       /* field radius r = support_radius / scale (0.65)          */
+  float iso_level;   /* level of f(x) = sum_j (r^2 - |x - x_j|^2)^3 / r^6       */
+  float margin;      /* node grid beyond the particles' bounds, in r (2)        */
+} psb_surface_params;
+
+psb_status psb_surface_default_params(psb_surface_params* params);
+psb_status psb_surface_create(psb_context ctx, psb_surface* out);
+psb_status psb_surface_destroy(psb_surface s);
+/* Extract from the fluid particles of a system's current state. */
+psb_status psb_surface_extract(psb_surface s, psb_system sys, const psb_surface_params* params,
+                               uint32_t* num_vertices, uint32_t* num_triangles);
+/* Extract from n device float4 positions; particles whose id (ids, device,
+ * may be NULL) is >= id_limit are left out.                                 */
+psb_status psb_surface_extract_points(psb_surface s, const float* pos4, const uint32_t* ids,
+                                      uint32_t n, uint32_t id_limit, float support_radius,
+                                      const psb_surface_params* params, uint32_t* num_vertices,
+                                      uint32_t* num_triangles);
+/* Host copies of the last mesh: vertices and unit normals (3 floats each),
+ * triangles (3 vertex indices each, wound with the normal facing the low
+ * side of the field).  Any pointer may be NULL.                             */
+psb_status psb_surface_get_mesh(psb_surface s, float* vertices3, float* normals3,
+                                uint32_t* triangles3);
+/* The node field of the last extraction (x fastest) and its grid; node
+ * (i, j, k) sits at origin + (i, j, k) * cell.  field may be NULL.         */
+psb_status psb_surface_get_field(psb_surface s, float* field, uint32_t dims[3], float origin[3],
+                                 float* cell);
+/* The case table in use: per case (bit k: corner k <= iso, corners numbered
+ * as MarchingCube_Kernel.cu:45-53) its triangle count and up to 15 edge
+ * indices (16 entries per case, -1 padded).                                 */
+psb_status psb_surface_case_table(uint8_t* triangle_counts, int8_t* triangle_edges);
+
 /* Device pointers of the system's sorted state (valid until the next update). */
 typedef struct psb_system_arrays {
   const float*    pos4;          /* m_dPos[1] after the last step   */

@@ -1260,6 +1260,245 @@ void get_particles(System<R>* s, float* pos4, float* vel4, float* vh4, float* ol
   }
 }
 
+// ---------------------------------------------------------------------------
+// Surface extraction (SurfaceSPH.cpp:29-190, evaluateIso_Kernel
+// UniformGridKernel.cu:98-138, MarchingCube_Kernel.cu:14-158), restated with
+// the documented differences: the field sums every particle within
+// r = h / scale, one vertex per crossed node-grid edge (numbered by node,
+// x fastest, then axis x, y, z), normals along -grad f, and a case table built
+// from the contours: on each face (corners counter-clockwise seen from
+// outside) every maximal run of corners <= iso is cut off by a segment from
+// the crossed edge entering the run to the one leaving it; segments chain
+// into loops, a loop starts at its smallest edge, loops go in that order and
+// are fanned from their start, wound towards the low side.
+struct McTab {
+  int ntri[256];
+  int tri[256][15];
+};
+
+const int kMcCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                             {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+const int kMcEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                            {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// faces, corners counter-clockwise seen from outside: -x +x -y +y -z +z
+const int kMcFace[6][4] = {{3, 7, 4, 0}, {1, 5, 6, 2}, {1, 2, 3, 0},
+                           {4, 7, 6, 5}, {4, 5, 1, 0}, {3, 2, 6, 7}};
+
+int mc_edge_between(int a, int b) {
+  for (int e = 0; e < 12; ++e)
+    if ((kMcEdge[e][0] == a && kMcEdge[e][1] == b) || (kMcEdge[e][0] == b && kMcEdge[e][1] == a)) return e;
+  return -1;
+}
+
+McTab mc_table() {
+  McTab t{};
+  for (int cs = 0; cs < 256; ++cs) {
+    int next[12];
+    for (int e = 0; e < 12; ++e) next[e] = -1;
+    for (const auto& f : kMcFace)
+      for (int i = 0; i < 4; ++i) {
+        const int c = f[i], prev = f[(i + 3) % 4];
+        const bool bc = (cs >> c) & 1, bp = (cs >> prev) & 1;
+        if (!bc || bp) continue;
+        int j = i;
+        while ((cs >> f[(j + 1) % 4]) & 1) j = (j + 1) % 4;
+        next[mc_edge_between(prev, c)] = mc_edge_between(f[j], f[(j + 1) % 4]);
+      }
+    bool seen[12] = {false};
+    int nt = 0;
+    for (int e0 = 0; e0 < 12; ++e0) {
+      if (next[e0] < 0 || seen[e0]) continue;
+      int loop[12], m = 0;
+      for (int e = e0; !seen[e]; e = next[e]) { seen[e] = true; loop[m++] = e; }
+      for (int k = 1; k + 1 < m; ++k) {
+        // low side: with corner 0 alone below (case 1) the loop e0 -> e3 -> e8
+        // runs clockwise seen from corner 0, so triangles take (0, k+1, k)
+        t.tri[cs][3 * nt] = loop[0];
+        t.tri[cs][3 * nt + 1] = loop[k + 1];
+        t.tri[cs][3 * nt + 2] = loop[k];
+        ++nt;
+      }
+    }
+    for (int k = 3 * nt; k < 15; ++k) t.tri[cs][k] = -1;
+    t.ntri[cs] = nt;
+  }
+  return t;
+}
+
+struct SurfaceOut {
+  std::vector<float> field, verts, norms;
+  std::vector<uint32_t> tris;
+  uint32_t dims[3] = {0, 0, 0};
+  float origin[3] = {0, 0, 0}, cell = 0;
+};
+
+inline int clamp_cell(float x, float mn, float cell, int n) {
+  float t = std::floor((x - mn) / cell);
+  if (!(t >= 0.0f && t < (float)n)) t = (t >= (float)n) ? (float)(n - 1) : 0.0f;
+  return (int)t;
+}
+
+void surface_extract(const float* x4, const uint32_t* ids, uint32_t n, uint32_t limit, float h,
+                     const psb_surface_params& prm, SurfaceOut& o) {
+  auto kept = [&](uint32_t p) { return !ids || ids[p] < limit; };
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  bool any = false;
+  for (uint32_t p = 0; p < n; ++p) {
+    if (!kept(p)) continue;
+    for (int a = 0; a < 3; ++a) {
+      const float v = x4[4 * p + a];
+      if (!any || v < lo[a]) lo[a] = v;
+      if (!any || v > hi[a]) hi[a] = v;
+    }
+    any = true;
+  }
+  if (!any) return;
+  const float r = h / prm.scale;
+  const double margin = (double)prm.margin * (double)r;
+  double mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = (double)lo[a] - margin;
+    mx[a] = (double)hi[a] + margin;
+    const double len = std::fabs(mx[a] - mn[a]);
+    o.dims[a] = (uint32_t)std::ceil(len / (double)prm.cell_size);
+    o.origin[a] = (float)((mn[a] + mx[a]) / 2 - len / 2 + (double)prm.cell_size / 2);
+  }
+  o.cell = prm.cell_size;
+  const int NX = (int)o.dims[0], NY = (int)o.dims[1], NZ = (int)o.dims[2];
+  // particle grid of cell r
+  Grid g;
+  for (int a = 0; a < 3; ++a) g.min[a] = (float)mn[a];
+  g.cell = r;
+  g.nx = (int)std::ceil((mx[0] - mn[0]) / (double)r) + 1;
+  g.ny = (int)std::ceil((mx[1] - mn[1]) / (double)r) + 1;
+  g.nz = (int)std::ceil((mx[2] - mn[2]) / (double)r) + 1;
+  std::vector<uint32_t> hash(n), perm(n), start(g.ncells() + 1);
+  uint32_t fl = 0;
+  grid_build(g, n, x4, hash.data(), perm.data(), start.data(), fl);
+  auto near = [&](float x, float y, float z, auto&& f) {
+    const int cx = clamp_cell(x, g.min[0], g.cell, g.nx);
+    const int cy = clamp_cell(y, g.min[1], g.cell, g.ny);
+    const int cz = clamp_cell(z, g.min[2], g.cell, g.nz);
+    for (int zz = std::max(cz - 1, 0); zz <= std::min(cz + 1, g.nz - 1); ++zz)
+      for (int yy = std::max(cy - 1, 0); yy <= std::min(cy + 1, g.ny - 1); ++yy) {
+        const uint64_t row = (uint64_t)g.nx * ((uint64_t)yy + (uint64_t)g.ny * zz);
+        const uint32_t q0 = start[row + std::max(cx - 1, 0)], q1 = start[row + std::min(cx + 1, g.nx - 1) + 1];
+        for (uint32_t q = q0; q < q1; ++q) f(perm[q]);
+      }
+  };
+  const float r2 = r * r;
+  const float inv_r6 = 1.0f / (r2 * r2 * r2);
+  const float iso = prm.iso_level;
+  auto node = [&](int i, int j, int k, float& x, float& y, float& z) {
+    x = o.origin[0] + (float)i * o.cell;
+    y = o.origin[1] + (float)j * o.cell;
+    z = o.origin[2] + (float)k * o.cell;
+  };
+  const uint64_t NN = (uint64_t)NX * NY * NZ;
+  o.field.assign(NN, 0.f);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < (int64_t)NN; ++t) {
+    const int i = (int)(t % NX), j = (int)((t / NX) % NY), k = (int)(t / ((int64_t)NX * NY));
+    float x, y, z;
+    node(i, j, k, x, y, z);
+    float acc = 0.f;
+    near(x, y, z, [&](uint32_t p) {
+      if (!kept(p)) return;
+      const float dx = x - x4[4 * p], dy = y - x4[4 * p + 1], dz = z - x4[4 * p + 2];
+      const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (d2 <= r2) {
+        const float w = r2 - d2;
+        acc = std::fma(w * w, w, acc);
+      }
+    });
+    o.field[t] = acc * inv_r6;
+  }
+  // crossed edges, vertex numbers
+  std::vector<uint8_t> ef(NN);
+  std::vector<uint32_t> vbase(NN + 1);
+  uint32_t nv = 0;
+  const uint64_t SY = (uint64_t)NX, SZ = (uint64_t)NX * NY;
+  for (uint64_t t = 0; t < NN; ++t) {
+    const int i = (int)(t % NX), j = (int)((t / NX) % NY), k = (int)(t / SZ);
+    const bool b0 = o.field[t] <= iso;
+    uint8_t f = 0;
+    if (i + 1 < NX && (o.field[t + 1] <= iso) != b0) f |= 1;
+    if (j + 1 < NY && (o.field[t + SY] <= iso) != b0) f |= 2;
+    if (k + 1 < NZ && (o.field[t + SZ] <= iso) != b0) f |= 4;
+    ef[t] = f;
+    vbase[t] = nv;
+    nv += (f & 1) + ((f >> 1) & 1) + ((f >> 2) & 1);
+  }
+  vbase[NN] = nv;
+  o.verts.assign(3 * (size_t)nv, 0.f);
+  o.norms.assign(3 * (size_t)nv, 0.f);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < (int64_t)NN; ++t) {
+    if (!ef[t]) continue;
+    const int i = (int)(t % NX), j = (int)((t / NX) % NY), k = (int)(t / SZ);
+    float ax, ay, az;
+    node(i, j, k, ax, ay, az);
+    const float va = o.field[t];
+    uint32_t v = vbase[t];
+    for (int axis = 0; axis < 3; ++axis) {
+      if (!((ef[t] >> axis) & 1)) continue;
+      const uint64_t tb = t + (axis == 0 ? 1 : axis == 1 ? SY : SZ);
+      float bx, by, bz;
+      node(i + (axis == 0), j + (axis == 1), k + (axis == 2), bx, by, bz);
+      const float vb = o.field[tb];
+      float px, py, pz;
+      if (std::fabs(iso - va) < 0.000001f) { px = ax; py = ay; pz = az; }
+      else if (std::fabs(iso - vb) < 0.000001f) { px = bx; py = by; pz = bz; }
+      else if (std::fabs(va - vb) < 0.000001f) { px = ax; py = ay; pz = az; }
+      else {
+        const float mu = (iso - va) / (vb - va);
+        px = ax + mu * (bx - ax); py = ay + mu * (by - ay); pz = az + mu * (bz - az);
+      }
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+      near(px, py, pz, [&](uint32_t p) {
+        if (!kept(p)) return;
+        const float dx = px - x4[4 * p], dy = py - x4[4 * p + 1], dz = pz - x4[4 * p + 2];
+        const float d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (d2 < r2) {
+          const float w = r2 - d2;
+          const float w2 = w * w;
+          gx = std::fma(dx, w2, gx); gy = std::fma(dy, w2, gy); gz = std::fma(dz, w2, gz);
+        }
+      });
+      const float gl = std::sqrt(dot3(gx, gy, gz, gx, gy, gz));
+      if (gl > 0.f) { gx = gx / gl; gy = gy / gl; gz = gz / gl; }
+      o.verts[3 * v] = px; o.verts[3 * v + 1] = py; o.verts[3 * v + 2] = pz;
+      o.norms[3 * v] = gx; o.norms[3 * v + 1] = gy; o.norms[3 * v + 2] = gz;
+      ++v;
+    }
+  }
+  // cubes, x fastest
+  static const McTab tab = mc_table();
+  // edge -> node offset and axis
+  const int em[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 2}, {0, 0, 1, 0}, {0, 0, 0, 2},
+                         {0, 1, 0, 0}, {1, 1, 0, 2}, {0, 1, 1, 0}, {0, 1, 0, 2},
+                         {0, 0, 0, 1}, {1, 0, 0, 1}, {1, 0, 1, 1}, {0, 0, 1, 1}};
+  o.tris.clear();
+  for (int k = 0; k + 1 < NZ; ++k)
+    for (int j = 0; j + 1 < NY; ++j)
+      for (int i = 0; i + 1 < NX; ++i) {
+        const uint64_t t = (uint64_t)i + SY * j + SZ * k;
+        int cs = 0;
+        for (int c = 0; c < 8; ++c) {
+          const uint64_t tc = t + kMcCorner[c][0] + SY * kMcCorner[c][1] + SZ * kMcCorner[c][2];
+          if (o.field[tc] <= iso) cs |= 1 << c;
+        }
+        for (int q = 0; q < 3 * tab.ntri[cs]; ++q) {
+          const int e = tab.tri[cs][q];
+          const uint64_t nd = t + em[e][0] + SY * em[e][1] + SZ * em[e][2];
+          const int ax = em[e][3];
+          uint32_t rank = 0;
+          for (int b = 0; b < ax; ++b) rank += (ef[nd] >> b) & 1;
+          o.tris.push_back(vbase[nd] + rank);
+        }
+      }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1609,5 +1848,37 @@ int oracle_system_arrays(void* hp, uint32_t* hash, uint32_t* perm, uint32_t* sta
   if (h->prec == 64) fill(h->d); else fill(h->f);
   return 0;
 }
+
+int oracle_surface_table(int* ntri, int* tri15) {
+  const McTab t = mc_table();
+  for (int c = 0; c < 256; ++c) {
+    ntri[c] = t.ntri[c];
+    for (int k = 0; k < 15; ++k) tri15[15 * c + k] = t.tri[c][k];
+  }
+  return 0;
+}
+
+void* oracle_surface_extract(const float* pos4, const uint32_t* ids, uint32_t n, uint32_t limit,
+                             float h, const psb_surface_params* prm, uint32_t* nv, uint32_t* nt) {
+  auto* o = new SurfaceOut;
+  surface_extract(pos4, ids, n, limit, h, *prm, *o);
+  *nv = (uint32_t)(o->verts.size() / 3);
+  *nt = (uint32_t)(o->tris.size() / 3);
+  return o;
+}
+
+int oracle_surface_get(void* hp, float* v3, float* n3, uint32_t* t3, float* field, uint32_t* dims,
+                       float* origin) {
+  auto* o = (SurfaceOut*)hp;
+  if (v3) std::copy(o->verts.begin(), o->verts.end(), v3);
+  if (n3) std::copy(o->norms.begin(), o->norms.end(), n3);
+  if (t3) std::copy(o->tris.begin(), o->tris.end(), t3);
+  if (field) std::copy(o->field.begin(), o->field.end(), field);
+  if (dims) for (int a = 0; a < 3; ++a) dims[a] = o->dims[a];
+  if (origin) for (int a = 0; a < 3; ++a) origin[a] = o->origin[a];
+  return 0;
+}
+
+void oracle_surface_free(void* hp) { delete (SurfaceOut*)hp; }
 
 }  // extern "C"

@@ -43,6 +43,12 @@ def lib():
         L.oracle_wall_sums.argtypes = [vp, C.c_uint32, vp, vp]
         L.oracle_system_set_spheres.argtypes = [vp, C.c_uint32, vp, vp, vp]
         L.oracle_collision_collider.argtypes = [vp, C.c_float, C.c_uint32, vp, vp, vp]
+        L.oracle_surface_table.argtypes = [vp, vp]
+        L.oracle_surface_extract.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_float, vp,
+                                             vp, vp]
+        L.oracle_surface_extract.restype = vp
+        L.oracle_surface_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_surface_free.argtypes = [vp]
         L.oracle_system_add_collider.argtypes = [vp, vp]
         L.oracle_system_count.restype = C.c_uint32
         L.oracle_system_count.argtypes = [vp]
@@ -132,6 +138,41 @@ def collision_collider(collider, dt, old_pos4, pos4, vel4):
     lib().oracle_collision_collider(C.addressof(collider), dt, x.shape[0], x0.ctypes.data,
                                     x.ctypes.data, v.ctypes.data)
     return x, v
+
+
+def surface_table():
+    """(triangle counts [256], edge indices [256, 15], -1 padded)."""
+    nt = np.zeros(256, np.int32)
+    tri = np.zeros((256, 15), np.int32)
+    lib().oracle_surface_table(nt.ctypes.data, tri.ctypes.data)
+    return nt, tri
+
+
+def surface_extract(pos4, h, sparams, ids=None, limit=0xFFFFFFFF):
+    """Iso-surface of particles: dict(vertices, normals, triangles, field,
+    dims, origin)."""
+    x = np.ascontiguousarray(pos4, np.float32)
+    idp = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    hnd = lib().oracle_surface_extract(x.ctypes.data, None if idp is None else idp.ctypes.data,
+                                       x.shape[0], limit, h, C.addressof(sparams), C.byref(nv),
+                                       C.byref(nt))
+    try:
+        dims = np.zeros(3, np.uint32)
+        org = np.zeros(3, np.float32)
+        lib().oracle_surface_get(hnd, None, None, None, None, dims.ctypes.data, org.ctypes.data)
+        out = {"vertices": np.zeros((nv.value, 3), np.float32),
+               "normals": np.zeros((nv.value, 3), np.float32),
+               "triangles": np.zeros((nt.value, 3), np.uint32),
+               "field": np.zeros(int(np.prod(dims.astype(np.int64))), np.float32)}
+        lib().oracle_surface_get(hnd, out["vertices"].ctypes.data, out["normals"].ctypes.data,
+                                 out["triangles"].ctypes.data, out["field"].ctypes.data, None,
+                                 None)
+        out["dims"] = dims
+        out["origin"] = org
+        return out
+    finally:
+        lib().oracle_surface_free(hnd)
 
 
 def wall_sums(params, xyz):

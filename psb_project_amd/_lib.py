@@ -225,6 +225,15 @@ class Collider(C.Structure):
         return cls.plane(corners, n, elast)
 
 
+class SurfaceParams(C.Structure):
+    _fields_ = [
+        ("cell_size", C.c_float),
+        ("scale", C.c_float),
+        ("iso_level", C.c_float),
+        ("margin", C.c_float),
+    ]
+
+
 BATCH_PHASES = ("hash_count", "neighbors_density", "density_epilogue", "pci_nonpressure",
                 "pci_density", "pci_force")
 
@@ -278,6 +287,15 @@ SIGNATURES = {
     "psb_system_count": (u32, [vp]),
     "psb_system_num_spheres": (u32, [vp]),
     "psb_system_add_collider": (i32, [vp, vp]),
+    "psb_surface_default_params": (i32, [vp]),
+    "psb_surface_create": (i32, [vp, C.POINTER(vp)]),
+    "psb_surface_destroy": (i32, [vp]),
+    "psb_surface_extract": (i32, [vp, vp, vp, C.POINTER(u32), C.POINTER(u32)]),
+    "psb_surface_extract_points": (i32, [vp, vp, vp, u32, u32, C.c_float, vp, C.POINTER(u32),
+                                         C.POINTER(u32)]),
+    "psb_surface_get_mesh": (i32, [vp, vp, vp, vp]),
+    "psb_surface_get_field": (i32, [vp, vp, vp, vp, vp]),
+    "psb_surface_case_table": (i32, [vp, vp]),
     "psb_system_clear_colliders": (i32, [vp]),
     "psb_collision_sphere": (i32, [vp, vp, vp, vp, vp, C.c_float, C.c_float, u32, vp, C.c_float,
                                    C.c_float, i32]),

@@ -316,6 +316,18 @@ static psb_status grid_sort(psb_grid g, uint32_t n, const ReorderArrays* ra, hip
   return PSB_OK;
 }
 
+// start[0..n] = exclusive prefix sums of count[0..n) (start[n] = total);
+// count is zeroed; bsum holds one entry per kScanTile counts
+psb_status scan_exclusive(uint32_t* count, uint64_t n, uint32_t* bsum, uint32_t* start,
+                          hipStream_t s) {
+  const uint32_t nt = (uint32_t)((n + kScanTile - 1) / kScanTile);
+  if (nt) hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, s, count, n, bsum);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, bsum, nt, start, n);
+  if (nt) hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, count, n, bsum, start);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
 psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
                       hipStream_t s) {
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);

@@ -797,6 +797,15 @@ psb_status psb_system_clear_colliders(psb_system s) {
   return upload_colliders(s);
 }
 
+psb_status psb_surface_extract(psb_surface surf, psb_system s, const psb_surface_params* prm,
+                               uint32_t* num_vertices, uint32_t* num_triangles) {
+  PSB_ARG(surf && s && prm, "null argument");
+  if (!s->has_state) { set_error("psb_surface_extract before psb_system_set_particles"); return PSB_ERR_STATE; }
+  const uint32_t limit = s->n_spheres ? s->n_fluid : 0xFFFFFFFFu;
+  return psb_surface_extract_points(surf, (const float*)s->bx, s->bid, s->n, limit,
+                                    s->prm.support_radius, prm, num_vertices, num_triangles);
+}
+
 psb_status psb_system_set_compact_list(psb_system s, int32_t on) {
   PSB_ARG(s, "null system");
   s->compact = on != 0;
