@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the surface-extraction and checkpoint timings")
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM double dam break")
     ap.add_argument("--dem-steps", type=int, default=1000)
     ap.add_argument("--dem-warmup", type=int, default=5)
@@ -270,6 +272,43 @@ def dem_leg(ctx, args, model):
     return out
 
 
+def state_file_leg(ctx, fs, sc, reps: int = 5):
+    """Surface extraction and checkpoint save / load on the timed run's final
+    state (the rows downstream of the step)."""
+    import tempfile
+    from psb_project_amd import surface
+    from psb_project_amd.system import FluidSystem
+    out = {}
+    s = surface.Surface(ctx)
+    p = surface.default_params(cell_size=sc.meta["s"])
+    nv, nt = s.run(fs, p)                                   # warm-up, buffers sized
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nv, nt = s.run(fs, p)
+    dt = (time.perf_counter() - t0) / reps
+    nodes = int(np.prod(s.field()["dims"].astype(np.int64)))
+    s.close()
+    out["surface"] = {"ms_per_extraction": round(1e3 * dt, 3), "nodes": nodes,
+                      "nodes_per_s": round(nodes / dt, 1), "vertices": nv, "triangles": nt,
+                      "cell_size": sc.meta["s"], "scale": p.scale, "iso_level": p.iso_level,
+                      "particles": sc.n}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "state.ckpt")
+        t0 = time.perf_counter()
+        fs.save_checkpoint(path)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        other = FluidSystem(ctx, sc.params, sc.n)
+        t0 = time.perf_counter()
+        other.load_checkpoint(path)
+        t_load = time.perf_counter() - t0
+        other.close()
+    out["checkpoint"] = {"save_ms": round(1e3 * t_save, 2), "load_ms": round(1e3 * t_load, 2),
+                         "bytes": size, "particles": sc.n}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -294,6 +333,11 @@ def main():
     elapsed, stats, start_state = timed_run(ctx, sys_, args.steps, split=not args.no_cpu_baseline)
     timing = sys_.timing()
     sys_.set_timing(False)
+    extras = None
+    if not args.no_extras:
+        print("bench: surface extraction and checkpoint of the final state", file=sys.stderr,
+              flush=True)
+        extras = state_file_leg(ctx, sys_, sc)
     sys_.close()
 
     value = sc.n * args.steps / elapsed
@@ -360,6 +404,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "dem": dem,
+        "state_files": extras,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
     }

@@ -35,13 +35,17 @@ class Surface:
         self.handle = h
         ctx.adopt(self)
 
-    def extract(self, system, params: SurfaceParams | None = None) -> dict:
-        """Surface of a FluidSystem's fluid particles (current state)."""
+    def run(self, system, params: SurfaceParams | None = None) -> tuple[int, int]:
+        """Extract on the device only; (vertex count, triangle count)."""
         params = params or default_params()
         nv, nt = C.c_uint32(), C.c_uint32()
         check(lib().psb_surface_extract(self.handle, system.handle, C.byref(params), C.byref(nv),
                                         C.byref(nt)))
-        return self._fetch(nv.value, nt.value)
+        return nv.value, nt.value
+
+    def extract(self, system, params: SurfaceParams | None = None) -> dict:
+        """Surface of a FluidSystem's fluid particles (current state)."""
+        return self._fetch(*self.run(system, params))
 
     def extract_points(self, pos4, support_radius: float, params: SurfaceParams | None = None,
                        ids=None, id_limit: int = 0xFFFFFFFF) -> dict:
@@ -56,11 +60,8 @@ class Surface:
                                                C.byref(params), C.byref(nv), C.byref(nt)))
         return self._fetch(nv.value, nt.value)
 
-    def _fetch(self, nv: int, nt: int) -> dict:
-        out = {"vertices": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
-               "triangles": np.zeros((nt, 3), np.uint32)}
-        check(lib().psb_surface_get_mesh(self.handle, out["vertices"].ctypes.data,
-                                         out["normals"].ctypes.data, out["triangles"].ctypes.data))
+    def field(self) -> dict:
+        """Node field of the last extraction (x fastest) and its grid."""
         dims = np.zeros(3, np.uint32)
         org = np.zeros(3, np.float32)
         cell = C.c_float()
@@ -68,7 +69,16 @@ class Surface:
                                           C.byref(cell)))
         field = np.zeros(int(np.prod(dims.astype(np.int64))), np.float32)
         check(lib().psb_surface_get_field(self.handle, field.ctypes.data, None, None, None))
-        out.update(field=field, dims=dims, origin=org, cell=cell.value)
+        return {"field": field, "dims": dims, "origin": org, "cell": cell.value}
+
+    def _fetch(self, nv: int, nt: int) -> dict:
+        """Mesh of the last extraction; nv, nt must be the counts it returned
+        (the copies fill arrays of exactly that size)."""
+        out = {"vertices": np.zeros((nv, 3), np.float32), "normals": np.zeros((nv, 3), np.float32),
+               "triangles": np.zeros((nt, 3), np.uint32)}
+        check(lib().psb_surface_get_mesh(self.handle, out["vertices"].ctypes.data,
+                                         out["normals"].ctypes.data, out["triangles"].ctypes.data))
+        out.update(self.field())
         return out
 
     def close(self) -> None:
