@@ -61,7 +61,8 @@ struct psb_batch_s {
   uint32_t* nlist = nullptr;
   uint16_t* nlist16 = nullptr;    // compact twin of nlist for the corrector passes
   uint32_t* flags = nullptr;          // [0] flags, [1] max neighbours
-  PciStatus* st = nullptr;            // [S]
+  PciStatus* st = nullptr;
+  ErrSlots es{};                  // spread per-iteration error maxima            // [S]
   uint32_t* done = nullptr;
   int32_t* h_final = nullptr;         // host-mapped: iteration at which every scene finished
   int32_t* d_final = nullptr;
@@ -162,6 +163,7 @@ static void buffers(psb_batch b, PciBuffers& pb, PciOut& o) {
   o.host_final = b->d_final;
   o.done = b->done;
   o.num_scenes = b->S;
+  o.es = b->es;
 }
 
 static psb_status enqueue_iter(psb_batch b, int k) {
@@ -177,7 +179,8 @@ static psb_status enqueue_iter(psb_batch b, int k) {
     const int dflags = (k == 0 && mlp) ? (kDensWriteFeature | kDensSkipUpdate) : 0;
     PSB_TRY(launch_pci_density_batch(bmap(b, BP_DENSITY), bshape(b, BP_DENSITY), b->n, b->c,
                                      nb.count, nb.list, cur, b->pres, b->rho_pred, b->dl, b->st,
-                                     k, b->feat, dflags, st, b->lanes[BP_DENSITY], b->nlist16));
+                                     k, b->feat, dflags, st, b->lanes[BP_DENSITY], b->nlist16,
+                                     b->es));
   }
   if (k == 0 && mlp) PSB_TRY(mlp_infer(b->mlp, b->feat, b->n, b->pres, cur, 1, b->c.pscale, st));
   {
@@ -193,7 +196,7 @@ static psb_status enqueue_prologue(psb_batch b) {
   hipStream_t st = b->ctx->stream;
   const bool warm = b->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = b->prm.warm_start == PSB_WARM_MLP;
-  PSB_TRY(launch_pci_reset(b->st, false, st, b->S, b->done));
+  PSB_TRY(launch_pci_reset(b->st, false, st, b->S, b->done, b->es));
   {
     BTimed t(b, BP_HASH);
     ReorderArrays ra;
@@ -342,6 +345,8 @@ static psb_status batch_alloc(psb_batch b) {
   PSB_TRY(A((void**)&b->nlist16, 2 * N * (size_t)b->prm.max_neighbors));
   PSB_TRY(A((void**)&b->flags, 64));
   PSB_TRY(A((void**)&b->st, sizeof(PciStatus) * b->S));
+  b->es.stride = err_slot_stride(b->S);
+  PSB_TRY(A((void**)&b->es.base, 4 * (size_t)kErrSlots * b->es.stride));
   PSB_TRY(A((void**)&b->done, 64));
   PSB_TRY(A((void**)&b->dsc, sizeof(BatchScene) * b->S));
   PSB_HIP(hipHostMalloc((void**)&b->h_final, 64, hipHostMallocMapped | hipHostMallocCoherent));
@@ -482,7 +487,7 @@ psb_status psb_batch_destroy(psb_batch b) {
   drop_graphs(b);
   void* ptrs[] = {b->ax, b->av, b->avh, b->bx, b->bv, b->bvh, b->aid, b->bid, b->ap, b->bp,
                   b->rho, b->pres, b->rho_pred, b->anp, b->dl, b->xp0, b->xp1, b->feat,
-                  b->ncount, b->nlist, b->nlist16, b->flags, b->st, b->done, b->dsc};
+                  b->ncount, b->nlist, b->nlist16, b->flags, b->st, b->done, b->dsc, b->es.base};
   for (void* p : ptrs) if (p) hipFree(p);
   for (auto p : b->dblk) if (p) hipFree(p);
   if (b->h_final) hipHostFree(b->h_final);
@@ -587,7 +592,7 @@ psb_status psb_batch_update(psb_batch b, int32_t steps) {
   hipStream_t st = b->ctx->stream;
   PSB_HIP(hipMemsetAsync(b->flags, 0, 8, st));
   PSB_HIP(hipMemsetAsync(b->grid->flags, 0, 4, st));
-  PSB_TRY(launch_pci_reset(b->st, true, st, b->S, b->done));
+  PSB_TRY(launch_pci_reset(b->st, true, st, b->S, b->done, b->es));
   for (int k = 0; k < steps; ++k) PSB_TRY(batch_step(b, b->step_counter++));
   return fetch_stats(b);
 }
@@ -697,7 +702,7 @@ psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) 
     if (e != hipSuccess) r = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
   }
   hipFree(save);
-  if (r == PSB_OK) r = launch_pci_reset(b->st, true, st, b->S, b->done);
+  if (r == PSB_OK) r = launch_pci_reset(b->st, true, st, b->S, b->done, b->es);
   b->use_graphs = graphs;
   b->lat_on = lat;
   if (block_sizes)   // threads per workgroup; negative: two threads per particle

@@ -29,6 +29,24 @@ struct PciBuffers {
   const uint32_t* aid;
 };
 
+// The per-iteration maximum compression error, spread over kErrSlots words
+// kErrSlotStride or more apart (different memory channels): the D pass's
+// workgroups take slot blockIdx % kErrSlots, so their atomicMax do not
+// serialise on one address (one address cost ~8.6 us per D launch over 1024
+// workgroups on MI355X).  Slot s, scene c, iteration k:
+// base[s * stride + c * kMaxIterSlots + k].  base == nullptr: the single
+// PciStatus::err word instead.
+constexpr int kErrSlots = 16;
+constexpr uint32_t kErrSlotStride = 16384;   // words (64 KiB)
+struct ErrSlots {
+  uint32_t* base = nullptr;
+  uint32_t stride = 0;
+};
+inline uint32_t err_slot_stride(uint32_t scenes) {
+  const uint64_t need = (uint64_t)scenes * kMaxIterSlots;
+  return (uint32_t)((need + kErrSlotStride - 1) / kErrSlotStride * kErrSlotStride);
+}
+
 struct PciOut {
   // fused step: end-of-step state
   float4* bx; float4* bv; float4* bvh; uint32_t* bid; float* bp;
@@ -41,6 +59,7 @@ struct PciOut {
   // batches: scenes finished this step; host_final is written by the last
   uint32_t* done;
   uint32_t num_scenes;
+  ErrSlots es;
 };
 
 // Kernel launch shape (grid, workgroup).
@@ -50,7 +69,7 @@ struct Shape {
 
 // one status per scene (scenes = 1 for a single system); done: batch counter
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes = 1,
-                            uint32_t* done = nullptr);
+                            uint32_t* done = nullptr, ErrSlots es = ErrSlots{});
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,
@@ -63,7 +82,7 @@ psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
                               hipStream_t s, const uint32_t* ids = nullptr, int lanes = 1,
-                              const uint16_t* list16 = nullptr);
+                              const uint16_t* list16 = nullptr, ErrSlots es = ErrSlots{});
 psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
                          uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
                          uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s);
@@ -84,7 +103,8 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags, hipStream_t s,
-                                    int lanes = 1, const uint16_t* list16 = nullptr);
+                                    int lanes = 1, const uint16_t* list16 = nullptr,
+                                    ErrSlots es = ErrSlots{});
 psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,
                                   const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                   const float4* X, float4* Xn, const float* rho_pred,

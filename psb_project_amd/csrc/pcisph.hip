@@ -29,9 +29,13 @@
 namespace psb {
 
 // one block per scene; block 0 also clears the batch's finished-scene count
-__global__ void k_pci_reset(PciStatus* st0, uint32_t* done) {
+__global__ void k_pci_reset(PciStatus* st0, uint32_t* done, ErrSlots es) {
   PciStatus* st = st0 + blockIdx.x;
   for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
+  if (es.base)
+    for (int k = threadIdx.x; k < kErrSlots * kMaxIterSlots; k += blockDim.x)
+      es.base[(size_t)(k / kMaxIterSlots) * es.stride + (size_t)blockIdx.x * kMaxIterSlots +
+              (k % kMaxIterSlots)] = 0u;
   if (threadIdx.x == 0) { st->final_iter = -1; st->wide = 0u; }
   if (done && blockIdx.x == 0 && threadIdx.x == 0) *done = 0u;
 }
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     const uint32_t* __restrict__ list, float4* X, float* __restrict__ pres,
     float* __restrict__ rho_pred, const float* __restrict__ dl, PciStatus* __restrict__ st0, int k,
     float* __restrict__ feat, int dflags, const uint32_t* __restrict__ ids,
-    const uint16_t* __restrict__ list16) {
+    const uint16_t* __restrict__ list16, ErrSlots es) {
   __shared__ float red[M::kMaxThreads / 64];
   PciStatus* __restrict__ st = st0 + map.scene_of_block();   // a block serves one scene
   {
@@ -307,7 +311,12 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
   if (threadIdx.x == 0) {
     float m = red[0];
     for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
-    if (m > 0.f) atomicMax(&st->err[k], __float_as_uint(m));
+    if (m > 0.f) {
+      uint32_t* dst = es.base ? es.base + (size_t)(blockIdx.x % kErrSlots) * es.stride +
+                                    (size_t)map.scene_of_block() * kMaxIterSlots + k
+                              : &st->err[k];
+      atomicMax(dst, __float_as_uint(m));
+    }
   }
 }
 
@@ -332,7 +341,15 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   if (!warm_pass) {
     const int fi = st->final_iter;
     if (fi >= 0 && fi < k) return;
-    err = __uint_as_float(st->err[k]);
+    if (o.es.base) {
+      const uint32_t* e = o.es.base + (size_t)map.scene_of_block() * kMaxIterSlots + k;
+      uint32_t m = 0u;
+#pragma unroll
+      for (int q = 0; q < kErrSlots; ++q) m = max(m, e[(size_t)q * o.es.stride]);
+      err = __uint_as_float(m);
+    } else {
+      err = __uint_as_float(st->err[k]);
+    }
     const int it = k + 1;
     fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it;
     if (fin && lead) {
@@ -444,9 +461,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
 
 // ---- launchers ----------------------------------------------------------------
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes,
-                            uint32_t* done) {
+                            uint32_t* done, ErrSlots es) {
   if (stats) hipLaunchKernelGGL(k_pci_stats_reset, dim3(scenes), dim3(64), 0, s, st);
-  hipLaunchKernelGGL(k_pci_reset, dim3(scenes), dim3(256), 0, s, st, done);
+  hipLaunchKernelGGL(k_pci_reset, dim3(scenes), dim3(256), 0, s, st, done, es);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -499,14 +516,14 @@ static psb_status density_impl(const M& map, const Shape& sh, uint32_t n, const 
                                const uint32_t* cnt, const uint32_t* list, float4* X, float* pres,
                                float* rho_pred, const float* dl, PciStatus* st, int k,
                                float* feat, int dflags, const uint32_t* ids, hipStream_t s,
-                               const uint16_t* l16) {
+                               const uint16_t* l16, ErrSlots es) {
   if (!n) return PSB_OK;
   if (c.mode == PSB_MODE_REFERENCE)
     hipLaunchKernelGGL((k_pci_density<true, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, 0, ids, l16);
+                       pres, rho_pred, dl, st, k, feat, 0, ids, l16, es);
   else
     hipLaunchKernelGGL((k_pci_density<false, M>), sh.grid, sh.block, 0, s, map, n, c, cnt, list, X,
-                       pres, rho_pred, dl, st, k, feat, dflags, ids, l16);
+                       pres, rho_pred, dl, st, k, feat, dflags, ids, l16, es);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -546,12 +563,12 @@ psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
                               hipStream_t s, const uint32_t* ids, int lanes,
-                              const uint16_t* list16) {
+                              const uint16_t* list16, ErrSlots es) {
   if (lanes == 2)
     return density_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, cnt, list, X, pres, rho_pred,
-                        dl, st, k, feat, dflags, ids, s, list16);
+                        dl, st, k, feat, dflags, ids, s, list16, es);
   return density_impl(SingleMap{}, single_shape(n), n, c, cnt, list, X, pres, rho_pred, dl, st, k,
-                      feat, dflags, ids, s, list16);
+                      feat, dflags, ids, s, list16, es);
 }
 
 psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32_t* cnt,
@@ -580,12 +597,13 @@ psb_status launch_pci_density_batch(const BatchMap& map, const Shape& sh, uint32
                                     const Consts& c, const uint32_t* cnt, const uint32_t* list,
                                     float4* X, float* pres, float* rho_pred, const float* dl,
                                     PciStatus* st, int k, float* feat, int dflags,
-                                    hipStream_t s, int lanes, const uint16_t* list16) {
+                                    hipStream_t s, int lanes, const uint16_t* list16,
+                                    ErrSlots es) {
   if (lanes == 2)
     return density_impl(BatchMapT<2>{map.sc, map.blk_scene, map.bsel}, sh, n, c, cnt, list, X,
-                        pres, rho_pred, dl, st, k, feat, dflags, nullptr, s, list16);
+                        pres, rho_pred, dl, st, k, feat, dflags, nullptr, s, list16, es);
   return density_impl(map, sh, n, c, cnt, list, X, pres, rho_pred, dl, st, k, feat, dflags,
-                      nullptr, s, list16);
+                      nullptr, s, list16, es);
 }
 
 psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t n,

@@ -71,6 +71,7 @@ struct psb_system_s {
   bool compact = true;
   uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
   PciStatus* st = nullptr;
+  ErrSlots es{};                  // spread per-iteration error maxima (PCISPH)
   int32_t* h_final = nullptr;     // host-mapped final_iter per step slot
   int32_t* d_final = nullptr;
   bool has_state = false;
@@ -170,6 +171,8 @@ static psb_status sys_alloc(psb_system s) {
     }
     PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
     PSB_TRY(A((void**)&s->nlist16, 2 * N * (size_t)s->prm.max_neighbors));
+    s->es.stride = err_slot_stride(1);
+    PSB_TRY(A((void**)&s->es.base, 4 * (size_t)kErrSlots * s->es.stride));
   }
   PSB_TRY(A((void**)&s->ncount, 4 * N));
   PSB_TRY(A((void**)&s->nlist, 4 * N * (size_t)s->prm.max_neighbors));
@@ -235,7 +238,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
   const bool feat = !ref && (mlp || s->rec != nullptr);
-  PSB_TRY(launch_pci_reset(s->st, false, st));
+  PSB_TRY(launch_pci_reset(s->st, false, st, 1, nullptr, s->es));
   PSB_TRY(grid_phase(s, step));
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
   Compact16 c16;
@@ -261,6 +264,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
   o.host_final = ref ? nullptr : s->d_final + slot;
+  o.es = s->es;
   const bool surf = ref || s->prm.surface_tension != 0.f;
   {
     Timed t(s, PH_NONPRESSURE, step);
@@ -283,7 +287,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16));
+                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16, s->es));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
@@ -392,7 +396,7 @@ psb_status psb_system_destroy(psb_system s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
-                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat};
+                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat, s->es.base};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
@@ -563,7 +567,7 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   hipStream_t st = s->ctx->stream;
   PSB_HIP(hipMemsetAsync(s->flags, 0, 8, st));
   PSB_HIP(hipMemsetAsync(s->grid->flags, 0, 4, st));
-  PSB_TRY(launch_pci_reset(s->st, true, st));
+  PSB_TRY(launch_pci_reset(s->st, true, st, 1, nullptr, s->es));
   s->step_final.assign(steps, -1);
   for (int k = 0; k < steps && s->n > 0; ++k) {
     if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k, k & 63));
