@@ -216,7 +216,7 @@ static psb_status enqueue_prologue(psb_batch b) {
     c16.wide_stride = sizeof(PciStatus) / sizeof(uint32_t);
     PSB_TRY(launch_neighbors_batch(b->grid, bmap(b, BP_NEIGHBORS), b->nblk[s], 64u << s, b->ax,
                                    b->n, b->c.h2, b->gd.R, nb, b->rho, b->c.mc, b->flags,
-                                   b->flags + 1, st, c16));
+                                   b->flags + 1, st, c16, b->es));
   }
   {
     BTimed t(b, BP_EPILOGUE);

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
                                                               float* __restrict__ rho, float mc,
                                                               uint32_t* __restrict__ flags,
                                                               uint32_t* __restrict__ maxcnt,
-                                                              Compact16 c16) {
+                                                              Compact16 c16, ErrSlots es) {
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -278,9 +278,22 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     if (cnt > cap) atomicOr(flags, PSB_FLAG_NEIGHBOR_OVERFLOW);
     if (wide) atomicOr(c16.wide + (size_t)sc * c16.wide_stride, 1u);
   }
-  if (maxcnt) {
-    uint32_t m = wave_max_u(cnt);
-    if ((threadIdx.x & 63u) == 0 && m) atomicMax(maxcnt, m);
+  if (maxcnt || es.base) {
+    // workgroup maximum, then one atomic per workgroup on a spread slot
+    __shared__ uint32_t red[M::kMaxThreads / 64];
+    const uint32_t m = wave_max_u(cnt);
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b = 0;
+      for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) b = max(b, red[w]);
+      if (b) {
+        uint32_t* dst = es.base ? es.base + (size_t)(blockIdx.x % kErrSlots) * es.stride +
+                                      (size_t)map.scene_of_block() * kMaxIterSlots + kCountEntry
+                                : maxcnt;
+        atomicMax(dst, b);
+      }
+    }
   }
 }
 
@@ -352,13 +365,14 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
 psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
-                                  uint32_t* maxcnt, hipStream_t s, const Compact16& c16) {
+                                  uint32_t* maxcnt, hipStream_t s, const Compact16& c16,
+                                  ErrSlots es) {
   if (!n) return PSB_OK;
   GridDev none{};
   dim3 gb(nblocks), tb(bsize);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, BatchMap>), gb, tb, 0, s, map, xs, n, none, g->start, \
-                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16)
+                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {
@@ -371,14 +385,15 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
 
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho, float mc, uint32_t* flags,
-                            uint32_t* maxcnt, hipStream_t s, const Compact16& c16) {
+                            uint32_t* maxcnt, hipStream_t s, const Compact16& c16,
+                            ErrSlots es) {
   if (!n) return PSB_OK;
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
   dim3 gb(blocks_for(n)), tb(kBlock);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, SingleMap>), gb, tb, 0, s, SingleMap{}, xs, n, gd,    \
-                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16)
+                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {

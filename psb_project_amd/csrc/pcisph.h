@@ -29,24 +29,6 @@ struct PciBuffers {
   const uint32_t* aid;
 };
 
-// The per-iteration maximum compression error, spread over kErrSlots words
-// kErrSlotStride or more apart (different memory channels): the D pass's
-// workgroups take slot blockIdx % kErrSlots, so their atomicMax do not
-// serialise on one address (one address cost ~8.6 us per D launch over 1024
-// workgroups on MI355X).  Slot s, scene c, iteration k:
-// base[s * stride + c * kMaxIterSlots + k].  base == nullptr: the single
-// PciStatus::err word instead.
-constexpr int kErrSlots = 16;
-constexpr uint32_t kErrSlotStride = 16384;   // words (64 KiB)
-struct ErrSlots {
-  uint32_t* base = nullptr;
-  uint32_t stride = 0;
-};
-inline uint32_t err_slot_stride(uint32_t scenes) {
-  const uint64_t need = (uint64_t)scenes * kMaxIterSlots;
-  return (uint32_t)((need + kErrSlotStride - 1) / kErrSlotStride * kErrSlotStride);
-}
-
 struct PciOut {
   // fused step: end-of-step state
   float4* bx; float4* bv; float4* bvh; uint32_t* bid; float* bp;

@@ -358,6 +358,13 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       st->last_err = __float_as_uint(err);
       atomicAdd(&st->total_iters, (unsigned long long)it);
       atomicMax(&st->worst_err, __float_as_uint(err));
+      if (o.es.base) {
+        // the neighbour search's largest count of this step -> flags[1]
+        const uint32_t* e = o.es.base + (size_t)map.scene_of_block() * kMaxIterSlots + kCountEntry;
+        uint32_t m = 0u;
+        for (int q = 0; q < kErrSlots; ++q) m = max(m, e[(size_t)q * o.es.stride]);
+        if (m) atomicMax(o.flags + 1, m);
+      }
       if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
       // tell the host, which stops queueing speculative iterations once
       // every scene has finished

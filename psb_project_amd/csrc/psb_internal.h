@@ -490,6 +490,27 @@ __device__ __forceinline__ void dist_and_inverse(float d2, float& dd, float& inv
   }
 }
 
+// Per-step maxima reported by many workgroups, spread over kErrSlots words
+// kErrSlotStride or more apart (different memory channels): a workgroup
+// takes slot blockIdx % kErrSlots, so the atomicMax do not serialise on one
+// address (one address cost ~8.6 us per PCISPH density launch over 1024
+// workgroups, and ~22 us per neighbour search over 4096 waves, on MI355X).
+// Slot s, scene c, entry k: base[s * stride + c * kMaxIterSlots + k]; entry k
+// < kMaxIterSlots - 1 is iteration k's compression error, entry
+// kMaxIterSlots - 1 the step's largest neighbour count.  base == nullptr:
+// single words instead (PciStatus::err, the count flag word).
+constexpr int kErrSlots = 16;
+constexpr uint32_t kErrSlotStride = 16384;   // words (64 KiB)
+constexpr int kCountEntry = kMaxIterSlots - 1;
+struct ErrSlots {
+  uint32_t* base = nullptr;
+  uint32_t stride = 0;
+};
+inline uint32_t err_slot_stride(uint32_t scenes) {
+  const uint64_t need = (uint64_t)scenes * kMaxIterSlots;
+  return (uint32_t)((need + kErrSlotStride - 1) / kErrSlotStride * kErrSlotStride);
+}
+
 // Neighbour list readers: the ELL list of 32-bit indices, and its compact
 // twin of 16-bit offsets (q - p + 32768, |q - p| <= 32767) that halves the
 // list bytes the corrector passes stream every iteration.
@@ -622,7 +643,7 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
                             uint32_t* flags, uint32_t* maxcnt, hipStream_t s,
-                            const Compact16& c16 = Compact16{});
+                            const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{});
 // batches: g spans the cells of every scene (scene s owns cells
 // [cell_off, cell_off + its grid)); nblocks x bsize threads, scene-aligned
 psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
@@ -632,6 +653,6 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                                   uint32_t* maxcnt, hipStream_t s,
-                                  const Compact16& c16 = Compact16{});
+                                  const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{});
 
 }  // namespace psb

@@ -250,7 +250,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   {
     Timed t(s, PH_NEIGHBORS, step);
     PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
-                             s->flags, s->flags + 1, st, c16));
+                             s->flags, s->flags + 1, st, c16, s->es));
   }
   {
     Timed t(s, PH_EPILOGUE, step);

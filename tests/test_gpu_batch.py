@@ -55,6 +55,8 @@ def test_batch_matches_single_systems(ctx, warm, walls):
         assert bs["iterations"] == stats["iterations"]
         assert bs["total_iterations"] == stats["total_iterations"]
         assert bs["max_error"] == stats["max_error"]
+    # batch-wide largest neighbour count over the update = max of the scenes'
+    assert b.stats(0)["max_neighbors_seen"] == max(x[1]["max_neighbors_seen"] for x in ref) > 0
     b.close()
 
 

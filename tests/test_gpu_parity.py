@@ -208,6 +208,7 @@ def test_wcsph_system_bitwise(ctx, oracle, mode, steps):
     g, o, gi, oi = run_pair(ctx, oracle, sc, steps)
     assert_state_equal(g.state(), o.state())
     assert gi[0]["flags"] == oi[0]["flags"]
+    assert gi[0]["max_neighbors_seen"] == oi[0]["max_neighbors_seen"] > 0
 
 
 @pytest.mark.parametrize("lanes,compact", [(1, True), (2, True), (1, False)])
@@ -221,6 +222,8 @@ def test_pcisph_system_bitwise(ctx, oracle, mode, lanes, compact):
         assert a["iterations"] == b["iterations"]
         assert a["max_error"] == np.float32(b["max_error"])
         assert a["flags"] == b["flags"]
+    # the step's largest neighbour count (spread slots, folded by the final force pass)
+    assert gi[0]["max_neighbors_seen"] == oi[0]["max_neighbors_seen"] > 0
     assert_state_equal(g.state(), o.state())
 
 
