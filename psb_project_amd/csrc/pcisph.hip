@@ -341,12 +341,18 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   if (!warm_pass) {
     const int fi = st->final_iter;
     if (fi >= 0 && fi < k) return;
+    uint32_t nbmax = 0u;
     if (o.es.base) {
-      const uint32_t* e = o.es.base + (size_t)map.scene_of_block() * kMaxIterSlots + k;
-      uint32_t m = 0u;
-#pragma unroll
-      for (int q = 0; q < kErrSlots; ++q) m = max(m, e[(size_t)q * o.es.stride]);
-      err = __uint_as_float(m);
+      // one vector load per wave: lanes 0..15 read iteration k's error
+      // slots, lanes 16..31 the neighbour-count slots; wave maxima
+      const uint32_t ln = threadIdx.x & 63u;
+      const size_t e0 = (size_t)map.scene_of_block() * kMaxIterSlots;
+      uint32_t v = 0u, vc = 0u;
+      if (ln < (uint32_t)kErrSlots) v = o.es.base[(size_t)ln * o.es.stride + e0 + k];
+      else if (ln < 2u * kErrSlots)
+        vc = o.es.base[(size_t)(ln - kErrSlots) * o.es.stride + e0 + kCountEntry];
+      err = __uint_as_float(wave_max_u(v));
+      nbmax = wave_max_u(vc);
     } else {
       err = __uint_as_float(st->err[k]);
     }
@@ -358,13 +364,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       st->last_err = __float_as_uint(err);
       atomicAdd(&st->total_iters, (unsigned long long)it);
       atomicMax(&st->worst_err, __float_as_uint(err));
-      if (o.es.base) {
-        // the neighbour search's largest count of this step -> flags[1]
-        const uint32_t* e = o.es.base + (size_t)map.scene_of_block() * kMaxIterSlots + kCountEntry;
-        uint32_t m = 0u;
-        for (int q = 0; q < kErrSlots; ++q) m = max(m, e[(size_t)q * o.es.stride]);
-        if (m) atomicMax(o.flags + 1, m);
-      }
+      // the neighbour search's largest count of this step -> flags[1]
+      if (nbmax) atomicMax(o.flags + 1, nbmax);
       if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
       // tell the host, which stops queueing speculative iterations once
       // every scene has finished
