@@ -15,6 +15,8 @@
 //       last step.
 //       --colliders adds a SphereCollision obstacle at the block's front and
 //       a PlaneCollision plate under it (printed as JSON lines "collider")
+//       --surface FILE writes the SurfaceSPH mesh after the K steps (binary:
+//       nv, nt, vertices, normals, triangles)
 //   state files, written after the K steps / read before them:
 //       --export-xml FILE / --export-txt FILE   ParticleExporter_XML / _Txt
 //       --checkpoint FILE                       binary checkpoint
@@ -41,7 +43,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
-  std::string print_xml, print_txt;
+  std::string print_xml, print_txt, surface_out;
   bool print = false, config = false, colliders = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
@@ -62,6 +64,7 @@ int main(int argc, char** argv) {
     else if (a == "--resume") resume = next();
     else if (a == "--print-xml") print_xml = next();
     else if (a == "--print-txt") print_txt = next();
+    else if (a == "--surface") surface_out = next();
     else { std::fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
   }
   SimulationData_PCI_SPHSystem water;
@@ -171,6 +174,20 @@ int main(int argc, char** argv) {
     if (!export_xml.empty() && !ParticleExporter_XML()._export(export_xml.c_str(), &sys)) return 1;
     if (!export_txt.empty() && !ParticleExporter_Txt()._export(export_txt.c_str(), &sys)) return 1;
     if (!checkpoint.empty()) sys.saveCheckpoint(checkpoint.c_str());
+    if (!surface_out.empty()) {
+      SurfaceSPH surf;
+      surf.params().cell_size = s;
+      if (!surf.extract(&sys, 0.5, steps)) return 1;
+      FILE* f = std::fopen(surface_out.c_str(), "wb");
+      if (!f) return 1;
+      const uint32_t nv = (uint32_t)surf.getVertexs().size(), nt = (uint32_t)surf.getIndexs().size() / 3;
+      std::fwrite(&nv, 4, 1, f);
+      std::fwrite(&nt, 4, 1, f);
+      for (const Vector3& q : surf.getVertexs()) { const float a3[3] = {q.x, q.y, q.z}; std::fwrite(a3, 4, 3, f); }
+      for (const Vector3& q : surf.getNormales()) { const float a3[3] = {q.x, q.y, q.z}; std::fwrite(a3, 4, 3, f); }
+      std::fwrite(surf.getIndexs().data(), 4, surf.getIndexs().size(), f);
+      std::fclose(f);
+    }
     std::vector<SPHParticle*> res = sys.getParticles();
     const psb_step_stats st = sys.stats();
     std::printf("particles %u steps %d iterations %d max_error %.6g\n", n, steps, st.iterations,

@@ -13,6 +13,7 @@
 //   ParticleExporter_Txt / ParticleLoader_Txt  "SPH n" text files
 //        (ParticleExporter_Txt.cpp:26-34)
 //   SPHSystem::saveCheckpoint / loadCheckpoint  bit-exact resume (C-ABI)
+//   SurfaceSPH                                 iso-surface mesh (SurfaceSPH.cpp:29-190)
 //
 // Differences that are deliberate: no GL (System() needs none), errors throw
 // std::runtime_error with psb_last_error() instead of being ignored, the host
@@ -522,6 +523,55 @@ class PCI_SPHSystem : public SPHSystem {
 
  protected:
   int32_t solver() const override { return PSB_SOLVER_PCISPH; }
+};
+
+// ---------------------------------------------------------------------------
+// Surface
+// ---------------------------------------------------------------------------
+// SurfaceSPH (SurfaceSPH.h, SurfaceSPH.cpp:29-190): extract(S, isoLevel, step)
+// fills the indexed mesh -- unique vertices, their normals, and the triangle
+// list (three indices per triangle) -- from the system's fluid particles.
+class SurfaceSPH {
+ public:
+  SurfaceSPH() { check(psb_surface_default_params(&prm)); }
+  ~SurfaceSPH() {
+    if (surf) psb_surface_destroy(surf);
+  }
+  SurfaceSPH(const SurfaceSPH&) = delete;
+  SurfaceSPH& operator=(const SurfaceSPH&) = delete;
+
+  // the node spacing, kernel scale and margin (SurfaceSPH.cpp:31-32 by default)
+  psb_surface_params& params() { return prm; }
+
+  bool extract(SPHSystem* S, double isoLevel, unsigned step = 0) {
+    (void)step;
+    if (!S || !S->handle()) return false;
+    if (!surf) check(psb_surface_create(S->context(), &surf));
+    prm.iso_level = (float)isoLevel;
+    uint32_t nv = 0, nt = 0;
+    check(psb_surface_extract(surf, S->handle(), &prm, &nv, &nt));
+    std::vector<float> v(3 * (size_t)nv), n(3 * (size_t)nv);
+    std::vector<uint32_t> t(3 * (size_t)nt);
+    check(psb_surface_get_mesh(surf, v.data(), n.data(), t.data()));
+    vertexs_cpu.resize(nv);
+    normales_cpu.resize(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+      vertexs_cpu[i] = Vector3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+      normales_cpu[i] = Vector3(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
+    }
+    indexs_cpu.assign(t.begin(), t.end());
+    return true;
+  }
+
+  const std::vector<Vector3>& getVertexs() const { return vertexs_cpu; }
+  const std::vector<Vector3>& getNormales() const { return normales_cpu; }
+  const std::vector<uint32_t>& getIndexs() const { return indexs_cpu; }
+
+ private:
+  psb_surface surf = nullptr;
+  psb_surface_params prm{};
+  std::vector<Vector3> vertexs_cpu, normales_cpu;
+  std::vector<uint32_t> indexs_cpu;
 };
 
 // ---------------------------------------------------------------------------
