@@ -531,6 +531,14 @@ psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
  * 32-bit list by itself.  Results do not depend on it.                     */
 psb_status psb_system_set_compact_list(psb_system sys, int32_t on);
 
+/* Neighbour rows staged in LDS by the PCISPH corrector passes (off by
+ * default; used with one thread per particle and the compact list): the
+ * sorted particles are cut into tiles of 512, each tile's neighbour rows are
+ * copied into LDS once per pass and the neighbours read from there.  A tile
+ * whose rows exceed 4,096 particles keeps the global path.  Results do not
+ * depend on it.                                                            */
+psb_status psb_system_set_halo(psb_system sys, int32_t on);
+
 /* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
  * sums split into even and odd list positions, one lane each), or 0 (the
  * default, 1).  Results do not depend on it: the sums always run in the

@@ -309,6 +309,7 @@ SIGNATURES = {
     "psb_system_set_timing": (i32, [vp, i32]),
     "psb_system_set_corrector_lanes": (i32, [vp, i32]),
     "psb_system_set_compact_list": (i32, [vp, i32]),
+    "psb_system_set_halo": (i32, [vp, i32]),
     "psb_system_set_state": (i32, [vp, u32, vp, vp, vp, vp, vp]),
     "psb_system_save_checkpoint": (i32, [vp, C.c_char_p]),
     "psb_system_load_checkpoint": (i32, [vp, C.c_char_p]),

@@ -199,7 +199,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
                                                               float* __restrict__ rho, float mc,
                                                               uint32_t* __restrict__ flags,
                                                               uint32_t* __restrict__ maxcnt,
-                                                              Compact16 c16, ErrSlots es) {
+                                                              Compact16 c16, ErrSlots es,
+                                                              const HaloTile* __restrict__ tiles) {
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -217,6 +218,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     const int x0 = max(cx - R, 0), x1 = min(cx + R, g.nx - 1);
     float S = 0.f;
     bool wide = false;
+    // staged tiles: the compact entry is the LDS slot q + sb of the row
+    const HaloTile* T = tiles ? tiles + p / kHaloTile : nullptr;
+    const bool hok = T && T->ok;
+    int32_t sb = 0;
     auto visit = [&](uint32_t q, const float4& xj) {
       const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
       const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
@@ -225,7 +230,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
           nlist[(uint64_t)cnt * n + p] = q;
           if (c16.list16) {
             const int d = (int)(q - p);
-            if (d >= -kCompactReach && d <= kCompactReach)
+            if (hok)
+              c16.list16[(uint64_t)cnt * n + p] = (uint16_t)((int32_t)q + sb);
+            else if (d >= -kCompactReach && d <= kCompactReach)
               c16.list16[(uint64_t)cnt * n + p] = (uint16_t)(d + 32768);
             else
               wide = true;
@@ -260,7 +267,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
         q0s[r] = ok ? start[base + x0] : 0u;
         q1s[r] = ok ? start[base + x1 + 1] : 0u;
       }
-      for (int r = 0; r < W * W; ++r) row(q0s[r], q1s[r]);
+      for (int r = 0; r < W * W; ++r) {
+        if (hok && q1s[r] > q0s[r]) sb = halo_slot_base(*T, cy + r % W - RT, cz + r / W - RT);
+        row(q0s[r], q1s[r]);
+      }
     } else {
       for (int dz = -R; dz <= R; ++dz) {
         const int z = cz + dz;
@@ -269,6 +279,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
           const int y = cy + dy;
           if (y < 0 || y >= g.ny) continue;
           const uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+          if (hok) sb = halo_slot_base(*T, y, z);
           row(start[base + x0], start[base + x1 + 1]);
         }
       }
@@ -329,6 +340,105 @@ static psb_status grid_sort(psb_grid g, uint32_t n, const ReorderArrays* ra, hip
   return PSB_OK;
 }
 
+// One workgroup per tile of kHaloTile sorted particles: the tile's halo rows
+// (HaloTile) and, per LDS slot, the sorted index staged there.
+__global__ __launch_bounds__(kHaloTile) void k_halo(const float4* __restrict__ xs, uint32_t n,
+                                                    GridDev g, const uint32_t* __restrict__ start,
+                                                    HaloTile* __restrict__ tiles,
+                                                    uint32_t* __restrict__ src) {
+  __shared__ int s_x0, s_x1, s_z0, s_z1, s_ok, s_nrows, s_total;
+  __shared__ int s_ylo[kHaloMaxZ], s_yhi[kHaloMaxZ], s_rowbase[kHaloMaxZ + 1];
+  __shared__ uint32_t s_gs[kHaloMaxRows], s_lb[kHaloMaxRows + 1];
+  const uint32_t t = blockIdx.x;
+  const uint32_t p = t * kHaloTile + threadIdx.x;
+  const int R = g.R;
+  if (threadIdx.x == 0) { s_x0 = 0x7FFFFFFF; s_x1 = -1; s_z0 = 0x7FFFFFFF; s_z1 = -1; }
+  if (threadIdx.x < kHaloMaxZ) { s_ylo[threadIdx.x] = 0x7FFFFFFF; s_yhi[threadIdx.x] = -1; }
+  if (threadIdx.x < kHaloMaxZ + 1) s_rowbase[threadIdx.x] = 0;
+  __syncthreads();
+  int cx = 0, cy = 0, cz = 0;
+  const bool v = p < n;
+  if (v) {
+    uint32_t bad = 0;
+    const float4 x = xs[p];
+    cx = cell_coord(x.x, g.minx, g.cell, g.nx, bad);
+    cy = cell_coord(x.y, g.miny, g.cell, g.ny, bad);
+    cz = cell_coord(x.z, g.minz, g.cell, g.nz, bad);
+    atomicMin(&s_x0, cx); atomicMax(&s_x1, cx);
+    atomicMin(&s_z0, cz); atomicMax(&s_z1, cz);
+  }
+  __syncthreads();
+  const int zlo = max(s_z0 - R, 0), zhi = min(s_z1 + R, g.nz - 1);
+  const bool zok = s_z1 >= 0 && zhi - zlo + 1 <= kHaloMaxZ;
+  if (v && zok)
+    for (int z = max(cz - R, zlo); z <= min(cz + R, zhi); ++z) {
+      atomicMin(&s_ylo[z - zlo], max(cy - R, 0));
+      atomicMax(&s_yhi[z - zlo], min(cy + R, g.ny - 1));
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = zok ? 1 : 0, rows = 0;
+    const int nzr = zok ? zhi - zlo + 1 : 0;
+    for (int i = 0; i < nzr; ++i) {
+      s_rowbase[i] = rows;
+      if (s_yhi[i] >= s_ylo[i]) rows += s_yhi[i] - s_ylo[i] + 1;
+      else s_ylo[i] = 0;   // no row in this layer
+    }
+    s_rowbase[nzr] = rows;
+    if (rows > kHaloMaxRows) ok = 0;
+    s_ok = ok;
+    s_nrows = ok ? rows : 0;
+  }
+  __syncthreads();
+  const int x0 = max(s_x0 - R, 0), x1 = min(s_x1 + R, g.nx - 1);
+  const int nrows = s_nrows;
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+    int i = 0;
+    while (s_rowbase[i + 1] <= r) ++i;
+    const int y = s_ylo[i] + (r - s_rowbase[i]), z = zlo + i;
+    const uint64_t base = (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+    const uint32_t gs = start[base + x0];
+    s_gs[r] = gs;
+    s_lb[r] = start[base + x1 + 1] - gs;   // count, scanned below
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int r = 0; r < nrows; ++r) { const uint32_t c = s_lb[r]; s_lb[r] = tot; tot += c; }
+    s_lb[nrows] = tot;
+    if (tot > (uint32_t)kHaloCap) s_ok = 0;
+    s_total = (int)tot;
+  }
+  __syncthreads();
+  HaloTile& T = tiles[t];
+  const int ok = s_ok;
+  if (threadIdx.x == 0) {
+    T.ok = ok; T.nrows = nrows; T.total = ok ? s_total : 0;
+    T.xlo = x0; T.xhi = x1; T.zlo = zlo; T.nz = zok ? zhi - zlo + 1 : 0; T.pad = 0;
+  }
+  if (threadIdx.x < kHaloMaxZ) T.ylo[threadIdx.x] = s_ylo[threadIdx.x];
+  if (threadIdx.x < kHaloMaxZ + 1) T.rowbase[threadIdx.x] = s_rowbase[threadIdx.x];
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) T.gstart[r] = s_gs[r];
+  for (int r = threadIdx.x; r <= nrows; r += blockDim.x) T.lbase[r] = s_lb[r];
+  if (!ok) return;
+  uint32_t* sr = src + (size_t)t * kHaloCap;
+  for (int r = 0; r < nrows; ++r) {
+    const uint32_t lb = s_lb[r], c = s_lb[r + 1] - lb, gs = s_gs[r];
+    for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) sr[lb + k] = gs + k;
+  }
+}
+
+psb_status launch_halo(psb_grid g, const float4* xs, uint32_t n, int R, HaloTile* tiles,
+                       uint32_t* src, hipStream_t s) {
+  if (!n) return PSB_OK;
+  GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+  gd.R = R;
+  const uint32_t nt = (n + kHaloTile - 1) / kHaloTile;
+  hipLaunchKernelGGL(k_halo, dim3(nt), dim3(kHaloTile), 0, s, xs, n, gd, g->start, tiles, src);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
 // start[0..n] = exclusive prefix sums of count[0..n) (start[n] = total);
 // count is zeroed; bsum holds one entry per kScanTile counts
 psb_status scan_exclusive(uint32_t* count, uint64_t n, uint32_t* bsum, uint32_t* start,
@@ -372,7 +482,8 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
   dim3 gb(nblocks), tb(bsize);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, BatchMap>), gb, tb, 0, s, map, xs, n, none, g->start, \
-                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es)
+                     r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es, \
+                     nullptr)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {
@@ -386,14 +497,15 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                             uint32_t* maxcnt, hipStream_t s, const Compact16& c16,
-                            ErrSlots es) {
+                            ErrSlots es, const HaloTile* tiles) {
   if (!n) return PSB_OK;
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
   dim3 gb(blocks_for(n)), tb(kBlock);
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, SingleMap>), gb, tb, 0, s, SingleMap{}, xs, n, gd,    \
-                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es)
+                     g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es, \
+                     tiles)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {

@@ -64,7 +64,8 @@ psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
                               hipStream_t s, const uint32_t* ids = nullptr, int lanes = 1,
-                              const uint16_t* list16 = nullptr, ErrSlots es = ErrSlots{});
+                              const uint16_t* list16 = nullptr, ErrSlots es = ErrSlots{},
+                              const HaloTile* tiles = nullptr, const uint32_t* halo_src = nullptr);
 psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint32_t stride,
                          uint32_t offset, float inv_ps, float* dfeat, float* dtarget,
                          uint64_t dcap, uint64_t base, uint32_t count, hipStream_t s);
@@ -72,7 +73,8 @@ psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
                             PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s,
-                            int lanes = 1, const uint16_t* list16 = nullptr);
+                            int lanes = 1, const uint16_t* list16 = nullptr,
+                            const HaloTile* tiles = nullptr, const uint32_t* halo_src = nullptr);
 
 // batches (scene-aligned blocks; see BatchMap)
 psb_status launch_pci_nonpressure_batch(const BatchMap& map, const Shape& sh, uint32_t n,

@@ -221,6 +221,21 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
 // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
 // in reference mode; template-delta update p = max(0, p + delta (rho - rho0))
 // in config mode.  X[p].w <- p/rho^2 (reference) or p (config).
+// Copies a TileMap tile's halo (X of every staged slot) into LDS; returns the
+// staged count, 0 when the tile keeps the global path.  Every thread of the
+// workgroup calls it (it ends with a barrier).
+template <class M>
+__device__ __forceinline__ uint32_t stage_halo(const M& map, const float4* __restrict__ X,
+                                               float4* hx) {
+  const uint32_t t = map.tile();
+  const HaloTile& T = map.tiles[t];
+  const uint32_t tot = T.ok ? (uint32_t)T.total : 0u;
+  const uint32_t* __restrict__ sr = map.src + (size_t)t * kHaloCap;
+  for (uint32_t k = threadIdx.x; k < tot; k += blockDim.x) hx[k] = X[sr[k]];
+  __syncthreads();
+  return tot;
+}
+
 template <bool REF, class M>
 __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     M map, uint32_t n, Consts c, const uint32_t* __restrict__ cnt,
@@ -229,11 +244,14 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     float* __restrict__ feat, int dflags, const uint32_t* __restrict__ ids,
     const uint16_t* __restrict__ list16, ErrSlots es) {
   __shared__ float red[M::kMaxThreads / 64];
+  __shared__ float4 hx[M::kHalo ? kHaloCap : 1];
   PciStatus* __restrict__ st = st0 + map.scene_of_block();   // a block serves one scene
   {
     const int fi = st->final_iter;
     if (fi >= 0 && fi < k) return;
   }
+  uint32_t staged = 0;   // staged particle count (TileMap tiles whose halo fits)
+  if constexpr (M::kHalo) staged = stage_halo(map, X, hx);
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -270,7 +288,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
         W22[a] = W22[a] + dot3(ux, uy, uz, ux, uy, uz);
       }
     };
-    if (list16 && st->wide == 0u)
+    if (staged)
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, staged - 1}, hx, pair);
+    else if (list16 && st->wide == 0u)
       for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
     else
       for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
@@ -375,6 +395,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
         __hip_atomic_store(o.host_final, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
+  __shared__ float4 hx[M::kHalo ? kHaloCap : 1];
+  uint32_t staged = 0;
+  if constexpr (M::kHalo) staged = stage_halo(map, X, hx);
   if (!valid) return;
   const float4 xi = X[p];
   constexpr int L = M::kLanes;
@@ -414,7 +437,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       sz2[a] = __builtin_fmaf(dz, cc, sz2[a]);
     }
   };
-  if (list16 && st->wide == 0u)
+  if (staged)
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, staged - 1}, hx, pair);
+  else if (list16 && st->wide == 0u)
     for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
   else
     for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
@@ -558,6 +583,10 @@ static Shape single_shape(uint32_t n, int lanes = 1) {
   return Shape{dim3(blocks_for((uint64_t)n * lanes)), dim3(kBlock)};
 }
 
+static Shape tile_shape(uint32_t n) {
+  return Shape{dim3((n + kHaloTile - 1) / kHaloTile), dim3(kHaloTile)};
+}
+
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,
@@ -571,7 +600,11 @@ psb_status launch_pci_density(uint32_t n, const Consts& c, const uint32_t* cnt,
                               const uint32_t* list, float4* X, float* pres, float* rho_pred,
                               const float* dl, PciStatus* st, int k, float* feat, int dflags,
                               hipStream_t s, const uint32_t* ids, int lanes,
-                              const uint16_t* list16, ErrSlots es) {
+                              const uint16_t* list16, ErrSlots es, const HaloTile* tiles,
+                              const uint32_t* halo_src) {
+  if (tiles && lanes == 1 && list16)
+    return density_impl(TileMap{tiles, halo_src}, tile_shape(n), n, c, cnt, list, X, pres,
+                        rho_pred, dl, st, k, feat, dflags, ids, s, list16, es);
   if (lanes == 2)
     return density_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, cnt, list, X, pres, rho_pred,
                         dl, st, k, feat, dflags, ids, s, list16, es);
@@ -583,7 +616,11 @@ psb_status launch_pci_force(uint32_t n, const Consts& c, bool step, const uint32
                             const uint32_t* list, const float4* X, float4* Xn,
                             const float* rho_pred, const float* pres, const PciBuffers& b,
                             PciStatus* st, int k, int warm_pass, const PciOut& o, hipStream_t s,
-                            int lanes, const uint16_t* list16) {
+                            int lanes, const uint16_t* list16, const HaloTile* tiles,
+                            const uint32_t* halo_src) {
+  if (tiles && lanes == 1 && list16)
+    return force_impl(TileMap{tiles, halo_src}, tile_shape(n), n, c, step, cnt, list, X, Xn,
+                      rho_pred, pres, b, st, k, warm_pass, o, s, list16);
   if (lanes == 2)
     return force_impl(SingleMapT<2>{}, single_shape(n, 2), n, c, step, cnt, list, X, Xn, rho_pred,
                       pres, b, st, k, warm_pass, o, s, list16);

@@ -146,6 +146,7 @@ template <int LANES = 1>
 struct SingleMapT {
   static constexpr int kMaxThreads = 256;
   static constexpr int kLanes = LANES;
+  static constexpr bool kHalo = false;
   __device__ __forceinline__ bool locate(uint32_t n, uint32_t& p, uint32_t& s, bool& lead) const {
     p = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LANES;
     s = 0;
@@ -162,10 +163,63 @@ struct SingleMapT {
 
 using SingleMap = SingleMapT<1>;
 
+// ---------------------------------------------------------------------------
+// Neighbour rows staged in LDS (single scene).  The sorted particles are cut
+// into tiles of kHaloTile; a tile's halo is every grid row (y', z') within
+// the stencil of one of its particles, each row over the x cells
+// [xlo, xhi] (the tile's x range widened by the stencil), i.e. a set of
+// contiguous sorted-index ranges.  The corrector passes copy the halo into
+// LDS once per launch and read every neighbour from there; the compact list
+// then holds LDS slots instead of index offsets.  A tile whose halo exceeds
+// the limits keeps the global path (ok = 0).
+// ---------------------------------------------------------------------------
+constexpr int kHaloTile = 512;      // particles per tile = threads per workgroup
+constexpr int kHaloCap = 4096;      // staged particles per tile (64 KiB of float4)
+constexpr int kHaloMaxZ = 13;       // z' layers
+constexpr int kHaloMaxRows = 224;
+
+struct HaloTile {
+  int32_t ok, nrows, total, xlo, xhi, zlo, nz, pad;
+  int32_t ylo[kHaloMaxZ], rowbase[kHaloMaxZ + 1];   // layer i: rows rowbase[i] .. rowbase[i+1)-1
+  uint32_t gstart[kHaloMaxRows];     // sorted index of the row's first particle (cell xlo)
+  uint32_t lbase[kHaloMaxRows + 1];  // LDS slot of that particle; lbase[nrows] = total
+};
+
+// slot of the staged copy of particle q (in row (y', z') of tile T) minus q
+__device__ __forceinline__ int32_t halo_slot_base(const HaloTile& T, int y, int z) {
+  const int i = z - T.zlo;
+  const int r = T.rowbase[i] + (y - T.ylo[i]);
+  return (int32_t)T.lbase[r] - (int32_t)T.gstart[r];
+}
+
+// TileMap: one scene, one workgroup per tile (blockDim = kHaloTile), tiles
+// XCD remapped like SingleMap; the corrector passes stage the tile's halo.
+struct TileMap {
+  static constexpr int kMaxThreads = kHaloTile;
+  static constexpr int kLanes = 1;
+  static constexpr bool kHalo = true;
+  const HaloTile* __restrict__ tiles;
+  const uint32_t* __restrict__ src;  // [tile][kHaloCap]: sorted index staged in each slot
+  __device__ __forceinline__ uint32_t tile() const { return xcd_block(blockIdx.x, gridDim.x); }
+  __device__ __forceinline__ bool locate(uint32_t n, uint32_t& p, uint32_t& s, bool& lead) const {
+    p = tile() * blockDim.x + threadIdx.x;
+    s = 0;
+    lead = blockIdx.x == 0 && threadIdx.x == 0;
+    return p < n;
+  }
+  __device__ __forceinline__ void patch(Consts&, uint32_t) const {}
+  __device__ __forceinline__ uint32_t scene_of_block() const { return 0; }
+  __device__ __forceinline__ GridDev grid(const GridDev& g, uint32_t, uint32_t& off) const {
+    off = 0;
+    return g;
+  }
+};
+
 template <int LANES = 1>
 struct BatchMapT {
   static constexpr int kMaxThreads = 512;
   static constexpr int kLanes = LANES;
+  static constexpr bool kHalo = false;
   const BatchScene* __restrict__ sc;
   const uint16_t* __restrict__ blk_scene;   // scene of each block, for this block size
   int bsel;                                 // particles per block = 64 << bsel
@@ -525,6 +579,14 @@ struct List16 {
   }
 };
 constexpr int kCompactReach = 32767;
+// LDS slots of a staged tile (clamped to the staged count)
+struct ListSlot {
+  const uint16_t* __restrict__ l;
+  uint32_t last;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t) const {
+    return min((uint32_t)l[i], last);
+  }
+};
 
 // Compact list output of the neighbour search: list16 (same ELL shape as
 // the 32-bit list) and a flag per scene, wide[scene * wide_stride], set when
@@ -643,7 +705,12 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
                             uint32_t* flags, uint32_t* maxcnt, hipStream_t s,
-                            const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{});
+                            const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{},
+                            const HaloTile* tiles = nullptr);
+// halo tables of the tiles of n sorted particles xs (after grid_build):
+// tiles[ceil(n / kHaloTile)], src[tiles * kHaloCap]
+psb_status launch_halo(psb_grid g, const float4* xs, uint32_t n, int R, HaloTile* tiles,
+                       uint32_t* src, hipStream_t s);
 // batches: g spans the cells of every scene (scene s owns cells
 // [cell_off, cell_off + its grid)); nblocks x bsize threads, scene-aligned
 psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,

@@ -69,6 +69,10 @@ struct psb_system_s {
   uint32_t* nlist = nullptr;
   uint16_t* nlist16 = nullptr;     // compact twin of nlist for the corrector passes (PCISPH)
   bool compact = true;
+  // neighbour rows staged in LDS by the corrector passes (single-lane, compact list)
+  bool halo = false;
+  HaloTile* tiles = nullptr;
+  uint32_t* halo_src = nullptr;
   uint32_t* flags = nullptr;      // [0] flags, [1] max neighbour count
   PciStatus* st = nullptr;
   ErrSlots es{};                  // spread per-iteration error maxima (PCISPH)
@@ -247,10 +251,14 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     c16.wide = &s->st->wide;
   }
   const uint16_t* l16 = c16.list16;
+  // halo staging: single lane, compact list
+  const HaloTile* tiles =
+      (s->halo && l16 && lanes == 1 && s->tiles && s->halo_src) ? s->tiles : nullptr;
+  if (tiles) PSB_TRY(launch_halo(s->grid, s->ax, s->n, s->gd.R, s->tiles, s->halo_src, st));
   {
     Timed t(s, PH_NEIGHBORS, step);
     PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
-                             s->flags, s->flags + 1, st, c16, s->es));
+                             s->flags, s->flags + 1, st, c16, s->es, tiles));
   }
   {
     Timed t(s, PH_EPILOGUE, step);
@@ -276,7 +284,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   if (warm) {
     Timed t(s, PH_PCI_FORCE, step);
     PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
-                             b, s->st, -1, 1, o, st, lanes, l16));
+                             b, s->st, -1, 1, o, st, lanes, l16, tiles, s->halo_src));
     std::swap(cur, nxt);
   }
   const int min_it = s->c.min_it, max_it = s->c.max_it;
@@ -287,7 +295,8 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16, s->es));
+                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16, s->es, tiles,
+                                 s->halo_src));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
@@ -299,7 +308,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     {
       Timed t(s, PH_PCI_FORCE, step, k);
       PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
-                               s->pres, b, s->st, k, 0, o, st, lanes, l16));
+                               s->pres, b, s->st, k, 0, o, st, lanes, l16, tiles, s->halo_src));
     }
     std::swap(cur, nxt);
     if (ref) continue;
@@ -396,7 +405,8 @@ psb_status psb_system_destroy(psb_system s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
-                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat, s->es.base};
+                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat, s->es.base,
+                  s->tiles, s->halo_src};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
@@ -808,6 +818,19 @@ psb_status psb_surface_extract(psb_surface surf, psb_system s, const psb_surface
   const uint32_t limit = s->n_spheres ? s->n_fluid : 0xFFFFFFFFu;
   return psb_surface_extract_points(surf, (const float*)s->bx, s->bid, s->n, limit,
                                     s->prm.support_radius, prm, num_vertices, num_triangles);
+}
+
+psb_status psb_system_set_halo(psb_system s, int32_t on) {
+  PSB_ARG(s, "null system");
+  if (on && !s->tiles && s->nlist16) {
+    // tile descriptors and staged-index tables, allocated on first use
+    PSB_HIP(hipSetDevice(s->ctx->device));
+    const size_t ntiles = ((s->cap ? s->cap : 1) + kHaloTile - 1) / kHaloTile;
+    PSB_HIP(hipMalloc((void**)&s->tiles, sizeof(HaloTile) * ntiles));
+    PSB_HIP(hipMalloc((void**)&s->halo_src, 4 * (size_t)kHaloCap * ntiles));
+  }
+  s->halo = on != 0;
+  return PSB_OK;
 }
 
 psb_status psb_system_set_compact_list(psb_system s, int32_t on) {

@@ -183,6 +183,11 @@ class FluidSystem:
                                         C.byref(ns)))
         return {"params": prm, "count": n.value, "fluid_count": nf.value, "num_spheres": ns.value}
 
+    def set_halo(self, on: bool) -> None:
+        """Stage each tile's neighbour rows in LDS in the corrector passes
+        (default off; single lane with the compact list)."""
+        check(lib().psb_system_set_halo(self.handle, 1 if on else 0))
+
     def set_compact_list(self, on: bool) -> None:
         """16-bit neighbour offsets in the corrector passes (default on)."""
         check(lib().psb_system_set_compact_list(self.handle, 1 if on else 0))

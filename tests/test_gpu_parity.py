@@ -251,6 +251,41 @@ def test_pcisph_lattice_walls_through_impact_bitwise(ctx, oracle, lanes):
     assert_state_equal(g.state(), o.state())
 
 
+@pytest.mark.parametrize("scene", ["lattice_walls_16", "random_cloud"])
+def test_halo_staging_keeps_results(ctx, oracle, scene):
+    """Neighbour rows staged in LDS against the global gathers and
+    the oracle: a coarse dam break through its wall impact (tiles that fit),
+    and a random cloud whose tiles span too many rows (global fallback)."""
+    from psb_project_amd.system import FluidSystem
+    if scene == "lattice_walls_16":
+        sc = scenes.pcisph_dam_break(n_side=16, s=0.02, lattice_walls=True)
+        steps, chunks = 60, 5
+    else:
+        sc = scenes.pcisph_dam_break(n_side=4)
+        rng = np.random.default_rng(12)
+        pos = np.zeros((12000, 4), np.float32)
+        pos[:, :3] = rng.uniform([0.02, 0.02, 0.02], [0.5, 0.3, 0.3], (12000, 3))
+        sc = scenes.Scene("cloud", sc.params, pos)
+        steps, chunks = 4, 2
+    runs = []
+    for halo in (True, False):
+        g = FluidSystem(ctx, sc.params, sc.n)
+        g.set_halo(halo)
+        g.set_particles(sc.pos4)
+        runs.append(g)
+    o = oracle.OracleSystem(sc.params, sc.n)
+    o.set_particles(sc.pos4)
+    for _ in range(chunks):
+        for g in runs:
+            g.update(steps)
+        o.update(steps)
+        assert runs[0].stats()["total_iterations"] == o.stats()["total_iterations"]
+    assert_state_equal(runs[0].state(), runs[1].state())
+    assert_state_equal(runs[0].state(), o.state())
+    for g in runs:
+        g.close()
+
+
 def test_compact_list_falls_back_on_wide_offsets(ctx):
     """A thin 180 x 100 x 4 slab on 5 mm cells: neighbours one or two cell
     layers apart in z sit more than 32767 places away in the sorted order,
