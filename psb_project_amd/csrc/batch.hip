@@ -342,7 +342,7 @@ static psb_status batch_alloc(psb_batch b) {
   PSB_TRY(A((void**)&b->xp0, 16 * N)); PSB_TRY(A((void**)&b->xp1, 16 * N));
   PSB_TRY(A((void**)&b->ncount, 4 * N));
   PSB_TRY(A((void**)&b->nlist, 4 * N * (size_t)b->prm.max_neighbors));
-  PSB_TRY(A((void**)&b->nlist16, 2 * N * (size_t)b->prm.max_neighbors));
+  PSB_TRY(A((void**)&b->nlist16, l16_bytes(N, (uint32_t)b->prm.max_neighbors)));
   PSB_TRY(A((void**)&b->flags, 64));
   PSB_TRY(A((void**)&b->st, sizeof(PciStatus) * b->S));
   b->es.stride = err_slot_stride(b->S);

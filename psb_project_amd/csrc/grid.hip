@@ -231,9 +231,9 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
           if (c16.list16) {
             const int d = (int)(q - p);
             if (hok)
-              c16.list16[(uint64_t)cnt * n + p] = (uint16_t)((int32_t)q + sb);
+              c16.list16[l16_at(cnt, p, n)] = (uint16_t)((int32_t)q + sb);
             else if (d >= -kCompactReach && d <= kCompactReach)
-              c16.list16[(uint64_t)cnt * n + p] = (uint16_t)(d + 32768);
+              c16.list16[l16_at(cnt, p, n)] = (uint16_t)(d + 32768);
             else
               wide = true;
           }

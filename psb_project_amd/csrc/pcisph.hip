@@ -289,11 +289,11 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
       }
     };
     if (staged)
-      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, staged - 1}, hx, pair);
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, n, staged - 1}, hx, pair);
     else if (list16 && st->wide == 0u)
-      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16, n}, X, pair);
     else
-      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list, n}, X, pair);
     const float S = split_total<L>(S2, lane);
     float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
     if (REF) {
@@ -438,11 +438,11 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
     }
   };
   if (staged)
-    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, staged - 1}, hx, pair);
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, n, staged - 1}, hx, pair);
   else if (list16 && st->wide == 0u)
-    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16}, X, pair);
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16, n}, X, pair);
   else
-    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list}, X, pair);
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32{list, n}, X, pair);
   float sx = split_total<L>(sx2, lane), sy = split_total<L>(sy2, lane),
         sz = split_total<L>(sz2, lane);
   if (lane != 0) return;   // the pair's second lane only summed

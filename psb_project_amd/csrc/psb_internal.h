@@ -565,26 +565,51 @@ inline uint32_t err_slot_stride(uint32_t scenes) {
   return (uint32_t)((need + kErrSlotStride - 1) / kErrSlotStride * kErrSlotStride);
 }
 
-// Neighbour list readers: the ELL list of 32-bit indices, and its compact
-// twin of 16-bit offsets (q - p + 32768, |q - p| <= 32767) that halves the
-// list bytes the corrector passes stream every iteration.
+// Neighbour list readers: the ELL list of 32-bit indices (entry k of
+// particle p at k * n + p), and its compact twin of 16-bit entries that
+// halves the list bytes the corrector passes stream every iteration.  The
+// compact list keeps 8 consecutive entries of one particle in one 16-byte
+// word, [k / 8][p][k % 8] (l16_at), so one vector load per thread brings 8
+// indices.  Entry: offset q - p + 32768 (|q - p| <= 32767), or, in a tile
+// staged in LDS, the slot of q's staged copy.
+__host__ __device__ __forceinline__ uint64_t l16_at(uint32_t k, uint32_t p, uint32_t n) {
+  return ((uint64_t)(k >> 3) * n + p) * 8u + (k & 7u);
+}
+constexpr int kL16Group = 8;
+inline size_t l16_bytes(size_t n, uint32_t cap) {
+  return 2u * (size_t)kL16Group * n * ((cap + kL16Group - 1) / kL16Group);
+}
 struct List32 {
+  static constexpr bool kGrouped = false;
   const uint32_t* __restrict__ l;
-  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t) const { return l[i]; }
+  uint32_t n;
+  __device__ __forceinline__ uint32_t at(uint32_t k, uint32_t p) const {
+    return l[(uint64_t)k * n + p];
+  }
 };
 struct List16 {
+  static constexpr bool kGrouped = true;
   const uint16_t* __restrict__ l;
-  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t p) const {
-    return p + (uint32_t)l[i] - 32768u;
+  uint32_t n;
+  __device__ __forceinline__ uint32_t decode(uint32_t e, uint32_t p) const {
+    return p + e - 32768u;
+  }
+  __device__ __forceinline__ uint32_t safe(uint32_t p) const { return p; }
+  __device__ __forceinline__ uint32_t at(uint32_t k, uint32_t p) const {
+    return decode(l[l16_at(k, p, n)], p);
   }
 };
 constexpr int kCompactReach = 32767;
 // LDS slots of a staged tile (clamped to the staged count)
 struct ListSlot {
+  static constexpr bool kGrouped = true;
   const uint16_t* __restrict__ l;
+  uint32_t n;
   uint32_t last;
-  __device__ __forceinline__ uint32_t operator()(uint64_t i, uint32_t) const {
-    return min((uint32_t)l[i], last);
+  __device__ __forceinline__ uint32_t decode(uint32_t e, uint32_t) const { return min(e, last); }
+  __device__ __forceinline__ uint32_t safe(uint32_t) const { return 0u; }
+  __device__ __forceinline__ uint32_t at(uint32_t k, uint32_t p) const {
+    return decode(l[l16_at(k, p, n)], p);
   }
 };
 
@@ -607,32 +632,53 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
                                                     uint32_t lane, const LI& list,
                                                     const float4* X, F&& f) {
   static_assert(U % 2 == 0, "U even keeps k & 1 static");
-  if (LANES == 1) {
+  if constexpr (LANES == 1 && LI::kGrouped) {
+    // one 16-byte load per 8 entries, the next group's load issued before
+    // the current group's gathers; entries past m select a safe index and
+    // are not summed
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
+    const uint32_t ng = (m + kL16Group - 1) / kL16Group;
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    if (ng) w = g4[p];
+    for (uint32_t g = 0; g < ng; ++g) {
+      const uint4 cur = w;
+      if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
+      const uint32_t e[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
+                             cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
+      const uint32_t k = g * kL16Group;
+      float4 xq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xq[u] = X[k + u < m ? list.decode(e[u], p) : list.safe(p)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k + u < m) f(u & 1, xq[u]);
+    }
+  } else if constexpr (LANES == 1) {
     uint32_t k = 0;
     for (; k + U <= m; k += U) {
       uint32_t q[U];
       float4 xq[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = list((uint64_t)(k + u) * n + p, p);
+      for (int u = 0; u < U; ++u) q[u] = list.at(k + u, p);
 #pragma unroll
       for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
 #pragma unroll
       for (int u = 0; u < U; ++u) f(u & 1, xq[u]);
     }
-    for (; k < m; ++k) f((int)(k & 1u), X[list((uint64_t)k * n + p, p)]);
+    for (; k < m; ++k) f((int)(k & 1u), X[list.at(k, p)]);
   } else {
     uint32_t k = lane;
     for (; k + 2 * (U - 1) < m; k += 2 * U) {
       uint32_t q[U];
       float4 xq[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = list((uint64_t)(k + 2 * u) * n + p, p);
+      for (int u = 0; u < U; ++u) q[u] = list.at(k + 2 * u, p);
 #pragma unroll
       for (int u = 0; u < U; ++u) xq[u] = X[q[u]];
 #pragma unroll
       for (int u = 0; u < U; ++u) f(0, xq[u]);
     }
-    for (; k < m; k += 2) f(0, X[list((uint64_t)k * n + p, p)]);
+    for (; k < m; k += 2) f(0, X[list.at(k, p)]);
   }
 }
 

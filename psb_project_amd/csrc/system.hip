@@ -174,7 +174,7 @@ static psb_status sys_alloc(psb_system s) {
       PSB_TRY(A((void**)&s->dl, 4 * N));
     }
     PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
-    PSB_TRY(A((void**)&s->nlist16, 2 * N * (size_t)s->prm.max_neighbors));
+    PSB_TRY(A((void**)&s->nlist16, l16_bytes(N, (uint32_t)s->prm.max_neighbors)));
     s->es.stride = err_slot_stride(1);
     PSB_TRY(A((void**)&s->es.base, 4 * (size_t)kErrSlots * s->es.stride));
   }
