@@ -158,6 +158,7 @@ static void buffers(psb_batch b, PciBuffers& pb, PciOut& o) {
   pb.anp = b->anp; pb.xp0 = b->xp0; pb.dl = b->dl;
   pb.feat = b->mlp ? b->feat : nullptr;
   pb.ax = b->ax; pb.avh = b->avh; pb.aid = b->aid;
+  pb.list16 = b->nlist16; pb.st = b->st;
   o = PciOut{};
   o.bx = b->bx; o.bv = b->bv; o.bvh = b->bvh; o.bid = b->bid; o.bp = b->bp; o.flags = b->flags;
   o.host_final = b->d_final;

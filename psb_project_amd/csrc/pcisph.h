@@ -27,6 +27,10 @@ struct PciBuffers {
   const float4* ax;   // start-of-step positions
   const float4* avh;  // start-of-step half-step velocities
   const uint32_t* aid;
+  // compact neighbour list for the non-pressure pass, used for the scenes
+  // whose wide flag (st[scene].wide) is clear; null: the 32-bit list
+  const uint16_t* list16;
+  const PciStatus* st;
 };
 
 struct PciOut {

@@ -51,101 +51,94 @@ __global__ void k_pci_stats_reset(PciStatus* st0) {
   }
 }
 
-// Pressure scaling of particle p in config mode: the template delta, capped by
-// the per-particle value 1/(beta (W1.W1 + W2)) of compute_Pressure_PCI
-// (PciSphSystem_Kernel.cu:44-63) evaluated once at the start positions.
-// With rigid-sphere boundary particles (static, v.w = 0) they add to W1 only.
-__device__ __forceinline__ float local_delta(const Consts& c, uint32_t p, uint32_t n,
-                                             const uint32_t* __restrict__ cnt,
-                                             const uint32_t* __restrict__ list,
-                                             const float4* __restrict__ xs,
-                                             const float4* __restrict__ vs) {
-  const float4 xi = xs[p];
-  float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
-  auto pair = [&](const float4& xj, bool moves) {
-    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-    if (d2 > 0.f && d2 < c.h2) {
-      const float dd = sqrtf(d2);
-      const float invd = 1.0f / dd;
-      const float hd = c.h - dd;
-      const float gw = c.c_spiky * (hd * hd);
-      const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
-      W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-      if (moves) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
-    }
-  };
-  if (c.n_fluid == kNoBodies)
-    for_neighbors<kUnroll>(p, n, cnt[p], list, xs,
-                           [&](uint32_t, const float4& xj) { pair(xj, true); });
-  else
-    for_neighbors2<kUnroll>(p, n, cnt[p], list, xs, vs,
-                            [&](uint32_t, const float4& xj, const float4& vj) {
-                              pair(xj, vj.w > 0.f);
-                            });
-  if (c.wall) {
-    // the static wall particles add to sum gradW, not to sum |gradW|^2
-    float S, gx, gy, gz;
-    wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, gx, gy, gz);
-    W1x = W1x + gx; W1y = W1y + gy; W1z = W1z + gz;
-  }
-  const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
-  if (!(den > 0.f)) return c.delta;
-  const float loc = 1.0f / den;
-  return loc < c.delta ? loc : c.delta;
-}
-
-// The same neighbour sums plus the warm-start features 1..7 of particle p
-// (velocity divergence, neighbour count, previous pressure, |v|, previous
-// pressure gradient); xs[q].w holds the previous-step pressure, vs[q].w 1/rho.
-__device__ __forceinline__ float delta_and_features(const Consts& c, uint32_t p, uint32_t n,
-                                                    const uint32_t* __restrict__ cnt,
-                                                    const uint32_t* __restrict__ list,
-                                                    const float4* __restrict__ xs,
-                                                    const float4* __restrict__ vs,
-                                                    float* __restrict__ feat) {
+// The config-mode pair sums of the non-pressure pass in one walk over
+// particle p's list: the viscosity (and surface) sums of internal_sums, and
+// the pressure scaling delta_i -- the template delta, capped by the
+// per-particle value 1/(beta (W1.W1 + W2)) of compute_Pressure_PCI
+// (PciSphSystem_Kernel.cu:44-63) evaluated once at the start positions
+// (rigid-sphere boundary particles, static with v.w = 0, add to W1 only).
+// FEAT: also the warm-start features 1..7 of p (velocity divergence,
+// neighbour count, previous pressure, |v|, previous pressure gradient);
+// xs[q].w then holds the previous-step pressure, vs[q].w 1/rho.  Each sum
+// keeps its own sequential order over the list.
+template <bool SURF, bool FEAT, class LI>
+__device__ __forceinline__ PairSums np_sums(const Consts& c, uint32_t p, uint32_t n, uint32_t m,
+                                            const LI& list, const float4* __restrict__ xs,
+                                            const float4* __restrict__ vs,
+                                            float* __restrict__ feat, float& dl) {
+  PairSums s;
+  s.fp[0] = s.fp[1] = s.fp[2] = 0.f;
+  s.fv[0] = s.fv[1] = s.fv[2] = 0.f;
+  s.fs = 0.f;
+  s.nn[0] = s.nn[1] = s.nn[2] = 0.f;
   const float4 xi = xs[p], vi = vs[p];
+  const bool local = FEAT || c.delta_local_cap;
+  const bool bodies = c.n_fluid != kNoBodies;
   float W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
   float div = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-  const uint32_t m = cnt[p];
-  for_neighbors2<kUnroll>(p, n, m, list, xs, vs, [&](uint32_t, const float4& xj,
-                                                     const float4& vj) {
+  for_pairs2<kUnroll>(p, n, m, list, xs, vs, [&](const float4& xj, const float4& vj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-    if (d2 > 0.f && d2 < c.h2) {
-      const float dd = sqrtf(d2);
-      const float invd = 1.0f / dd;
+    if (d2 > 0.f) {
+      float dd, invd = 0.f;
+      if (local) dist_and_inverse(d2, dd, invd);
+      else dd = sqrt_rn(d2);
       const float hd = c.h - dd;
-      const float gw = c.c_spiky * (hd * hd);
-      const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
-      W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-      if (vj.w > 0.f) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);   // boundary: W1 only
-      const float vol = c.m * vj.w;
-      const float t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
-      div = __builtin_fmaf(t, vol, div);
-      const float dp = (xj.w - xi.w) * vol;
-      gx = __builtin_fmaf(ux, dp, gx);
-      gy = __builtin_fmaf(uy, dp, gy);
-      gz = __builtin_fmaf(uz, dp, gz);
+      const float rinvj = vj.w;
+      const float lv = hd * rinvj;
+      s.fv[0] = __builtin_fmaf(vj.x - vi.x, lv, s.fv[0]);
+      s.fv[1] = __builtin_fmaf(vj.y - vi.y, lv, s.fv[1]);
+      s.fv[2] = __builtin_fmaf(vj.z - vi.z, lv, s.fv[2]);
+      if (SURF) {
+        const float a = c.h2 - d2;
+        const float b = c.h2x3 - 7.0f * d2;
+        s.fs = __builtin_fmaf(a * b, rinvj, s.fs);
+        const float an = (a * a) * rinvj;
+        s.nn[0] = __builtin_fmaf(dx, an, s.nn[0]);
+        s.nn[1] = __builtin_fmaf(dy, an, s.nn[1]);
+        s.nn[2] = __builtin_fmaf(dz, an, s.nn[2]);
+      }
+      if (local && d2 < c.h2) {
+        const float gw = c.c_spiky * (hd * hd);
+        const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
+        W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+        const bool moves = (FEAT || bodies) ? vj.w > 0.f : true;
+        if (moves) W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        if (FEAT) {
+          const float vol = c.m * vj.w;
+          const float t = dot3(vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, ux, uy, uz);
+          div = __builtin_fmaf(t, vol, div);
+          const float dp = (xj.w - xi.w) * vol;
+          gx = __builtin_fmaf(ux, dp, gx);
+          gy = __builtin_fmaf(uy, dp, gy);
+          gz = __builtin_fmaf(uz, dp, gz);
+        }
+      }
     }
   });
-  feat[1 * (size_t)n + p] = div * c.dt;
-  feat[2 * (size_t)n + p] = (float)m * 0.03125f;
-  feat[3 * (size_t)n + p] = xi.w * c.inv_ps;
-  feat[4 * (size_t)n + p] = sqrtf(dot3(vi.x, vi.y, vi.z, vi.x, vi.y, vi.z)) * c.dt_over_h;
-  feat[5 * (size_t)n + p] = gx * c.h_over_ps;
-  feat[6 * (size_t)n + p] = gy * c.h_over_ps;
-  feat[7 * (size_t)n + p] = gz * c.h_over_ps;
-  if (!c.delta_local_cap) return c.delta;
+  if (FEAT) {
+    feat[1 * (size_t)n + p] = div * c.dt;
+    feat[2 * (size_t)n + p] = (float)m * 0.03125f;
+    feat[3 * (size_t)n + p] = xi.w * c.inv_ps;
+    feat[4 * (size_t)n + p] = sqrtf(dot3(vi.x, vi.y, vi.z, vi.x, vi.y, vi.z)) * c.dt_over_h;
+    feat[5 * (size_t)n + p] = gx * c.h_over_ps;
+    feat[6 * (size_t)n + p] = gy * c.h_over_ps;
+    feat[7 * (size_t)n + p] = gz * c.h_over_ps;
+  }
+  dl = c.delta;
+  if (!c.delta_local_cap) return s;
   if (c.wall) {
+    // the static wall particles add to sum gradW, not to sum |gradW|^2
     float S, wgx, wgy, wgz;
     wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, wgx, wgy, wgz);
     W1x = W1x + wgx; W1y = W1y + wgy; W1z = W1z + wgz;
   }
   const float den = c.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2);
-  if (!(den > 0.f)) return c.delta;
-  const float loc = 1.0f / den;
-  return loc < c.delta ? loc : c.delta;
+  if (den > 0.f) {
+    const float loc = 1.0f / den;
+    if (loc < c.delta) dl = loc;
+  }
+  return s;
 }
 
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:117-170)
@@ -170,7 +163,19 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
       for (int k = 1; k < 8; ++k) b.feat[(size_t)k * n + p] = 0.f;
     return;
   }
-  PairSums s = internal_sums<false, SURF>(c, p, n, cnt, list, xs, vs);
+  // the compact list unless this scene's offsets overflowed it
+  const PciStatus* stc = b.st ? b.st + map.scene_of_block() : nullptr;
+  const bool c16 = b.list16 && stc && stc->wide == 0u;
+  const uint32_t m = cnt[p];
+  PairSums s;
+  float dl = c.delta;
+  if (REF) {
+    s = c16 ? internal_sums_l<false, SURF>(c, p, n, m, List16{b.list16, n}, xs, vs)
+            : internal_sums_l<false, SURF>(c, p, n, m, List32{list, n}, xs, vs);
+  } else {
+    s = c16 ? np_sums<SURF, FEAT>(c, p, n, m, List16{b.list16, n}, xs, vs, b.feat, dl)
+            : np_sums<SURF, FEAT>(c, p, n, m, List32{list, n}, xs, vs, b.feat, dl);
+  }
   const float ri = rho[p];
   const float sigma = REF ? 0.0728f : c.sigma;
   float fV[3] = {s.fv[0] * c.cv, s.fv[1] * c.cv, s.fv[2] * c.cv};
@@ -194,8 +199,7 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
       if (o.F) o.F[p] = make_float4(0.f, gy * ri, 0.f, 0.f);
     }
   } else {
-    if (FEAT) b.dl[p] = delta_and_features(c, p, n, cnt, list, xs, vs, b.feat);
-    else b.dl[p] = c.delta_local_cap ? local_delta(c, p, n, cnt, list, xs, vs) : c.delta;
+    b.dl[p] = dl;
     const float inv = 1.0f / ri;
     const float a0 = (fV[0] + fS[0]) * inv + c.gx, a1 = (fV[1] + fS[1]) * inv + c.gy,
                 a2 = (fV[2] + fS[2]) * inv + c.gz;
@@ -279,8 +283,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
       if (!REF) w = fmaxf(w, 0.f);
       S2[a] = __builtin_fmaf(w * w, w, S2[a]);
       if (REF && d2 > 0.f) {
-        const float dd = sqrtf(d2);
-        const float invd = 1.0f / dd;
+        float dd, invd;
+        dist_and_inverse(d2, dd, invd);
         const float hd = c.h - dd;
         const float gw = c.c_spiky * (hd * hd);
         const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;

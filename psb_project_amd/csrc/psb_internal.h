@@ -533,6 +533,10 @@ __device__ __forceinline__ float rcp_rn_fast(float d) {
   q = __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), r, q);
   return __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), r, q);
 }
+// sqrtf(x), on the fast path for x >= kFastSqrtMin
+__device__ __forceinline__ float sqrt_rn(float x) {
+  return __builtin_expect(x >= kFastSqrtMin, 1) ? sqrt_rn_fast(x) : sqrtf(x);
+}
 // dd = sqrtf(d2), invd = 1.0f / dd for d2 > 0
 __device__ __forceinline__ void dist_and_inverse(float d2, float& dd, float& invd) {
   if (__builtin_expect(d2 >= kFastSqrtMin, 1)) {
@@ -679,6 +683,58 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
       for (int u = 0; u < U; ++u) f(0, xq[u]);
     }
     for (; k < m; k += 2) f(0, X[list.at(k, p)]);
+  }
+}
+
+// Visit a list (List32 / List16 accessor) in order, gathering two vec4
+// arrays per neighbour: f(xq, vq).  The grouped compact list brings 8
+// indices per vector load (next group's load issued first); entries past m
+// select a safe index and are not visited.
+template <int U, class LI, class F>
+__device__ __forceinline__ void for_pairs2(uint32_t p, uint32_t n, uint32_t m, const LI& list,
+                                           const float4* __restrict__ X,
+                                           const float4* __restrict__ V, F&& f) {
+  if constexpr (LI::kGrouped) {
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
+    const uint32_t ng = (m + kL16Group - 1) / kL16Group;
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    if (ng) w = g4[p];
+    for (uint32_t g = 0; g < ng; ++g) {
+      const uint4 cur = w;
+      if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
+      const uint32_t e[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
+                             cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
+      const uint32_t k = g * kL16Group;
+#pragma unroll
+      for (int h = 0; h < 8; h += 4) {
+        float4 xq[4], vq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t q = k + h + u < m ? list.decode(e[h + u], p) : list.safe(p);
+          xq[u] = X[q];
+          vq[u] = V[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (k + h + u < m) f(xq[u], vq[u]);
+      }
+    }
+  } else {
+    uint32_t k = 0;
+    for (; k + U <= m; k += U) {
+      uint32_t q[U];
+      float4 xq[U], vq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) q[u] = list.at(k + u, p);
+#pragma unroll
+      for (int u = 0; u < U; ++u) { xq[u] = X[q[u]]; vq[u] = V[q[u]]; }
+#pragma unroll
+      for (int u = 0; u < U; ++u) f(xq[u], vq[u]);
+    }
+    for (; k < m; ++k) {
+      const uint32_t q = list.at(k, p);
+      f(X[q], V[q]);
+    }
   }
 }
 

@@ -12,13 +12,13 @@ struct PairSums {
 // Pair loop of internalForces (SphKernel.cu:47-78) and of
 // compute_Viscosity_SurfaceTension_Force_PCI (PciSphSystem_Kernel.cu:134-158)
 // with uniform material: xs[q].w = p_q/rho_q^2 (used when PRESS),
-// vs[q].w = 1/rho_q.  Sums exclude d = 0 (the particle itself).
-template <bool PRESS, bool SURF>
-__device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, uint32_t n,
-                                                  const uint32_t* __restrict__ cnt,
-                                                  const uint32_t* __restrict__ list,
-                                                  const float4* __restrict__ xs,
-                                                  const float4* __restrict__ vs) {
+// vs[q].w = 1/rho_q.  Sums exclude d = 0 (the particle itself).  LI: list
+// accessor (List32, List16) over the particle's m entries.
+template <bool PRESS, bool SURF, class LI>
+__device__ __forceinline__ PairSums internal_sums_l(const Consts& c, uint32_t p, uint32_t n,
+                                                    uint32_t m, const LI& list,
+                                                    const float4* __restrict__ xs,
+                                                    const float4* __restrict__ vs) {
   PairSums s;
   s.fp[0] = s.fp[1] = s.fp[2] = 0.f;
   s.fv[0] = s.fv[1] = s.fv[2] = 0.f;
@@ -26,16 +26,16 @@ __device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, u
   s.nn[0] = s.nn[1] = s.nn[2] = 0.f;
   const float4 xi = xs[p], vi = vs[p];
   const float pvi = xi.w;
-  for_neighbors2<kUnroll>(p, n, cnt[p], list, xs, vs, [&](uint32_t, const float4& xj,
-                                                         const float4& vj) {
+  for_pairs2<kUnroll>(p, n, m, list, xs, vs, [&](const float4& xj, const float4& vj) {
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     if (d2 > 0.f) {
-      const float dd = sqrtf(d2);
+      float dd, invd = 0.f;
+      if (PRESS) dist_and_inverse(d2, dd, invd);
+      else dd = sqrt_rn(d2);
       const float hd = c.h - dd;
       const float rinvj = vj.w;
       if (PRESS) {
-        const float invd = 1.0f / dd;
         const float gw = c.c_spiky * (hd * hd);
         const float cc = gw * (pvi + xj.w);
         s.fp[0] = __builtin_fmaf(dx * invd, cc, s.fp[0]);
@@ -58,6 +58,15 @@ __device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, u
     }
   });
   return s;
+}
+
+template <bool PRESS, bool SURF>
+__device__ __forceinline__ PairSums internal_sums(const Consts& c, uint32_t p, uint32_t n,
+                                                  const uint32_t* __restrict__ cnt,
+                                                  const uint32_t* __restrict__ list,
+                                                  const float4* __restrict__ xs,
+                                                  const float4* __restrict__ vs) {
+  return internal_sums_l<PRESS, SURF>(c, p, n, cnt[p], List32{list, n}, xs, vs);
 }
 
 // SphKernel.cu:80-88: f_S = (n (fs * -sigma)) / |n| when |n| >= l.

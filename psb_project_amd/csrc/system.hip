@@ -269,6 +269,10 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
   b.feat = feat ? s->feat : nullptr;
   b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
+  // staged tiles hold LDS slots in the compact list, which only the
+  // corrector passes decode: the non-pressure pass then reads the 32-bit list
+  b.list16 = tiles ? nullptr : l16;
+  b.st = s->st;
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
   o.host_final = ref ? nullptr : s->d_final + slot;
