@@ -631,6 +631,12 @@ struct Compact16 {
 // increasing k, added as even + odd.  LANES = 1: one thread keeps both
 // (f(acc = k & 1, ...)); LANES = 2: lane l of the particle's pair keeps the
 // sum over k = l (mod 2) (f(acc = 0, ...)) and split_total adds the pair.
+// the 8 entries of one compact-list word, in entry order
+__device__ __forceinline__ void unpack_l16(const uint4& c, uint32_t* e) {
+  e[0] = c.x & 0xFFFFu; e[1] = c.x >> 16; e[2] = c.y & 0xFFFFu; e[3] = c.y >> 16;
+  e[4] = c.z & 0xFFFFu; e[5] = c.z >> 16; e[6] = c.w & 0xFFFFu; e[7] = c.w >> 16;
+}
+
 template <int LANES, int U, class LI, class F>
 __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint32_t m,
                                                     uint32_t lane, const LI& list,
@@ -638,24 +644,35 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
   static_assert(U % 2 == 0, "U even keeps k & 1 static");
   if constexpr (LANES == 1 && LI::kGrouped) {
     // one 16-byte load per 8 entries, the next group's load issued before
-    // the current group's gathers; entries past m select a safe index and
-    // are not summed
+    // the current group's gathers.  Full groups run without per-entry
+    // tests (so every gather of a group is in flight at once); in the last,
+    // partial group the entries past m select a safe index and are not
+    // summed.
     const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
-    const uint32_t ng = (m + kL16Group - 1) / kL16Group;
+    const uint32_t ng = (m + kL16Group - 1) / kL16Group, nfull = m / kL16Group;
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     if (ng) w = g4[p];
-    for (uint32_t g = 0; g < ng; ++g) {
+    for (uint32_t g = 0; g < nfull; ++g) {
       const uint4 cur = w;
       if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
-      const uint32_t e[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
-                             cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
-      const uint32_t k = g * kL16Group;
+      uint32_t e[8];
+      unpack_l16(cur, e);
       float4 xq[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) xq[u] = X[k + u < m ? list.decode(e[u], p) : list.safe(p)];
+      for (int u = 0; u < 8; ++u) xq[u] = X[list.decode(e[u], p)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f(u & 1, xq[u]);
+    }
+    if (nfull < ng) {
+      uint32_t e[8];
+      unpack_l16(w, e);
+      const uint32_t r = m - nfull * kL16Group;   // 1..7 entries left
+      float4 xq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xq[u] = X[(uint32_t)u < r ? list.decode(e[u], p) : list.safe(p)];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (k + u < m) f(u & 1, xq[u]);
+        if ((uint32_t)u < r) f(u & 1, xq[u]);
     }
   } else if constexpr (LANES == 1) {
     uint32_t k = 0;
