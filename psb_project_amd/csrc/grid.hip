@@ -189,6 +189,8 @@ __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict
 // row ranges are loaded first (independent table loads) and the candidates of
 // a row are gathered U at a time before being tested in order.  RT = 0: any
 // stencil width, rows visited one after the other.
+constexpr int kRowSlots = 8;   // candidate loads in flight per stencil row
+
 template <bool DENS, int RT, class M>
 __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float4* xs, uint32_t n,
                                                               GridDev g0,
@@ -245,16 +247,17 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
         }
       }
     };
+    // a row's candidates 8 at a time: one round of loads covers a typical
+    // row (slots past the row's end reload its last particle, unvisited)
     auto row = [&](uint32_t q0, uint32_t q1) {
-      uint32_t q = q0;
-      for (; q + kUnroll <= q1; q += kUnroll) {
-        float4 xq[kUnroll];
+      for (uint32_t q = q0; q < q1; q += kRowSlots) {
+        float4 xq[kRowSlots];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) xq[u] = xs[q + u];
+        for (int u = 0; u < kRowSlots; ++u) xq[u] = xs[min(q + u, q1 - 1)];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) visit(q + u, xq[u]);
+        for (int u = 0; u < kRowSlots; ++u)
+          if (q + u < q1) visit(q + u, xq[u]);
       }
-      for (; q < q1; ++q) visit(q, xs[q]);
     };
     if (RT > 0) {
       constexpr int W = 2 * RT + 1;
