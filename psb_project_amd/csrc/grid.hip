@@ -10,7 +10,9 @@
 //   1. k_hash_count   hash[i] = cell of x_i;  slot[i] = atomicAdd(count[c])
 //   2. k_scan_*       start = exclusive prefix sum of count over all cells
 //                     (start[c] = #particles with hash < c, start[ncells] = n);
-//                     the scan also re-zeroes count for the next build
+//                     the per-tile sums come from step 1 (tsum), so the scan
+//                     reads count once; k_rank_reorder re-zeroes the occupied
+//                     cells' counts for the next build
 //   3. k_scatter      tmp[start[c] + slot[i]] = i      (order within a cell
 //                     still depends on the atomics)
 //   4. k_rank_reorder rank of i among its cell's tmp entries by index ->
@@ -31,7 +33,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const floa
                                                                uint32_t* __restrict__ hash,
                                                                uint32_t* __restrict__ slot,
                                                                uint32_t* __restrict__ count,
-                                                               uint32_t* __restrict__ flags) {
+                                                               uint32_t* __restrict__ flags,
+                                                               uint32_t* __restrict__ tsum) {
   uint32_t i, sc;
   bool lead;
   if (!map.locate(n, i, sc, lead)) return;
@@ -45,6 +48,18 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const floa
   uint32_t c = off + (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
   hash[i] = c;
   slot[i] = atomicAdd(&count[c], 1u);
+  // per-scan-tile particle counts (the scan's first pass), one atomic per
+  // distinct tile of the wave: the particles arrive in last step's sorted
+  // order, so a wave's cells mostly share a tile
+  const uint32_t t = c / kScanTile;
+  uint64_t todo = __ballot(1);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t tl = (uint32_t)__shfl((int)t, leader, 64);
+    const uint64_t same = __ballot(t == tl) & todo;
+    if ((int)(threadIdx.x & 63u) == leader) atomicAdd(&tsum[tl], (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
   uint32_t f = 0;
   if (bad) f |= PSB_FLAG_OUT_OF_GRID;
   if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) f |= PSB_FLAG_NONFINITE;
@@ -95,23 +110,29 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// exclusive scan of the tile sums in one workgroup; start[ncells] = total
-__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ bsum, uint32_t nb,
+// exclusive scan of the tile sums in one workgroup: dst[k] = sum of src[<k]
+// (dst may be src); CONSUME zeroes src for the next build; start[ncells] = total
+template <bool CONSUME>
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* src, uint32_t* dst, uint32_t nb,
                                                     uint32_t* __restrict__ start, uint64_t ncells) {
   __shared__ uint32_t lds[4];
   uint32_t carry = 0;
   for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
     uint32_t k = b0 + threadIdx.x;
-    uint32_t v = k < nb ? bsum[k] : 0u;
+    uint32_t v = k < nb ? src[k] : 0u;
     uint32_t tot;
     uint32_t ex = block_excl_scan(v, lds, tot);
-    if (k < nb) bsum[k] = carry + ex;
+    if (k < nb) {
+      dst[k] = carry + ex;
+      if (CONSUME) src[k] = 0u;
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) start[ncells] = carry;
 }
 
-// start[c] = tile offset + in-tile exclusive prefix; count[c] = 0
+// start[c] = tile offset + in-tile exclusive prefix; ZERO: count[c] = 0
+template <bool ZERO>
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ count, uint64_t ncells,
                                                     const uint32_t* __restrict__ bsum,
                                                     uint32_t* __restrict__ start) {
@@ -141,12 +162,15 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ count
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       s4[j] = make_uint4(v[4 * j] + off, v[4 * j + 1] + off, v[4 * j + 2] + off, v[4 * j + 3] + off);
-      c4[j] = make_uint4(0u, 0u, 0u, 0u);
+      if (ZERO) c4[j] = make_uint4(0u, 0u, 0u, 0u);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < 16; ++j)
-      if (base + j < ncells) { start[base + j] = v[j] + off; count[base + j] = 0u; }
+      if (base + j < ncells) {
+        start[base + j] = v[j] + off;
+        if (ZERO) count[base + j] = 0u;
+      }
   }
 }
 
@@ -160,11 +184,13 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ha
 }
 
 // Stable rank within the cell + gather of the particle state.
+// The cell's first particle re-zeroes its count for the next build.
 template <bool STATE>
 __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict__ hash, uint32_t n,
                                                       const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ tmp,
-                                                      uint32_t* __restrict__ perm, ReorderArrays ra) {
+                                                      uint32_t* __restrict__ perm, ReorderArrays ra,
+                                                      uint32_t* __restrict__ count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t c = hash[i];
@@ -174,6 +200,7 @@ __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict
     for (uint32_t q = s; q < e; ++q) rank += (tmp[q] < i) ? 1u : 0u;
   const uint32_t dst = s + rank;
   perm[dst] = i;
+  if (rank == 0) count[c] = 0u;
   if (STATE) {
     ra.dx[dst] = ra.sx[i];
     ra.dv[dst] = ra.sv[i];
@@ -322,20 +349,22 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ src, f
 
 // counting sort of the hashed particles + cell ranges + reorder
 static psb_status grid_sort(psb_grid g, uint32_t n, const ReorderArrays* ra, hipStream_t s) {
+  // the tile sums came with the hashes (tsum); the scan's second pass
+  // leaves count alone, the reorder re-zeroes the occupied cells
   const uint32_t nb = blocks_for(n);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, g->bsum, g->nscan, g->start, g->ncells);
-  hipLaunchKernelGGL(k_scan_apply, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells, g->bsum,
-                     g->start);
+  hipLaunchKernelGGL(k_scan_tiles<true>, dim3(1), dim3(kBlock), 0, s, g->tsum, g->bsum, g->nscan,
+                     g->start, g->ncells);
+  hipLaunchKernelGGL(k_scan_apply<false>, dim3(g->nscan), dim3(kBlock), 0, s, g->count, g->ncells,
+                     g->bsum, g->start);
   if (n) {
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, g->hash, g->slot, n, g->start, g->tmp);
     if (ra) {
       hipLaunchKernelGGL(k_rank_reorder<true>, dim3(nb), dim3(kBlock), 0, s, g->hash, n, g->start,
-                         g->tmp, g->perm, *ra);
+                         g->tmp, g->perm, *ra, g->count);
     } else {
       ReorderArrays none{};
       hipLaunchKernelGGL(k_rank_reorder<false>, dim3(nb), dim3(kBlock), 0, s, g->hash, n, g->start,
-                         g->tmp, g->perm, none);
+                         g->tmp, g->perm, none, g->count);
     }
   }
   PSB_HIP(hipGetLastError());
@@ -448,8 +477,8 @@ psb_status scan_exclusive(uint32_t* count, uint64_t n, uint32_t* bsum, uint32_t*
                           hipStream_t s) {
   const uint32_t nt = (uint32_t)((n + kScanTile - 1) / kScanTile);
   if (nt) hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, s, count, n, bsum);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, bsum, nt, start, n);
-  if (nt) hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, count, n, bsum, start);
+  hipLaunchKernelGGL(k_scan_tiles<false>, dim3(1), dim3(kBlock), 0, s, bsum, bsum, nt, start, n);
+  if (nt) hipLaunchKernelGGL(k_scan_apply<true>, dim3(nt), dim3(kBlock), 0, s, count, n, bsum, start);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -459,7 +488,7 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   if (n) {
     hipLaunchKernelGGL(k_hash_count<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags);
+                       SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags, g->tsum);
   }
   return grid_sort(g, n, ra, s);
 }
@@ -470,7 +499,7 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
   if (n) {
     GridDev none{};
     hipLaunchKernelGGL(k_hash_count<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, pos, n,
-                       none, g->hash, g->slot, g->count, g->flags);
+                       none, g->hash, g->slot, g->count, g->flags, g->tsum);
   }
   return grid_sort(g, n, ra, s);
 }
@@ -530,6 +559,8 @@ static psb_status grid_alloc(psb_grid g) {
   PSB_HIP(hipMalloc(&g->tmp, 4 * cap));
   PSB_HIP(hipMalloc(&g->perm, 4 * cap));
   PSB_HIP(hipMalloc(&g->bsum, 4 * (size_t)g->nscan));
+  PSB_HIP(hipMalloc(&g->tsum, 4 * (size_t)g->nscan));
+  PSB_HIP(hipMemsetAsync(g->tsum, 0, 4 * (size_t)g->nscan, g->ctx->stream));
   PSB_HIP(hipMalloc(&g->flags, 64));
   PSB_HIP(hipMemsetAsync(g->count, 0, 4 * (g->nscan * (size_t)kScanTile), g->ctx->stream));
   PSB_HIP(hipMemsetAsync(g->flags, 0, 64, g->ctx->stream));
@@ -585,7 +616,7 @@ psb_status psb_finalize_grid(psb_grid g) {
   if (!g) return PSB_OK;
   if (g->ctx) { hipSetDevice(g->ctx->device); hipStreamSynchronize(g->ctx->stream); }
   hipFree(g->count); hipFree(g->start); hipFree(g->hash); hipFree(g->slot);
-  hipFree(g->tmp); hipFree(g->perm); hipFree(g->bsum); hipFree(g->flags);
+  hipFree(g->tmp); hipFree(g->perm); hipFree(g->bsum); hipFree(g->tsum); hipFree(g->flags);
   delete g;
   return PSB_OK;
 }
