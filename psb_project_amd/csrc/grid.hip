@@ -26,6 +26,8 @@
 
 namespace psb {
 
+constexpr int kTileSlots = 16;   // copies of the per-tile count table (atomic spreading)
+
 // M: SingleMap or BatchMap (a batch hashes into its scene's cell range).
 template <class M>
 __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const float4* __restrict__ pos,
@@ -34,7 +36,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const floa
                                                                uint32_t* __restrict__ slot,
                                                                uint32_t* __restrict__ count,
                                                                uint32_t* __restrict__ flags,
-                                                               uint32_t* __restrict__ tsum) {
+                                                               uint32_t* __restrict__ tsum,
+                                                               uint32_t nscan) {
   uint32_t i, sc;
   bool lead;
   if (!map.locate(n, i, sc, lead)) return;
@@ -49,15 +52,17 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const floa
   hash[i] = c;
   slot[i] = atomicAdd(&count[c], 1u);
   // per-scan-tile particle counts (the scan's first pass), one atomic per
-  // distinct tile of the wave: the particles arrive in last step's sorted
-  // order, so a wave's cells mostly share a tile
+  // distinct tile of the wave (the particles arrive in last step's sorted
+  // order, so a wave's cells mostly share a tile), spread over kTileSlots
+  // copies of the table by workgroup
   const uint32_t t = c / kScanTile;
+  uint32_t* __restrict__ ts = tsum + (size_t)(blockIdx.x % kTileSlots) * nscan;
   uint64_t todo = __ballot(1);
   while (todo) {
     const int leader = __ffsll((unsigned long long)todo) - 1;
     const uint32_t tl = (uint32_t)__shfl((int)t, leader, 64);
     const uint64_t same = __ballot(t == tl) & todo;
-    if ((int)(threadIdx.x & 63u) == leader) atomicAdd(&tsum[tl], (uint32_t)__popcll(same));
+    if ((int)(threadIdx.x & 63u) == leader) atomicAdd(&ts[tl], (uint32_t)__popcll(same));
     todo &= ~same;
   }
   uint32_t f = 0;
@@ -111,7 +116,8 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 }
 
 // exclusive scan of the tile sums in one workgroup: dst[k] = sum of src[<k]
-// (dst may be src); CONSUME zeroes src for the next build; start[ncells] = total
+// (dst may be src).  CONSUME: src holds kTileSlots tables of nb partial sums
+// (added here) and is zeroed for the next build.  start[ncells] = total.
 template <bool CONSUME>
 __global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* src, uint32_t* dst, uint32_t nb,
                                                     uint32_t* __restrict__ start, uint64_t ncells) {
@@ -119,13 +125,21 @@ __global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* src, uint32_t* dst
   uint32_t carry = 0;
   for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
     uint32_t k = b0 + threadIdx.x;
-    uint32_t v = k < nb ? src[k] : 0u;
+    uint32_t v = 0u;
+    if (k < nb) {
+      if (CONSUME) {
+#pragma unroll
+        for (int j = 0; j < kTileSlots; ++j) {
+          v += src[(size_t)j * nb + k];
+          src[(size_t)j * nb + k] = 0u;
+        }
+      } else {
+        v = src[k];
+      }
+    }
     uint32_t tot;
     uint32_t ex = block_excl_scan(v, lds, tot);
-    if (k < nb) {
-      dst[k] = carry + ex;
-      if (CONSUME) src[k] = 0u;
-    }
+    if (k < nb) dst[k] = carry + ex;
     carry += tot;
   }
   if (threadIdx.x == 0) start[ncells] = carry;
@@ -488,7 +502,8 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   if (n) {
     hipLaunchKernelGGL(k_hash_count<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags, g->tsum);
+                       SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags, g->tsum,
+                       g->nscan);
   }
   return grid_sort(g, n, ra, s);
 }
@@ -499,7 +514,7 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
   if (n) {
     GridDev none{};
     hipLaunchKernelGGL(k_hash_count<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, pos, n,
-                       none, g->hash, g->slot, g->count, g->flags, g->tsum);
+                       none, g->hash, g->slot, g->count, g->flags, g->tsum, g->nscan);
   }
   return grid_sort(g, n, ra, s);
 }
@@ -559,8 +574,8 @@ static psb_status grid_alloc(psb_grid g) {
   PSB_HIP(hipMalloc(&g->tmp, 4 * cap));
   PSB_HIP(hipMalloc(&g->perm, 4 * cap));
   PSB_HIP(hipMalloc(&g->bsum, 4 * (size_t)g->nscan));
-  PSB_HIP(hipMalloc(&g->tsum, 4 * (size_t)g->nscan));
-  PSB_HIP(hipMemsetAsync(g->tsum, 0, 4 * (size_t)g->nscan, g->ctx->stream));
+  PSB_HIP(hipMalloc(&g->tsum, 4 * (size_t)kTileSlots * g->nscan));
+  PSB_HIP(hipMemsetAsync(g->tsum, 0, 4 * (size_t)kTileSlots * g->nscan, g->ctx->stream));
   PSB_HIP(hipMalloc(&g->flags, 64));
   PSB_HIP(hipMemsetAsync(g->count, 0, 4 * (g->nscan * (size_t)kScanTile), g->ctx->stream));
   PSB_HIP(hipMemsetAsync(g->flags, 0, 64, g->ctx->stream));

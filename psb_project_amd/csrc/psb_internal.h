@@ -803,7 +803,7 @@ struct psb_grid_s {
   uint32_t* tmp = nullptr;         // [cap]
   uint32_t* perm = nullptr;        // [cap]
   uint32_t* bsum = nullptr;        // [scan blocks]: tile offsets
-  uint32_t* tsum = nullptr;        // [scan blocks]: tile counts, zero between builds
+  uint32_t* tsum = nullptr;        // [16][scan blocks]: tile counts, zero between builds
   uint32_t* flags = nullptr;       // [1]
   uint32_t nscan = 0;
 };
