@@ -243,7 +243,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
                                                               uint32_t* __restrict__ flags,
                                                               uint32_t* __restrict__ maxcnt,
                                                               Compact16 c16, ErrSlots es,
-                                                              const HaloTile* __restrict__ tiles) {
+                                                              const HaloTile* __restrict__ tiles,
+                                                              EpiArgs ep) {
   uint32_t p, sc;
   bool lead;
   const bool valid = map.locate(n, p, sc, lead);
@@ -329,7 +330,17 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
       }
     }
     ncount[p] = cnt < cap ? cnt : cap;
-    if (DENS) rho[p] = S * mc;
+    if (DENS) {
+      if (ep.on) {
+        // the density epilogue, fused (x.w written here is not read by the
+        // search: it tests positions only)
+        Consts cc = ep.c;
+        map.patch(cc, sc);
+        density_epilogue_one(cc, ep, p, S * mc, rho);
+      } else {
+        rho[p] = S * mc;
+      }
+    }
     if (cnt > cap) atomicOr(flags, PSB_FLAG_NEIGHBOR_OVERFLOW);
     if (wide) atomicOr(c16.wide + (size_t)sc * c16.wide_stride, 1u);
   }
@@ -523,14 +534,16 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                                   uint32_t* maxcnt, hipStream_t s, const Compact16& c16,
-                                  ErrSlots es) {
+                                  ErrSlots es, const EpiArgs* ep) {
   if (!n) return PSB_OK;
   GridDev none{};
   dim3 gb(nblocks), tb(bsize);
+  EpiArgs e{};
+  if (ep && rho) e = *ep;
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, BatchMap>), gb, tb, 0, s, map, xs, n, none, g->start, \
                      r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es, \
-                     nullptr)
+                     nullptr, e)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {
@@ -544,15 +557,17 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                             uint32_t* maxcnt, hipStream_t s, const Compact16& c16,
-                            ErrSlots es, const HaloTile* tiles) {
+                            ErrSlots es, const HaloTile* tiles, const EpiArgs* ep) {
   if (!n) return PSB_OK;
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
   gd.R = R;
   dim3 gb(blocks_for(n)), tb(kBlock);
+  EpiArgs e{};
+  if (ep && rho) e = *ep;
 #define PSB_NB(D_, R_)                                                                       \
   hipLaunchKernelGGL((k_neighbors<D_, R_, SingleMap>), gb, tb, 0, s, SingleMap{}, xs, n, gd,    \
                      g->start, r2, nb.capacity, nb.count, nb.list, rho, mc, flags, maxcnt, c16, es, \
-                     tiles)
+                     tiles, e)
   if (rho) {
     if (R == 1) PSB_NB(true, 1); else if (R == 2) PSB_NB(true, 2); else PSB_NB(true, 0);
   } else {

@@ -108,6 +108,9 @@ psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, 
 psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, const float4* pos,
                                const float4* vel, float* rho, float* pres, float4* xs, float4* vs,
                                hipStream_t s);
+// the epilogue's arguments, for the fused path of launch_neighbors
+EpiArgs make_epilogue(const Consts& c, int pcisph, int warm, int xw_prev, float* pres,
+                      const float* pprev, float4* xs, float4* vs, const uint32_t* ids);
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
                                    int xw_prev, float* rho, float* pres, const float* pprev,
                                    float4* xs, float4* vs, hipStream_t s,

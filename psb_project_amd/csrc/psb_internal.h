@@ -459,6 +459,50 @@ __device__ __forceinline__ float eos(const Consts& c, float rho) {
   return pr;
 }
 
+// Per-particle density epilogue after the neighbour sums.  WCSPH: p =
+// EOS(rho); x.w = p/rho^2; v.w = 1/rho.  PCISPH: p = 0 or the reordered
+// previous pressure; v.w = 1/rho; with xw_prev, x.w = previous pressure
+// (gathered by the warm-start feature sums); lattice walls add their density.
+// Rigid-sphere boundary particles get p = 0 and v.w = 0: zero volume in
+// every neighbour sum that weights by 1/rho_j (viscosity, surface normals,
+// warm-start features).  The neighbour kernel runs it when `on`.
+struct EpiArgs {
+  int on, pcisph, warm, xw_prev;
+  Consts c;
+  float* pres;
+  const float* pprev;
+  float4* xs;
+  float4* vs;
+  const uint32_t* ids;
+};
+__device__ __forceinline__ void density_epilogue_one(const Consts& c, const EpiArgs& e, uint32_t p,
+                                                     float r, float* __restrict__ rho) {
+  if (!e.pcisph) {
+    const float pr = eos(c, r);
+    rho[p] = r;
+    e.pres[p] = pr;
+    e.xs[p].w = pr / (r * r);
+    e.vs[p].w = 1.0f / r;
+  } else if (is_boundary(c, e.ids, p)) {
+    rho[p] = r;
+    e.pres[p] = 0.f;
+    if (e.xw_prev) e.xs[p].w = 0.f;
+    e.vs[p].w = 0.f;
+  } else {
+    e.pres[p] = e.warm ? e.pprev[p] : 0.f;
+    if (e.xw_prev) e.xs[p].w = e.pprev[p];
+    float rr = r;
+    if (c.wall) {
+      const float4 x = e.xs[p];
+      float S, gx, gy, gz;
+      wall_sums<true, false>(c, x.x, x.y, x.z, S, gx, gy, gz);
+      rr = r + S * c.mc;
+    }
+    rho[p] = rr;
+    e.vs[p].w = 1.0f / rr;
+  }
+}
+
 // Visit the ELL neighbour list of particle p in order.  The list loads and
 // gathers of U neighbours are issued before any of them is consumed, so a
 // wave keeps U dependent load chains in flight; f(q, xq) still sees the
@@ -826,7 +870,7 @@ psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, 
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
                             uint32_t* flags, uint32_t* maxcnt, hipStream_t s,
                             const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{},
-                            const HaloTile* tiles = nullptr);
+                            const HaloTile* tiles = nullptr, const EpiArgs* ep = nullptr);
 // halo tables of the tiles of n sorted particles xs (after grid_build):
 // tiles[ceil(n / kHaloTile)], src[tiles * kHaloCap]
 psb_status launch_halo(psb_grid g, const float4* xs, uint32_t n, int R, HaloTile* tiles,
@@ -840,6 +884,7 @@ psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblo
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
                                   uint32_t* maxcnt, hipStream_t s,
-                                  const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{});
+                                  const Compact16& c16 = Compact16{}, ErrSlots es = ErrSlots{},
+                                  const EpiArgs* ep = nullptr);
 
 }  // namespace psb

@@ -18,43 +18,16 @@
 
 namespace psb {
 
-// WCSPH: p = EOS(rho); x.w = p/rho^2; v.w = 1/rho.  PCISPH: p = 0 or the
-// reordered previous pressure; v.w = 1/rho; with xw_prev, x.w = previous
-// pressure (gathered by the warm-start feature sums).  Rigid-sphere boundary
-// particles get p = 0 and v.w = 0: zero volume in every neighbour sum that
-// weights by 1/rho_j (viscosity, surface normals, warm-start features).
+// The density epilogue as its own pass (density_epilogue_one).
 template <class M>
-__global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(
-    M map, uint32_t n, Consts c, int pcisph, int warm, int xw_prev, float* __restrict__ rho,
-    float* __restrict__ pres, const float* __restrict__ pprev, float4* __restrict__ xs,
-    float4* __restrict__ vs, const uint32_t* __restrict__ ids) {
+__global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(M map, uint32_t n, EpiArgs e,
+                                                                     float* __restrict__ rho) {
   uint32_t p, sc;
   bool lead;
   if (!map.locate(n, p, sc, lead)) return;
+  Consts c = e.c;
   map.patch(c, sc);
-  const float r = rho[p];
-  if (!pcisph) {
-    const float pr = eos(c, r);
-    pres[p] = pr;
-    xs[p].w = pr / (r * r);
-    vs[p].w = 1.0f / r;
-  } else if (is_boundary(c, ids, p)) {
-    pres[p] = 0.f;
-    if (xw_prev) xs[p].w = 0.f;
-    vs[p].w = 0.f;
-  } else {
-    pres[p] = warm ? pprev[p] : 0.f;
-    if (xw_prev) xs[p].w = pprev[p];
-    float rr = r;
-    if (c.wall) {
-      const float4 x = xs[p];
-      float S, gx, gy, gz;
-      wall_sums<true, false>(c, x.x, x.y, x.z, S, gx, gy, gz);
-      rr = r + S * c.mc;
-      rho[p] = rr;
-    }
-    vs[p].w = 1.0f / rr;
-  }
+  density_epilogue_one(c, e, p, rho[p], rho);
 }
 
 // Density from an existing list (standalone evaluate_densities_forces);
@@ -218,12 +191,23 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
   return PSB_OK;
 }
 
+EpiArgs make_epilogue(const Consts& c, int pcisph, int warm, int xw_prev, float* pres,
+                      const float* pprev, float4* xs, float4* vs, const uint32_t* ids) {
+  EpiArgs e{};
+  e.on = 1;
+  e.pcisph = pcisph; e.warm = warm; e.xw_prev = xw_prev;
+  e.c = c;
+  e.pres = pres; e.pprev = pprev; e.xs = xs; e.vs = vs; e.ids = ids;
+  return e;
+}
+
 psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
                                    int xw_prev, float* rho, float* pres, const float* pprev,
                                    float4* xs, float4* vs, hipStream_t s, const uint32_t* ids) {
   if (!n) return PSB_OK;
   hipLaunchKernelGGL(k_density_epilogue<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                     SingleMap{}, n, c, pcisph, warm, xw_prev, rho, pres, pprev, xs, vs, ids);
+                     SingleMap{}, n, make_epilogue(c, pcisph, warm, xw_prev, pres, pprev, xs, vs, ids),
+                     rho);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -233,8 +217,8 @@ psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, 
                                          float* rho, float* pres, const float* pprev, float4* xs,
                                          float4* vs, hipStream_t s) {
   if (!n) return PSB_OK;
-  hipLaunchKernelGGL(k_density_epilogue<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, n, c, 1,
-                     warm, xw_prev, rho, pres, pprev, xs, vs, (const uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_density_epilogue<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, n,
+                     make_epilogue(c, 1, warm, xw_prev, pres, pprev, xs, vs, nullptr), rho);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

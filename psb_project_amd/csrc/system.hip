@@ -209,14 +209,11 @@ static psb_status wcsph_step(psb_system s, int step) {
   PSB_TRY(grid_phase(s, step));
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
   {
+    // neighbour search + density + EOS epilogue in one pass
     Timed t(s, PH_NEIGHBORS, step);
+    const EpiArgs ep = make_epilogue(s->c, 0, 0, 0, s->pres, nullptr, s->ax, s->av, nullptr);
     PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
-                             s->flags, s->flags + 1, st));
-  }
-  {
-    Timed t(s, PH_EPILOGUE, step);
-    PSB_TRY(launch_density_epilogue(s->n, s->c, 0, 0, 0, s->rho, s->pres, nullptr, s->ax, s->av,
-                                    st));
+                             s->flags, s->flags + 1, st, Compact16{}, ErrSlots{}, nullptr, &ep));
   }
   {
     Timed t(s, PH_FORCES, step);
@@ -256,14 +253,12 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       (s->halo && l16 && lanes == 1 && s->tiles && s->halo_src) ? s->tiles : nullptr;
   if (tiles) PSB_TRY(launch_halo(s->grid, s->ax, s->n, s->gd.R, s->tiles, s->halo_src, st));
   {
+    // neighbour search + density + epilogue in one pass
     Timed t(s, PH_NEIGHBORS, step);
+    const EpiArgs ep = make_epilogue(s->c, 1, warm ? 1 : 0, feat ? 1 : 0, s->pres, s->ap, s->ax,
+                                     s->av, s->aid);
     PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
-                             s->flags, s->flags + 1, st, c16, s->es, tiles));
-  }
-  {
-    Timed t(s, PH_EPILOGUE, step);
-    PSB_TRY(launch_density_epilogue(s->n, s->c, 1, warm ? 1 : 0, feat ? 1 : 0, s->rho, s->pres,
-                                    s->ap, s->ax, s->av, st, s->aid));
+                             s->flags, s->flags + 1, st, c16, s->es, tiles, &ep));
   }
   PciBuffers b{};
   b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
