@@ -23,6 +23,7 @@
 // contiguous particle range [start[c0], start[c1 + 1]) because the hash is
 // x-fastest, so the neighbour kernel does two table loads per row.
 #include "psb_internal.h"
+#include "pcisph.h"
 
 namespace psb {
 
@@ -37,7 +38,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_hash_count(M map, const floa
                                                                uint32_t* __restrict__ count,
                                                                uint32_t* __restrict__ flags,
                                                                uint32_t* __restrict__ tsum,
-                                                               uint32_t nscan) {
+                                                               uint32_t nscan, StepReset rs) {
+  if (rs.st && blockIdx.x < rs.scenes) pci_reset_scene(rs.st, blockIdx.x, rs.done, rs.es);
   uint32_t i, sc;
   bool lead;
   if (!map.locate(n, i, sc, lead)) return;
@@ -509,12 +511,17 @@ psb_status scan_exclusive(uint32_t* count, uint64_t n, uint32_t* bsum, uint32_t*
 }
 
 psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
-                      hipStream_t s) {
+                      hipStream_t s, const StepReset* reset) {
   GridDev gd = grid_dev(g->desc, g->desc.cell_size);
+  StepReset rs{};
+  if (reset) {
+    if (n && blocks_for(n) >= reset->scenes) rs = *reset;   // rides on the hashing pass
+    else PSB_TRY(launch_pci_reset(reset->st, false, s, reset->scenes, reset->done, reset->es));
+  }
   if (n) {
     hipLaunchKernelGGL(k_hash_count<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        SingleMap{}, pos, n, gd, g->hash, g->slot, g->count, g->flags, g->tsum,
-                       g->nscan);
+                       g->nscan, rs);
   }
   return grid_sort(g, n, ra, s);
 }
@@ -525,7 +532,8 @@ psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, u
   if (n) {
     GridDev none{};
     hipLaunchKernelGGL(k_hash_count<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, pos, n,
-                       none, g->hash, g->slot, g->count, g->flags, g->tsum, g->nscan);
+                       none, g->hash, g->slot, g->count, g->flags, g->tsum, g->nscan,
+                       StepReset{});
   }
   return grid_sort(g, n, ra, s);
 }

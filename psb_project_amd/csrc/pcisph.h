@@ -17,6 +17,28 @@ struct PciStatus {
   uint32_t err[kMaxIterSlots];     // float bits: max error of D(k)
 };
 
+// Per-step reset of one scene's status and error slots, by one workgroup.
+__device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, uint32_t* done,
+                                                ErrSlots es) {
+  PciStatus* st = st0 + scene;
+  for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
+  if (es.base)
+    for (int k = threadIdx.x; k < kErrSlots * kMaxIterSlots; k += blockDim.x)
+      es.base[(size_t)(k / kMaxIterSlots) * es.stride + (size_t)scene * kMaxIterSlots +
+              (k % kMaxIterSlots)] = 0u;
+  if (threadIdx.x == 0) { st->final_iter = -1; st->wide = 0u; }
+  if (done && scene == 0 && threadIdx.x == 0) *done = 0u;
+}
+
+// The per-step reset as a job for the first kernel of the step (the grid
+// build's hashing pass runs it on its first `scenes` workgroups).
+struct StepReset {
+  PciStatus* st;
+  uint32_t* done;
+  ErrSlots es;
+  uint32_t scenes;
+};
+
 struct PciBuffers {
   float4* fa;    // reference mode: Facc + f_viscosity (force density)
   float4* fs;    // reference mode: f_surface

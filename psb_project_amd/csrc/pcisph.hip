@@ -30,14 +30,7 @@ namespace psb {
 
 // one block per scene; block 0 also clears the batch's finished-scene count
 __global__ void k_pci_reset(PciStatus* st0, uint32_t* done, ErrSlots es) {
-  PciStatus* st = st0 + blockIdx.x;
-  for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
-  if (es.base)
-    for (int k = threadIdx.x; k < kErrSlots * kMaxIterSlots; k += blockDim.x)
-      es.base[(size_t)(k / kMaxIterSlots) * es.stride + (size_t)blockIdx.x * kMaxIterSlots +
-              (k % kMaxIterSlots)] = 0u;
-  if (threadIdx.x == 0) { st->final_iter = -1; st->wide = 0u; }
-  if (done && blockIdx.x == 0 && threadIdx.x == 0) *done = 0u;
+  pci_reset_scene(st0, blockIdx.x, done, es);
 }
 
 __global__ void k_pci_stats_reset(PciStatus* st0) {

@@ -864,8 +864,10 @@ struct ReorderArrays {
 };
 
 // grid.hip
+struct StepReset;   // pcisph.h
+// reset: the PCISPH per-step status reset, run by the hashing pass
 psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderArrays* ra,
-                      hipStream_t s);
+                      hipStream_t s, const StepReset* reset = nullptr);
 psb_status launch_neighbors(psb_grid g, const float4* xs, uint32_t n, float r2, int R,
                             psb_neighbors nb, float* rho_sum_scaled, float mc,
                             uint32_t* flags, uint32_t* maxcnt, hipStream_t s,

@@ -196,12 +196,12 @@ static psb_status sys_alloc(psb_system s) {
   return PSB_OK;
 }
 
-static psb_status grid_phase(psb_system s, int step) {
+static psb_status grid_phase(psb_system s, int step, const StepReset* reset = nullptr) {
   ReorderArrays ra;
   ra.sx = s->bx; ra.sv = s->bv; ra.svh = s->bvh; ra.sid = s->bid; ra.sp = s->bp;
   ra.dx = s->ax; ra.dv = s->av; ra.dvh = s->avh; ra.did = s->aid; ra.dp = s->ap;
   Timed t(s, PH_GRID, step);
-  return grid_build(s->grid, s->bx, s->n, &ra, s->ctx->stream);
+  return grid_build(s->grid, s->bx, s->n, &ra, s->ctx->stream, reset);
 }
 
 static psb_status wcsph_step(psb_system s, int step) {
@@ -239,8 +239,9 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
   const bool feat = !ref && (mlp || s->rec != nullptr);
-  PSB_TRY(launch_pci_reset(s->st, false, st, 1, nullptr, s->es));
-  PSB_TRY(grid_phase(s, step));
+  // the status reset rides on the grid build's first kernel
+  const StepReset reset{s->st, nullptr, s->es, 1u};
+  PSB_TRY(grid_phase(s, step, &reset));
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
   Compact16 c16;
   if (s->compact && s->nlist16) {
