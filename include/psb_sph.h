@@ -532,7 +532,8 @@ psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
 psb_status psb_system_set_compact_list(psb_system sys, int32_t on);
 
 /* Neighbour rows staged in LDS by the PCISPH corrector passes (off by
- * default; used with one thread per particle and the compact list): the
+ * default; used with one thread per particle, psb_system_set_corrector_lanes
+ * (sys, 1), and the compact list): the
  * sorted particles are cut into tiles of 512, each tile's neighbour rows are
  * copied into LDS once per pass and the neighbours read from there.  A tile
  * whose rows exceed 4,096 particles keeps the global path.  Results do not
@@ -541,8 +542,9 @@ psb_status psb_system_set_halo(psb_system sys, int32_t on);
 
 /* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
  * sums split into even and odd list positions, one lane each), or 0 (the
- * default, 1).  Results do not depend on it: the sums always run in the
- * even + odd order.                                                        */
+ * default: 2 while one lane per particle would leave at most 4 wavefronts
+ * per SIMD, i.e. n <= 1024 x compute units, else 1).  Results do not depend
+ * on it: the sums always run in the even + odd order.                      */
 psb_status psb_system_set_corrector_lanes(psb_system sys, int32_t lanes);
 
 /* Per-phase timing with HIP events on the system stream (off by default):

@@ -718,6 +718,38 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
       for (int u = 0; u < 8; ++u)
         if ((uint32_t)u < r) f(u & 1, xq[u]);
     }
+  } else if constexpr (LANES == 2 && LI::kGrouped) {
+    // two lanes per particle on the grouped list: both load the group's
+    // word, lane l takes its entries k = l (mod 2) (4 gathers per group)
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
+    const uint32_t ng = (m + kL16Group - 1) / kL16Group, nfull = m / kL16Group;
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    if (ng) w = g4[p];
+    for (uint32_t g = 0; g < nfull; ++g) {
+      const uint4 cur = w;
+      if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
+      uint32_t e[8];
+      unpack_l16(cur, e);
+      float4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xq[u] = X[list.decode(lane ? e[2 * u + 1] : e[2 * u], p)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f(0, xq[u]);
+    }
+    if (nfull < ng) {
+      uint32_t e[8];
+      unpack_l16(w, e);
+      const uint32_t r = m - nfull * kL16Group;   // 1..7 entries left
+      float4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = 2u * u + lane;
+        xq[u] = X[j < r ? list.decode(lane ? e[2 * u + 1] : e[2 * u], p) : list.safe(p)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (2u * u + lane < r) f(0, xq[u]);
+    }
   } else if constexpr (LANES == 1) {
     uint32_t k = 0;
     for (; k + U <= m; k += U) {

@@ -102,7 +102,8 @@ struct psb_system_s {
   float4* d_soff = nullptr;
   float4* d_fb = nullptr;
   int sphere_cur = 0;
-  int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: one)
+  int32_t corrector_lanes = 0;   // threads per particle in the corrector passes (0: by size)
+  int32_t cus = 0;               // compute units of the device
   std::vector<ColliderDev> colliders;   // after the box, in order
   ColliderDev* d_coll = nullptr;
 };
@@ -228,8 +229,13 @@ static psb_status wcsph_step(psb_system s, int step) {
 // double the waves (8 per SIMD at n = 262,144 on 256 CUs) but measured no
 // faster on the 262,144-particle dam break (pci_density 19.9 us against
 // 17.0 us, pci_force 18.7 us against 18.9 us).
+// Two lanes per particle when one would leave at most 4 wavefronts per SIMD
+// (n <= 1024 x CUs): the doubled grid then hides more of the gathers'
+// latency (262,144 particles: corrector passes ~4 % faster); above that
+// one lane per particle already fills the device.
 static int corrector_lanes(psb_system s) {
-  return s->corrector_lanes == 2 ? 2 : 1;
+  if (s->corrector_lanes) return s->corrector_lanes == 2 ? 2 : 1;
+  return s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus ? 2 : 1;
 }
 
 static psb_status pcisph_step(psb_system s, int step, int slot) {
@@ -387,6 +393,10 @@ psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm, uint32_
   if (stt == PSB_OK) {
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
+  }
+  if (stt == PSB_OK) {
+    hipError_t e = hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e != hipSuccess) stt = hip_fail(e, "hipDeviceGetAttribute", __FILE__, __LINE__);
   }
   if (stt == PSB_OK) stt = grid_create(ctx, prm->grid, capacity, &s->grid);
   if (stt == PSB_OK) stt = sys_alloc(s);

@@ -193,7 +193,8 @@ class FluidSystem:
         check(lib().psb_system_set_compact_list(self.handle, 1 if on else 0))
 
     def set_corrector_lanes(self, lanes: int) -> None:
-        """Threads per particle in the corrector passes: 1, 2, or 0 (default, 1)."""
+        """Threads per particle in the corrector passes: 1, 2, or 0 (default:
+        2 up to 1024 particles per compute unit, else 1)."""
         check(lib().psb_system_set_corrector_lanes(self.handle, int(lanes)))
 
     # -- timing ------------------------------------------------------------

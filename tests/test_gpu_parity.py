@@ -270,6 +270,7 @@ def test_halo_staging_keeps_results(ctx, oracle, scene):
     runs = []
     for halo in (True, False):
         g = FluidSystem(ctx, sc.params, sc.n)
+        g.set_corrector_lanes(1)   # staging runs with one lane per particle
         g.set_halo(halo)
         g.set_particles(sc.pos4)
         runs.append(g)
