@@ -230,8 +230,9 @@ __global__ __launch_bounds__(256) void k_rank_reorder(const uint32_t* __restrict
 // density sum of the same pairs rides along (SphKernel.cu:9-27).
 // RT > 0: the stencil half-width is a compile-time constant; the (2RT+1)^2
 // row ranges are loaded first (independent table loads) and the candidates of
-// a row are gathered U at a time before being tested in order.  RT = 0: any
-// stencil width, rows visited one after the other.
+// a row are gathered kRowSlots at a time before being tested in order.  RT =
+// 0: any stencil width, rows visited one after the other.  With ep.on the
+// density epilogue (density_epilogue_one) runs on the particle's sum.
 constexpr int kRowSlots = 8;   // candidate loads in flight per stencil row
 
 template <bool DENS, int RT, class M>

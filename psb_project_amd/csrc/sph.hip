@@ -5,8 +5,8 @@
 // (ForceExt_Constante.cu:15-24), collisionBox (BoxCollision_Kernel.cu:7-34).
 //
 // The fused system step runs two neighbour passes per WCSPH step:
-//   k_neighbors<DENS> (grid.hip) + epilogue: list, rho, p = EOS(rho),
-//       x.w = p / rho^2, v.w = 1 / rho                       (k_density_epilogue)
+//   k_neighbors<DENS> (grid.hip) with the density epilogue inside: list,
+//       rho, p = EOS(rho), x.w = p / rho^2, v.w = 1 / rho
 //   k_wcsph_forces: pressure + viscosity (+ surface) force densities, gravity
 //       g*rho, leapfrog, box, velocity interpolation, all in registers; the
 //       new state goes to the B buffers.
