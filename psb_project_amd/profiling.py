@@ -11,6 +11,11 @@ traffic counters.
             [--trace KERNEL_TRACE_CSV]
         formats rocprofv3 --stats kernel_stats.csv (top kernels by time) and,
         from the per-dispatch trace, the average over full dispatches;
+    python -m psb_project_amd.profiling counters CSV [CSV ...] --kernel REGEX --cus N
+            --commands TEXT --out FILE
+        per-kernel averages of every counter in rocprofv3 --pmc counter
+        collections (dispatches of at least 5 us), per wavefront where that
+        reads better, and the derived issue and busy fractions;
     python -m psb_project_amd.profiling traffic --kernel REGEX --fetch CSV --write CSV
             --copy-fetch CSV --copy-write CSV --copy-bytes B --alg-bytes A
             --commands TEXT --out FILE
@@ -116,6 +121,65 @@ def cmd_kernel_stats(a) -> int:
     return 0
 
 
+def cmd_counters(a) -> int:
+    """Average every counter per dispatch, per kernel, over the given
+    counter-collection files (one per --pmc pass), plus derived figures:
+    VALU issue time at 4 cycles per wave64 instruction on 4 SIMDs per CU,
+    TA/TD busy per CU, L1 lines per vector read -- as fractions of the
+    dispatch duration at the given clock."""
+    rx = re.compile(a.kernel)
+    vals = defaultdict(lambda: defaultdict(list))
+    durs = defaultdict(list)
+    for path in a.csv:
+        per = defaultdict(lambda: defaultdict(float))
+        meta = {}
+        for r in _rows(path):
+            k = r.get("Kernel_Name", "")
+            if not rx.search(k):
+                continue
+            d = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            per[d][r["Counter_Name"]] += float(r["Counter_Value"])
+            meta[d] = (re.sub(r"\(.*$", "", k).replace("void ", ""),
+                       (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        for d, cs in per.items():
+            name, us = meta[d]
+            if us < a.full_min_us:
+                continue
+            durs[(name, path)].append(us)
+            for c, v in cs.items():
+                vals[name][c].append(v)
+    out = [f"# rocprofv3 counters: {a.title}", "", "commands:",
+           *[f"  {c}" for c in a.commands.split(";;")], "",
+           f"dispatches of at least {a.full_min_us} us; {a.cus} CUs x 4 SIMDs, {a.ghz} GHz", ""]
+    for name in sorted(vals):
+        cs = {c: sum(v) / len(v) for c, v in vals[name].items()}
+        us = [x for (n, _), l in durs.items() if n == name for x in l]
+        dur = sum(us) / len(us)
+        cyc = dur * 1e3 * a.ghz
+        out.append(f"{name}  (mean duration under counters {dur:.2f} us)")
+        waves = cs.get("SQ_WAVES")
+        for c in sorted(cs):
+            line = f"  {c:32s} {cs[c]:16.1f}"
+            if waves and c.startswith("SQ_INSTS"):
+                line += f"   {cs[c] / waves:10.1f} per wavefront"
+            out.append(line)
+        if "SQ_INSTS_VALU" in cs:
+            issue = cs["SQ_INSTS_VALU"] / (4 * a.cus) * 4
+            out.append(f"  -> VALU issue {issue / (a.ghz * 1e3):.2f} us = {issue / cyc:.0%} of the dispatch")
+        for c, lab in (("TA_TA_BUSY_sum", "TA busy"), ("TD_TD_BUSY_sum", "TD busy")):
+            if c in cs:
+                out.append(f"  -> {lab} {cs[c] / a.cus / cyc:.0%} per CU")
+        if "TCP_TOTAL_CACHE_ACCESSES_sum" in cs and cs.get("SQ_INSTS_VMEM_RD"):
+            out.append(f"  -> L1 accesses per vector read "
+                       f"{cs['TCP_TOTAL_CACHE_ACCESSES_sum'] / cs['SQ_INSTS_VMEM_RD']:.1f}")
+        out.append("")
+    text = "\n".join(out)
+    with open(a.out, "w") as f:
+        f.write(text)
+    print(text)
+    return 0
+
+
 def cmd_traffic(a) -> int:
     fk, nd = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE", a.full_min_us)
     wk, _ = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE", a.full_min_us)
@@ -165,6 +229,15 @@ def main(argv=None) -> int:
     k.add_argument("--split-regex", default=r"k_pci_(density|force)",
                    help="kernels whose early-exit launches are split off")
     k.add_argument("--out", required=True)
+    q = sub.add_parser("counters")
+    q.add_argument("csv", nargs="+")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--cus", type=int, default=256)
+    q.add_argument("--ghz", type=float, default=2.4)
+    q.add_argument("--commands", required=True)
+    q.add_argument("--title", default="")
+    q.add_argument("--full-min-us", type=float, default=5.0)
+    q.add_argument("--out", required=True)
     t = sub.add_parser("traffic")
     for name in ("kernel", "fetch", "write", "copy_fetch", "copy_write", "commands", "out"):
         t.add_argument("--" + name.replace("_", "-"), required=True)
@@ -173,7 +246,8 @@ def main(argv=None) -> int:
     t.add_argument("--title", default="")
     t.add_argument("--full-min-us", type=float, default=5.0)
     a = ap.parse_args(argv)
-    return {"copy": cmd_copy, "kernel-stats": cmd_kernel_stats, "traffic": cmd_traffic}[a.cmd](a)
+    return {"copy": cmd_copy, "kernel-stats": cmd_kernel_stats, "traffic": cmd_traffic,
+            "counters": cmd_counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
