@@ -251,6 +251,22 @@ def test_pcisph_lattice_walls_through_impact_bitwise(ctx, oracle, lanes):
     assert_state_equal(g.state(), o.state())
 
 
+@pytest.mark.parametrize("cap", [37, 29])
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_pcisph_list_capacity_off_group(ctx, oracle, cap, lanes):
+    """A list capacity that is not a multiple of the compact list's 8-entry
+    groups (37), and one below the lattice's 33 neighbours (29: lists
+    truncated in order, overflow flagged); one and two lanes; bitwise."""
+    sc = scenes.pcisph_dam_break(n_side=16, s=0.02, lattice_walls=True)
+    sc.params.max_neighbors = cap
+    g, o, gi, oi = run_pair(ctx, oracle, sc, 20, chunks=3, lanes=lanes)
+    for a, b in zip(gi, oi):
+        assert a["iterations"] == b["iterations"]
+        assert a["flags"] == b["flags"]
+    assert bool(gi[0]["flags"] & FLAG_NEIGHBOR_OVERFLOW) == (cap == 29)
+    assert_state_equal(g.state(), o.state())
+
+
 @pytest.mark.parametrize("scene", ["lattice_walls_16", "random_cloud"])
 def test_halo_staging_keeps_results(ctx, oracle, scene):
     """Neighbour rows staged in LDS against the global gathers and
