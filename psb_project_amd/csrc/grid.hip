@@ -293,12 +293,13 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
       }
     };
     // a row's candidates 8 at a time: one round of loads covers a typical
-    // row (slots past the row's end reload its last particle, unvisited)
+    // row (slots past the row's end load nothing and are not visited)
     auto row = [&](uint32_t q0, uint32_t q1) {
       for (uint32_t q = q0; q < q1; q += kRowSlots) {
         float4 xq[kRowSlots];
 #pragma unroll
-        for (int u = 0; u < kRowSlots; ++u) xq[u] = xs[min(q + u, q1 - 1)];
+        for (int u = 0; u < kRowSlots; ++u)
+          if (q + u < q1) xq[u] = xs[q + u];
 #pragma unroll
         for (int u = 0; u < kRowSlots; ++u)
           if (q + u < q1) visit(q + u, xq[u]);
