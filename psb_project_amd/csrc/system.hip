@@ -225,22 +225,29 @@ static psb_status wcsph_step(psb_system s, int step) {
   return PSB_OK;
 }
 
-// Threads per particle in the corrector passes.  One unless set: two lanes
-// double the waves (8 per SIMD at n = 262,144 on 256 CUs) but measured no
-// faster on the 262,144-particle dam break (pci_density 19.9 us against
-// 17.0 us, pci_force 18.7 us against 18.9 us).
-// Two lanes per particle when one would leave at most 4 wavefronts per SIMD
-// (n <= 1024 x CUs): the doubled grid then hides more of the gathers'
-// latency (262,144 particles: corrector passes ~4 % faster); above that
-// one lane per particle already fills the device.
-static int corrector_lanes(psb_system s) {
-  if (s->corrector_lanes) return s->corrector_lanes == 2 ? 2 : 1;
-  return s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus ? 2 : 1;
+// Threads per particle in the density (D) and force (F) passes, as set, or
+// by default (measured on MI355X, event-timed): D takes two lanes (262,144
+// particles 16.9 -> 16.4 us; 2.1 M 184 -> 172 us); F takes two while one
+// lane per particle would leave at most 4 wavefronts per SIMD (n <= 1024 x
+// CUs: 262,144 particles 19.3 -> 18.2 us) and one above, where the second
+// lane's duplicated per-particle work outweighs the latency it hides (2.1 M
+// 213 against 220 us).  LDS staging runs with one lane in both.
+struct Lanes {
+  int d, f;
+};
+static Lanes corrector_lanes(psb_system s) {
+  if (s->corrector_lanes) {
+    const int l = s->corrector_lanes == 2 ? 2 : 1;
+    return Lanes{l, l};
+  }
+  if (s->halo) return Lanes{1, 1};
+  const bool small = s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus;
+  return Lanes{2, small ? 2 : 1};
 }
 
 static psb_status pcisph_step(psb_system s, int step, int slot) {
   hipStream_t st = s->ctx->stream;
-  const int lanes = corrector_lanes(s);
+  const Lanes lanes = corrector_lanes(s);
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
   const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
@@ -257,7 +264,8 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   const uint16_t* l16 = c16.list16;
   // halo staging: single lane, compact list
   const HaloTile* tiles =
-      (s->halo && l16 && lanes == 1 && s->tiles && s->halo_src) ? s->tiles : nullptr;
+      (s->halo && l16 && lanes.d == 1 && lanes.f == 1 && s->tiles && s->halo_src) ? s->tiles
+                                                                                     : nullptr;
   if (tiles) PSB_TRY(launch_halo(s->grid, s->ax, s->n, s->gd.R, s->tiles, s->halo_src, st));
   {
     // neighbour search + density + epilogue in one pass
@@ -290,7 +298,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   if (warm) {
     Timed t(s, PH_PCI_FORCE, step);
     PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
-                             b, s->st, -1, 1, o, st, lanes, l16, tiles, s->halo_src));
+                             b, s->st, -1, 1, o, st, lanes.f, l16, tiles, s->halo_src));
     std::swap(cur, nxt);
   }
   const int min_it = s->c.min_it, max_it = s->c.max_it;
@@ -301,7 +309,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
       Timed t(s, PH_PCI_DENSITY, step, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
       PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
-                                 s->st, k, s->feat, dflags, st, s->aid, lanes, l16, s->es, tiles,
+                                 s->st, k, s->feat, dflags, st, s->aid, lanes.d, l16, s->es, tiles,
                                  s->halo_src));
     }
     if (k == 0 && mlp) {
@@ -314,7 +322,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     {
       Timed t(s, PH_PCI_FORCE, step, k);
       PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
-                               s->pres, b, s->st, k, 0, o, st, lanes, l16, tiles, s->halo_src));
+                               s->pres, b, s->st, k, 0, o, st, lanes.f, l16, tiles, s->halo_src));
     }
     std::swap(cur, nxt);
     if (ref) continue;
