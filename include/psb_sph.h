@@ -542,10 +542,10 @@ psb_status psb_system_set_halo(psb_system sys, int32_t on);
 
 /* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
  * sums split into even and odd list positions, one lane each), or 0 (the
- * default: 2 in the density pass; 2 in the force pass while one lane per
- * particle would leave at most 4 wavefronts per SIMD, i.e. n <= 1024 x
- * compute units, else 1; 1 in both with LDS staging on).  Results do not
- * depend on it: the sums always run in the even + odd order.               */
+ * default: 2 while one lane per particle would leave at most 4 wavefronts
+ * per SIMD, i.e. n <= 1024 x compute units, else 1; 1 with LDS staging
+ * on).  Results do not depend on it: the sums always run in the even + odd
+ * order.                                                                   */
 psb_status psb_system_set_corrector_lanes(psb_system sys, int32_t lanes);
 
 /* Per-phase timing with HIP events on the system stream (off by default):

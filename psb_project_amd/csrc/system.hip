@@ -226,12 +226,13 @@ static psb_status wcsph_step(psb_system s, int step) {
 }
 
 // Threads per particle in the density (D) and force (F) passes, as set, or
-// by default (measured on MI355X, event-timed): D takes two lanes (262,144
-// particles 16.9 -> 16.4 us; 2.1 M 184 -> 172 us); F takes two while one
-// lane per particle would leave at most 4 wavefronts per SIMD (n <= 1024 x
-// CUs: 262,144 particles 19.3 -> 18.2 us) and one above, where the second
-// lane's duplicated per-particle work outweighs the latency it hides (2.1 M
-// 213 against 220 us).  LDS staging runs with one lane in both.
+// by default two while one lane per particle would leave at most 4
+// wavefronts per SIMD (n <= 1024 x CUs), else one.  Measured on MI355X,
+// event-timed: 262,144 particles, D 16.9 -> 16.4 us and F 19.3 -> 18.2 us
+// with two; 2.1 M (DEM scene) two lanes win in D early (184 -> 172 us) but
+// lose later in the run (46.6 -> 61.9 us after 400 steps) and in F (213 ->
+// 220 us): the second lane duplicates the per-particle work.  LDS staging
+// runs with one lane.
 struct Lanes {
   int d, f;
 };
@@ -241,8 +242,8 @@ static Lanes corrector_lanes(psb_system s) {
     return Lanes{l, l};
   }
   if (s->halo) return Lanes{1, 1};
-  const bool small = s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus;
-  return Lanes{2, small ? 2 : 1};
+  const int l = s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus ? 2 : 1;
+  return Lanes{l, l};
 }
 
 static psb_status pcisph_step(psb_system s, int step, int slot) {

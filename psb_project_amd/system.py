@@ -194,8 +194,7 @@ class FluidSystem:
 
     def set_corrector_lanes(self, lanes: int) -> None:
         """Threads per particle in the corrector passes: 1, 2, or 0 (default:
-        2 in the density pass; 2 in the force pass up to 1024 particles per
-        compute unit, else 1)."""
+        2 up to 1024 particles per compute unit, else 1)."""
         check(lib().psb_system_set_corrector_lanes(self.handle, int(lanes)))
 
     # -- timing ------------------------------------------------------------
