@@ -253,10 +253,12 @@ def evaluate(ctx: Context, scenes_list, steps: int, model: PressureMLP | None,
 
 def train_and_evaluate(ctx: Context, make_scene, train_ids, eval_ids, steps: int = 200,
                        stride: int = 16, epochs: int = 4, batch: int = 8192, lr: float = 1e-3,
-                       seed: int = 1, log=None):
-    """Record plain-PCISPH converged pressures on the training scenes, train
-    the network on the GPU, then compare warm starts on the held-out scenes.
-    Returns (model, summary)."""
+                       seed: int = 1, log=None, eval_steps: int | None = None):
+    """Record plain-PCISPH converged pressures on the training scenes (`steps`
+    steps each, every `stride`-th particle), train the network on the GPU,
+    then compare warm starts on the held-out scenes over `eval_steps` steps
+    (default: `steps`).  Returns (model, summary)."""
+    eval_steps = steps if eval_steps is None else eval_steps
     train_scenes = [make_scene(i) for i in train_ids]
     n_per = sum((sc.n + stride - 1) // stride for sc in train_scenes) * steps
     ds = Dataset(ctx, n_per)
@@ -269,10 +271,11 @@ def train_and_evaluate(ctx: Context, make_scene, train_ids, eval_ids, steps: int
     samples = ds.count
     ds.close()
     summary = {"train_scenes": list(train_ids), "eval_scenes": list(eval_ids), "steps": steps,
+               "eval_steps": eval_steps, "stride": stride,
                "samples": int(samples), "epochs": epochs, "lr": lr, "batch": batch,
                "train_plain_iters_per_step": float(np.mean(plain_iters)),
                "mse_per_epoch": [float(x) for x in losses]}
-    summary["eval"] = evaluate(ctx, [make_scene(i) for i in eval_ids], steps, model)
+    summary["eval"] = evaluate(ctx, [make_scene(i) for i in eval_ids], eval_steps, model)
     return model, summary
 
 
