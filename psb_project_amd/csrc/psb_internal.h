@@ -581,12 +581,13 @@ __device__ __forceinline__ float rcp_rn_fast(float d) {
 __device__ __forceinline__ float sqrt_rn(float x) {
   return __builtin_expect(x >= kFastSqrtMin, 1) ? sqrt_rn_fast(x) : sqrtf(x);
 }
-// dd = sqrtf(d2), invd = 1.0f / dd for d2 > 0
+// dd = sqrtf(d2), invd = 1.0f / dd for d2 > 0: the fast path always (no
+// branch around it), redone by the full expansions in the rare lanes below
+// kFastSqrtMin
 __device__ __forceinline__ void dist_and_inverse(float d2, float& dd, float& invd) {
-  if (__builtin_expect(d2 >= kFastSqrtMin, 1)) {
-    dd = sqrt_rn_fast(d2);
-    invd = rcp_rn_fast(dd);
-  } else {
+  dd = sqrt_rn_fast(d2);
+  invd = rcp_rn_fast(dd);
+  if (__builtin_expect(d2 < kFastSqrtMin, 0)) {
     dd = sqrtf(d2);
     invd = 1.0f / dd;
   }
