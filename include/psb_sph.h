@@ -677,7 +677,9 @@ typedef struct psb_scene_desc {
   psb_grid_desc grid;         /* scene grid; cell_size = material grid's     */
 } psb_scene_desc;
 
-/* Scene-batch step phases, for psb_batch_autotune / psb_batch_get_timing.   */
+/* Scene-batch step phases, for psb_batch_autotune / psb_batch_get_timing.
+ * The density epilogue runs inside the neighbors_density phase; its own
+ * phase slot is kept for the layout and launches nothing.                  */
 #define PSB_BATCH_PHASES 6  /* hash_count, neighbors_density, density_epilogue,
                                pci_nonpressure, pci_density, pci_force       */
 

@@ -197,15 +197,16 @@ static psb_status enqueue_prologue(psb_batch b) {
   hipStream_t st = b->ctx->stream;
   const bool warm = b->prm.warm_start == PSB_WARM_PREVIOUS;
   const bool mlp = b->prm.warm_start == PSB_WARM_MLP;
-  PSB_TRY(launch_pci_reset(b->st, false, st, b->S, b->done, b->es));
   {
+    // the per-step status reset rides on the hashing pass
     BTimed t(b, BP_HASH);
     ReorderArrays ra;
     ra.sx = b->bx; ra.sv = b->bv; ra.svh = b->bvh; ra.sid = b->bid; ra.sp = b->bp;
     ra.dx = b->ax; ra.dv = b->av; ra.dvh = b->avh; ra.did = b->aid; ra.dp = b->ap;
     const int s = b->bsel[BP_HASH];
+    const StepReset reset{b->st, b->done, b->es, b->S};
     PSB_TRY(grid_build_batch(b->grid, bmap(b, BP_HASH), b->nblk[s], 64u << s, b->bx, b->n, &ra,
-                             st));
+                             st, &reset));
   }
   psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
   {
@@ -215,16 +216,12 @@ static psb_status enqueue_prologue(psb_batch b) {
     c16.list16 = b->nlist16;
     c16.wide = &b->st[0].wide;
     c16.wide_stride = sizeof(PciStatus) / sizeof(uint32_t);
+    // with the density epilogue inside (the epilogue phase launches nothing)
+    const EpiArgs ep = make_epilogue(b->c, 1, warm ? 1 : 0, mlp ? 1 : 0, b->pres, b->ap, b->ax,
+                                     b->av, nullptr);
     PSB_TRY(launch_neighbors_batch(b->grid, bmap(b, BP_NEIGHBORS), b->nblk[s], 64u << s, b->ax,
                                    b->n, b->c.h2, b->gd.R, nb, b->rho, b->c.mc, b->flags,
-                                   b->flags + 1, st, c16, b->es));
-  }
-  {
-    BTimed t(b, BP_EPILOGUE);
-    const int s = b->bsel[BP_EPILOGUE];
-    PSB_TRY(launch_density_epilogue_batch(bmap(b, BP_EPILOGUE), b->nblk[s], 64u << s, b->n, b->c,
-                                          warm ? 1 : 0, mlp ? 1 : 0, b->rho, b->pres, b->ap,
-                                          b->ax, b->av, st));
+                                   b->flags + 1, st, c16, b->es, &ep));
   }
   PciBuffers pb; PciOut o;
   buffers(b, pb, o);

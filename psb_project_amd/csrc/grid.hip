@@ -530,12 +530,16 @@ psb_status grid_build(psb_grid g, const float4* pos, uint32_t n, const ReorderAr
 
 psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
                             const float4* pos, uint32_t n, const ReorderArrays* ra,
-                            hipStream_t s) {
+                            hipStream_t s, const StepReset* reset) {
+  StepReset rs{};
+  if (reset) {
+    if (n && nblocks >= reset->scenes) rs = *reset;   // rides on the hashing pass
+    else PSB_TRY(launch_pci_reset(reset->st, false, s, reset->scenes, reset->done, reset->es));
+  }
   if (n) {
     GridDev none{};
     hipLaunchKernelGGL(k_hash_count<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, pos, n,
-                       none, g->hash, g->slot, g->count, g->flags, g->tsum, g->nscan,
-                       StepReset{});
+                       none, g->hash, g->slot, g->count, g->flags, g->tsum, g->nscan, rs);
   }
   return grid_sort(g, n, ra, s);
 }

@@ -914,7 +914,7 @@ psb_status launch_halo(psb_grid g, const float4* xs, uint32_t n, int R, HaloTile
 // [cell_off, cell_off + its grid)); nblocks x bsize threads, scene-aligned
 psb_status grid_build_batch(psb_grid g, const BatchMap& map, uint32_t nblocks, uint32_t bsize,
                             const float4* pos, uint32_t n, const ReorderArrays* ra,
-                            hipStream_t s);
+                            hipStream_t s, const StepReset* reset = nullptr);
 psb_status launch_neighbors_batch(psb_grid g, const BatchMap& map, uint32_t nblocks,
                                   uint32_t bsize, const float4* xs, uint32_t n, float r2, int R,
                                   psb_neighbors nb, float* rho, float mc, uint32_t* flags,
