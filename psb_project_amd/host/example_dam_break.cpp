@@ -13,6 +13,8 @@
 //       record: params (psb_sph_params), n, initial float4 positions, final
 //       float4 positions / velocities (by particle id), iterations of the
 //       last step.
+//       --spheres adds two rigid DEM spheres (SPHSystem::setSpheres) and
+//       prints their parameters and final state;
 //       --colliders adds a SphereCollision obstacle at the block's front and
 //       a PlaneCollision plate under it (printed as JSON lines "collider")
 //       --surface FILE writes the SurfaceSPH mesh after the K steps (binary:
@@ -44,7 +46,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
   std::string print_xml, print_txt, surface_out;
-  bool print = false, config = false, colliders = false;
+  bool print = false, config = false, colliders = false, spheres = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -57,6 +59,7 @@ int main(int argc, char** argv) {
     else if (a == "--out") out = next();
     else if (a == "--config") config = true;
     else if (a == "--colliders") colliders = true;
+    else if (a == "--spheres") spheres = true;
     else if (a == "--export-xml") export_xml = next();
     else if (a == "--export-txt") export_txt = next();
     else if (a == "--checkpoint") checkpoint = next();
@@ -156,6 +159,18 @@ int main(int argc, char** argv) {
       sys.addObjectCollision(ob);
       sys.addObjectCollision(plate);
     }
+    if (spheres) {
+      // two rigid DEM spheres in the flow's path
+      psb_sphere_params sp{};
+      sp.radius = 0.2f * L; sp.density = 500.f; sp.spring = 1e4f; sp.damping = 0.3f;
+      const std::vector<Vector3> centers = {Vector3(1.6f * L, 0.25f * L, 0.5f * L),
+                                            Vector3(2.2f * L, 0.25f * L, 1.2f * L)};
+      sys.setSpheres(centers, sp);
+      for (const Vector3& c : centers)
+        std::printf("{\"sphere\": [%.9g, %.9g, %.9g], \"radius\": %.9g, \"density\": %.9g, "
+                    "\"spring\": %.9g, \"damping\": %.9g}\n", c.x, c.y, c.z, sp.radius, sp.density,
+                    sp.spring, sp.damping);
+    }
     EmitterBox emitter(Vector3(0.5f * L, 0.5f * L, 0.5f * L), L - s, L - s, L - s, &water);
     std::vector<SPHParticle*> ps = emitter.emitParticles();
     const uint32_t n = (uint32_t)ps.size();
@@ -187,6 +202,13 @@ int main(int argc, char** argv) {
       for (const Vector3& q : surf.getNormales()) { const float a3[3] = {q.x, q.y, q.z}; std::fwrite(a3, 4, 3, f); }
       std::fwrite(surf.getIndexs().data(), 4, surf.getIndexs().size(), f);
       std::fclose(f);
+    }
+    if (spheres) {
+      const PCI_SPHSystem::SphereState ss = sys.getSpheres();
+      for (size_t k = 0; k < ss.centers.size(); ++k)
+        std::printf("{\"sphere_final\": [%.9g, %.9g, %.9g], \"velocity\": [%.9g, %.9g, %.9g]}\n",
+                    ss.centers[k].x, ss.centers[k].y, ss.centers[k].z, ss.velocities[k].x,
+                    ss.velocities[k].y, ss.velocities[k].z);
     }
     std::vector<SPHParticle*> res = sys.getParticles();
     const psb_step_stats st = sys.stats();

@@ -421,8 +421,26 @@ class SPHSystem : public System {
     }
     const uint32_t n = (uint32_t)particles.size();
     if (sys) { psb_system_destroy(sys); sys = nullptr; }
-    check(psb_system_create(ctx, &prm, n, &sys));
+    // capacity: the fluid plus the rigid spheres' boundary particles
+    const uint32_t nsph = (uint32_t)sphere_centers.size();
+    const uint32_t extra = nsph ? nsph * psb_sphere_particle_count(&prm, &sphere_prm) : 0u;
+    check(psb_system_create(ctx, &prm, n + extra, &sys));
     for (const psb_collider& k : colliders) check(psb_system_add_collider(sys, &k));
+    if (lanes >= 0) check(psb_system_set_corrector_lanes(sys, lanes));
+    if (compact >= 0) check(psb_system_set_compact_list(sys, compact));
+    if (halo >= 0) check(psb_system_set_halo(sys, halo));
+    if (mlp) check(psb_system_set_mlp(sys, mlp));
+    if (nsph) {
+      std::vector<float> c(3 * (size_t)nsph), u(3 * (size_t)nsph, 0.f);
+      for (uint32_t k = 0; k < nsph; ++k) {
+        c[3 * k] = sphere_centers[k].x; c[3 * k + 1] = sphere_centers[k].y; c[3 * k + 2] = sphere_centers[k].z;
+        if (k < sphere_velocities.size()) {
+          u[3 * k] = sphere_velocities[k].x; u[3 * k + 1] = sphere_velocities[k].y;
+          u[3 * k + 2] = sphere_velocities[k].z;
+        }
+      }
+      check(psb_system_set_spheres(sys, nsph, c.data(), u.data(), &sphere_prm));
+    }
     std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
       const Vector3 p = particles[i]->getPos(), u = particles[i]->getVel();
@@ -486,6 +504,48 @@ class SPHSystem : public System {
     dirty = true;
   }
 
+  // Rigid DEM spheres coupled to the fluid through boundary particles
+  // (psb_system_set_spheres; no counterpart in the reference's SPH classes).
+  // Set before init(): the system's capacity then holds their boundary
+  // particles too, which follow the fluid and are not in getParticles().
+  void setSpheres(const std::vector<Vector3>& centers, const psb_sphere_params& sp,
+                  const std::vector<Vector3>& velocities = {}) {
+    sphere_centers = centers;
+    sphere_velocities = velocities;
+    sphere_prm = sp;
+  }
+  struct SphereState {
+    std::vector<Vector3> centers, velocities, fluid_forces;
+  };
+  // centres, velocities and the last step's fluid force of every sphere
+  SphereState getSpheres() const {
+    SphereState o;
+    if (!sys) return o;
+    const uint32_t m = psb_system_num_spheres(sys);
+    std::vector<float> c(3 * (size_t)m), v(3 * (size_t)m), f(3 * (size_t)m);
+    if (m) check(psb_system_get_spheres(sys, c.data(), v.data(), f.data()));
+    for (uint32_t k = 0; k < m; ++k) {
+      o.centers.emplace_back(c[3 * k], c[3 * k + 1], c[3 * k + 2]);
+      o.velocities.emplace_back(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+      o.fluid_forces.emplace_back(f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+    }
+    return o;
+  }
+
+  // Learned warm start (params().warm_start = PSB_WARM_MLP): the network
+  // that predicts each step's starting pressure; the caller keeps it alive.
+  void setWarmStartModel(psb_mlp m) {
+    mlp = m;
+    if (sys) check(psb_system_set_mlp(sys, m));
+  }
+
+  // Speed knobs of the corrector passes (results do not depend on them):
+  // threads per particle (0 = by size, 1, 2), the 16-bit neighbour list,
+  // neighbour rows staged in LDS.  Applied at init() when set before it.
+  void setCorrectorLanes(int l) { lanes = l; if (sys) check(psb_system_set_corrector_lanes(sys, l)); }
+  void setCompactList(bool on) { compact = on; if (sys) check(psb_system_set_compact_list(sys, on)); }
+  void setHalo(bool on) { halo = on; if (sys) check(psb_system_set_halo(sys, on)); }
+
   // State in a given storage order (ids: particle index of each slot), e.g.
   // from ParticleLoader_XML::restore; marks the host mirror stale.
   void setState(const std::vector<float>& pos4, const std::vector<float>& vel4,
@@ -514,6 +574,10 @@ class SPHSystem : public System {
   psb_system sys = nullptr;
   psb_sph_params prm{};
   bool dirty = false;
+  std::vector<Vector3> sphere_centers, sphere_velocities;
+  psb_sphere_params sphere_prm{};
+  psb_mlp mlp = nullptr;
+  int lanes = -1, compact = -1, halo = -1;   // -1: library default
 };
 
 // PCI_SPHSystem (PciSphSystem.h): PCISPH on the GPU through psb_system.
