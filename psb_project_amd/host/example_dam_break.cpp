@@ -13,6 +13,8 @@
 //       record: params (psb_sph_params), n, initial float4 positions, final
 //       float4 positions / velocities (by particle id), iterations of the
 //       last step.
+//       --batch runs the scene as scene 0 of a two-scene SceneBatch instead
+//       (with --config) and writes scene 0's result;
 //       --spheres adds two rigid DEM spheres (SPHSystem::setSpheres) and
 //       prints their parameters and final state;
 //       --colliders adds a SphereCollision obstacle at the block's front and
@@ -46,7 +48,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
   std::string print_xml, print_txt, surface_out;
-  bool print = false, config = false, colliders = false, spheres = false;
+  bool print = false, config = false, colliders = false, spheres = false, batch = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -60,6 +62,7 @@ int main(int argc, char** argv) {
     else if (a == "--config") config = true;
     else if (a == "--colliders") colliders = true;
     else if (a == "--spheres") spheres = true;
+    else if (a == "--batch") batch = true;
     else if (a == "--export-xml") export_xml = next();
     else if (a == "--export-txt") export_txt = next();
     else if (a == "--checkpoint") checkpoint = next();
@@ -185,6 +188,46 @@ int main(int argc, char** argv) {
       return 1;
     }
     if (!resume.empty()) sys.loadCheckpoint(resume.c_str());
+    if (batch) {
+      // the same scene as scene 0 of a SceneBatch, beside a second scene (the
+      // block's first half in the same box); scene 0's result goes to --out
+      const psb_sph_params m = sys.params();
+      psb_scene_desc d{};
+      d.count = n;
+      for (int a = 0; a < 3; ++a) { d.box_min[a] = m.box_min[a]; d.box_max[a] = m.box_max[a]; }
+      d.grid = m.grid;
+      psb_scene_desc d1 = d;
+      d1.count = n / 2;
+      SceneBatch sb(sys.context(), m, {d, d1});
+      sb.setParticles(0, x0.data());
+      sb.setParticles(1, x0.data());
+      sb.update(steps);
+      std::vector<float> bx, bv;
+      std::vector<uint32_t> ids;
+      sb.getParticles(0, n, bx, bv, ids);
+      const psb_step_stats st = sb.stats(0);
+      std::printf("batch scenes %u particles %u steps %d iterations %d\n", sb.numScenes(), n, steps,
+                  st.iterations);
+      if (!out.empty()) {
+        FILE* f = std::fopen(out.c_str(), "wb");
+        if (!f) return 1;
+        std::fwrite(&m, sizeof(m), 1, f);
+        std::fwrite(&n, 4, 1, f);
+        std::fwrite(x0.data(), 4, x0.size(), f);
+        std::vector<float> x(4 * (size_t)n, 0.f), v(4 * (size_t)n, 0.f);
+        for (uint32_t k = 0; k < n; ++k)
+          for (int a = 0; a < 3; ++a) {
+            x[4 * (size_t)ids[k] + a] = bx[4 * (size_t)k + a];
+            v[4 * (size_t)ids[k] + a] = bv[4 * (size_t)k + a];
+          }
+        std::fwrite(x.data(), 4, x.size(), f);
+        std::fwrite(v.data(), 4, v.size(), f);
+        const int32_t it = st.iterations;
+        std::fwrite(&it, 4, 1, f);
+        std::fclose(f);
+      }
+      return 0;
+    }
     for (int k = 0; k < steps; ++k) sys.update();
     if (!export_xml.empty() && !ParticleExporter_XML()._export(export_xml.c_str(), &sys)) return 1;
     if (!export_txt.empty() && !ParticleExporter_Txt()._export(export_txt.c_str(), &sys)) return 1;

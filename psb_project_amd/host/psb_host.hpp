@@ -589,6 +589,66 @@ class PCI_SPHSystem : public SPHSystem {
   int32_t solver() const override { return PSB_SOLVER_PCISPH; }
 };
 
+// SceneBatch: many independent PCISPH scenes of one material advanced by one
+// launch per phase (psb_batch_*; no counterpart in the reference, which runs
+// one System per process).  Each scene stops at its own tolerance and equals
+// the same scene run alone, bit for bit.
+class SceneBatch {
+ public:
+  SceneBatch(psb_context ctx, const psb_sph_params& material,
+             const std::vector<psb_scene_desc>& scenes) {
+    check(psb_batch_create(ctx, &material, (uint32_t)scenes.size(), scenes.data(), &b));
+  }
+  ~SceneBatch() { if (b) psb_batch_destroy(b); }
+  SceneBatch(const SceneBatch&) = delete;
+  SceneBatch& operator=(const SceneBatch&) = delete;
+
+  uint32_t numScenes() const { return psb_batch_num_scenes(b); }
+  // FP32 float4 arrays of one scene (vel4 / vh4 may be null: zero)
+  void setParticles(uint32_t scene, const float* pos4, const float* vel4 = nullptr,
+                    const float* vh4 = nullptr) {
+    check(psb_batch_set_particles(b, scene, pos4, vel4, vh4));
+  }
+  void update(int32_t steps = 1) { check(psb_batch_update(b, steps)); }
+  psb_step_stats stats(uint32_t scene) const {
+    psb_step_stats s{};
+    check(psb_batch_get_stats(b, scene, &s));
+    return s;
+  }
+  // one scene's state in its sorted order, with the particle ids
+  void getParticles(uint32_t scene, uint32_t count, std::vector<float>& pos4,
+                    std::vector<float>& vel4, std::vector<uint32_t>& ids) const {
+    pos4.assign(4 * (size_t)count, 0.f);
+    vel4.assign(4 * (size_t)count, 0.f);
+    ids.assign(count, 0u);
+    check(psb_batch_get_particles(b, scene, pos4.data(), vel4.data(), nullptr, nullptr, nullptr,
+                                  ids.data()));
+  }
+  void setWarmStartModel(psb_mlp m) { check(psb_batch_set_mlp(b, m)); }
+  // launch shapes per phase (PSB_BATCH_PHASES), as psb_batch_autotune returns them
+  std::vector<int32_t> autotune(int32_t steps = 2) {
+    std::vector<int32_t> bs(PSB_BATCH_PHASES, 0);
+    check(psb_batch_autotune(b, steps, bs.data()));
+    return bs;
+  }
+  void setBlockSizes(const std::vector<int32_t>& bs) { check(psb_batch_set_block_sizes(b, bs.data())); }
+  void setGraphs(bool on) { check(psb_batch_set_graphs(b, on)); }
+  void setLatency(bool on) { check(psb_batch_set_latency(b, on)); }
+  struct Latency {
+    double p50_ms = 0, p99_ms = 0, max_ms = 0;
+    uint64_t samples = 0;
+  };
+  Latency latency() const {
+    Latency l;
+    check(psb_batch_get_latency(b, &l.p50_ms, &l.p99_ms, &l.max_ms, &l.samples));
+    return l;
+  }
+  psb_batch handle() const { return b; }
+
+ private:
+  psb_batch b = nullptr;
+};
+
 // ---------------------------------------------------------------------------
 // Surface
 // ---------------------------------------------------------------------------

@@ -153,6 +153,22 @@ def test_cpp_spheres_match_python_binding(ctx, tmp_path):
     fs.close()
 
 
+def test_cpp_scene_batch_matches_single_system(ctx, tmp_path):
+    """SceneBatch in the C++ API: the dam break as scene 0 of a two-scene
+    batch ends bit for bit where the single PCI_SPHSystem run ends."""
+    recs = []
+    for extra in ([], ["--batch"]):
+        out = tmp_path / f"run{len(recs)}.bin"
+        r = subprocess.run([str(EXE), "--steps", "20", "--side", "12", "--config", *extra,
+                            "--out", str(out)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        recs.append(read_record(out))
+    (_, n0, x00, x_single, v_single, _), (_, n1, x01, x_batch, v_batch, _) = recs
+    assert n0 == n1 and np.array_equal(x00, x01)
+    assert np.array_equal(x_single[:, :3], x_batch[:, :3])
+    assert np.array_equal(v_single[:, :3], v_batch[:, :3])
+
+
 def test_cpp_surface_matches_python_binding(ctx, tmp_path):
     """SurfaceSPH::extract in the C++ API returns the mesh psb_surface_extract
     gives the Python binding for the same state."""
