@@ -121,10 +121,6 @@ psb_status launch_pci_force_batch(const BatchMap& map, const Shape& sh, uint32_t
                                   const float* pres, const PciBuffers& b, PciStatus* st, int k,
                                   int warm_pass, const PciOut& o, hipStream_t s, int lanes = 1,
                                   const uint16_t* list16 = nullptr);
-psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, uint32_t bsize,
-                                         uint32_t n, const Consts& c, int warm, int xw_prev,
-                                         float* rho, float* pres, const float* pprev, float4* xs,
-                                         float4* vs, hipStream_t s);
 
 // sph.hip
 psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, const float4* pos,
@@ -133,10 +129,6 @@ psb_status launch_density_list(uint32_t n, const Consts& c, psb_neighbors nb, co
 // the epilogue's arguments, for the fused path of launch_neighbors
 EpiArgs make_epilogue(const Consts& c, int pcisph, int warm, int xw_prev, float* pres,
                       const float* pprev, float4* xs, float4* vs, const uint32_t* ids);
-psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   int xw_prev, float* rho, float* pres, const float* pprev,
-                                   float4* xs, float4* vs, hipStream_t s,
-                                   const uint32_t* ids = nullptr);
 psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
                                     const uint32_t* list, const float4* xs, const float4* vs,
                                     const float* rho, const float* pres, const float4* avh,

@@ -18,18 +18,6 @@
 
 namespace psb {
 
-// The density epilogue as its own pass (density_epilogue_one).
-template <class M>
-__global__ __launch_bounds__(M::kMaxThreads) void k_density_epilogue(M map, uint32_t n, EpiArgs e,
-                                                                     float* __restrict__ rho) {
-  uint32_t p, sc;
-  bool lead;
-  if (!map.locate(n, p, sc, lead)) return;
-  Consts c = e.c;
-  map.patch(c, sc);
-  density_epilogue_one(c, e, p, rho[p], rho);
-}
-
 // Density from an existing list (standalone evaluate_densities_forces);
 // copies x, v into the scratch vec4 arrays with their w lanes filled.
 __global__ __launch_bounds__(256) void k_density_list(uint32_t n, Consts c,
@@ -199,28 +187,6 @@ EpiArgs make_epilogue(const Consts& c, int pcisph, int warm, int xw_prev, float*
   e.c = c;
   e.pres = pres; e.pprev = pprev; e.xs = xs; e.vs = vs; e.ids = ids;
   return e;
-}
-
-psb_status launch_density_epilogue(uint32_t n, const Consts& c, int pcisph, int warm,
-                                   int xw_prev, float* rho, float* pres, const float* pprev,
-                                   float4* xs, float4* vs, hipStream_t s, const uint32_t* ids) {
-  if (!n) return PSB_OK;
-  hipLaunchKernelGGL(k_density_epilogue<SingleMap>, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                     SingleMap{}, n, make_epilogue(c, pcisph, warm, xw_prev, pres, pprev, xs, vs, ids),
-                     rho);
-  PSB_HIP(hipGetLastError());
-  return PSB_OK;
-}
-
-psb_status launch_density_epilogue_batch(const BatchMap& map, uint32_t nblocks, uint32_t bsize,
-                                         uint32_t n, const Consts& c, int warm, int xw_prev,
-                                         float* rho, float* pres, const float* pprev, float4* xs,
-                                         float4* vs, hipStream_t s) {
-  if (!n) return PSB_OK;
-  hipLaunchKernelGGL(k_density_epilogue<BatchMap>, dim3(nblocks), dim3(bsize), 0, s, map, n,
-                     make_epilogue(c, 1, warm, xw_prev, pres, pprev, xs, vs, nullptr), rho);
-  PSB_HIP(hipGetLastError());
-  return PSB_OK;
 }
 
 psb_status launch_wcsph_step_forces(uint32_t n, const Consts& c, bool surf, const uint32_t* cnt,
