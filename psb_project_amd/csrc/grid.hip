@@ -153,6 +153,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ count
                                                     const uint32_t* __restrict__ bsum,
                                                     uint32_t* __restrict__ start) {
   __shared__ uint32_t lds[4];
+  const uint32_t toff = bsum[blockIdx.x];   // loaded with the counts, not after the block scan
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * 16;
   uint32_t v[16];
   const bool full = (uint64_t)(blockIdx.x + 1) * kScanTile <= ncells;
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ count
 #pragma unroll
   for (int j = 0; j < 16; ++j) { uint32_t t = v[j]; v[j] = s; s += t; }
   uint32_t tot;
-  uint32_t off = block_excl_scan(s, lds, tot) + bsum[blockIdx.x];
+  uint32_t off = block_excl_scan(s, lds, tot) + toff;
   if (full) {
     uint4* s4 = reinterpret_cast<uint4*>(start + base);
     uint4* c4 = reinterpret_cast<uint4*>(count + base);
