@@ -273,25 +273,23 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     auto visit = [&](uint32_t q, const float4& xj) {
       const float dx = xi.x - xj.x, dy_ = xi.y - xj.y, dz_ = xi.z - xj.z;
       const float d2 = dot3(dx, dy_, dz_, dx, dy_, dz_);
-      if (d2 <= r2) {
-        if (cnt < cap) {
-          nlist[(uint64_t)cnt * n + p] = q;
-          if (c16.list16) {
-            const int d = (int)(q - p);
-            if (hok)
-              c16.list16[l16_at(cnt, p, n)] = (uint16_t)((int32_t)q + sb);
-            else if (d >= -kCompactReach && d <= kCompactReach)
-              c16.list16[l16_at(cnt, p, n)] = (uint16_t)(d + 32768);
-            else
-              wide = true;
-          }
-        }
-        cnt++;
-        if (DENS) {
-          const float w = r2 - d2;
-          S = __builtin_fmaf(w * w, w, S);
+      const bool acc = d2 <= r2;
+      if (DENS) {
+        // a rejected candidate adds +0, which leaves S (never -0) unchanged
+        const float w = acc ? r2 - d2 : 0.f;
+        S = __builtin_fmaf(w * w, w, S);
+      }
+      if (acc && cnt < cap) {
+        nlist[(uint64_t)cnt * n + p] = q;
+        if (c16.list16) {
+          const int d = (int)(q - p);
+          const bool fits = d >= -kCompactReach && d <= kCompactReach;
+          const uint16_t e = hok ? (uint16_t)((int32_t)q + sb) : (uint16_t)(d + 32768);
+          if (hok || fits) c16.list16[l16_at(cnt, p, n)] = e;
+          wide = wide || (!hok && !fits);
         }
       }
+      cnt += acc ? 1u : 0u;
     };
     // a row's candidates 8 at a time: one round of loads covers a typical
     // row (slots past the row's end load nothing and are not visited)
