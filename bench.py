@@ -75,6 +75,7 @@ def parse():
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=200)
+    ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
@@ -158,7 +159,7 @@ def learned_leg(ctx, sc, args):
     model, summary = learned.train_and_evaluate(
         ctx, lambda i: scenes.pcisph_dam_break(n_side=n_side, scene_id=100 + i),
         range(args.train_seeds), range(args.train_seeds, args.train_seeds + args.eval_seeds),
-        steps=args.train_steps)
+        steps=args.train_steps, epochs=args.train_epochs)
     train_s = time.perf_counter() - t0
     p = sc.params.copy()
     p.warm_start = WARM_MLP
