@@ -8,7 +8,7 @@
 //   k_body_react      force of the fluid on every boundary particle: the
 //                     opposite of the pressure force it exerted, at the
 //                     final predicted positions and pressures;
-//   k_body_step       one thread per sphere: fluid force summed in
+//   k_body_step       one wave per sphere: fluid force summed in
 //                     boundary-id order, gravity, DEM contacts with every
 //                     other sphere (index order) and the six box walls,
 //                     symplectic Euler;
@@ -49,25 +49,46 @@ __global__ __launch_bounds__(256) void k_body_react(uint32_t n, Consts c,
   fb[ids[p] - c.n_fluid] = make_float4(sx * s, sy * s, sz * s, 0.f);
 }
 
+__device__ __forceinline__ float lane_value(float v, uint32_t u) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)u));
+}
+
+// One wave per sphere: the lanes load the boundary forces and evaluate the
+// sphere pairs 64 at a time, and every lane then adds them up lane by lane,
+// in boundary-id and sphere-index order (the sums of one thread per sphere).
 __global__ __launch_bounds__(256) void k_body_step(SphereArgs a, Consts c) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.count) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (j >= a.count) return;   // j is uniform over the wave
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  for (uint32_t l = 0; l < a.per; ++l) {
-    const float4 f = a.fb[(size_t)j * a.per + l];
-    fx = fx + f.x; fy = fy + f.y; fz = fz + f.z;
+  for (uint32_t l0 = 0; l0 < a.per; l0 += 64u) {
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (l0 + lane < a.per) f = a.fb[(size_t)j * a.per + l0 + lane];
+    const uint32_t m = min(64u, a.per - l0);
+    for (uint32_t u = 0; u < m; ++u) {
+      fx = fx + lane_value(f.x, u);
+      fy = fy + lane_value(f.y, u);
+      fz = fz + lane_value(f.z, u);
+    }
   }
-  a.f_fluid[j] = make_float4(fx, fy, fz, 0.f);
+  if (lane == 0) a.f_fluid[j] = make_float4(fx, fy, fz, 0.f);
   fx = fx + a.mass * c.gx;
   fy = fy + a.mass * c.gy;
   fz = fz + a.mass * c.gz;
   const float4 cj = a.c_in[j], vj = a.v_in[j];
-  for (uint32_t i = 0; i < a.count; ++i) {
-    if (i == j) continue;
-    float gx, gy, gz;
-    dem_pair<false>(cj, a.c_in[i], a.v_in[i], vj, a.radius, a.radius, a.spring, a.damping, gx, gy,
-                    gz);
-    fx = fx + gx; fy = fy + gy; fz = fz + gz;
+  for (uint32_t i0 = 0; i0 < a.count; i0 += 64u) {
+    const uint32_t i = i0 + lane;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (i < a.count && i != j)
+      dem_pair<false>(cj, a.c_in[i], a.v_in[i], vj, a.radius, a.radius, a.spring, a.damping, gx,
+                      gy, gz);
+    const uint32_t m = min(64u, a.count - i0);
+    for (uint32_t u = 0; u < m; ++u) {
+      if (i0 + u == j) continue;
+      fx = fx + lane_value(gx, u);
+      fy = fy + lane_value(gy, u);
+      fz = fz + lane_value(gz, u);
+    }
   }
   // walls: signed distance d of the centre (positive inside), overlap
   // R - d along the inward normal, dashpot on the normal velocity
@@ -87,6 +108,7 @@ __global__ __launch_bounds__(256) void k_body_step(SphereArgs a, Consts c) {
   const float inv_m = 1.0f / a.mass;
   const float vx = vj.x + (fx * inv_m) * c.dt, vy = vj.y + (fy * inv_m) * c.dt,
               vzz = vj.z + (fz * inv_m) * c.dt;
+  if (lane != 0) return;
   a.v_out[j] = make_float4(vx, vy, vzz, 0.f);
   a.c_out[j] = make_float4(cj.x + vx * c.dt, cj.y + vy * c.dt, cj.z + vzz * c.dt, 0.f);
 }
@@ -116,7 +138,7 @@ psb_status launch_bodies(uint32_t n, const Consts& c, const uint32_t* cnt, const
   if (!a.count) return PSB_OK;
   hipLaunchKernelGGL(k_body_react, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, cnt, list, X,
                      ids, a.fb);
-  hipLaunchKernelGGL(k_body_step, dim3(blocks_for(a.count)), dim3(kBlock), 0, s, a, c);
+  hipLaunchKernelGGL(k_body_step, dim3(blocks_for((uint64_t)a.count * 64u)), dim3(kBlock), 0, s, a, c);
   hipLaunchKernelGGL(k_body_particles, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, c, a, ids, bx,
                      bv, bvh, bid, bp);
   PSB_HIP(hipGetLastError());
