@@ -55,7 +55,7 @@ __device__ __forceinline__ float lane_value(float v, uint32_t u) {
 
 // One wave per sphere: the lanes load the boundary forces and evaluate the
 // sphere pairs 64 at a time, and every lane then adds them up lane by lane,
-// in boundary-id and sphere-index order (the sums of one thread per sphere).
+// in boundary-id and sphere-index order (each sum in the same serial order as a single loop).
 __global__ __launch_bounds__(256) void k_body_step(SphereArgs a, Consts c) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
