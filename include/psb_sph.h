@@ -131,6 +131,14 @@ typedef struct psb_sph_params {
                                full template over-correct and diverge; 0.6
                                holds there.  The warm-start feature scale
                                keeps the unrelaxed delta.                     */
+  int32_t retry_levels;     /* PCISPH config mode, fused system step: a step
+                               whose corrector ends above the tolerance after
+                               max_iterations is redone from its start state
+                               as two steps of dt/2 (the plain corrector,
+                               p = 0 start), each of which may be redone as
+                               two of dt/4, down to dt / 2^retry_levels; the
+                               step still advances dt.  0 disables (the
+                               reference's fixed step).  At most 4.           */
 } psb_sph_params;
 
 #define PSB_WALL_CLAMP      0  /* box walls act only through the collider (clamp) */
@@ -386,6 +394,9 @@ typedef struct psb_step_stats {
   int64_t  total_iterations; /* corrector iterations over the last update call */
   float    worst_error;    /* largest final max_error over the last update call */
   int32_t  max_neighbors_seen; /* largest neighbour count of the last step   */
+  int32_t  retried_steps;  /* steps of the last update call redone as substeps
+                              (params.retry_levels)                          */
+  int32_t  substeps;       /* substeps those retries ran                     */
 } psb_step_stats;
 
 /* SPHSystem::SPHSystem + init (SphSystem.cpp:6-113): allocate every device

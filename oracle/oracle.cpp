@@ -623,6 +623,7 @@ template <class R> struct System {
   R last_err = 0;
   int64_t total_iters = 0;
   R worst_err = 0;
+  int retried_steps = 0, substeps = 0;   // retries of the last update call
   int steps_done = 0;
   uint32_t max_nbr_seen = 0;
   std::vector<int> iter_hist;
@@ -737,17 +738,88 @@ template <class R> struct System {
 
   // PCI_SPHSystem::update (PciSphSystem.cpp:163-204) -> pci_SPH_update
   // (PciSphSystem.cu:8-68).
-  void pcisph_step() {
-    build_grid();
-    neighbors_density();
+  void pcisph_step() { pcisph_advance(0, false); }
+
+  // Constants of a retry substep at level L: the step size dt / 2^L and
+  // everything derived from it (beta, the template delta, feature scales).
+  Derived<R> level_derived(int level) const {
+    psb_sph_params p = prm;
+    const float scale = std::ldexp(1.0f, -level);
+    p.dt = prm.dt * scale;
+    if (p.delta > 0.f) p.delta = prm.delta / (scale * scale);
+    return derive<R>(p);
+  }
+
+  // One step of dt / 2^level.  A config-mode step whose corrector ends above
+  // the tolerance is redone from its start state A as two steps of half the
+  // size, down to params.retry_levels, each restarting from p = 0 without
+  // warm start; the first half reuses A's grid, lists and densities.
+  void pcisph_advance(int level, bool from_a) {
+    const Derived<R> saved = d;
+    if (level > 0) d = level_derived(level);
+    const bool ref = prm.mode == PSB_MODE_REFERENCE;
+    const bool retry_ok = !ref && level < prm.retry_levels;
+    pci_prologue(from_a);
+    const bool warm = !ref && !from_a && prm.warm_start == PSB_WARM_PREVIOUS;
+    const bool mlp = !ref && !from_a && prm.warm_start == PSB_WARM_MLP;
+    const bool feat_now = !ref && !from_a && (mlp || feat_on);
+    std::vector<V4<R>>* cur = &xp0;
+    std::vector<V4<R>>* nxt = &xp1;
+    if (warm) {
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < (int64_t)n; ++k) (*cur)[k].w = pres[k];
+      pci_force(*cur, *nxt, false);
+      std::swap(cur, nxt);
+    }
+    const int min_it = ref ? 3 : std::max(1, prm.min_iterations);
+    const int max_it = ref ? 3 : std::max(min_it, prm.max_iterations);
+    int it = 0;
+    R err = 0;
+    while (true) {
+      err = pci_density(*cur, it == 0 && feat_now, it == 0 && mlp);
+      if (it == 0 && mlp) mlp_pressure(*cur);
+      it++;
+      bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it;
+      pci_force(*cur, *nxt, fin);
+      if (fin) break;
+      std::swap(cur, nxt);
+    }
+    const bool failed = !ref && err > (R)prm.tolerance;
+    last_iters = it; last_err = err;
+    total_iters += it;
+    if (failed && retry_ok) {
+      if (level == 0) retried_steps++;
+      substeps += 2;
+      d = saved;
+      pcisph_advance(level + 1, true);
+      pcisph_advance(level + 1, false);
+      return;
+    }
+    if (failed) flags |= PSB_FLAG_NOT_CONVERGED;
+    worst_err = std::max(worst_err, err);
+    iter_hist.push_back(it); err_hist.push_back(err);
+    if (!ref && n_spheres) bodies_step(*cur);
+    d = saved;
+  }
+
+  // The PCISPH step up to its corrector loop: grid build, neighbours and
+  // densities, initial pressure, non-pressure forces and the first
+  // prediction (xp0).  The corrector probe of the C API stops here.
+  // from_a: a retry from the step's start state A, whose grid, lists and
+  // densities (walls included) are kept; the pressure starts at 0.
+  void pci_prologue(bool from_a = false) {
+    if (!from_a) {
+      build_grid();
+      neighbors_density();
+    }
     const bool ref = prm.mode == PSB_MODE_REFERENCE;
     const R sigma = ref ? (R)0.0728f : d.sigma;
     const bool surf = ref ? true : (prm.surface_tension != 0.f);
     const R gx = ref ? (R)0 : d.gx, gz = ref ? (R)0 : d.gz;
     const R gy = ref ? (R)-9.81f : d.gy;
-    const bool warm = !ref && prm.warm_start == PSB_WARM_PREVIOUS;
-    const bool mlp = !ref && prm.warm_start == PSB_WARM_MLP;
-    const bool feat_now = !ref && (mlp || feat_on);
+    const bool warm = !ref && !from_a && prm.warm_start == PSB_WARM_PREVIOUS;
+    const bool mlp = !ref && !from_a && prm.warm_start == PSB_WARM_MLP;
+    const bool feat_now = !ref && !from_a && (mlp || feat_on);
     // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
@@ -759,7 +831,7 @@ template <class R> struct System {
       }
       pres[k] = warm ? Ap[k] : (R)0;
       if (feat_now) Ax[k].w = Ap[k];
-      if (d.wall) {
+      if (d.wall && !from_a) {
         R Sw, gx_, gy_, gz_;
         wall_sums(d, true, false, Ax[k].x, Ax[k].y, Ax[k].z, Sw, gx_, gy_, gz_);
         rho[k] = rho[k] + Sw * d.mc;
@@ -810,32 +882,6 @@ template <class R> struct System {
       }
       xp0[p] = xp;
     }
-    std::vector<V4<R>>* cur = &xp0;
-    std::vector<V4<R>>* nxt = &xp1;
-    if (warm) {
-#pragma omp parallel for schedule(static)
-      for (int64_t k = 0; k < (int64_t)n; ++k) (*cur)[k].w = pres[k];
-      pci_force(*cur, *nxt, false);
-      std::swap(cur, nxt);
-    }
-    const int min_it = ref ? 3 : std::max(1, prm.min_iterations);
-    const int max_it = ref ? 3 : std::max(min_it, prm.max_iterations);
-    int it = 0;
-    R err = 0;
-    while (true) {
-      err = pci_density(*cur, it == 0 && feat_now, it == 0 && mlp);
-      if (it == 0 && mlp) mlp_pressure(*cur);
-      it++;
-      bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it;
-      if (fin && !ref && err > (R)prm.tolerance) flags |= PSB_FLAG_NOT_CONVERGED;
-      pci_force(*cur, *nxt, fin);
-      if (fin) break;
-      std::swap(cur, nxt);
-    }
-    last_iters = it; last_err = err;
-    total_iters += it; worst_err = std::max(worst_err, err);
-    iter_hist.push_back(it); err_hist.push_back(err);
-    if (!ref && n_spheres) bodies_step(*cur);
   }
 
   // Rigid spheres after the corrector, X = the final iteration's predicted
@@ -1171,6 +1217,7 @@ template <class R> struct System {
 
   void update(int steps) {
     flags = 0; total_iters = 0; worst_err = 0; steps_done = 0; max_nbr_seen = 0;
+    retried_steps = 0; substeps = 0;
     for (int s = 0; s < steps; ++s) {
       if (prm.solver == PSB_SOLVER_PCISPH) pcisph_step(); else wcsph_step();
       steps_done++;
@@ -1741,15 +1788,113 @@ int oracle_system_get(void* hp, float* pos4, float* vel4, float* vh4, float* old
   return 0;
 }
 
-// stats: flags, last_iters, last_err, total_iters, worst_err, steps, max_nbr
-int oracle_system_stats(void* hp, double* out /* 7 */) {
+// stats: flags, last_iters, last_err, total_iters, worst_err, steps, max_nbr,
+// retried_steps, substeps
+int oracle_system_stats(void* hp, double* out /* 9 */) {
   Handle* h = (Handle*)hp;
   auto fill = [&](auto* s) {
     out[0] = s->flags; out[1] = s->last_iters; out[2] = (double)s->last_err;
     out[3] = (double)s->total_iters; out[4] = (double)s->worst_err; out[5] = s->steps_done;
-    out[6] = s->max_nbr_seen;
+    out[6] = s->max_nbr_seen; out[7] = s->retried_steps; out[8] = s->substeps;
   };
   if (h->prec == 64) fill(h->d); else fill(h->f);
+  return 0;
+}
+
+// Corrector probe: the PCISPH step of the particles set last, stopped after
+// its non-pressure pass (pci_prologue), then single density and force passes
+// of the corrector on caller-given predicted states.  All arrays are in the
+// sorted order of the prologue (ids gives the caller's particle index); vec4
+// arrays are n x 4 floats.  Any pointer may be null.
+int oracle_pci_prologue(void* hp) {
+  Handle* h = (Handle*)hp;
+  if (h->prec == 64) h->d->pci_prologue(); else h->f->pci_prologue();
+  return 0;
+}
+
+extern "C++" {
+template <class R> static void put4(float* dst, const std::vector<V4<R>>& v, uint32_t n) {
+  if (!dst) return;
+  for (uint32_t i = 0; i < n; ++i) {
+    dst[4 * i] = (float)v[i].x; dst[4 * i + 1] = (float)v[i].y;
+    dst[4 * i + 2] = (float)v[i].z; dst[4 * i + 3] = (float)v[i].w;
+  }
+}
+template <class R> static void get4(std::vector<V4<R>>& v, const float* src, uint32_t n) {
+  if (!src) return;
+  for (uint32_t i = 0; i < n; ++i)
+    v[i] = {(R)src[4 * i], (R)src[4 * i + 1], (R)src[4 * i + 2], (R)src[4 * i + 3]};
+}
+template <class R> static void put1(float* dst, const std::vector<R>& v, uint32_t n) {
+  if (dst) for (uint32_t i = 0; i < n; ++i) dst[i] = (float)v[i];
+}
+template <class R> static void get1(std::vector<R>& v, const float* src, uint32_t n) {
+  if (src) for (uint32_t i = 0; i < n; ++i) v[i] = (R)src[i];
+}
+}  // extern "C++"
+
+// x0 / vh0: start-of-step positions and half-step velocities; xp: first
+// prediction (w = 0); anp: config-mode non-pressure acceleration; fa / fs:
+// reference-mode forcesAccum (gravity + viscosity) and surface force;
+// delta: config-mode per-particle pressure scaling; pres, rho_pred: the
+// corrector's pressure and predicted density; rho: start-of-step density.
+int oracle_pci_get(void* hp, float* x0, float* vh0, float* xp, float* anp, float* fa, float* fs,
+                   float* delta, float* pres, float* rho_pred, float* rho, uint32_t* ids) {
+  Handle* h = (Handle*)hp;
+  auto run = [&](auto* s) {
+    const uint32_t n = s->n;
+    put4(x0, s->Ax, n); put4(vh0, s->Avh, n); put4(xp, s->xp0, n); put4(anp, s->anp, n);
+    put4(fa, s->fa, n); put4(fs, s->fs4, n);
+    put1(delta, s->dl, n); put1(pres, s->pres, n); put1(rho_pred, s->rho_pred, n);
+    put1(rho, s->rho, n);
+    if (ids) for (uint32_t i = 0; i < n; ++i) ids[i] = s->Aid[i];
+  };
+  if (h->prec == 64) run(h->d); else run(h->f);
+  return 0;
+}
+
+int oracle_pci_set(void* hp, const float* x0, const float* vh0, const float* anp, const float* fa,
+                   const float* fs, const float* delta, const float* pres, const float* rho_pred) {
+  Handle* h = (Handle*)hp;
+  auto run = [&](auto* s) {
+    const uint32_t n = s->n;
+    get4(s->Ax, x0, n); get4(s->Avh, vh0, n); get4(s->anp, anp, n); get4(s->fa, fa, n);
+    get4(s->fs4, fs, n);
+    get1(s->dl, delta, n); get1(s->pres, pres, n); get1(s->rho_pred, rho_pred, n);
+  };
+  if (h->prec == 64) run(h->d); else run(h->f);
+  return 0;
+}
+
+// One density pass (compute_Density_PCI + compute_Pressure_PCI) on the
+// predicted positions xp (xyz in; w out: the pressure term the force pass
+// gathers); returns the max compression error.
+double oracle_pci_density(void* hp, float* xp) {
+  Handle* h = (Handle*)hp;
+  auto run = [&](auto* s) {
+    using R = typename std::remove_reference<decltype(s->rho[0])>::type;
+    std::vector<V4<R>> X(s->n);
+    get4(X, xp, s->n);
+    double e = (double)s->pci_density(X);
+    put4(xp, X, s->n);
+    return e;
+  };
+  return h->prec == 64 ? run(h->d) : run(h->f);
+}
+
+// One non-final force pass (compute_Pressure_Force_PCI + restore +
+// integrateSPH_LeapFrog_Forces) on xp (w = pressure term); the next
+// prediction goes to xn (w = 0).
+int oracle_pci_force(void* hp, const float* xp, float* xn) {
+  Handle* h = (Handle*)hp;
+  auto run = [&](auto* s) {
+    using R = typename std::remove_reference<decltype(s->rho[0])>::type;
+    std::vector<V4<R>> X(s->n), Xn(s->n);
+    get4(X, xp, s->n);
+    s->pci_force(X, Xn, false);
+    put4(xn, Xn, s->n);
+  };
+  if (h->prec == 64) run(h->d); else run(h->f);
   return 0;
 }
 

@@ -64,6 +64,12 @@ def lib():
         L.oracle_system_stats.argtypes = [vp, vp]
         L.oracle_system_arrays.argtypes = [vp, vp, vp, vp, vp, vp]
         L.oracle_system_enable_features.argtypes = [vp, C.c_int]
+        L.oracle_pci_prologue.argtypes = [vp]
+        L.oracle_pci_get.argtypes = [vp] * 12
+        L.oracle_pci_set.argtypes = [vp] * 9
+        L.oracle_pci_density.restype = C.c_double
+        L.oracle_pci_density.argtypes = [vp, vp]
+        L.oracle_pci_force.argtypes = [vp, vp, vp]
         L.oracle_system_features.argtypes = [vp, vp]
         L.oracle_system_set_mlp.argtypes = [vp, vp, vp, vp]
         L.oracle_system_pressure_scale.restype = C.c_double
@@ -261,10 +267,10 @@ class OracleSystem:
         return o
 
     def stats(self) -> dict:
-        out = np.zeros(7, np.float64)
+        out = np.zeros(9, np.float64)
         lib().oracle_system_stats(self.h, out.ctypes.data)
         keys = ("flags", "iterations", "max_error", "total_iterations", "worst_error", "steps",
-                "max_neighbors_seen")
+                "max_neighbors_seen", "retried_steps", "substeps")
         return {k: (int(v) if k not in ("max_error", "worst_error") else float(v))
                 for k, v in zip(keys, out)}
 
@@ -304,6 +310,56 @@ class OracleSystem:
                                             o["velocities"].ctypes.data,
                                             o["fluid_force"].ctypes.data)
         return o
+
+    # -- corrector probe: the PCISPH step split at its passes ----------------
+    _PCI_VEC = ("x0", "vh0", "xp", "anp", "fa", "fs")
+    _PCI_SCAL = ("delta", "pres", "rho_pred", "rho")
+
+    def pci_prologue(self) -> dict:
+        """Run the step of the particles set last up to its corrector loop
+        (grid, neighbours, densities, non-pressure pass, first prediction)
+        and return its arrays in sorted order: x0, vh0 (start state), xp
+        (first prediction), anp (config non-pressure acceleration), fa, fs
+        (reference forcesAccum and surface force), delta, pres, rho_pred,
+        rho, ids."""
+        lib().oracle_pci_prologue(self.h)
+        return self.pci_get()
+
+    def pci_get(self) -> dict:
+        n = self.n
+        o = {k: np.zeros((n, 4), np.float32) for k in self._PCI_VEC}
+        o.update({k: np.zeros(n, np.float32) for k in self._PCI_SCAL})
+        o["ids"] = np.zeros(n, np.uint32)
+        lib().oracle_pci_get(self.h, *(o[k].ctypes.data for k in self._PCI_VEC),
+                             *(o[k].ctypes.data for k in self._PCI_SCAL[:3]),
+                             o["rho"].ctypes.data, o["ids"].ctypes.data)
+        return o
+
+    def pci_set(self, **arrays):
+        """Overwrite prologue arrays (sorted order): x0, vh0, anp, fa, fs,
+        delta, pres, rho_pred."""
+        keys = ("x0", "vh0", "anp", "fa", "fs", "delta", "pres", "rho_pred")
+        bad = set(arrays) - set(keys)
+        if bad:
+            raise KeyError(sorted(bad))
+        arrs = [None if arrays.get(k) is None else _f32(arrays[k]) for k in keys]
+        lib().oracle_pci_set(self.h, *(_p(a) for a in arrs))
+
+    def pci_density(self, xp):
+        """One density pass on predicted positions xp (n, 4); returns
+        (xp with w = the pressure term, max compression error); the pressure
+        is read with pci_get()["pres"]."""
+        x = np.array(xp, np.float32, copy=True)
+        e = lib().oracle_pci_density(self.h, x.ctypes.data)
+        return x, float(e)
+
+    def pci_force(self, xp):
+        """One non-final force pass on xp (w = pressure term): the next
+        prediction (n, 4)."""
+        x = _f32(xp)
+        out = np.zeros_like(x)
+        lib().oracle_pci_force(self.h, x.ctypes.data, out.ctypes.data)
+        return out
 
     def enable_features(self, on: bool = True):
         lib().oracle_system_enable_features(self.h, 1 if on else 0)

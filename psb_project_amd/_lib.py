@@ -82,6 +82,7 @@ class Params(C.Structure):
         ("delta_local_cap", C.c_int32),
         ("wall_model", C.c_int32),
         ("relaxation", C.c_float),
+        ("retry_levels", C.c_int32),
     ]
 
     def copy(self) -> "Params":
@@ -124,6 +125,8 @@ class StepStats(C.Structure):
         ("total_iterations", C.c_int64),
         ("worst_error", C.c_float),
         ("max_neighbors_seen", C.c_int32),
+        ("retried_steps", C.c_int32),
+        ("substeps", C.c_int32),
     ]
 
 

@@ -19,14 +19,17 @@ struct PciStatus {
 
 // Per-step reset of one scene's status and error slots, by one workgroup.
 __device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, uint32_t* done,
-                                                ErrSlots es) {
+                                                ErrSlots es, bool keep_wide = false) {
   PciStatus* st = st0 + scene;
   for (int k = threadIdx.x; k < kMaxIterSlots; k += blockDim.x) st->err[k] = 0u;
   if (es.base)
     for (int k = threadIdx.x; k < kErrSlots * kMaxIterSlots; k += blockDim.x)
       es.base[(size_t)(k / kMaxIterSlots) * es.stride + (size_t)scene * kMaxIterSlots +
               (k % kMaxIterSlots)] = 0u;
-  if (threadIdx.x == 0) { st->final_iter = -1; st->wide = 0u; }
+  if (threadIdx.x == 0) {
+    st->final_iter = -1;
+    if (!keep_wide) st->wide = 0u;
+  }
   if (done && scene == 0 && threadIdx.x == 0) *done = 0u;
 }
 
@@ -55,6 +58,11 @@ struct PciBuffers {
   const PciStatus* st;
 };
 
+// Added to the iteration a force pass stores into PciOut.host_final when the
+// step ends above the tolerance (config mode), so the host learns both when
+// and whether the corrector finished.
+constexpr int32_t kFinalFailed = 1 << 16;
+
 struct PciOut {
   // fused step: end-of-step state
   float4* bx; float4* bv; float4* bvh; uint32_t* bid; float* bp;
@@ -78,6 +86,9 @@ struct Shape {
 // one status per scene (scenes = 1 for a single system); done: batch counter
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes = 1,
                             uint32_t* done = nullptr, ErrSlots es = ErrSlots{});
+// reset of a step redone from its start state: the neighbour lists (and the
+// compact list's overflow flag) of that state stay valid
+psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es);
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,

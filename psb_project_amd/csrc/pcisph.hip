@@ -33,6 +33,10 @@ __global__ void k_pci_reset(PciStatus* st0, uint32_t* done, ErrSlots es) {
   pci_reset_scene(st0, blockIdx.x, done, es);
 }
 
+__global__ void k_pci_retry_reset(PciStatus* st0, ErrSlots es) {
+  pci_reset_scene(st0, blockIdx.x, nullptr, es, true);
+}
+
 __global__ void k_pci_stats_reset(PciStatus* st0) {
   PciStatus* st = st0 + blockIdx.x;
   if (threadIdx.x == 0) {
@@ -380,16 +384,21 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       st->last_iters = it;
       st->last_err = __float_as_uint(err);
       atomicAdd(&st->total_iters, (unsigned long long)it);
-      atomicMax(&st->worst_err, __float_as_uint(err));
+      // an attempt that will be redone as substeps leaves the worst error
+      // and the flag to the attempts that are kept
+      const bool retried = !REF && err > c.tolerance && c.retry_ok;
+      if (!retried) atomicMax(&st->worst_err, __float_as_uint(err));
       // the neighbour search's largest count of this step -> flags[1]
       if (nbmax) atomicMax(o.flags + 1, nbmax);
-      if (!REF && err > c.tolerance) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
+      if (!REF && err > c.tolerance && !retried) atomicOr(o.flags, PSB_FLAG_NOT_CONVERGED);
       // tell the host, which stops queueing speculative iterations once
       // every scene has finished
       bool all = true;
       if (o.done) all = atomicAdd(o.done, 1u) + 1u == o.num_scenes;
-      if (all && o.host_final)
-        __hip_atomic_store(o.host_final, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (all && o.host_final) {
+        const int32_t hv = (!REF && err > c.tolerance) ? (k | kFinalFailed) : k;
+        __hip_atomic_store(o.host_final, hv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
   __shared__ float4 hx[M::kHalo ? kHaloCap : 1];
@@ -490,6 +499,12 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
 }
 
 // ---- launchers ----------------------------------------------------------------
+psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es) {
+  hipLaunchKernelGGL(k_pci_retry_reset, dim3(1), dim3(256), 0, s, st, es);
+  PSB_HIP(hipGetLastError());
+  return PSB_OK;
+}
+
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes,
                             uint32_t* done, ErrSlots es) {
   if (stats) hipLaunchKernelGGL(k_pci_stats_reset, dim3(scenes), dim3(64), 0, s, st);

@@ -47,6 +47,7 @@ psb_status hip_fail(hipError_t e, const char* what, const char* file, int line);
 constexpr uint32_t kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = 4096;      // cells per scan workgroup (256 x 16)
 constexpr int kMaxIterSlots = 256;        // PCISPH iteration slots per step
+constexpr int kMaxRetryLevels = 4;        // dt halvings of a failed PCISPH step
 
 inline uint32_t blocks_for(uint64_t n, uint32_t b = kBlock) {
   return (uint32_t)((n + b - 1) / b);
@@ -78,6 +79,10 @@ struct Consts {
   float tolerance;
   int eos, mode, box_enabled, min_it, max_it;
   int delta_local_cap;
+  // 1: a step ending above the tolerance will be redone as substeps (the
+  // fused system step with params.retry_levels), so the force pass leaves
+  // NOT_CONVERGED and the worst error to the attempt that is kept
+  int retry_ok;
   // warm-start features: pressure scale and feature normalisers
   float pscale, inv_ps, h_over_ps, dt_over_h, e_scale;
   // lattice walls (PSB_WALL_LATTICE): implicit boundary particles at

@@ -87,6 +87,7 @@ static psb_status check_params(const psb_sph_params& p) {
               p.max_iterations <= kMaxIterSlots - 1,
           "PCISPH iterations: 1 <= min <= max <= 255");
   PSB_ARG(p.wall_model == PSB_WALL_CLAMP || p.wall_model == PSB_WALL_LATTICE, "params.wall_model");
+  PSB_ARG(p.retry_levels >= 0 && p.retry_levels <= kMaxRetryLevels, "params.retry_levels in [0, 4]");
   if (p.wall_model == PSB_WALL_LATTICE && p.box_enabled) {
     const double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * p.support_radius;
     PSB_ARG(p.lattice_spacing >= 0.f && p.support_radius <= 4.0 * s,
@@ -181,6 +182,7 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   c->min_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.min_iterations;
   c->max_it = p.mode == PSB_MODE_REFERENCE ? 3 : p.max_iterations;
   c->delta_local_cap = p.mode == PSB_MODE_CONFIG && p.delta_local_cap ? 1 : 0;
+  c->retry_ok = 0;   // set by the fused system step, which does the retries
   {
     double s = p.lattice_spacing > 0.f ? (double)p.lattice_spacing : 0.5 * h;
     const double ps = p.delta > 0.f ? (double)p.delta * rho0 * 0.01

@@ -246,16 +246,47 @@ static Lanes corrector_lanes(psb_system s) {
   return Lanes{l, l};
 }
 
-static psb_status pcisph_step(psb_system s, int step, int slot) {
+// Constants of a retry substep at level L (dt / 2^L): everything make_consts
+// derives from dt (beta, the template delta, the warm-start feature scales)
+// recomputed, the system's patched fields (bodies, colliders) kept.
+static psb_status level_consts(psb_system s, int level, Consts* out) {
+  if (level == 0) { *out = s->c; return PSB_OK; }
+  psb_sph_params p = s->prm;
+  const float scale = std::ldexp(1.0f, -level);
+  p.dt = s->prm.dt * scale;
+  if (p.delta > 0.f) p.delta = s->prm.delta / (scale * scale);
+  PSB_TRY(make_consts(p, out, nullptr));
+  out->n_fluid = s->c.n_fluid;
+  out->coll = s->c.coll;
+  out->ncoll = s->c.ncoll;
+  out->retry_ok = level < s->prm.retry_levels ? 1 : 0;
+  return PSB_OK;
+}
+
+// One attempt at a PCISPH step with constants c.  from_a: restart from the
+// start state A of the current step (a retry: the grid, the lists and the
+// densities of A are still valid, the pressure starts at 0 and no warm start
+// or recorder runs); otherwise the step begins with the grid build of B.
+// Returns the final iteration and whether it ended above the tolerance.
+static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool from_a,
+                                 int* final_out, bool* failed_out) {
   hipStream_t st = s->ctx->stream;
   const Lanes lanes = corrector_lanes(s);
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
-  const bool warm = !ref && s->prm.warm_start == PSB_WARM_PREVIOUS;
-  const bool mlp = !ref && s->prm.warm_start == PSB_WARM_MLP;
-  const bool feat = !ref && (mlp || s->rec != nullptr);
-  // the status reset rides on the grid build's first kernel
+  const bool warm = !ref && !from_a && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = !ref && !from_a && s->prm.warm_start == PSB_WARM_MLP;
+  const bool feat = !ref && !from_a && (mlp || s->rec != nullptr);
+  const int att = (int)s->step_final.size();   // attempt index of this update call
+  const int slot = att & 63;
+  s->step_final.push_back(-1);
   const StepReset reset{s->st, nullptr, s->es, 1u};
-  PSB_TRY(grid_phase(s, step, &reset));
+  if (!from_a) {
+    // the status reset rides on the grid build's first kernel
+    PSB_TRY(grid_phase(s, att, &reset));
+  } else {
+    PSB_TRY(launch_pci_retry_reset(s->st, st, s->es));
+    PSB_HIP(hipMemsetAsync(s->pres, 0, 4 * (size_t)s->n, st));
+  }
   psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
   Compact16 c16;
   if (s->compact && s->nlist16) {
@@ -267,13 +298,13 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   const HaloTile* tiles =
       (s->halo && l16 && lanes.d == 1 && lanes.f == 1 && s->tiles && s->halo_src) ? s->tiles
                                                                                      : nullptr;
-  if (tiles) PSB_TRY(launch_halo(s->grid, s->ax, s->n, s->gd.R, s->tiles, s->halo_src, st));
-  {
+  if (!from_a) {
+    if (tiles) PSB_TRY(launch_halo(s->grid, s->ax, s->n, s->gd.R, s->tiles, s->halo_src, st));
     // neighbour search + density + epilogue in one pass
-    Timed t(s, PH_NEIGHBORS, step);
-    const EpiArgs ep = make_epilogue(s->c, 1, warm ? 1 : 0, feat ? 1 : 0, s->pres, s->ap, s->ax,
+    Timed t(s, PH_NEIGHBORS, att);
+    const EpiArgs ep = make_epilogue(c, 1, warm ? 1 : 0, feat ? 1 : 0, s->pres, s->ap, s->ax,
                                      s->av, s->aid);
-    PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
+    PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, c.h2, s->gd.R, nb, s->rho, c.mc,
                              s->flags, s->flags + 1, st, c16, s->es, tiles, &ep));
   }
   PciBuffers b{};
@@ -290,39 +321,39 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
   o.es = s->es;
   const bool surf = ref || s->prm.surface_tension != 0.f;
   {
-    Timed t(s, PH_NONPRESSURE, step);
-    PSB_TRY(launch_pci_nonpressure(s->n, s->c, surf, warm, false, s->ncount, s->nlist, s->ax, s->av,
+    Timed t(s, PH_NONPRESSURE, att);
+    PSB_TRY(launch_pci_nonpressure(s->n, c, surf, warm, false, s->ncount, s->nlist, s->ax, s->av,
                                    s->rho, s->avh, s->pres, b, o, st));
   }
   float4* cur = s->xp0;
   float4* nxt = s->xp1;
   if (warm) {
-    Timed t(s, PH_PCI_FORCE, step);
-    PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
+    Timed t(s, PH_PCI_FORCE, att);
+    PSB_TRY(launch_pci_force(s->n, c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred, s->pres,
                              b, s->st, -1, 1, o, st, lanes.f, l16, tiles, s->halo_src));
     std::swap(cur, nxt);
   }
-  const int min_it = s->c.min_it, max_it = s->c.max_it;
+  const int min_it = c.min_it, max_it = c.max_it;
   int final_known = -1;
   if (!ref) s->h_final[slot] = -1;
   for (int k = 0; k < max_it; ++k) {
     {
-      Timed t(s, PH_PCI_DENSITY, step, k);
+      Timed t(s, PH_PCI_DENSITY, att, k);
       const int dflags = (k == 0 && feat) ? (kDensWriteFeature | (mlp ? kDensSkipUpdate : 0)) : 0;
-      PSB_TRY(launch_pci_density(s->n, s->c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
+      PSB_TRY(launch_pci_density(s->n, c, s->ncount, s->nlist, cur, s->pres, s->rho_pred, s->dl,
                                  s->st, k, s->feat, dflags, st, s->aid, lanes.d, l16, s->es, tiles,
                                  s->halo_src));
     }
     if (k == 0 && mlp) {
       // the network sets the starting pressure (and the w lane of the
       // predicted positions the force pass gathers)
-      Timed t(s, PH_MLP, step);
-      PSB_TRY(mlp_infer(s->mlp, s->feat, s->n, s->pres, cur, 1, s->c.pscale, st,
-                        s->n_spheres ? s->aid : nullptr, s->c.n_fluid));
+      Timed t(s, PH_MLP, att);
+      PSB_TRY(mlp_infer(s->mlp, s->feat, s->n, s->pres, cur, 1, c.pscale, st,
+                        s->n_spheres ? s->aid : nullptr, c.n_fluid));
     }
     {
-      Timed t(s, PH_PCI_FORCE, step, k);
-      PSB_TRY(launch_pci_force(s->n, s->c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
+      Timed t(s, PH_PCI_FORCE, att, k);
+      PSB_TRY(launch_pci_force(s->n, c, true, s->ncount, s->nlist, cur, nxt, s->rho_pred,
                                s->pres, b, s->st, k, 0, o, st, lanes.f, l16, tiles, s->halo_src));
     }
     std::swap(cur, nxt);
@@ -342,8 +373,23 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     final_known = __atomic_load_n(&s->h_final[slot], __ATOMIC_ACQUIRE);
     if (final_known < 0) final_known = max_it - 1;
   }
-  if ((int)s->step_final.size() <= step) s->step_final.resize(step + 1, -1);
-  s->step_final[step] = final_known;
+  *failed_out = (final_known & kFinalFailed) != 0;
+  final_known &= kFinalFailed - 1;
+  s->step_final[att] = final_known;
+  *final_out = final_known;
+  (void)step;
+  return PSB_OK;
+}
+
+// After an accepted attempt: the rigid spheres (config 3) and the
+// training-set recorder.
+static psb_status pcisph_finish(psb_system s, const Consts& c, int final_known, bool from_a) {
+  hipStream_t st = s->ctx->stream;
+  const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
+  const bool warm = !ref && !from_a && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = !ref && !from_a && s->prm.warm_start == PSB_WARM_MLP;
+  const bool feat = !ref && !from_a && (mlp || s->rec != nullptr);
+  const int att = (int)s->step_final.size() - 1;
   if (s->n_spheres) {
     // the pressure force of the final iteration used the positions its
     // density pass read
@@ -359,7 +405,7 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     a.damping = s->sphere.damping;
     a.wall_damping = s->sphere.damping;
     for (int ax = 0; ax < 3; ++ax) { a.box_min[ax] = s->prm.box_min[ax]; a.box_max[ax] = s->prm.box_max[ax]; }
-    PSB_TRY(launch_bodies(s->n, s->c, s->ncount, s->nlist, xfin, s->aid, a, s->bx, s->bv, s->bvh,
+    PSB_TRY(launch_bodies(s->n, c, s->ncount, s->nlist, xfin, s->aid, a, s->bx, s->bv, s->bvh,
                           s->bid, s->bp, st));
     s->sphere_cur ^= 1;
   }
@@ -371,14 +417,36 @@ static psb_status pcisph_step(psb_system s, int step, int slot) {
     uint64_t cnt = offset < s->n ? (uint64_t)(s->n - offset + stride - 1) / stride : 0;
     if (d->count + cnt > d->capacity) cnt = d->capacity - d->count;
     if (cnt) {
-      Timed t(s, PH_RECORD, step);
-      PSB_TRY(launch_record(s->feat, s->pres, s->n, stride, offset, s->c.inv_ps, d->feat,
+      Timed t(s, PH_RECORD, att);
+      PSB_TRY(launch_record(s->feat, s->pres, s->n, stride, offset, c.inv_ps, d->feat,
                             d->target, d->capacity, d->count, (uint32_t)cnt, st));
       d->count += cnt;
     }
   }
   return PSB_OK;
 }
+
+// Advance one step of dt / 2^level.  A step whose corrector ends above the
+// tolerance is redone from its start state A as two steps of half the size
+// (params.retry_levels deep); the failed attempt's end state in B is
+// overwritten by the first half, which reuses A's grid and lists.
+static psb_status pcisph_advance(psb_system s, int step, int level, bool from_a) {
+  Consts c;
+  PSB_TRY(level_consts(s, level, &c));
+  int fin = 0;
+  bool failed = false;
+  PSB_TRY(pcisph_attempt(s, step, c, from_a, &fin, &failed));
+  if (failed && c.retry_ok) {
+    if (level == 0) s->stats.retried_steps += 1;
+    s->stats.substeps += 2;
+    PSB_TRY(pcisph_advance(s, step, level + 1, true));
+    PSB_TRY(pcisph_advance(s, step, level + 1, false));
+    return PSB_OK;
+  }
+  return pcisph_finish(s, c, fin, from_a);
+}
+
+static psb_status pcisph_step(psb_system s, int step) { return pcisph_advance(s, step, 0, false); }
 
 static psb_status ensure_features(psb_system s) {
   if (s->feat) return PSB_OK;
@@ -399,6 +467,8 @@ psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm, uint32_
   s->prm = *prm;
   s->cap = capacity;
   psb_status stt = make_consts(*prm, &s->c, &s->gd);
+  s->c.retry_ok = (prm->mode == PSB_MODE_CONFIG && prm->solver == PSB_SOLVER_PCISPH &&
+                   prm->retry_levels > 0) ? 1 : 0;
   if (stt == PSB_OK) {
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
@@ -597,9 +667,11 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   PSB_HIP(hipMemsetAsync(s->flags, 0, 8, st));
   PSB_HIP(hipMemsetAsync(s->grid->flags, 0, 4, st));
   PSB_TRY(launch_pci_reset(s->st, true, st, 1, nullptr, s->es));
-  s->step_final.assign(steps, -1);
+  s->step_final.clear();
+  s->stats.retried_steps = 0;
+  s->stats.substeps = 0;
   for (int k = 0; k < steps && s->n > 0; ++k) {
-    if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k, k & 63));
+    if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k));
     else PSB_TRY(wcsph_step(s, k));
     s->step_counter += 1;
     if (s->recs.size() > 4096) PSB_TRY(collect_timing(s));

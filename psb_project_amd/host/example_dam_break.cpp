@@ -8,6 +8,9 @@
 //   example_dam_break --print-xml FILE | --print-txt FILE   (no GPU needed)
 //       particle files read by ParticleLoader_XML::load / ParticleLoader_Txt,
 //       one JSON line per particle
+//   example_dam_break --check-materials A.xml B.xml         (no GPU needed)
+//       two EmitterBox blocks, one per material, through the one-material
+//       check of SPHSystem::init: prints "uniform" or the refusal (exit 3)
 //   example_dam_break --steps K --side N --out FILE [--config]
 //       runs K steps of an N^3 block (spacing 5 mm) and writes the binary
 //       record: params (psb_sph_params), n, initial float4 positions, final
@@ -47,7 +50,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
-  std::string print_xml, print_txt, surface_out;
+  std::string print_xml, print_txt, surface_out, check_a, check_b;
   bool print = false, config = false, colliders = false, spheres = false, batch = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
@@ -71,7 +74,28 @@ int main(int argc, char** argv) {
     else if (a == "--print-xml") print_xml = next();
     else if (a == "--print-txt") print_txt = next();
     else if (a == "--surface") surface_out = next();
+    else if (a == "--check-materials") { check_a = next(); check_b = next(); }
     else { std::fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
+  }
+  if (!check_a.empty()) {
+    SimulationData_PCI_SPHSystem ma, mb;
+    if (!ma.loadConfiguration(check_a.c_str()) || !mb.loadConfiguration(check_b.c_str())) {
+      std::fprintf(stderr, "cannot read the materials\n");
+      return 1;
+    }
+    std::vector<SPHParticle*> ps = EmitterBox({0.1f, 0.1f, 0.1f}, 0.05f, 0.05f, 0.05f, &ma).emitParticles();
+    std::vector<SPHParticle*> pb = EmitterBox({0.3f, 0.1f, 0.1f}, 0.05f, 0.05f, 0.05f, &mb).emitParticles();
+    ps.insert(ps.end(), pb.begin(), pb.end());
+    int rc = 0;
+    try {
+      checkUniformMaterial(ps, "SPHSystem::init");
+      std::printf("uniform %zu\n", ps.size());
+    } catch (const std::invalid_argument& e) {
+      std::printf("refused: %s\n", e.what());
+      rc = 3;
+    }
+    for (auto* q : ps) delete q;
+    return rc;
   }
   SimulationData_PCI_SPHSystem water;
   if (!material.empty() && !water.loadConfiguration(material.c_str())) {
@@ -137,6 +161,7 @@ int main(int argc, char** argv) {
       p.min_iterations = 3;
       p.max_iterations = 50;
       p.delta_local_cap = 1;
+      p.retry_levels = 2;
       p.max_neighbors = 96;
     }
     p.lattice_spacing = s;

@@ -344,6 +344,34 @@ class EmitterBox {
   Vector3 worldPosition;
 };
 
+// The reference packs every particle's own material into per-particle
+// arrays (SphSystem.cpp:157-167, Emitter.cpp:141-175) and mixes them per pair
+// (h = max(h_i, h_j), (mu_i + mu_j) / 2: PciSphSystem_Kernel.cu:51,79,101,138,144).
+// The C-ABI carries one material per system, so a particle set that mixes
+// materials is refused (std::invalid_argument) instead of running with
+// particle 0's material.  Particles sharing one SimulationData object, or
+// objects with equal values, form one material.
+inline void checkUniformMaterial(const std::vector<SPHParticle*>& ps, const char* who) {
+  if (ps.empty()) return;
+  const SimulationData_SPHSystem* d = ps[0]->getData();
+  for (size_t k = 1; k < ps.size(); ++k) {
+    const SimulationData_SPHSystem* e = ps[k]->getData();
+    if (e == d) continue;
+    if (e->getParticleMass() != d->getParticleMass() ||
+        e->getSupportRadius() != d->getSupportRadius() ||
+        e->getRestDensity() != d->getRestDensity() ||
+        e->getGasStiffness() != d->getGasStiffness() ||
+        e->getViscosity() != d->getViscosity() ||
+        e->getSurfaceTension() != d->getSurfaceTension() ||
+        e->getThreshold() != d->getThreshold() ||
+        e->getParticleRadius() != d->getParticleRadius())
+      throw std::invalid_argument(
+          std::string(who) + ": particle " + std::to_string(k) +
+          " has a different material than particle 0; one material per system is supported "
+          "(split the scene into one system per material)");
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Systems
 // ---------------------------------------------------------------------------
@@ -396,6 +424,7 @@ class SPHSystem : public System {
     particles = std::move(ps);
     if (particles.empty()) throw std::runtime_error("SPHSystem::init without particles");
     const SimulationData_SPHSystem* d = particles[0]->getData();
+    checkUniformMaterial(particles, "SPHSystem::init");
     prm.solver = solver();
     prm.mass = d->getParticleMass();
     prm.support_radius = d->getSupportRadius();

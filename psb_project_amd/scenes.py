@@ -135,12 +135,17 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
                      scene_id: int = 0, box=(1.28, 0.64, 0.64), tolerance: float = 0.01,
                      min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
                      calibrate_mass: bool = True, lattice_walls: bool = False,
-                     relaxation: float = 1.0) -> Scene:
+                     relaxation: float = 1.0, retry_levels: int | None = None) -> Scene:
     """Plain PCISPH dam break (64^3 block at the x-min end of the box).
 
     lattice_walls: the box walls also act as implicit boundary particles on
     the fluid lattice continued past them (config mode); otherwise they act
     only through the box collider.
+    retry_levels: a step whose corrector ends above the tolerance is redone
+    as two steps of dt/2, recursively this many levels deep (default 2 in
+    config mode, 0 in reference mode).  At dt = 5e-4 the corrector diverges
+    at the wall impacts of the 262,144-particle dam break (errors up to
+    115 % after 50 iterations); the same states converge at dt/2.
     """
     p = default_params()
     p.mode = mode
@@ -171,6 +176,7 @@ def pcisph_dam_break(n_side=64, s: float = 0.005, mode: int = MODE_CONFIG,
     p.delta_local_cap = 1
     p.wall_model = WALL_LATTICE if lattice_walls else WALL_CLAMP
     p.relaxation = relaxation
+    p.retry_levels = (2 if mode == MODE_CONFIG else 0) if retry_levels is None else retry_levels
     pos = lattice(n_side, s, seed=2000 + scene_id)
     return Scene("pcisph_dam_break", p, pos, meta={"s": s, "n_side": n_side})
 

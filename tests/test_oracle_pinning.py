@@ -288,7 +288,7 @@ def test_pcisph_tolerance_contract(oracle):
 
 
 def test_pcisph_zero_tolerance_hits_cap(oracle):
-    sc = scenes.pcisph_dam_break(n_side=10, tolerance=0.0, max_it=7)
+    sc = scenes.pcisph_dam_break(n_side=10, tolerance=0.0, max_it=7, retry_levels=0)
     osys = oracle.OracleSystem(sc.params, sc.n)
     osys.set_particles(sc.pos4)
     osys.update(2)
@@ -301,7 +301,7 @@ def test_pcisph_corrector_reduces_error(oracle):
     """More corrector iterations leave a smaller compression error."""
     errs = []
     for it in (1, 3, 8):
-        sc = scenes.pcisph_dam_break(n_side=12, tolerance=0.0, min_it=it, max_it=it,
+        sc = scenes.pcisph_dam_break(n_side=12, tolerance=0.0, min_it=it, max_it=it, retry_levels=0,
                                      calibrate_mass=False)   # m = rho0 s^3: 1 % over-dense
         osys = oracle.OracleSystem(sc.params, sc.n)
         osys.set_particles(sc.pos4)
