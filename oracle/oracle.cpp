@@ -772,7 +772,12 @@ template <class R> struct System {
       std::swap(cur, nxt);
     }
     const int min_it = ref ? 3 : std::max(1, prm.min_iterations);
-    const int max_it = ref ? 3 : std::max(min_it, prm.max_iterations);
+    int max_it = ref ? 3 : std::max(min_it, prm.max_iterations);
+    // the deepest retry level may iterate longer: halving dt does not help a
+    // particle whose density barely responds to pressure (a near-coincident
+    // pair), which only more iterations separate
+    if (!ref && level > 0 && level == prm.retry_levels)
+      max_it = std::min(4 * prm.max_iterations, 255);
     int it = 0;
     R err = 0;
     while (true) {

@@ -48,6 +48,12 @@ constexpr uint32_t kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr uint32_t kScanTile = 4096;      // cells per scan workgroup (256 x 16)
 constexpr int kMaxIterSlots = 256;        // PCISPH iteration slots per step
 constexpr int kMaxRetryLevels = 4;        // dt halvings of a failed PCISPH step
+constexpr int kRetryIterFactor = 4;       // iteration cap of the deepest retry level
+// corrector iteration cap of the deepest retry level (params.retry_levels)
+inline int retry_max_iterations(const psb_sph_params& p) {
+  const int m = p.max_iterations * kRetryIterFactor;
+  return m < kMaxIterSlots - 1 ? m : kMaxIterSlots - 1;
+}
 
 inline uint32_t blocks_for(uint64_t n, uint32_t b = kBlock) {
   return (uint32_t)((n + b - 1) / b);

@@ -248,7 +248,11 @@ static Lanes corrector_lanes(psb_system s) {
 
 // Constants of a retry substep at level L (dt / 2^L): everything make_consts
 // derives from dt (beta, the template delta, the warm-start feature scales)
-// recomputed, the system's patched fields (bodies, colliders) kept.
+// recomputed, the system's patched fields (bodies, colliders) kept.  The
+// deepest level may iterate up to kRetryIterFactor x max_iterations: halving
+// dt does not help a particle whose density barely responds to pressure (a
+// near-coincident pair: the poly6 density is flat in d at d -> 0, while the
+// spiky pressure force is not), which only more iterations separate.
 static psb_status level_consts(psb_system s, int level, Consts* out) {
   if (level == 0) { *out = s->c; return PSB_OK; }
   psb_sph_params p = s->prm;
@@ -260,6 +264,7 @@ static psb_status level_consts(psb_system s, int level, Consts* out) {
   out->coll = s->c.coll;
   out->ncoll = s->c.ncoll;
   out->retry_ok = level < s->prm.retry_levels ? 1 : 0;
+  if (level == s->prm.retry_levels) out->max_it = retry_max_iterations(s->prm);
   return PSB_OK;
 }
 
@@ -1034,7 +1039,7 @@ struct CkptHeader {
   int64_t step_counter;
 };
 constexpr char kCkptMagic[8] = {'P', 'S', 'B', 'C', 'K', 'P', 'T', '1'};
-constexpr uint32_t kCkptVersion = 1;
+constexpr uint32_t kCkptVersion = 2;   // 2: psb_sph_params.retry_levels, retry stats
 
 struct Fnv {
   uint64_t h = 1469598103934665603ull;
