@@ -8,25 +8,36 @@ search, non-pressure forces, the tolerance-driven corrector, integration,
 box collision, velocity interpolation) with all state resident in HBM.  The
 default workload is the 262,144-particle PCISPH dam break (64^3 block,
 s = 5 mm, 1.28 x 0.64 x 0.64 m box, 256 x 128 x 128 cells, 1 % tolerance,
-3..50 iterations).  W untimed steps, then K steps timed between two stream
-synchronisations; per-kernel durations come from HIP events recorded on the
-library's stream around every launch of every T-th timed step.
+3..50 iterations, failed steps redone as dt/2 substeps).  W untimed steps,
+then K steps timed between two stream synchronisations (`value`); per-kernel
+durations come from HIP events on the library's stream around every launch
+of every T-th timed step, and the roofline's from the corrector passes
+replayed back to back on the final state of the timed region.
 
-Then the batched serving workload: 64 independent scenes (log-uniform
-4,096..131,072 particles, dam breaks and drops) advanced by one launch per
-phase, block sizes autotuned, steps launched on the stream, timed with and
-without the learned start, with per-scene step latency p50/p99 (the plain
-workload also replayed from HIP graphs, reported as value_graphs).
+`full_run` times the workload's whole configured horizon (2000 steps from
+the initial state), plain and with the learned start, whatever K is, with
+the tolerance accounting of each: iterations per step, steps redone as
+substeps, the worst final error and the flags.
 
-The learned warm start is trained the way it is meant to be used
+Then the learned warm start is trained the way it is meant to be used
 (untimed): plain PCISPH records converged pressures on 16 seeded scenes of
 the same shape for 200 steps, the MLP is trained on the GPU, and 4 held-out
 seeds compare plain, previous-pressure and learned starts.  The same K steps
 of the workload are then timed again with the learned start.
 
-Prints one JSON line with the metric, the roofline of the dominant kernel,
-the CPU baseline (the oracle on this host, a bounded sample) and the learned
-warm start's iterations and throughput.
+Then the batched serving workload: 64 independent scenes (log-uniform
+4,096..131,072 particles, dam breaks and drops) advanced by one launch per
+phase, block sizes autotuned, steps launched on the stream, timed with and
+without the learned start, with per-scene step latency p50/p99 (the plain
+workload also replayed from HIP graphs, reported as value_graphs); and the
+2.1 M-particle double dam break with 512 DEM spheres, whose corrector passes
+(a working set beyond the 256 MB Infinity Cache) give a second roofline.
+
+The CPU baseline is the oracle on this host's cores over the same step window
+as `value`, from the GPU state at the start of that window (the trajectories
+are bit-identical).
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -80,6 +91,8 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the 2000-step full-horizon block")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the surface-extraction and checkpoint timings")
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM double dam break")
@@ -97,14 +110,33 @@ def read_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """CPU model and core counts of this host (lscpu)."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    return info
+
+
 def cpu_baseline(sc, start_state, first_step: int, steps: int):
-    """The FP32 oracle on this host's cores, started from the GPU state at the
-    middle of the timed region (the trajectories are bit-identical, so the
-    CPU sees the corrector iteration counts of those GPU steps)."""
+    """The FP32 oracle on this host's cores (OpenMP, threads bound close),
+    over the GPU's timed window from the GPU state at its start (the
+    trajectories are bit-identical, so the CPU runs the same iterations)."""
+    # binding must be set before the OpenMP runtime starts (first oracle load)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import pyoracle
     pyoracle.build()
-    cores = os.cpu_count() or 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, cores))))
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(cpu["physical_cores"] or cpu["logical_cpus"] or 1)))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = pyoracle.OracleSystem(sc.params, sc.n)
     o.set_particles(start_state["pos"], start_state["vel"], start_state["vh"])
@@ -118,35 +150,90 @@ def cpu_baseline(sc, start_state, first_step: int, steps: int):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
+        "cpu_model": cpu["model"],
+        "host_physical_cores": cpu["physical_cores"],
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "iters_per_step": round(st["total_iterations"] / steps, 3),
         "sample": f"{sc.name} {sc.n} particles, steps {first_step + 1}..{first_step + steps} "
-                  f"(from the GPU state at the middle of the timed region), FP32 oracle, "
-                  f"{threads} OpenMP threads, "
+                  f"(the GPU's timed window, from the GPU state at its start), FP32 oracle, "
+                  f"{threads} OpenMP threads bound close, "
                   f"{st['total_iterations'] / steps:.2f} iters/step",
     }
 
 
-def timed_run(ctx, fs, steps: int, split: bool):
-    """K steps between stream synchronisations; with `split` the state at the
-    midpoint is copied out (untimed) for the CPU baseline."""
-    half = steps // 2 if split else steps
-    total, stats, mid = 0.0, [], None
-    for k, nsteps in enumerate((half, steps - half)):
-        ctx.sync()
-        t0 = time.perf_counter()
-        if nsteps:
-            fs.update(nsteps)
-        ctx.sync()
-        total += time.perf_counter() - t0
-        if nsteps:
-            stats.append(fs.stats())
-        if k == 0 and split:
-            mid = fs.state()
-    out = {"total_iterations": sum(x["total_iterations"] for x in stats),
-           "worst_error": max((x["worst_error"] for x in stats), default=0.0),
-           "flags": 0}
-    for x in stats:
-        out["flags"] |= x["flags"]
-    return total, out, mid
+def timed_run(ctx, fs, steps: int):
+    """K steps between stream synchronisations, with the tolerance accounting."""
+    ctx.sync()
+    t0 = time.perf_counter()
+    if steps:
+        fs.update(steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    st = fs.stats() if steps else {}
+    out = {"total_iterations": st.get("total_iterations", 0),
+           "worst_error": st.get("worst_error", 0.0),
+           "flags": st.get("flags", 0),
+           "retried_steps": st.get("retried_steps", 0),
+           "substeps": st.get("substeps", 0)}
+    return elapsed, out
+
+
+def tolerance_report(sc, steps, st):
+    return {"iters_per_step": round(st["total_iterations"] / max(1, steps), 3),
+            "retried_steps": st["retried_steps"], "substeps": st["substeps"],
+            "max_density_error": st["worst_error"], "flags": st["flags"],
+            "within_tolerance": bool(st["worst_error"] <= sc.params.tolerance * (1 + 1e-6)
+                                     and (st["flags"] & 0x9) == 0)}
+
+
+def full_run(ctx, sc, model=None):
+    """The workload's configured horizon (2000 steps from the initial state)
+    as one timed block, plain or with the learned start."""
+    from psb_project_amd._lib import WARM_MLP
+    from psb_project_amd.system import FluidSystem
+    steps = int(sc.meta.get("steps", 2000))
+    p = sc.params.copy()
+    if model is not None:
+        p.warm_start = WARM_MLP
+    fs = FluidSystem(ctx, p, sc.n)
+    if model is not None:
+        fs.set_mlp(model)
+    fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
+    elapsed, st = timed_run(ctx, fs, steps)
+    fs.close()
+    out = {"steps": steps, "value": round(sc.n * steps / elapsed, 1),
+           "ms_per_step": round(1e3 * elapsed / steps, 4)}
+    out.update(tolerance_report(sc, steps, st))
+    return out
+
+
+def trace_average(kernel: str, workload: str):
+    """Full-dispatch average (us) of a kernel in the committed rocprofv3
+    summary of this workload, if there is one."""
+    f = ROOT / "profiles" / f"kernels_{workload}.txt"
+    if not f.exists():
+        return None
+    text = f.read_text()
+    i = text.find("full dispatches")
+    if i < 0:
+        return None
+    m = re.search(rf"^\S*k_{kernel}\S*.*?\s+(\d+)\s+([0-9.]+)\s+(\d+)\s+([0-9.]+)\s*$",
+                  text[i:], flags=re.M)
+    return float(m.group(2)) if m else None
+
+
+def roofline_of(kernel: str, n: int, us: float, workload: str):
+    alg = ALG_BYTES[kernel] * n
+    ach = alg / (us * 1e-6) / 1e9
+    r = {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+         "traffic": read_traffic(kernel, workload), "alg_bytes_per_launch": alg,
+         "avg_launch_us": round(us, 3)}
+    tr = trace_average(kernel, workload)
+    if tr:
+        r["trace_avg_us"] = tr
+        r["frac_trace"] = round(alg / (tr * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    return r
 
 
 def learned_leg(ctx, sc, args):
@@ -168,22 +255,22 @@ def learned_leg(ctx, sc, args):
     fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
     if args.warmup:
         fs.update(args.warmup)
-    elapsed, st, _ = timed_run(ctx, fs, args.steps, split=False)
+    elapsed, st = timed_run(ctx, fs, args.steps)
     fs.close()
     ev = summary["eval"]
-    return model, {
+    out = {
         "value": round(sc.n * args.steps / elapsed, 1),
         "ms_per_step": round(1e3 * elapsed / args.steps, 4),
-        "iters_per_step": round(st["total_iterations"] / args.steps, 3),
-        "max_density_error": st["worst_error"],
-        "flags": st["flags"],
+    }
+    out.update(tolerance_report(sc, args.steps, st))
+    return model, dict(out, **{
         "heldout": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                         for kk, vv in v.items()} for k, v in ev.items()},
         "train": {"scenes": args.train_seeds, "steps": args.train_steps,
                   "samples": summary["samples"], "epochs": summary["epochs"],
                   "mse_per_epoch": [round(x, 5) for x in summary["mse_per_epoch"]],
                   "seconds": round(train_s, 1)},
-    }
+    })
 
 
 def batched_leg(ctx, args, model):
@@ -222,6 +309,8 @@ def batched_leg(ctx, args, model):
         its = [b.stats(i)["total_iterations"] / args.batch_steps for i in range(b.num_scenes)]
         worst = max(b.stats(i)["worst_error"] for i in range(b.num_scenes))
         flags = b.stats(0)["flags"]
+        over = sum(1 for i in range(b.num_scenes)
+                   if b.stats(i)["worst_error"] > scs[i].params.tolerance * (1 + 1e-6))
         out[name] = {
             "value": round(b.n * args.batch_steps / dt, 1),
             "ms_per_step": round(1e3 * dt / args.batch_steps, 4),
@@ -232,11 +321,15 @@ def batched_leg(ctx, args, model):
             "latency_samples": lat["samples"],
             "max_density_error": worst,
             "flags": flags,
+            "scenes_over_tolerance": over,
+            "within_tolerance": over == 0 and (flags & 0x9) == 0,
             "block_sizes": tuned,
         }
         if name == "plain":
             out[name]["value_graphs"] = graph_value
         b.close()
+    if "learned" in out:
+        out["learned_vs_plain_batched"] = round(out["learned"]["value"] / out["plain"]["value"], 3)
     out["scenes"] = args.batch_scenes
     out["particles"] = sum(sc.n for sc in scenes.batched_scenes(num_scenes=args.batch_scenes))
     out["steps"] = args.batch_steps
@@ -266,16 +359,21 @@ def dem_leg(ctx, args, model):
                        m["spring"], m["damping"])
         fs.set_particles(sc.pos4)
         fs.update(args.dem_warmup)
-        elapsed, st, _ = timed_run(ctx, fs, args.dem_steps, split=False)
+        elapsed, st = timed_run(ctx, fs, args.dem_steps)
         sp = fs.spheres()
         out[name] = {
             "value": round(sc.n * args.dem_steps / elapsed, 1),
             "ms_per_step": round(1e3 * elapsed / args.dem_steps, 4),
-            "iters_per_step": round(st["total_iterations"] / args.dem_steps, 3),
-            "max_density_error": st["worst_error"],
-            "flags": st["flags"],
             "sphere_speed_max": round(float(np.linalg.norm(sp["velocities"], axis=1).max()), 4),
         }
+        out[name].update(tolerance_report(sc, args.dem_steps, st))
+        if name == "plain":
+            # corrector passes on the final state: a working set beyond the
+            # 256 MB Infinity Cache (positions 34 MB, compact list 0.4 GB)
+            t = fs.time_corrector(reps=20)
+            out["roofline"] = roofline_of("pci_force", fs.n, t["pci_force"], "double_dam_break_dem")
+            out["roofline_density"] = roofline_of("pci_density", fs.n, t["pci_density"],
+                                                  "double_dam_break_dem")
         out["boundary_particles"] = fs.n - sc.n
         out["fluid_particles"] = sc.n
         fs.close()
@@ -333,6 +431,7 @@ def main():
     from psb_project_amd.system import FluidSystem
 
     sc = scenes.pcisph_dam_break() if args.workload == "pcisph_dam_break" else scenes.wcsph_dam_break()
+    pcisph = sc.params.solver == 1
     ctx = Context(0)
     sys_ = FluidSystem(ctx, sc.params, sc.n)
     sys_.set_particles(sc.pos4, sc.vel4, sc.vh4)
@@ -340,12 +439,34 @@ def main():
     if args.warmup:
         sys_.update(args.warmup)
     ctx.sync()
+    start_state = None if args.no_cpu_baseline else sys_.state()
     sys_.reset_timing()
     sys_.set_timing(not args.no_timing, every=args.timing_every)
     print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
-    elapsed, stats, start_state = timed_run(ctx, sys_, args.steps, split=not args.no_cpu_baseline)
+    elapsed, stats = timed_run(ctx, sys_, args.steps)
     timing = sys_.timing()
     sys_.set_timing(False)
+    kernels = {k: {"avg_us": 1e3 * t / c, "launches": c, "total_ms": t}
+               for k, (t, c) in timing.items() if c}
+    roofline = None
+    if pcisph and args.steps:
+        # the dominant kernel of the step by sampled time; its duration from
+        # the corrector passes replayed back to back on the final state
+        t = sys_.time_corrector(reps=50)
+        pair = ("pci_force", "pci_density")
+        if all(k in kernels for k in pair):
+            dom = max(pair, key=lambda k: kernels[k]["total_ms"])
+        else:
+            dom = max(pair, key=lambda k: t[k])
+        roofline = roofline_of(dom, sc.n, t[dom], sc.name)
+        if dom in kernels:
+            roofline["in_step_event_us"] = round(kernels[dom]["avg_us"], 3)
+        other = "pci_density" if dom == "pci_force" else "pci_force"
+        roofline["other_kernel"] = roofline_of(other, sc.n, t[other], sc.name)
+    elif kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+        if dom in ALG_BYTES:
+            roofline = roofline_of(dom, sc.n, kernels[dom]["avg_us"], sc.name)
     extras = None
     if not args.no_extras:
         print("bench: surface extraction and checkpoint of the final state", file=sys.stderr,
@@ -354,43 +475,41 @@ def main():
     sys_.close()
 
     value = sc.n * args.steps / elapsed
-    iters = stats["total_iterations"] / max(1, args.steps)
-    kernels = {k: {"avg_us": 1e3 * t / c, "launches": c, "total_ms": t}
-               for k, (t, c) in timing.items() if c}
-    roofline = None
-    if kernels:
-        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-        per = ALG_BYTES.get(dom)
-        if per is not None:
-            alg = per * sc.n
-            ach = alg / (kernels[dom]["avg_us"] * 1e-6) / 1e9
-            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": read_traffic(dom, sc.name),
-                        "alg_bytes_per_launch": alg,
-                        "avg_launch_us": round(kernels[dom]["avg_us"], 3),
-                        "sampled_launches": kernels[dom]["launches"]}
+    full = None
+    if pcisph and not args.no_full_run:
+        print("bench: full 2000-step horizon, plain", file=sys.stderr, flush=True)
+        full = {"plain": full_run(ctx, sc)}
     learned = None
     model = None
-    if not args.no_learned and sc.params.solver == 1:
+    if not args.no_learned and pcisph:
         print("bench: learned warm start (train untimed, then time)", file=sys.stderr, flush=True)
         model, learned = learned_leg(ctx, sc, args)
+        if full is not None:
+            print("bench: full 2000-step horizon, learned", file=sys.stderr, flush=True)
+            full["learned"] = full_run(ctx, sc, model)
+            pl, le = full["plain"], full["learned"]
+            full["learned_vs_plain"] = round(le["value"] / pl["value"], 3)
+            full["iteration_reduction"] = round(1 - le["iters_per_step"] / pl["iters_per_step"], 4)
     batched = None
-    if not args.no_batch and sc.params.solver == 1:
+    if not args.no_batch and pcisph:
         print("bench: batched scenes", file=sys.stderr, flush=True)
         batched = batched_leg(ctx, args, model)
-        if model is not None and "learned" in batched:
-            batched["learned_vs_plain_single"] = round(batched["learned"]["value"] / value, 3)
+        if "learned" in batched and full is not None:
+            # side figure: batched learned against the plain single scene's
+            # full horizon (different scenes and horizons)
+            batched["learned_batched_vs_plain_single_full"] = round(
+                batched["learned"]["value"] / full["plain"]["value"], 3)
     dem = None
-    if not args.no_dem and sc.params.solver == 1:
+    if not args.no_dem and pcisph:
         print("bench: double dam break with DEM spheres", file=sys.stderr, flush=True)
         dem = dem_leg(ctx, args, model)
     if model is not None:
         model.close()
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.steps:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sc, start_state, args.warmup + args.steps // 2, args.cpu_steps)
+        cpu_steps = min(args.steps, args.cpu_steps)
+        cpu = cpu_baseline(sc, start_state, args.warmup, cpu_steps)
     line = {
         "metric": "particle-steps/s at <=1% density error; PCISPH iters/step, plain vs learned",
         "value": round(value, 1),
@@ -398,7 +517,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": round(1e3 * elapsed / max(1, args.steps), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -407,20 +526,22 @@ def main():
         "config": {"workload": sc.name, "particles": sc.n,
                    "grid": list(sc.params.grid.dims), "tolerance": sc.params.tolerance,
                    "min_iterations": sc.params.min_iterations,
-                   "max_iterations": sc.params.max_iterations, "warm_start": "none",
+                   "max_iterations": sc.params.max_iterations,
+                   "retry_levels": sc.params.retry_levels, "warm_start": "none",
                    "parallelism": "single-gpu"},
-        "iters_per_step": round(iters, 3),
-        "max_density_error": stats["worst_error"],
-        "flags": stats["flags"],
+    }
+    line.update(tolerance_report(sc, args.steps, stats))
+    line.update({
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "full_run": full,
         "learned": learned,
         "batched": batched,
         "dem": dem,
         "state_files": extras,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},
-    }
+    })
     print(json.dumps(line), flush=True)
     ctx.close()
 

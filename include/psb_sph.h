@@ -570,6 +570,17 @@ psb_status psb_system_set_timing(psb_system sys, int32_t every);
 int32_t    psb_system_get_timing(psb_system sys, int32_t cap, const char** names,
                                  double* total_ms, int64_t* launches);
 psb_status psb_system_reset_timing(psb_system sys);
+/* Kernel durations of the corrector passes on the system's current state
+ * (no counterpart in the reference; a measurement helper for the
+ * benchmark): the density pass and the force pass of iteration 0 are each
+ * launched `reps` times back to back on the system's stream between two HIP
+ * events, so the per-launch figure carries one event pair per `reps`
+ * launches instead of one per launch.  Needs a PCISPH config-mode system
+ * that has stepped; the replays rewrite only scratch (predicted positions,
+ * corrector pressure), so the next update continues from the same state.
+ * Outputs are microseconds per launch.                                     */
+psb_status psb_system_time_corrector(psb_system sys, int32_t reps, double* us_density,
+                                     double* us_force);
 
 /* ------------------------------------------------------------------------ */
 /* Learned PCISPH warm start.  No counterpart in the reference, whose        */

@@ -319,6 +319,7 @@ SIGNATURES = {
     "psb_checkpoint_info": (i32, [C.c_char_p, vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
     "psb_system_get_timing": (i32, [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_int64)]),
     "psb_system_reset_timing": (i32, [vp]),
+    "psb_system_time_corrector": (i32, [vp, i32, C.POINTER(f64), C.POINTER(f64)]),
     "psb_mlp_create": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
     "psb_mlp_destroy": (i32, [vp]),
     "psb_mlp_num_params": (i32, []),

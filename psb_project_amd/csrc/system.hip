@@ -810,6 +810,61 @@ psb_status psb_system_reset_timing(psb_system s) {
   return PSB_OK;
 }
 
+psb_status psb_system_time_corrector(psb_system s, int32_t reps, double* us_density,
+                                     double* us_force) {
+  PSB_ARG(s && reps > 0 && us_density && us_force, "bad argument");
+  PSB_ARG(s->prm.solver == PSB_SOLVER_PCISPH && s->prm.mode == PSB_MODE_CONFIG,
+          "psb_system_time_corrector needs a PCISPH config-mode system");
+  if (!s->stepped || s->n == 0) { set_error("psb_system_time_corrector before a step"); return PSB_ERR_STATE; }
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const Lanes lanes = corrector_lanes(s);
+  // iteration 0 of a corrector that never finalises: the force pass writes
+  // the next prediction only
+  Consts c = s->c;
+  c.min_it = 2;
+  c.max_it = kMaxIterSlots - 1;
+  c.retry_ok = 0;
+  const uint16_t* l16 = (s->compact && s->nlist16) ? s->nlist16 : nullptr;
+  const HaloTile* tiles =
+      (s->halo && l16 && lanes.d == 1 && lanes.f == 1 && s->tiles && s->halo_src) ? s->tiles
+                                                                                     : nullptr;
+  PciBuffers b{};
+  b.fa = s->fa; b.fs = s->fs; b.anp = s->anp; b.xp0 = s->xp0; b.dl = s->dl;
+  b.ax = s->ax; b.avh = s->avh; b.aid = s->aid;
+  b.list16 = tiles ? nullptr : l16;
+  b.st = s->st;
+  PciOut o{};
+  o.flags = s->flags;
+  o.es = s->es;
+  PSB_TRY(launch_pci_retry_reset(s->st, st, s->es));
+  hipEvent_t e[3];
+  for (auto& x : e) PSB_HIP(hipEventCreate(&x));
+  psb_status stt = PSB_OK;
+  auto run = [&]() -> psb_status {
+    PSB_HIP(hipEventRecord(e[0], st));
+    for (int r = 0; r < reps; ++r)
+      PSB_TRY(launch_pci_density(s->n, c, s->ncount, s->nlist, s->xp0, s->pres, s->rho_pred,
+                                 s->dl, s->st, 0, nullptr, 0, st, s->aid, lanes.d, l16, s->es,
+                                 tiles, s->halo_src));
+    PSB_HIP(hipEventRecord(e[1], st));
+    for (int r = 0; r < reps; ++r)
+      PSB_TRY(launch_pci_force(s->n, c, true, s->ncount, s->nlist, s->xp0, s->xp1, s->rho_pred,
+                               s->pres, b, s->st, 0, 0, o, st, lanes.f, l16, tiles, s->halo_src));
+    PSB_HIP(hipEventRecord(e[2], st));
+    PSB_HIP(hipEventSynchronize(e[2]));
+    float md = 0.f, mf = 0.f;
+    PSB_HIP(hipEventElapsedTime(&md, e[0], e[1]));
+    PSB_HIP(hipEventElapsedTime(&mf, e[1], e[2]));
+    *us_density = 1e3 * (double)md / reps;
+    *us_force = 1e3 * (double)mf / reps;
+    return PSB_OK;
+  };
+  stt = run();
+  for (auto& x : e) hipEventDestroy(x);
+  return stt;
+}
+
 float psb_system_pressure_scale(psb_system s) { return s ? s->c.pscale : 0.f; }
 
 uint32_t psb_sphere_particle_count(const psb_sph_params* prm, const psb_sphere_params* sp) {

@@ -229,7 +229,7 @@ def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28
                          sphere_density: float = 500.0, spring: float = 1e4,
                          damping_ratio: float = 0.3, seed: int = 3000, tolerance: float = 0.01,
                          min_it: int = 3, max_it: int = 50, warm_start: int = WARM_NONE,
-                         lattice_walls: bool = True, relaxation: float = 0.6) -> Scene:
+                         lattice_walls: bool = True, relaxation: float = 1.0) -> Scene:
     """PCISPH double dam break with rigid DEM spheres.
 
     Two fluid blocks of block = (nx, ny, nz) lattice particles at the x-min
@@ -240,10 +240,12 @@ def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28
     at least 2 radii + s apart and one radius + s from the walls.  Contact
     spring k_n = spring; the normal dashpot coefficient is
     c = 2 damping_ratio sqrt(k_n m / 2) (m / 2: reduced mass of two spheres).
-    The corrector is under-relaxed (delta = relaxation * template): with
-    boundary-particle walls under 128-layer columns the full template
-    over-corrects and diverges within 30 steps; 0.6 holds the collapse.
-    meta holds the sphere centres and parameters.
+    The corrector runs the full template delta (relaxation 1) with failed
+    steps redone as substeps (retry_levels 2): measured on MI355X over the
+    first 400 steps, every step ends within the 1 % tolerance (17-49
+    iterations per step), where under-relaxation (0.6) without retries
+    capped most steps after the collapse and with retries failed even at
+    dt/4.  meta holds the sphere centres and parameters.
     """
     nx, ny, nz = block
     sc = pcisph_dam_break(n_side=1, s=s, box=box, tolerance=tolerance, min_it=min_it,

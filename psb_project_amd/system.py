@@ -202,6 +202,13 @@ class FluidSystem:
         """Per-kernel HIP events on every `every`-th step (off when `on` is false)."""
         check(lib().psb_system_set_timing(self.handle, int(every) if on else 0))
 
+    def time_corrector(self, reps: int = 50) -> dict:
+        """Per-launch durations (us) of the density and force passes,
+        replayed `reps` times back to back on the current state."""
+        d, f = C.c_double(), C.c_double()
+        check(lib().psb_system_time_corrector(self.handle, int(reps), C.byref(d), C.byref(f)))
+        return {"pci_density": d.value, "pci_force": f.value}
+
     def reset_timing(self) -> None:
         check(lib().psb_system_reset_timing(self.handle))
 
