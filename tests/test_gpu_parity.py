@@ -568,7 +568,7 @@ def test_pcisph_retry_with_spheres_bitwise(ctx, oracle):
     run, each substep moves the spheres by its own dt."""
     from psb_project_amd.system import FluidSystem
     sc = scenes.double_dam_break_dem(block=(12, 12, 8), box=(0.24, 0.12, 0.12), num_spheres=6,
-                                     max_it=3, tolerance=0.004)
+                                     max_it=3, tolerance=0.002)
     sc.params.retry_levels = 2
     m = sc.meta
     cap = sc.n + 93 * 6
