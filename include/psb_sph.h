@@ -381,6 +381,26 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
                               float* forces_accum4, uint32_t n,
                               int32_t* iterations, float* max_error);
 
+/* Allocation-free psb_pci_sph_update.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as psb_pci_sph_update, but its scratch
+ * (about 120 B per particle) is a static device array of each GPU, sized
+ * for PSB_NOALLOC_MAX_PARTICLES particles (the reference's own capacity,
+ * 65,536; 7.9 MB per GPU, held until the process ends), whose device
+ * address psb_init looks up.  n above the maximum returns
+ * PSB_ERR_CAPACITY.  The call allocates nothing; every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other
+ * contexts or threads on the same device) race: serialise them.          */
+#define PSB_NOALLOC_MAX_PARTICLES 65536u
+psb_status psb_pci_sph_update_noalloc(psb_context ctx, const psb_sph_params* prm,
+                                      psb_neighbors nb, float* old_pos4, float* pos4,
+                                      const float* vel4, float* vel_inter_av4,
+                                      float* vel_inter_ap4, float* density, float* pressure,
+                                      float* normals4, float* f_viscosity4, float* f_surface4,
+                                      float* f_pressure4, float* forces_accum4, uint32_t n,
+                                      int32_t* iterations, float* max_error);
+
 /* ------------------------------------------------------------------------ */
 /* Fused system step (SPHSystem / PCI_SPHSystem)                              */
 /* ------------------------------------------------------------------------ */

@@ -282,6 +282,9 @@ SIGNATURES = {
     "psb_collision_box": (i32, [vp, vp, vp, vp, f32, f32, u32, fp3, fp3, i32, f32]),
     "psb_pci_sph_update": (i32, [vp, C.POINTER(Params), Neighbors, vp, vp, vp, vp, vp, vp, vp,
                                  vp, vp, vp, vp, vp, u32, C.POINTER(i32), C.POINTER(f32)]),
+    "psb_pci_sph_update_noalloc": (i32, [vp, C.POINTER(Params), Neighbors, vp, vp, vp, vp, vp, vp,
+                                         vp, vp, vp, vp, vp, vp, u32, C.POINTER(i32),
+                                         C.POINTER(f32)]),
     "psb_system_create": (i32, [vp, C.POINTER(Params), u32, C.POINTER(vp)]),
     "psb_system_destroy": (i32, [vp]),
     "psb_system_get_params": (i32, [vp, C.POINTER(Params)]),

@@ -83,6 +83,17 @@ struct Shape {
   dim3 grid, block;
 };
 
+// Scratch bytes of psb_pci_sph_update for n particles: seven float4 arrays
+// (start-state copies, forcesAccum, surface force, non-pressure acceleration,
+// two predictions), the start pressure and delta (2 x 4 B), the status
+// aligned to 256 B.
+constexpr size_t pci_update_scratch_bytes(size_t n) {
+  return 7 * 16 * n + 8 * n + sizeof(PciStatus) + 256;
+}
+// per-device static scratch of psb_pci_sph_update_noalloc, looked up once per
+// device with that device current (psb_init)
+psb_status static_scratch_setup(int device);
+
 // one status per scene (scenes = 1 for a single system); done: batch counter
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes = 1,
                             uint32_t* done = nullptr, ErrSlots es = ErrSlots{});

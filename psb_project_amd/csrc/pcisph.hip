@@ -21,6 +21,8 @@
 // returns immediately, so the host can queue iterations ahead without
 // waiting for each error.
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "psb_internal.h"
 #include "pcisph.h"
@@ -708,14 +710,48 @@ psb_status launch_record(const float* feat, const float* pres, uint32_t n, uint3
 
 using namespace psb;
 
-extern "C" {
+namespace psb {
 
-psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_neighbors nb,
-                              float* old_pos4, float* pos4, const float* vel4,
-                              float* vel_inter_av4, float* vel_inter_ap4, float* density,
-                              float* pressure, float* normals4, float* f_viscosity4,
-                              float* f_surface4, float* f_pressure4, float* forces_accum4,
-                              uint32_t n, int32_t* iterations, float* max_error) {
+// ---------------------------------------------------------------------------
+// Static scratch of psb_pci_sph_update_noalloc: one array per GPU, held until
+// the process ends, sized for PSB_NOALLOC_MAX_PARTICLES.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is a host-side
+// stand-in); it passes the device address that hipGetSymbolAddress returned
+// for the current device at psb_init.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = pci_update_scratch_bytes(PSB_NOALLOC_MAX_PARTICLES);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices] = {};
+
+psb_status static_scratch_setup(int device) {
+  PSB_ARG(device >= 0 && device < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PSB_OK;
+  void* ptr = nullptr;
+  PSB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));   // device `device` is current
+  g_scratch_ptr[device] = static_cast<char*>(ptr);
+  return PSB_OK;
+}
+
+static char* static_scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+// The solve of psb_pci_sph_update on a caller-chosen scratch area of at
+// least pci_update_scratch_bytes(n) bytes (null: the context's scratch,
+// grown on demand).
+static psb_status pci_sph_update_on(psb_context ctx, const psb_sph_params* prm, psb_neighbors nb,
+                                    float* old_pos4, float* pos4, const float* vel4,
+                                    float* vel_inter_av4, float* vel_inter_ap4, float* density,
+                                    float* pressure, float* normals4, float* f_viscosity4,
+                                    float* f_surface4, float* f_pressure4, float* forces_accum4,
+                                    uint32_t n, int32_t* iterations, float* max_error,
+                                    char* fixed_scratch) {
   PSB_ARG(ctx && prm, "null argument");
   PSB_ARG(n == 0 || (old_pos4 && pos4 && vel4 && vel_inter_av4 && vel_inter_ap4 && density &&
                      pressure),
@@ -728,8 +764,11 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   PSB_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const size_t v4 = 16 * (size_t)n;
-  PSB_TRY(ensure_scratch(ctx, 7 * v4 + sizeof(PciStatus) + 256 + 8 * (size_t)n));
-  char* base = (char*)ctx->scratch;
+  char* base = fixed_scratch;
+  if (!base) {
+    PSB_TRY(ensure_scratch(ctx, pci_update_scratch_bytes(n)));
+    base = (char*)ctx->scratch;
+  }
   float4* xs = (float4*)base;
   float4* vs = (float4*)(base + v4);
   PciBuffers b{};
@@ -795,6 +834,42 @@ psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_ne
   if (iterations) *iterations = iters;
   if (max_error) *max_error = err;
   return PSB_OK;
+}
+
+}  // namespace psb
+
+extern "C" {
+
+psb_status psb_pci_sph_update(psb_context ctx, const psb_sph_params* prm, psb_neighbors nb,
+                              float* old_pos4, float* pos4, const float* vel4,
+                              float* vel_inter_av4, float* vel_inter_ap4, float* density,
+                              float* pressure, float* normals4, float* f_viscosity4,
+                              float* f_surface4, float* f_pressure4, float* forces_accum4,
+                              uint32_t n, int32_t* iterations, float* max_error) {
+  return pci_sph_update_on(ctx, prm, nb, old_pos4, pos4, vel4, vel_inter_av4, vel_inter_ap4,
+                           density, pressure, normals4, f_viscosity4, f_surface4, f_pressure4,
+                           forces_accum4, n, iterations, max_error, nullptr);
+}
+
+psb_status psb_pci_sph_update_noalloc(psb_context ctx, const psb_sph_params* prm,
+                                      psb_neighbors nb, float* old_pos4, float* pos4,
+                                      const float* vel4, float* vel_inter_av4,
+                                      float* vel_inter_ap4, float* density, float* pressure,
+                                      float* normals4, float* f_viscosity4, float* f_surface4,
+                                      float* f_pressure4, float* forces_accum4, uint32_t n,
+                                      int32_t* iterations, float* max_error) {
+  PSB_ARG(ctx, "null argument");
+  if (n > PSB_NOALLOC_MAX_PARTICLES) {
+    set_error("psb_pci_sph_update_noalloc: n = " + std::to_string(n) + " exceeds the static "
+              "scratch's maximum of " + std::to_string(PSB_NOALLOC_MAX_PARTICLES) +
+              " particles (use psb_pci_sph_update)");
+    return PSB_ERR_CAPACITY;
+  }
+  char* base = static_scratch_ptr(ctx->device);
+  if (!base) { set_error("static scratch not set up on this device (psb_init)"); return PSB_ERR_STATE; }
+  return pci_sph_update_on(ctx, prm, nb, old_pos4, pos4, vel4, vel_inter_av4, vel_inter_ap4,
+                           density, pressure, normals4, f_viscosity4, f_surface4, f_pressure4,
+                           forces_accum4, n, iterations, max_error, base);
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "psb_internal.h"
+#include "pcisph.h"
 
 namespace psb {
 
@@ -303,6 +304,10 @@ psb_status psb_init(int32_t device, psb_context* out) {
   e = hipMalloc(&ctx->d_flags, 64);
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_pinned, 4096, hipHostMallocDefault);
   if (e != hipSuccess) { psb_finalize(ctx); return hip_fail(e, "context buffers", __FILE__, __LINE__); }
+  // device address of this GPU's copy of the allocation-free entry point's
+  // static scratch (psb_pci_sph_update_noalloc)
+  const psb_status ss = static_scratch_setup(device);
+  if (ss != PSB_OK) { psb_finalize(ctx); return ss; }
   *out = ctx;
   return PSB_OK;
 }
