@@ -759,10 +759,11 @@ template <class R> struct System {
     if (level > 0) d = level_derived(level);
     const bool ref = prm.mode == PSB_MODE_REFERENCE;
     const bool retry_ok = !ref && level < prm.retry_levels;
-    pci_prologue(from_a);
-    const bool warm = !ref && !from_a && prm.warm_start == PSB_WARM_PREVIOUS;
-    const bool mlp = !ref && !from_a && prm.warm_start == PSB_WARM_MLP;
-    const bool feat_now = !ref && !from_a && (mlp || feat_on);
+    const bool plain = level > 0;   // substeps run the plain corrector
+    pci_prologue(from_a, plain);
+    const bool warm = !ref && !plain && prm.warm_start == PSB_WARM_PREVIOUS;
+    const bool mlp = !ref && !plain && prm.warm_start == PSB_WARM_MLP;
+    const bool feat_now = !ref && !plain && (mlp || feat_on);
     std::vector<V4<R>>* cur = &xp0;
     std::vector<V4<R>>* nxt = &xp1;
     if (warm) {
@@ -811,8 +812,9 @@ template <class R> struct System {
   // densities, initial pressure, non-pressure forces and the first
   // prediction (xp0).  The corrector probe of the C API stops here.
   // from_a: a retry from the step's start state A, whose grid, lists and
-  // densities (walls included) are kept; the pressure starts at 0.
-  void pci_prologue(bool from_a = false) {
+  // densities (walls included) are kept; the pressure starts at 0.  plain:
+  // a retry substep, without warm start or features.
+  void pci_prologue(bool from_a = false, bool plain = false) {
     if (!from_a) {
       build_grid();
       neighbors_density();
@@ -822,9 +824,9 @@ template <class R> struct System {
     const bool surf = ref ? true : (prm.surface_tension != 0.f);
     const R gx = ref ? (R)0 : d.gx, gz = ref ? (R)0 : d.gz;
     const R gy = ref ? (R)-9.81f : d.gy;
-    const bool warm = !ref && !from_a && prm.warm_start == PSB_WARM_PREVIOUS;
-    const bool mlp = !ref && !from_a && prm.warm_start == PSB_WARM_MLP;
-    const bool feat_now = !ref && !from_a && (mlp || feat_on);
+    const bool warm = !ref && !plain && prm.warm_start == PSB_WARM_PREVIOUS;
+    const bool mlp = !ref && !plain && prm.warm_start == PSB_WARM_MLP;
+    const bool feat_now = !ref && !plain && (mlp || feat_on);
     // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)n; ++k) {

@@ -92,6 +92,21 @@ struct psb_batch_s {
   std::vector<TRec> trecs;
   double t_phase[BP_COUNT] = {};
   int64_t step_counter = 0;
+  // retries (params.retry_levels): per level, the scenes being redone as
+  // substeps and their block tables (scene records with blk0 for the subset)
+  struct RetrySet {
+    std::vector<uint32_t> scenes;
+    BatchScene* dsc = nullptr;
+    uint32_t* dlist = nullptr;
+    uint16_t* dblk[kBatchBlockSizes] = {};
+    uint32_t nblk[kBatchBlockSizes] = {};
+  };
+  RetrySet rs[kMaxRetryLevels + 1];
+  std::vector<int32_t> retried, substeps;   // per scene, over the current update call
+  std::vector<char> lat_retried[2];         // scenes redone in the step of each parity
+  hipEvent_t ev_retry_end[2] = {};
+  std::vector<hipEvent_t> ev_riter;         // iteration ends of retry attempts
+  PciStatus* h_rstat = nullptr;             // pinned [S] status copy for the retry decision
 };
 
 namespace psb {
@@ -284,12 +299,170 @@ static psb_status read_latency(psb_batch b, int par) {
   PSB_HIP(hipEventSynchronize(b->ev_copy[par]));
   const PciStatus* hs = b->h_stat + (size_t)par * b->S;
   for (uint32_t s = 0; s < b->S; ++s) {
+    if (!b->lat_retried[par].empty() && b->lat_retried[par][s]) {
+      // redone as substeps: the step ends with the retries
+      float ms = 0.f;
+      PSB_HIP(hipEventElapsedTime(&ms, b->ev_start[par], b->ev_retry_end[par]));
+      b->lat_ms.push_back(ms);
+      continue;
+    }
     const int f = hs[s].final_iter;
     if (f < 0 || f >= (int)b->ev_iter[par].size()) continue;
     float ms = 0.f;
     PSB_HIP(hipEventElapsedTime(&ms, b->ev_start[par], b->ev_iter[par][f]));
     b->lat_ms.push_back(ms);
   }
+  return PSB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Retries: a scene whose corrector ends above the tolerance is redone from
+// its start state as two substeps of dt/2 (params.retry_levels deep), exactly
+// as the single-system step does, with the other scenes left as they are.
+// The launches cover only the redone scenes (their own block tables); the
+// grid build of a second half runs over every scene (one counting sort over
+// the global cells), which re-sorts the finished scenes' end states into A
+// without changing them.
+// ---------------------------------------------------------------------------
+static psb_status batch_level_consts(psb_batch b, int level, Consts* out) {
+  if (level == 0) { *out = b->c; return PSB_OK; }
+  psb_sph_params p = b->prm;
+  const float scale = std::ldexp(1.0f, -level);
+  p.dt = b->prm.dt * scale;
+  if (p.delta > 0.f) p.delta = b->prm.delta / (scale * scale);
+  PSB_TRY(make_consts(p, out, nullptr));
+  out->retry_ok = level < b->prm.retry_levels ? 1 : 0;
+  if (level == b->prm.retry_levels) out->max_it = retry_max_iterations(b->prm);
+  return PSB_OK;
+}
+
+static psb_status retry_set(psb_batch b, int level, const std::vector<uint32_t>& scenes) {
+  psb_batch_s::RetrySet& R = b->rs[level];
+  auto A = [&](void** p, size_t bytes) -> psb_status { PSB_HIP(hipMalloc(p, bytes)); return PSB_OK; };
+  if (!R.dsc) {
+    PSB_TRY(A((void**)&R.dsc, sizeof(BatchScene) * b->S));
+    PSB_TRY(A((void**)&R.dlist, 4 * (size_t)b->S));
+    for (int t = 0; t < kBatchBlockSizes; ++t) PSB_TRY(A((void**)&R.dblk[t], 2 * (size_t)b->nblk[t]));
+  }
+  R.scenes = scenes;
+  std::vector<BatchScene> hs(b->hsc);
+  for (int t = 0; t < kBatchBlockSizes; ++t) {
+    const uint32_t bs = 64u << t;
+    std::vector<uint16_t> tab;
+    for (uint32_t s : scenes) {
+      hs[s].blk0[t] = (uint32_t)tab.size();
+      tab.insert(tab.end(), (hs[s].n + bs - 1) / bs, (uint16_t)s);
+    }
+    R.nblk[t] = (uint32_t)tab.size();
+    PSB_HIP(hipMemcpy(R.dblk[t], tab.data(), 2 * tab.size(), hipMemcpyHostToDevice));
+  }
+  PSB_HIP(hipMemcpy(R.dsc, hs.data(), sizeof(BatchScene) * b->S, hipMemcpyHostToDevice));
+  PSB_HIP(hipMemcpy(R.dlist, scenes.data(), 4 * scenes.size(), hipMemcpyHostToDevice));
+  return PSB_OK;
+}
+
+static BatchMap rmap(psb_batch b, const psb_batch_s::RetrySet& R, int phase) {
+  const int t = btable(b, phase);
+  return BatchMap{R.dsc, R.dblk[t], t};
+}
+
+static Shape rshape(psb_batch b, const psb_batch_s::RetrySet& R, int phase) {
+  return Shape{dim3(R.nblk[btable(b, phase)]), dim3(64u << b->bsel[phase])};
+}
+
+// Scenes of `scenes` whose last attempt ended above the tolerance.
+static psb_status failed_scenes(psb_batch b, const std::vector<uint32_t>& scenes,
+                                std::vector<uint32_t>* out) {
+  PSB_HIP(hipMemcpyAsync(b->h_rstat, b->st, sizeof(PciStatus) * b->S, hipMemcpyDeviceToHost,
+                         b->ctx->stream));
+  PSB_HIP(hipStreamSynchronize(b->ctx->stream));
+  out->clear();
+  for (uint32_t s : scenes) {
+    float e;
+    std::memcpy(&e, &b->h_rstat[s].last_err, 4);
+    if (e > b->prm.tolerance) out->push_back(s);
+  }
+  return PSB_OK;
+}
+
+// One plain-corrector attempt of dt / 2^level for the scenes of rs[level]:
+// from their start state A (the first half of a redo) or from B after a
+// grid build (the second half).
+static psb_status retry_attempt(psb_batch b, int level, bool from_a) {
+  const psb_batch_s::RetrySet& R = b->rs[level];
+  const uint32_t cnt = (uint32_t)R.scenes.size();
+  Consts c;
+  PSB_TRY(batch_level_consts(b, level, &c));
+  hipStream_t st = b->ctx->stream;
+  psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
+  PciBuffers pb; PciOut o;
+  buffers(b, pb, o);
+  pb.feat = nullptr;
+  o.num_scenes = cnt;
+  if (!from_a) {
+    ReorderArrays ra;
+    ra.sx = b->bx; ra.sv = b->bv; ra.svh = b->bvh; ra.sid = b->bid; ra.sp = b->bp;
+    ra.dx = b->ax; ra.dv = b->av; ra.dvh = b->avh; ra.did = b->aid; ra.dp = b->ap;
+    const int sh = b->bsel[BP_HASH];
+    const StepReset reset{b->st, b->done, b->es, b->S};
+    PSB_TRY(grid_build_batch(b->grid, bmap(b, BP_HASH), b->nblk[sh], 64u << sh, b->bx, b->n, &ra,
+                             st, &reset));
+    const int sn = b->bsel[BP_NEIGHBORS];
+    Compact16 c16;
+    c16.list16 = b->nlist16;
+    c16.wide = &b->st[0].wide;
+    c16.wide_stride = sizeof(PciStatus) / sizeof(uint32_t);
+    const EpiArgs ep = make_epilogue(c, 1, 0, 0, b->pres, b->ap, b->ax, b->av, nullptr);
+    PSB_TRY(launch_neighbors_batch(b->grid, rmap(b, R, BP_NEIGHBORS), R.nblk[sn], 64u << sn,
+                                   b->ax, b->n, c.h2, b->gd.R, nb, b->rho, c.mc, b->flags,
+                                   b->flags + 1, st, c16, b->es, &ep));
+  } else {
+    PSB_TRY(launch_pci_retry_reset(b->st, st, b->es, R.dlist, cnt));
+    PSB_HIP(hipMemsetAsync(b->done, 0, 4, st));
+    for (uint32_t s : R.scenes)
+      PSB_HIP(hipMemsetAsync(b->pres + b->hsc[s].p0, 0, 4 * (size_t)b->hsc[s].n, st));
+  }
+  PSB_TRY(launch_pci_nonpressure_batch(rmap(b, R, BP_NONPRESSURE), rshape(b, R, BP_NONPRESSURE),
+                                       b->n, c, b->prm.surface_tension != 0.f, false, nb.count,
+                                       nb.list, b->ax, b->av, b->rho, b->avh, b->pres, pb, o, st));
+  *b->h_final = -1;
+  for (int k = 0; k < c.max_it; ++k) {
+    float4* cur = (k & 1) ? b->xp1 : b->xp0;
+    float4* nxt = (k & 1) ? b->xp0 : b->xp1;
+    PSB_TRY(launch_pci_density_batch(rmap(b, R, BP_DENSITY), rshape(b, R, BP_DENSITY), b->n, c,
+                                     nb.count, nb.list, cur, b->pres, b->rho_pred, b->dl, b->st,
+                                     k, nullptr, 0, st, b->lanes[BP_DENSITY], b->nlist16, b->es));
+    PSB_TRY(launch_pci_force_batch(rmap(b, R, BP_FORCE), rshape(b, R, BP_FORCE), b->n, c,
+                                   nb.count, nb.list, cur, nxt, b->rho_pred, b->pres, pb, b->st, k,
+                                   0, o, st, b->lanes[BP_FORCE], b->nlist16));
+    if (k + 1 >= c.min_it) PSB_HIP(hipEventRecord(b->ev_riter[k], st));
+    if (k >= c.min_it) {
+      PSB_HIP(hipEventSynchronize(b->ev_riter[k - 1]));
+      if (__atomic_load_n(b->h_final, __ATOMIC_ACQUIRE) >= 0) break;
+    }
+  }
+  PSB_HIP(hipStreamSynchronize(st));
+  return PSB_OK;
+}
+
+static psb_status retry_redo(psb_batch b, const std::vector<uint32_t>& scenes, int level);
+
+static psb_status retry_half(psb_batch b, const std::vector<uint32_t>& scenes, int level,
+                             bool from_a) {
+  PSB_TRY(retry_set(b, level, scenes));
+  PSB_TRY(retry_attempt(b, level, from_a));
+  if (level >= b->prm.retry_levels) return PSB_OK;
+  std::vector<uint32_t> failed;
+  PSB_TRY(failed_scenes(b, scenes, &failed));
+  if (!failed.empty()) PSB_TRY(retry_redo(b, failed, level + 1));
+  return PSB_OK;
+}
+
+// Redo `scenes` (failed at level - 1) as two substeps of dt / 2^level.
+static psb_status retry_redo(psb_batch b, const std::vector<uint32_t>& scenes, int level) {
+  for (uint32_t s : scenes) b->substeps[s] += 2;
+  PSB_TRY(retry_half(b, scenes, level, true));
+  PSB_TRY(retry_half(b, scenes, level, false));
   return PSB_OK;
 }
 
@@ -318,10 +491,23 @@ static psb_status batch_step(psb_batch b, int64_t t) {
   // before the next step resets the flag
   if (final_known < 0) PSB_HIP(hipStreamSynchronize(st));
   if (b->lat_on) {
+    // (before any retry: a second half's grid build resets every scene's
+    // final iteration)
     PSB_HIP(hipMemcpyAsync(b->h_stat + (size_t)par * b->S, b->st, sizeof(PciStatus) * b->S,
                            hipMemcpyDeviceToHost, st));
     PSB_HIP(hipEventRecord(b->ev_copy[par], st));
     b->pending = par;
+  }
+  if (b->prm.retry_levels > 0) {
+    std::vector<uint32_t> all(b->S), failed;
+    for (uint32_t s = 0; s < b->S; ++s) all[s] = s;
+    PSB_TRY(failed_scenes(b, all, &failed));
+    b->lat_retried[par].assign(b->S, 0);
+    if (!failed.empty()) {
+      for (uint32_t s : failed) { b->retried[s] += 1; b->lat_retried[par][s] = 1; }
+      PSB_TRY(retry_redo(b, failed, 1));
+    }
+    PSB_HIP(hipEventRecord(b->ev_retry_end[par], st));
   }
   if (b->tuning) PSB_TRY(collect_tuning(b));
   return PSB_OK;
@@ -351,6 +537,13 @@ static psb_status batch_alloc(psb_batch b) {
   PSB_HIP(hipHostGetDevicePointer((void**)&b->d_final, b->h_final, 0));
   *b->h_final = -1;
   PSB_HIP(hipHostMalloc((void**)&b->h_stat, 2 * sizeof(PciStatus) * b->S, hipHostMallocDefault));
+  PSB_HIP(hipHostMalloc((void**)&b->h_rstat, sizeof(PciStatus) * b->S, hipHostMallocDefault));
+  for (int p = 0; p < 2; ++p) PSB_HIP(hipEventCreate(&b->ev_retry_end[p]));
+  for (int k = 0; k < kMaxIterSlots; ++k) {
+    hipEvent_t e;
+    PSB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->ev_riter.push_back(e);
+  }
   hipStream_t st = b->ctx->stream;
   PSB_HIP(hipMemsetAsync(b->flags, 0, 64, st));
   PSB_HIP(hipMemsetAsync(b->st, 0, sizeof(PciStatus) * b->S, st));
@@ -399,6 +592,8 @@ static psb_status fetch_stats(psb_batch b) {
     o.total_iterations = (int64_t)hs[s].total_iters;
     std::memcpy(&o.worst_error, &hs[s].worst_err, 4);
     o.max_neighbors_seen = (int32_t)b->last_maxnb;
+    o.retried_steps = b->retried[s];
+    o.substeps = b->substeps[s];
   }
   return PSB_OK;
 }
@@ -462,6 +657,9 @@ psb_status psb_batch_create(psb_context ctx, const psb_sph_params* material, uin
   b->n = (uint32_t)parts;
   b->has_state.assign(num_scenes, 0);
   b->stats.assign(num_scenes, psb_step_stats{});
+  b->retried.assign(num_scenes, 0);
+  b->substeps.assign(num_scenes, 0);
+  b->c.retry_ok = material->retry_levels > 0 ? 1 : 0;
   {
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
@@ -490,6 +688,14 @@ psb_status psb_batch_destroy(psb_batch b) {
   for (auto p : b->dblk) if (p) hipFree(p);
   if (b->h_final) hipHostFree(b->h_final);
   if (b->h_stat) hipHostFree(b->h_stat);
+  if (b->h_rstat) hipHostFree(b->h_rstat);
+  for (auto& R : b->rs) {
+    if (R.dsc) hipFree(R.dsc);
+    if (R.dlist) hipFree(R.dlist);
+    for (auto p : R.dblk) if (p) hipFree(p);
+  }
+  for (int p = 0; p < 2; ++p) if (b->ev_retry_end[p]) hipEventDestroy(b->ev_retry_end[p]);
+  for (auto e : b->ev_riter) hipEventDestroy(e);
   for (int p = 0; p < 2; ++p) {
     if (b->ev_start[p]) hipEventDestroy(b->ev_start[p]);
     if (b->ev_copy[p]) hipEventDestroy(b->ev_copy[p]);
@@ -591,6 +797,8 @@ psb_status psb_batch_update(psb_batch b, int32_t steps) {
   PSB_HIP(hipMemsetAsync(b->flags, 0, 8, st));
   PSB_HIP(hipMemsetAsync(b->grid->flags, 0, 4, st));
   PSB_TRY(launch_pci_reset(b->st, true, st, b->S, b->done, b->es));
+  std::fill(b->retried.begin(), b->retried.end(), 0);
+  std::fill(b->substeps.begin(), b->substeps.end(), 0);
   for (int k = 0; k < steps; ++k) PSB_TRY(batch_step(b, b->step_counter++));
   return fetch_stats(b);
 }

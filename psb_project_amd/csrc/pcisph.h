@@ -98,8 +98,10 @@ psb_status static_scratch_setup(int device);
 psb_status launch_pci_reset(PciStatus* st, bool stats, hipStream_t s, uint32_t scenes = 1,
                             uint32_t* done = nullptr, ErrSlots es = ErrSlots{});
 // reset of a step redone from its start state: the neighbour lists (and the
-// compact list's overflow flag) of that state stay valid
-psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es);
+// compact list's overflow flag) of that state stay valid.  scenes: device
+// list of `count` scene indices (null: scenes 0 .. count-1)
+psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es,
+                                  const uint32_t* scenes = nullptr, uint32_t count = 1);
 psb_status launch_pci_nonpressure(uint32_t n, const Consts& c, bool surf, bool warm, bool out,
                                   const uint32_t* cnt, const uint32_t* list, const float4* xs,
                                   const float4* vs, const float* rho, const float4* avh,

@@ -35,8 +35,8 @@ __global__ void k_pci_reset(PciStatus* st0, uint32_t* done, ErrSlots es) {
   pci_reset_scene(st0, blockIdx.x, done, es);
 }
 
-__global__ void k_pci_retry_reset(PciStatus* st0, ErrSlots es) {
-  pci_reset_scene(st0, blockIdx.x, nullptr, es, true);
+__global__ void k_pci_retry_reset(PciStatus* st0, ErrSlots es, const uint32_t* scenes) {
+  pci_reset_scene(st0, scenes ? scenes[blockIdx.x] : blockIdx.x, nullptr, es, true);
 }
 
 __global__ void k_pci_stats_reset(PciStatus* st0) {
@@ -501,8 +501,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
 }
 
 // ---- launchers ----------------------------------------------------------------
-psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es) {
-  hipLaunchKernelGGL(k_pci_retry_reset, dim3(1), dim3(256), 0, s, st, es);
+psb_status launch_pci_retry_reset(PciStatus* st, hipStream_t s, ErrSlots es,
+                                  const uint32_t* scenes, uint32_t count) {
+  if (!count) return PSB_OK;
+  hipLaunchKernelGGL(k_pci_retry_reset, dim3(count), dim3(256), 0, s, st, es, scenes);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

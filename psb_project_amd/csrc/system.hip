@@ -270,17 +270,18 @@ static psb_status level_consts(psb_system s, int level, Consts* out) {
 
 // One attempt at a PCISPH step with constants c.  from_a: restart from the
 // start state A of the current step (a retry: the grid, the lists and the
-// densities of A are still valid, the pressure starts at 0 and no warm start
-// or recorder runs); otherwise the step begins with the grid build of B.
+// densities of A are still valid, the pressure starts at 0); otherwise the
+// step begins with the grid build of B.  plain: a retry substep, which runs
+// the plain corrector (p = 0 start, no warm start, features or recorder).
 // Returns the final iteration and whether it ended above the tolerance.
 static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool from_a,
-                                 int* final_out, bool* failed_out) {
+                                 bool plain, int* final_out, bool* failed_out) {
   hipStream_t st = s->ctx->stream;
   const Lanes lanes = corrector_lanes(s);
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
-  const bool warm = !ref && !from_a && s->prm.warm_start == PSB_WARM_PREVIOUS;
-  const bool mlp = !ref && !from_a && s->prm.warm_start == PSB_WARM_MLP;
-  const bool feat = !ref && !from_a && (mlp || s->rec != nullptr);
+  const bool warm = !ref && !plain && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = !ref && !plain && s->prm.warm_start == PSB_WARM_MLP;
+  const bool feat = !ref && !plain && (mlp || s->rec != nullptr);
   const int att = (int)s->step_final.size();   // attempt index of this update call
   const int slot = att & 63;
   s->step_final.push_back(-1);
@@ -388,12 +389,12 @@ static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool f
 
 // After an accepted attempt: the rigid spheres (config 3) and the
 // training-set recorder.
-static psb_status pcisph_finish(psb_system s, const Consts& c, int final_known, bool from_a) {
+static psb_status pcisph_finish(psb_system s, const Consts& c, int final_known, bool plain) {
   hipStream_t st = s->ctx->stream;
   const bool ref = s->prm.mode == PSB_MODE_REFERENCE;
-  const bool warm = !ref && !from_a && s->prm.warm_start == PSB_WARM_PREVIOUS;
-  const bool mlp = !ref && !from_a && s->prm.warm_start == PSB_WARM_MLP;
-  const bool feat = !ref && !from_a && (mlp || s->rec != nullptr);
+  const bool warm = !ref && !plain && s->prm.warm_start == PSB_WARM_PREVIOUS;
+  const bool mlp = !ref && !plain && s->prm.warm_start == PSB_WARM_MLP;
+  const bool feat = !ref && !plain && (mlp || s->rec != nullptr);
   const int att = (int)s->step_final.size() - 1;
   if (s->n_spheres) {
     // the pressure force of the final iteration used the positions its
@@ -434,13 +435,14 @@ static psb_status pcisph_finish(psb_system s, const Consts& c, int final_known, 
 // Advance one step of dt / 2^level.  A step whose corrector ends above the
 // tolerance is redone from its start state A as two steps of half the size
 // (params.retry_levels deep); the failed attempt's end state in B is
-// overwritten by the first half, which reuses A's grid and lists.
+// overwritten by the first half, which reuses A's grid and lists.  Substeps
+// (level > 0) run the plain corrector.
 static psb_status pcisph_advance(psb_system s, int step, int level, bool from_a) {
   Consts c;
   PSB_TRY(level_consts(s, level, &c));
   int fin = 0;
   bool failed = false;
-  PSB_TRY(pcisph_attempt(s, step, c, from_a, &fin, &failed));
+  PSB_TRY(pcisph_attempt(s, step, c, from_a, level > 0, &fin, &failed));
   if (failed && c.retry_ok) {
     if (level == 0) s->stats.retried_steps += 1;
     s->stats.substeps += 2;
@@ -448,7 +450,7 @@ static psb_status pcisph_advance(psb_system s, int step, int level, bool from_a)
     PSB_TRY(pcisph_advance(s, step, level + 1, false));
     return PSB_OK;
   }
-  return pcisph_finish(s, c, fin, from_a);
+  return pcisph_finish(s, c, fin, level > 0);
 }
 
 static psb_status pcisph_step(psb_system s, int step) { return pcisph_advance(s, step, 0, false); }

@@ -177,6 +177,36 @@ def test_batch_latency_samples(ctx):
     b.close()
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_batch_retries_match_single_systems(ctx, graphs):
+    """A 0.2 % tolerance with a 3-iteration cap makes scenes fail their
+    steps (four of the five here); the batch redoes each failed scene alone as substeps (its other
+    scenes untouched), so every scene still equals the single system with
+    the same retries, bit for bit, with its retry counts; every scene-step
+    gives one latency sample."""
+    scs = scenes.batched_scenes(num_scenes=5, min_count=300, max_count=5000, seed=83,
+                                tolerance=0.002, max_it=3)
+    b = BatchSystem(ctx, scs)
+    b.set_all()
+    b.set_graphs(graphs)
+    b.set_latency(True)
+    b.update(40)
+    ref = single_runs(ctx, scs, 40)
+    retried = 0
+    for i, (st, stats) in enumerate(ref):
+        assert_same(b.state(i), st, f"scene {i}")
+        bs = b.stats(i)
+        for k in ("iterations", "total_iterations", "retried_steps", "substeps"):
+            assert bs[k] == stats[k], (i, k, bs[k], stats[k])
+        assert bs["max_error"] == stats["max_error"]
+        assert bs["worst_error"] == stats["worst_error"]
+        retried += bs["retried_steps"]
+    assert retried > 0
+    lat = b.latency()
+    assert lat["samples"] == 40 * len(scs)
+    b.close()
+
+
 def test_batch_errors(ctx):
     scs = small_batch(WARM_NONE, False, n=2, seed=82)
     b = BatchSystem(ctx, scs)
