@@ -324,11 +324,9 @@ static psb_status read_latency(psb_batch b, int par) {
 // the global cells), which re-sorts the finished scenes' end states into A
 // without changing them.
 // ---------------------------------------------------------------------------
-// Switched off: the first run of the batch retry test on MI355X ended with a
-// GPU fault whose cause is not found yet; batches keep failed scenes'
-// steps (flagged NOT_CONVERGED) until it is.
-constexpr bool kBatchRetries = false;
-
+// Level constants start from make_consts, which leaves no collider or body
+// behind (a batch has neither), so the final force pass of a retry walks no
+// collider list.
 static psb_status batch_level_consts(psb_batch b, int level, Consts* out) {
   if (level == 0) { *out = b->c; return PSB_OK; }
   psb_sph_params p = b->prm;
@@ -396,7 +394,7 @@ static psb_status failed_scenes(psb_batch b, const std::vector<uint32_t>& scenes
 static psb_status retry_attempt(psb_batch b, int level, bool from_a) {
   const psb_batch_s::RetrySet& R = b->rs[level];
   const uint32_t cnt = (uint32_t)R.scenes.size();
-  Consts c;
+  Consts c{};
   PSB_TRY(batch_level_consts(b, level, &c));
   hipStream_t st = b->ctx->stream;
   psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
@@ -503,7 +501,7 @@ static psb_status batch_step(psb_batch b, int64_t t) {
     PSB_HIP(hipEventRecord(b->ev_copy[par], st));
     b->pending = par;
   }
-  if (kBatchRetries && b->prm.retry_levels > 0) {
+  if (b->prm.retry_levels > 0) {
     std::vector<uint32_t> all(b->S), failed;
     for (uint32_t s = 0; s < b->S; ++s) all[s] = s;
     PSB_TRY(failed_scenes(b, all, &failed));
@@ -664,7 +662,7 @@ psb_status psb_batch_create(psb_context ctx, const psb_sph_params* material, uin
   b->stats.assign(num_scenes, psb_step_stats{});
   b->retried.assign(num_scenes, 0);
   b->substeps.assign(num_scenes, 0);
-  b->c.retry_ok = (kBatchRetries && material->retry_levels > 0) ? 1 : 0;
+  b->c.retry_ok = material->retry_levels > 0 ? 1 : 0;
   {
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) stt = hip_fail(e, "hipSetDevice", __FILE__, __LINE__);

@@ -760,7 +760,7 @@ static psb_status pci_sph_update_on(psb_context ctx, const psb_sph_params* prm, 
           "null array");
   PSB_ARG(nb.n == n && nb.count && nb.list, "neighbour list stride must equal n");
   PSB_ARG(prm->warm_start != PSB_WARM_MLP, "the MLP warm start needs a psb_system");
-  Consts c;
+  Consts c{};
   PSB_TRY(make_consts(*prm, &c, nullptr));
   if (!n) return PSB_OK;
   PSB_HIP(hipSetDevice(ctx->device));

@@ -142,6 +142,9 @@ psb_status make_collider(const psb_collider& in, ColliderDev* out) {
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   PSB_TRY(check_params(p));
+  // every field starts defined: no bodies, no colliders (coll = nullptr,
+  // ncoll = 0) until a system patches them in
+  *c = Consts{};
   const double h = p.support_radius, m = p.mass, rho0 = p.rest_density, dt = p.dt, pi = M_PI;
   c->h = (float)h;
   c->h2 = (float)(h * h);
@@ -260,7 +263,7 @@ psb_status psb_default_params(psb_sph_params* p) {
 
 psb_status psb_derive(const psb_sph_params* p, psb_derived* out) {
   PSB_ARG(p && out, "null argument");
-  Consts c;
+  Consts c{};
   GridDev g;
   PSB_TRY(make_consts(*p, &c, &g));
   const double h = p->support_radius, pi = M_PI;

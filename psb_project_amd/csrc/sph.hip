@@ -222,7 +222,7 @@ psb_status psb_evaluate_densities_forces(psb_context ctx, const psb_sph_params* 
                                          float* forces_accum4) {
   PSB_ARG(ctx && prm && (n == 0 || (pos4 && vel4 && density && pressure)), "null argument");
   PSB_ARG(nb.n == n && nb.count && nb.list, "neighbour list stride must equal n");
-  Consts c;
+  Consts c{};
   PSB_TRY(make_consts(*prm, &c, nullptr));
   if (!n) return PSB_OK;
   PSB_HIP(hipSetDevice(ctx->device));

@@ -438,7 +438,7 @@ static psb_status pcisph_finish(psb_system s, const Consts& c, int final_known, 
 // overwritten by the first half, which reuses A's grid and lists.  Substeps
 // (level > 0) run the plain corrector.
 static psb_status pcisph_advance(psb_system s, int step, int level, bool from_a) {
-  Consts c;
+  Consts c{};
   PSB_TRY(level_consts(s, level, &c));
   int fin = 0;
   bool failed = false;

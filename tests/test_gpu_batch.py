@@ -177,8 +177,6 @@ def test_batch_latency_samples(ctx):
     b.close()
 
 
-@pytest.mark.skip(reason="batch retries are switched off (kBatchRetries): their first GPU run "
-                         "ended with a GPU fault whose cause is not found yet")
 @pytest.mark.parametrize("graphs", [False, True])
 def test_batch_retries_match_single_systems(ctx, graphs):
     """A 0.2 % tolerance with a 3-iteration cap makes scenes fail their
