@@ -177,14 +177,17 @@ def train(ctx: Context, dataset: Dataset, pressure_scale: float, epochs: int = 3
     rng = np.random.default_rng(seed)
     losses = []
     idx_dev = ctx.array(4 * batch)
-    nb = count // batch
+    # every sample once per epoch: full minibatches, then the remainder as a
+    # smaller last one (a dataset smaller than one batch still trains)
+    nb = -(-count // batch)
     for ep in range(epochs):
         perm = rng.permutation(count).astype(np.uint32)
         ep_loss = 0.0
         for b in range(nb):
-            idx_dev.upload(perm[b * batch:(b + 1) * batch])
-            ep_loss += model.train_step(fptr, tptr, cap, idx_dev.ptr, batch, lr)
-        losses.append(ep_loss / max(1, nb))
+            part = perm[b * batch:(b + 1) * batch]
+            idx_dev.upload(part)
+            ep_loss += model.train_step(fptr, tptr, cap, idx_dev.ptr, len(part), lr)
+        losses.append(ep_loss / nb)
         if log:
             log(f"epoch {ep}: mse {losses[-1]:.5f}")
     idx_dev.free()
