@@ -225,12 +225,18 @@ def full_run(ctx, sc, model=None):
 
 
 def trace_average(kernel: str, workload: str):
-    """Full-dispatch average (us) of a kernel in the committed rocprofv3
-    summary of this workload, if there is one."""
+    """Average duration (us) of a kernel in the committed rocprofv3 summary of
+    this workload: its corrector replay on the final state (what the
+    roofline times) when the summary has one, else its full dispatches."""
     f = ROOT / "profiles" / f"kernels_{workload}.txt"
     if not f.exists():
         return None
     text = f.read_text()
+    i = text.find("corrector passes replayed")
+    if i >= 0:
+        m = re.search(rf"^\S*k_{kernel}\S*.*?\s+(\d+)\s+([0-9.]+)\s*$", text[i:], flags=re.M)
+        if m:
+            return float(m.group(2))
     i = text.find("full dispatches")
     if i < 0:
         return None
@@ -347,11 +353,49 @@ def batched_leg(ctx, args, model):
         b.close()
     if "learned" in out:
         out["learned_vs_plain_batched"] = round(out["learned"]["value"] / out["plain"]["value"], 3)
+    # the same scenes and steps through the plain single-system path, one
+    # scene after another: the baseline the batched, warm-started serving
+    # path replaces
+    seq = plain_sequential(ctx, args)
+    out["plain_sequential"] = seq
+    for name in ("plain", "learned"):
+        if name in out:
+            out[f"{name}_batched_vs_plain_sequential"] = round(out[name]["value"] / seq["value"], 3)
     out["scenes"] = args.batch_scenes
     out["particles"] = sum(sc.n for sc in scenes.batched_scenes(num_scenes=args.batch_scenes))
     out["steps"] = args.batch_steps
     out["graphs"] = False
     return out
+
+
+def plain_sequential(ctx, args):
+    """The batched workload's scenes advanced one after another, each by the
+    plain single-system step (FluidSystem, zero start), over the same warm-up
+    and timed steps: total particle-steps over the summed timed time."""
+    from psb_project_amd import scenes
+    from psb_project_amd._lib import WARM_NONE
+    from psb_project_amd.system import FluidSystem
+    scs = scenes.batched_scenes(num_scenes=args.batch_scenes, warm_start=WARM_NONE)
+    total_t = 0.0
+    total_ps = 0
+    its = []
+    worst = 0.0
+    flags = 0
+    for sc in scs:
+        fs = FluidSystem(ctx, sc.params, sc.n)
+        fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
+        fs.update(args.batch_warmup)
+        elapsed, st = timed_run(ctx, fs, args.batch_steps)
+        fs.close()
+        total_t += elapsed
+        total_ps += sc.n * args.batch_steps
+        its.append(st["total_iterations"] / args.batch_steps)
+        worst = max(worst, st["worst_error"])
+        flags |= st["flags"]
+    return {"value": round(total_ps / total_t, 1), "seconds": round(total_t, 3),
+            "iters_per_step_mean": round(float(np.mean(its)), 3), "max_density_error": worst,
+            "flags": flags, "within_tolerance": worst <= scs[0].params.tolerance * (1 + 1e-6)
+            and (flags & 0x9) == 0}
 
 
 def dem_leg(ctx, args, model):

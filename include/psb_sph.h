@@ -135,10 +135,16 @@ typedef struct psb_sph_params {
                                whose corrector ends above the tolerance after
                                max_iterations is redone from its start state
                                as two steps of dt/2 (the plain corrector,
-                               p = 0 start), each of which may be redone as
-                               two of dt/4, down to dt / 2^retry_levels; the
+                               p = 0 start, delta halved as well), each of
+                               which may be redone as two of dt/4 (delta
+                               quartered), down to dt / 2^retry_levels; the
                                step still advances dt.  0 disables (the
-                               reference's fixed step).  At most 4.           */
+                               reference's fixed step).  At most 4.  In
+                               config mode a corrector whose error, after
+                               min_iterations and above the tolerance, grows
+                               past twice the smallest error of its earlier
+                               iterations stops there as failed (it diverges:
+                               iterating on would only build pressure).       */
 } psb_sph_params;
 
 #define PSB_WALL_CLAMP      0  /* box walls act only through the collider (clamp) */

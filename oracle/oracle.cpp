@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "../include/psb_sph.h"
@@ -45,6 +46,33 @@
 namespace {
 
 template <class R> struct V4 { R x, y, z, w; };
+
+// Loops over fewer items than this run on the calling thread: thread start-up
+// and barriers would cost more than the work (small test scenes).
+constexpr int64_t kOmpMinItems = 2048;
+
+// 1/sqrt(x) of the config-mode pressure force (x normal, > 0).  FP32: the
+// kernel's sequence, operation for operation -- the bit-pattern seed
+// 0x5F375A86 - (bits(x) >> 1) (C. Lomont, "Fast inverse square root", 2003)
+// and three Newton steps y <- y (3/2 - (x/2) y^2), each an fp32 multiply or
+// fma -- so both give the same bits.  FP64: 1/sqrt(x).
+inline float rsqrt_pair(float x) {
+  int32_t i;
+  std::memcpy(&i, &x, 4);
+  i = 0x5F375A86 - (i >> 1);
+  float y;
+  std::memcpy(&y, &i, 4);
+  const float hx = 0.5f * x;
+  y = y * std::fma(-(hx * y), y, 1.5f);
+  y = y * std::fma(-(hx * y), y, 1.5f);
+  y = y * std::fma(-(hx * y), y, 1.5f);
+  return y;
+}
+inline double rsqrt_pair(double x) { return 1.0 / std::sqrt(x); }
+// squared distances below this are taken at this value (self entry: dx = 0)
+constexpr float kMinPairD2 = 0x1p-96f;
+// the corrector's divergence stop: error above this factor times its smallest so far
+constexpr float kDivergeFactor = 2.0f;
 
 // ---------------------------------------------------------------------------
 // Derived constants.  Computed in double from the float parameters, then
@@ -58,6 +86,7 @@ template <class R> struct Derived {
   R beta;         // 2 (m dt / rho0)^2
   R delta;        // template delta
   R tait_b, inv_rho0, rho0, k, kp;   // kp = -m / rho0^2
+  R kps;          // kp * c_spiky, rounded once
   R cv, cs, cn;   // mu m c_visc, m c_surf_lap, m c_surf_grad
   R m, dt, sigma, l, gx, gy, gz;
   R box_lo[3], box_hi[3], one_plus_e;
@@ -113,6 +142,7 @@ template <class R> Derived<R> derive(const psb_sph_params& p) {
   d.rho0 = (R)rho0;
   d.k = (R)p.gas_stiffness;
   d.kp = (R)(-m / (rho0 * rho0));
+  d.kps = (R)((-m / (rho0 * rho0)) * (-45.0 / (pi * std::pow(h, 6))));
   d.cv = (R)((double)p.viscosity * m * 45.0 / (pi * std::pow(h, 6)));
   d.cs = (R)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));
   d.cn = (R)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));
@@ -284,7 +314,7 @@ uint32_t neighbors(const Grid& g, const uint32_t* start, uint32_t n, const V4<R>
                    R radius2, int Rs, uint32_t cap, uint32_t* count, uint32_t* list,
                    R* rho_sum, uint32_t* maxcnt) {
   uint32_t flags = 0, mx = 0;
-#pragma omp parallel for schedule(static) reduction(| : flags) reduction(max : mx)
+#pragma omp parallel for schedule(static) reduction(| : flags) reduction(max : mx) if(n >= kOmpMinItems)
   for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
     const uint32_t p = (uint32_t)pi;
     const V4<R> xi = xs[p];
@@ -661,7 +691,7 @@ template <class R> struct System {
       x4[4 * i + 3] = 0.f;
     }
     grid_build(g, n, x4.data(), hash.data(), perm.data(), start.data(), flags);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
       uint32_t i = perm[k];
       Ax[k] = Bx[i]; Av[k] = Bv[i]; Avh[k] = Bvh[i]; Aid[k] = Bid[i]; Ap[k] = Bp[i];
@@ -674,7 +704,7 @@ template <class R> struct System {
     flags |= neighbors<R>(g, start.data(), n, Ax.data(), d.h2, d.R_stencil,
                           (uint32_t)prm.max_neighbors, count.data(), list.data(), S.data(), &mx);
     max_nbr_seen = std::max(max_nbr_seen, mx);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t k = 0; k < (int64_t)n; ++k) rho[k] = S[k] * d.mc;
   }
 
@@ -697,7 +727,7 @@ template <class R> struct System {
     build_grid();
     neighbors_density();
     const bool surf = prm.surface_tension != 0.f;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
       R r = rho[k];
       R pr = eos(d, prm, r);
@@ -706,7 +736,7 @@ template <class R> struct System {
       Av[k].w = (R)1 / r;
     }
     uint32_t zf = 0;
-#pragma omp parallel for schedule(static) reduction(| : zf)
+#pragma omp parallel for schedule(static) reduction(| : zf) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       PairSums<R> s = surf ? internal_sums<R, true, true>(d, p, n, count.data(), list.data(), Ax.data(), Av.data())
@@ -741,12 +771,14 @@ template <class R> struct System {
   void pcisph_step() { pcisph_advance(0, false); }
 
   // Constants of a retry substep at level L: the step size dt / 2^L and
-  // everything derived from it (beta, the template delta, feature scales).
+  // everything derived from it (beta, the template delta, feature scales),
+  // with delta also relaxed by 2^-L (on top of params.relaxation).
   Derived<R> level_derived(int level) const {
     psb_sph_params p = prm;
     const float scale = std::ldexp(1.0f, -level);
     p.dt = prm.dt * scale;
     if (p.delta > 0.f) p.delta = prm.delta / (scale * scale);
+    p.relaxation = (prm.relaxation > 0.f ? prm.relaxation : 1.0f) * scale;
     return derive<R>(p);
   }
 
@@ -767,7 +799,7 @@ template <class R> struct System {
     std::vector<V4<R>>* cur = &xp0;
     std::vector<V4<R>>* nxt = &xp1;
     if (warm) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
       for (int64_t k = 0; k < (int64_t)n; ++k) (*cur)[k].w = pres[k];
       pci_force(*cur, *nxt, false);
       std::swap(cur, nxt);
@@ -781,11 +813,17 @@ template <class R> struct System {
       max_it = std::min(4 * prm.max_iterations, 255);
     int it = 0;
     R err = 0;
+    // smallest error of the earlier iterations: a config-mode corrector that
+    // grows past twice it (after min_it, above the tolerance) stops, failed
+    R min_prev = std::numeric_limits<R>::infinity();
     while (true) {
       err = pci_density(*cur, it == 0 && feat_now, it == 0 && mlp);
       if (it == 0 && mlp) mlp_pressure(*cur);
       it++;
-      bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it;
+      const bool diverged = !ref && it >= min_it && err > (R)prm.tolerance &&
+                            err > (R)kDivergeFactor * min_prev;
+      bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it || diverged;
+      min_prev = std::min(min_prev, err);
       pci_force(*cur, *nxt, fin);
       if (fin) break;
       std::swap(cur, nxt);
@@ -828,7 +866,7 @@ template <class R> struct System {
     const bool mlp = !ref && !plain && prm.warm_start == PSB_WARM_MLP;
     const bool feat_now = !ref && !plain && (mlp || feat_on);
     // initial pressure (PciSphSystem_Kernel.cu:131 resets it to zero)
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t k = 0; k < (int64_t)n; ++k) {
       if (!ref && bnd((uint32_t)k)) {      // sphere boundary particle: p = 0, zero volume
         pres[k] = 0;
@@ -847,7 +885,7 @@ template <class R> struct System {
     }
     // non-pressure forces + first prediction (PciSphSystem_Kernel.cu:117-170,
     // SphKernel.cu:95-109)
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       if (!ref && bnd(p)) {               // static during the corrector
@@ -900,7 +938,7 @@ template <class R> struct System {
   void bodies_step(const std::vector<V4<R>>& X) {
     const uint32_t nb = n_spheres * per_sphere;
     std::vector<V4<R>> fbb(nb);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       if (!bnd(p)) continue;
@@ -970,7 +1008,7 @@ template <class R> struct System {
     }
     sph_c = nc;
     sph_v = nv;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       if (!bnd(p)) continue;
@@ -1071,7 +1109,7 @@ template <class R> struct System {
     std::vector<float> f(8 * (size_t)n);
     for (size_t i = 0; i < f.size(); ++i) f[i] = (float)feat[i];
     Mlp m = make_mlp(mlp_w.data(), mlp_mu, mlp_sd, true);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       double x[kIn], z1[kHid], h1[kHid], z2[kHid], h2[kHid];
       m.input(f.data(), n, (uint64_t)kk, x);
@@ -1087,7 +1125,7 @@ template <class R> struct System {
   R pci_density(std::vector<V4<R>>& X, bool write_feature = false, bool skip_update = false) {
     const bool ref = prm.mode == PSB_MODE_REFERENCE;
     R emax = 0;
-#pragma omp parallel for schedule(static) reduction(max : emax)
+#pragma omp parallel for schedule(static) reduction(max : emax) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       if (!ref && bnd(p)) {               // sphere boundary particle: no pressure, no error
@@ -1157,7 +1195,7 @@ template <class R> struct System {
   // start state (restore_Pos_Vel_PCI, :190-202).
   void pci_force(std::vector<V4<R>>& X, std::vector<V4<R>>& Xn, bool fin) {
     const bool ref = prm.mode == PSB_MODE_REFERENCE;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       const V4<R> xi = X[p];
@@ -1182,21 +1220,18 @@ template <class R> struct System {
             sx2[a] = fmaR(dx * invd, c, sx2[a]); sy2[a] = fmaR(dy * invd, c, sy2[a]);
             sz2[a] = fmaR(dz * invd, c, sz2[a]);
           }
-        } else if (d2 > (R)0 && d2 < d.h2) {
-          R dd = std::sqrt(d2);
-          R invd = (R)1 / dd;
-          R hd = d.h - dd;
-          R gw = d.c_spiky * (hd * hd);
-          R c = (gw * invd) * (xi.w + xj.w);
+        } else {
+          // (h - d)^2 / d (p_i + p_j), r = 1/d, d = d2 r; zero past h through
+          // the clamp, zero for the self entry (dx = 0); c_spiky kp applied to
+          // the sum
+          const R d2c = std::max(d2, (R)kMinPairD2);
+          const R r = rsqrt_pair(d2c);
+          const R hd = std::max(fmaR(-d2c, r, d.h), (R)0);
+          const R c = ((hd * hd) * r) * (xi.w + xj.w);
           sx2[a] = fmaR(dx, c, sx2[a]); sy2[a] = fmaR(dy, c, sy2[a]); sz2[a] = fmaR(dz, c, sz2[a]);
         }
       }
       R sx = sx2[0] + sx2[1], sy = sy2[0] + sy2[1], sz = sz2[0] + sz2[1];
-      if (!ref && d.wall) {   // wall particles push with p_i only (p_b = 0)
-        R Sw, gx_, gy_, gz_;
-        wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, gx_, gy_, gz_);
-        sx = fmaR(gx_, xi.w, sx); sy = fmaR(gy_, xi.w, sy); sz = fmaR(gz_, xi.w, sz);
-      }
       const V4<R> x0 = Ax[p], vh0 = Avh[p];
       V4<R> vh, x;
       if (ref) {
@@ -1209,7 +1244,14 @@ template <class R> struct System {
         vh = {vh0.x + (F0 * inv) * d.dt, vh0.y + (F1 * inv) * d.dt, vh0.z + (F2 * inv) * d.dt, 0};
       } else {
         const V4<R> a = anp[p];
-        R a0 = a.x + sx * d.kp, a1 = a.y + sy * d.kp, a2 = a.z + sz * d.kp;
+        R fP0 = sx * d.kps, fP1 = sy * d.kps, fP2 = sz * d.kps;
+        if (d.wall) {   // wall particles push with p_i only (p_b = 0)
+          R Sw, gx_, gy_, gz_;
+          wall_sums(d, false, true, xi.x, xi.y, xi.z, Sw, gx_, gy_, gz_);
+          const R kpw = d.kp * xi.w;
+          fP0 = fmaR(gx_, kpw, fP0); fP1 = fmaR(gy_, kpw, fP1); fP2 = fmaR(gz_, kpw, fP2);
+        }
+        R a0 = a.x + fP0, a1 = a.y + fP1, a2 = a.z + fP2;
         vh = {vh0.x + a0 * d.dt, vh0.y + a1 * d.dt, vh0.z + a2 * d.dt, 0};
       }
       x = {x0.x + vh.x * d.dt, x0.y + vh.y * d.dt, x0.z + vh.z * d.dt, 0};
@@ -1450,7 +1492,7 @@ void surface_extract(const float* x4, const uint32_t* ids, uint32_t n, uint32_t 
   };
   const uint64_t NN = (uint64_t)NX * NY * NZ;
   o.field.assign(NN, 0.f);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(NN >= kOmpMinItems)
   for (int64_t t = 0; t < (int64_t)NN; ++t) {
     const int i = (int)(t % NX), j = (int)((t / NX) % NY), k = (int)(t / ((int64_t)NX * NY));
     float x, y, z;
@@ -1486,7 +1528,7 @@ void surface_extract(const float* x4, const uint32_t* ids, uint32_t n, uint32_t 
   vbase[NN] = nv;
   o.verts.assign(3 * (size_t)nv, 0.f);
   o.norms.assign(3 * (size_t)nv, 0.f);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(NN >= kOmpMinItems)
   for (int64_t t = 0; t < (int64_t)NN; ++t) {
     if (!ef[t]) continue;
     const int i = (int)(t % NX), j = (int)((t / NX) % NY), k = (int)(t / SZ);
@@ -1559,6 +1601,11 @@ void surface_extract(const float* x4, const uint32_t* ids, uint32_t n, uint32_t 
 // C entry points (ctypes).  All arrays are host memory.
 // ===========================================================================
 extern "C" {
+
+// rsqrt_pair on n inputs (the pressure force's 1/sqrt)
+void oracle_rsqrt_pair(const float* x, float* out, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = rsqrt_pair(x[i]);
+}
 
 int oracle_derive(const psb_sph_params* p, double* out /* 16 */) {
   Derived<double> d = derive<double>(*p);
@@ -1679,7 +1726,7 @@ int oracle_interaction_system(int mode, uint32_t n, const float* pos4, const flo
                               const float* damping, const uint32_t* count, const uint32_t* list) {
   const bool ref = mode == PSB_MODE_REFERENCE;
   const float zero[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
   for (int64_t aa = 0; aa < (int64_t)n; ++aa) {
     const uint32_t a = (uint32_t)aa;
     const float* xa = pos4 + 4 * (size_t)a;
@@ -1937,7 +1984,7 @@ double oracle_system_pressure_scale(void* hp) {
 int oracle_mlp_forward(const float* w, const float* mu, const float* sd, const float* feat,
                        uint64_t n, float* out, int emulate_bf16) {
   Mlp m = make_mlp(w, mu, sd, emulate_bf16 != 0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
   for (int64_t s = 0; s < (int64_t)n; ++s) {
     double x[kIn], z1[kHid], h1[kHid], z2[kHid], h2[kHid];
     m.input(feat, n, (uint64_t)s, x);

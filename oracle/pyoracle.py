@@ -33,6 +33,8 @@ def lib():
         L = C.CDLL(str(LIB))
         vp = C.c_void_p
         L.oracle_derive.argtypes = [vp, vp]
+        L.oracle_rsqrt_pair.argtypes = [vp, vp, C.c_uint32]
+        L.oracle_rsqrt_pair.restype = None
         L.oracle_grid_build.restype = C.c_uint32
         L.oracle_grid_build.argtypes = [C.c_uint32, vp, vp, vp, vp, vp]
         L.oracle_neighbors.restype = C.c_uint32
@@ -83,6 +85,14 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data
+
+
+def rsqrt_pair(x) -> np.ndarray:
+    """The pressure force's 1/sqrt on FP32 inputs (normal, > 0)."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.empty_like(x)
+    lib().oracle_rsqrt_pair(_p(x), _p(out), x.size)
+    return out
 
 
 def derive(params) -> np.ndarray:

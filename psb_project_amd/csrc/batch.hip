@@ -333,6 +333,10 @@ static psb_status batch_level_consts(psb_batch b, int level, Consts* out) {
   const float scale = std::ldexp(1.0f, -level);
   p.dt = b->prm.dt * scale;
   if (p.delta > 0.f) p.delta = b->prm.delta / (scale * scale);
+  // the corrector under-relaxed with the step: a step that failed over-
+  // corrects as readily at any dt (the pressure's displacement scales with
+  // delta dt^2), so each level halves delta on top of params.relaxation
+  p.relaxation = (b->prm.relaxation > 0.f ? b->prm.relaxation : 1.0f) * scale;
   PSB_TRY(make_consts(p, out, nullptr));
   out->retry_ok = level < b->prm.retry_levels ? 1 : 0;
   if (level == b->prm.retry_levels) out->max_it = retry_max_iterations(b->prm);
@@ -644,8 +648,8 @@ psb_status psb_batch_create(psb_context ctx, const psb_sph_params* material, uin
     sc.n = d.count;
     cells += (uint64_t)d.grid.dims[0] * d.grid.dims[1] * d.grid.dims[2];
     parts += d.count;
-    if (cells >= (1ull << 31) || parts >= (1ull << 31)) {
-      set_error("batch exceeds 2^31 cells or particles");
+    if (cells >= (1ull << 31) || parts > kMaxParticles) {
+      set_error("batch exceeds 2^31 cells or 2^27 particles");
       stt = PSB_ERR_INVALID_ARGUMENT;
       break;
     }

@@ -14,8 +14,16 @@ struct PciStatus {
   unsigned long long total_iters;  // iterations since reset
   uint32_t wide;                   // 1: some compact neighbour offset overflowed this step
   uint32_t pad;
+  // float bits: smallest error of iterations 0..k, written by F(k) into
+  // [k & 1] and read by F(k + 1) from there (each launch reads the other slot)
+  uint32_t min_err[2];
   uint32_t err[kMaxIterSlots];     // float bits: max error of D(k)
 };
+
+// A config-mode corrector stops as failed once its error, after
+// min_iterations and above the tolerance, exceeds this factor times the
+// smallest error of its earlier iterations.
+constexpr float kDivergeFactor = 2.0f;
 
 // Per-step reset of one scene's status and error slots, by one workgroup.
 __device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, uint32_t* done,
@@ -29,6 +37,7 @@ __device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, 
   if (threadIdx.x == 0) {
     st->final_iter = -1;
     if (!keep_wide) st->wide = 0u;
+    st->min_err[0] = st->min_err[1] = 0x7F800000u;   // +inf
   }
   if (done && scene == 0 && threadIdx.x == 0) *done = 0u;
 }

@@ -380,7 +380,11 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
       err = __uint_as_float(st->err[k]);
     }
     const int it = k + 1;
-    fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it;
+    // smallest error of iterations 0 .. k-1 (+inf at k = 0)
+    const float minp = __uint_as_float(st->min_err[(k + 1) & 1]);
+    const bool diverged = !REF && it >= c.min_it && err > c.tolerance && err > kDivergeFactor * minp;
+    fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it || diverged;
+    if (lead && !REF) st->min_err[k & 1] = __float_as_uint(fminf(minp, err));
     if (fin && lead) {
       st->final_iter = k;
       st->last_iters = it;
@@ -434,12 +438,14 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
         sy2[a] = __builtin_fmaf(dy * invd, cc, sy2[a]);
         sz2[a] = __builtin_fmaf(dz * invd, cc, sz2[a]);
       }
-    } else if (d2 > 0.f && d2 < c.h2) {
-      float dd, invd;
-      dist_and_inverse(d2, dd, invd);
-      const float hd = c.h - dd;
-      const float gw = c.c_spiky * (hd * hd);
-      const float cc = (gw * invd) * (xi.w + wj);
+    } else {
+      // (h - d)^2 / d (p_i + p_j) with r = 1/d from rsqrt_nr and d = d2 r;
+      // no branch: past h the clamp makes (h - d) zero, and the self entry
+      // (dx = 0) adds zero.  c_spiky kp multiplies the sum afterwards.
+      const float d2c = fmaxf(d2, kMinPairD2);
+      const float r = rsqrt_nr(d2c);
+      const float hd = fmaxf(__builtin_fmaf(-d2c, r, c.h), 0.f);
+      const float cc = ((hd * hd) * r) * (xi.w + wj);
       sx2[a] = __builtin_fmaf(dx, cc, sx2[a]);
       sy2[a] = __builtin_fmaf(dy, cc, sy2[a]);
       sz2[a] = __builtin_fmaf(dz, cc, sz2[a]);
@@ -454,15 +460,6 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   float sx = split_total<L>(sx2, lane), sy = split_total<L>(sy2, lane),
         sz = split_total<L>(sz2, lane);
   if (lane != 0) return;   // the pair's second lane only summed
-  if (!REF && c.wall) {
-    // wall particles push with the particle's own pressure only (p_b = 0:
-    // F = -m psi_b p_i / rho0^2 gradW, psi_b = m on the fluid lattice)
-    float S, gx, gy, gz;
-    wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, gx, gy, gz);
-    sx = __builtin_fmaf(gx, xi.w, sx);
-    sy = __builtin_fmaf(gy, xi.w, sy);
-    sz = __builtin_fmaf(gz, xi.w, sz);
-  }
   const float4 x0 = b.ax[p], vh0 = b.avh[p];
   float4 vh;
   float fP0 = 0.f, fP1 = 0.f, fP2 = 0.f;
@@ -477,7 +474,17 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
                      vh0.z + (F2 * inv) * c.dt, 0.f);
   } else {
     const float4 a = b.anp[p];
-    fP0 = sx * c.kp; fP1 = sy * c.kp; fP2 = sz * c.kp;
+    fP0 = sx * c.kps; fP1 = sy * c.kps; fP2 = sz * c.kps;
+    if (c.wall) {
+      // wall particles push with the particle's own pressure only (p_b = 0:
+      // F = -m psi_b p_i / rho0^2 gradW, psi_b = m on the fluid lattice)
+      float S, gx, gy, gz;
+      wall_sums<false, true>(c, xi.x, xi.y, xi.z, S, gx, gy, gz);
+      const float kpw = c.kp * xi.w;
+      fP0 = __builtin_fmaf(gx, kpw, fP0);
+      fP1 = __builtin_fmaf(gy, kpw, fP1);
+      fP2 = __builtin_fmaf(gz, kpw, fP2);
+    }
     float a0 = a.x + fP0, a1 = a.y + fP1, a2 = a.z + fP2;
     vh = make_float4(vh0.x + a0 * c.dt, vh0.y + a1 * c.dt, vh0.z + a2 * c.dt, 0.f);
   }

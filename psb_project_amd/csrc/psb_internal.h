@@ -49,6 +49,7 @@ constexpr uint32_t kScanTile = 4096;      // cells per scan workgroup (256 x 16)
 constexpr int kMaxIterSlots = 256;        // PCISPH iteration slots per step
 constexpr int kMaxRetryLevels = 4;        // dt halvings of a failed PCISPH step
 constexpr int kRetryIterFactor = 4;       // iteration cap of the deepest retry level
+constexpr uint32_t kMaxParticles = 1u << 27;  // particles per system or batch (gather4)
 // corrector iteration cap of the deepest retry level (params.retry_levels)
 inline int retry_max_iterations(const psb_sph_params& p) {
   const int m = p.max_iterations * kRetryIterFactor;
@@ -80,6 +81,7 @@ struct ColliderDev {
 
 struct Consts {
   float h, h2, h2x3, c_spiky, mc, beta, delta, tait_b, inv_rho0, rho0, k, kp;
+  float kps;   // kp * c_spiky (config-mode pressure force, applied to the pair sum)
   float cv, cs, cn, m, dt, sigma, l, gx, gy, gz;
   float box_lo[3], box_hi[3], one_plus_e;
   float tolerance;
@@ -604,6 +606,29 @@ __device__ __forceinline__ void dist_and_inverse(float d2, float& dd, float& inv
   }
 }
 
+// 1/sqrt(x) for normal x > 0 from fp32 adds, multiplies and fmas only, so a
+// CPU restatement computes the same bits: the bit-pattern seed
+// 0x5F375A86 - (bits(x) >> 1) (C. Lomont, "Fast inverse square root", 2003;
+// relative error below 3.5 %) and three Newton steps y <- y (3/2 - (x/2) y^2)
+// (error 3.5e-2 -> 1.8e-3 -> 4.7e-6 -> fp32 rounding).  No transcendental
+// unit, no compare or select: 12 full-rate VALU instructions.
+__host__ __device__ __forceinline__ float rsqrt_nr(float x) {
+  int32_t i;
+  __builtin_memcpy(&i, &x, 4);
+  i = 0x5F375A86 - (i >> 1);
+  float y;
+  __builtin_memcpy(&y, &i, 4);
+  const float hx = 0.5f * x;
+  y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
+  y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
+  y = y * __builtin_fmaf(-(hx * y), y, 1.5f);
+  return y;
+}
+// Squared distances below this are taken at this value in the config-mode
+// pressure force, so rsqrt_nr sees a normal input and the self entry
+// (d2 = 0, dx = 0) adds exactly zero.
+constexpr float kMinPairD2 = 0x1p-96f;
+
 // Per-step maxima reported by many workgroups, spread over kErrSlots words
 // kErrSlotStride or more apart (different memory channels): a workgroup
 // takes slot blockIdx % kErrSlots, so the atomicMax do not serialise on one
@@ -693,6 +718,14 @@ __device__ __forceinline__ void unpack_l16(const uint4& c, uint32_t* e) {
   e[4] = c.z & 0xFFFFu; e[5] = c.z >> 16; e[6] = c.w & 0xFFFFu; e[7] = c.w >> 16;
 }
 
+// X[i] through a 32-bit byte offset (i < 2^27 particles: every system and
+// batch is created below that), so the gather takes the array's base from
+// scalar registers and the offset from one VGPR: one shift-add per
+// neighbour instead of a 64-bit address computation.
+__device__ __forceinline__ float4 gather4(const float4* X, uint32_t i) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(X) + (i << 4));
+}
+
 template <int LANES, int U, class LI, class F>
 __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint32_t m,
                                                     uint32_t lane, const LI& list,
@@ -715,7 +748,7 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
       unpack_l16(cur, e);
       float4 xq[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) xq[u] = X[list.decode(e[u], p)];
+      for (int u = 0; u < 8; ++u) xq[u] = gather4(X, list.decode(e[u], p));
 #pragma unroll
       for (int u = 0; u < 8; ++u) f(u & 1, xq[u]);
     }
@@ -725,7 +758,8 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
       const uint32_t r = m - nfull * kL16Group;   // 1..7 entries left
       float4 xq[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) xq[u] = X[(uint32_t)u < r ? list.decode(e[u], p) : list.safe(p)];
+      for (int u = 0; u < 8; ++u)
+        xq[u] = gather4(X, (uint32_t)u < r ? list.decode(e[u], p) : list.safe(p));
 #pragma unroll
       for (int u = 0; u < 8; ++u)
         if ((uint32_t)u < r) f(u & 1, xq[u]);
@@ -735,28 +769,29 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
     // word, lane l takes its entries k = l (mod 2) (4 gathers per group)
     const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
     const uint32_t ng = (m + kL16Group - 1) / kL16Group, nfull = m / kL16Group;
+    const uint32_t sh = lane << 4;   // this lane's half of each 32-bit pair of entries
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     if (ng) w = g4[p];
     for (uint32_t g = 0; g < nfull; ++g) {
       const uint4 cur = w;
       if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
-      uint32_t e[8];
-      unpack_l16(cur, e);
+      const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
       float4 xq[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xq[u] = X[list.decode(lane ? e[2 * u + 1] : e[2 * u], p)];
+      for (int u = 0; u < 4; ++u)
+        xq[u] = gather4(X, list.decode(__builtin_amdgcn_ubfe(wd[u], sh, 16), p));
 #pragma unroll
       for (int u = 0; u < 4; ++u) f(0, xq[u]);
     }
     if (nfull < ng) {
-      uint32_t e[8];
-      unpack_l16(w, e);
+      const uint32_t wd[4] = {w.x, w.y, w.z, w.w};
       const uint32_t r = m - nfull * kL16Group;   // 1..7 entries left
       float4 xq[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = 2u * u + lane;
-        xq[u] = X[j < r ? list.decode(lane ? e[2 * u + 1] : e[2 * u], p) : list.safe(p)];
+        xq[u] = gather4(X, j < r ? list.decode(__builtin_amdgcn_ubfe(wd[u], sh, 16), p)
+                                 : list.safe(p));
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)

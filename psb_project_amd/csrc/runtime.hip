@@ -166,6 +166,7 @@ psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g) {
   c->rho0 = (float)rho0;
   c->k = p.gas_stiffness;
   c->kp = (float)(-m / (rho0 * rho0));
+  c->kps = (float)((-m / (rho0 * rho0)) * (-45.0 / (pi * std::pow(h, 6))));
   c->cv = (float)((double)p.viscosity * m * 45.0 / (pi * std::pow(h, 6)));
   c->cs = (float)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));
   c->cn = (float)(m * -945.0 / (32.0 * pi * std::pow(h, 9)));

@@ -248,17 +248,19 @@ static Lanes corrector_lanes(psb_system s) {
 
 // Constants of a retry substep at level L (dt / 2^L): everything make_consts
 // derives from dt (beta, the template delta, the warm-start feature scales)
-// recomputed, the system's patched fields (bodies, colliders) kept.  The
-// deepest level may iterate up to kRetryIterFactor x max_iterations: halving
-// dt does not help a particle whose density barely responds to pressure (a
-// near-coincident pair: the poly6 density is flat in d at d -> 0, while the
-// spiky pressure force is not), which only more iterations separate.
+// recomputed, delta relaxed by 2^-L, the system's patched fields (bodies,
+// colliders) kept.  The deepest level may iterate up to kRetryIterFactor x
+// max_iterations: its relaxed corrector converges more slowly.
 static psb_status level_consts(psb_system s, int level, Consts* out) {
   if (level == 0) { *out = s->c; return PSB_OK; }
   psb_sph_params p = s->prm;
   const float scale = std::ldexp(1.0f, -level);
   p.dt = s->prm.dt * scale;
   if (p.delta > 0.f) p.delta = s->prm.delta / (scale * scale);
+  // the corrector under-relaxed with the step: a step that failed over-
+  // corrects as readily at any dt (the pressure's displacement scales with
+  // delta dt^2), so each level halves delta on top of params.relaxation
+  p.relaxation = (s->prm.relaxation > 0.f ? s->prm.relaxation : 1.0f) * scale;
   PSB_TRY(make_consts(p, out, nullptr));
   out->n_fluid = s->c.n_fluid;
   out->coll = s->c.coll;
@@ -469,6 +471,7 @@ psb_status psb_system_create(psb_context ctx, const psb_sph_params* prm, uint32_
                              psb_system* out) {
   PSB_ARG(ctx && prm && out, "null argument");
   *out = nullptr;
+  PSB_ARG(capacity <= kMaxParticles, "capacity above 2^27 particles");
   psb_system s = new psb_system_s();
   s->ctx = ctx;
   s->prm = *prm;

@@ -8,9 +8,10 @@ traffic counters.
         writes exactly mib MiB, so FETCH_SIZE / WRITE_SIZE of that kernel give
         the counter-to-bytes factor;
     python -m psb_project_amd.profiling kernel-stats STATS_CSV --command CMD --out FILE
-            [--trace KERNEL_TRACE_CSV]
+            [--trace KERNEL_TRACE_CSV] [--replay N]
         formats rocprofv3 --stats kernel_stats.csv (top kernels by time) and,
-        from the per-dispatch trace, the average over full dispatches;
+        from the per-dispatch trace, the average over full dispatches (and
+        over the last N dispatches: the bench's corrector replay);
     python -m psb_project_amd.profiling counters CSV [CSV ...] --kernel REGEX --cus N
             --commands TEXT --out FILE
         per-kernel averages of every counter in rocprofv3 --pmc counter
@@ -96,7 +97,8 @@ def cmd_kernel_stats(a) -> int:
         # average over full dispatches can be set beside the benchmark's.
         per = defaultdict(list)
         rx = re.compile(a.split_regex)
-        for r in _rows(a.trace):
+        trace = sorted(_rows(a.trace), key=lambda r: int(r["Start_Timestamp"]))
+        for r in trace:
             if not rx.search(r["Kernel_Name"]):
                 continue
             name = re.sub(r"\(.*$", "", r["Kernel_Name"]).replace("void ", "")
@@ -113,6 +115,16 @@ def cmd_kernel_stats(a) -> int:
                        f"{len(early):7d} {eavg:9.3f}")
             if len(out) > 60:
                 break
+        if a.replay:
+            # bench.py ends its timed run by replaying the corrector passes
+            # back to back on the final state (its roofline): the last
+            # `replay` dispatches of each split kernel
+            out += ["", f"corrector passes replayed on the final state (the last {a.replay} "
+                        f"dispatches of each):",
+                    f"{'kernel':60s} {'replay':>7s} {'avg_us':>9s}"]
+            for name, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+                last = d[-a.replay:]
+                out.append(f"{name[:60]:60s} {len(last):7d} {sum(last) / len(last):9.3f}")
     out.append("")
     text = "\n".join(out)
     with open(a.out, "w") as f:
@@ -228,6 +240,8 @@ def main(argv=None) -> int:
     k.add_argument("--full-min-us", type=float, default=5.0)
     k.add_argument("--split-regex", default=r"k_pci_(density|force)",
                    help="kernels whose early-exit launches are split off")
+    k.add_argument("--replay", type=int, default=0,
+                   help="average the last N dispatches of each split kernel separately")
     k.add_argument("--out", required=True)
     q = sub.add_parser("counters")
     q.add_argument("csv", nargs="+")

@@ -4,6 +4,8 @@ from the file in a new system agree bit for bit afterwards -- with the
 previous-pressure warm start (pressure carries over steps) and with rigid
 DEM spheres (sphere state and boundary particles) -- and bad files or
 mismatched systems are refused."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -120,3 +122,102 @@ def test_set_state_restores_storage_order(ctx):
         b.set_state(st["pos"], ids=np.zeros(sc.n, np.uint32))
     a.close()
     b.close()
+
+
+# ------------------------------------------------------------ resume on the oracle
+def read_checkpoint(path):
+    """The checkpoint's header and per-particle arrays, read independently of
+    the library: header (magic PSBCKPT1, version, the sizes of the three
+    records that follow, counts, step counter), psb_sph_params, sphere
+    parameters, statistics, a presence mask, then the arrays in storage order
+    (B and A positions, velocities, half-step velocities as float4; B and A
+    ids, B and A pressures, densities, predicted densities, pressures)."""
+    import ctypes as C
+    raw = Path(path).read_bytes()
+    assert raw[:8] == b"PSBCKPT1"
+    version, pbytes, sbytes, tbytes, n, n_fluid, n_sph, per, stepped, solver = \
+        np.frombuffer(raw, np.uint32, 10, 8)
+    off = 8 + 40 + 8
+    from psb_project_amd._lib import Params
+    assert pbytes == C.sizeof(Params)
+    params = Params.from_buffer_copy(raw[off:off + pbytes])
+    off += int(pbytes + sbytes + tbytes)
+    mask = int(np.frombuffer(raw, np.uint32, 1, off)[0]); off += 4
+    names = [("bx", 4), ("bv", 4), ("bvh", 4), ("ax", 4), ("av", 4), ("avh", 4), ("bid", 1),
+             ("aid", 1), ("bp", 1), ("ap", 1), ("rho", 1), ("rho_pred", 1), ("pres", 1)]
+    out = {"params": params, "n": int(n)}
+    for i, (name, w) in enumerate(names):
+        if not mask & (1 << i):
+            continue
+        dt = np.uint32 if name.endswith("id") else np.float32
+        a = np.frombuffer(raw, dt, int(n) * w, off)
+        out[name] = a.reshape(int(n), 4) if w == 4 else a
+        off += 4 * int(n) * w
+    return out
+
+
+def oracle_from_storage(oracle, params, x4, v4, vh4):
+    o = oracle.OracleSystem(params, x4.shape[0])
+    o.set_particles(np.ascontiguousarray(x4), np.ascontiguousarray(v4), np.ascontiguousarray(vh4))
+    return o
+
+
+@pytest.mark.parametrize("solver", ["pcisph", "wcsph"])
+def test_checkpoint_resumes_on_the_oracle(ctx, oracle, tmp_path, solver):
+    """A checkpoint the GPU saved at step 15, read by this test's own parser
+    and fed to the CPU oracle in its storage order, continues exactly as the
+    GPU run does: 10 more steps bit for bit (positions, velocities,
+    densities, iteration counts), ids mapped through the file's id array."""
+    sc = (scenes.pcisph_dam_break(n_side=16) if solver == "pcisph"
+          else scenes.wcsph_dam_break(n_side=14))
+    fs = FluidSystem(ctx, sc.params, sc.n)
+    fs.set_particles(sc.pos4)
+    fs.update(15)
+    path = tmp_path / "s.ckpt"
+    fs.save_checkpoint(path)
+    fs.update(10)
+    g = fs.state()
+    ck = read_checkpoint(path)
+    assert ck["n"] == sc.n and bytes(ck["params"]) == bytes(sc.params)
+    o = oracle_from_storage(oracle, ck["params"], ck["bx"], ck["bv"], ck["bvh"])
+    o.update(10)
+    c = o.state()
+    for k in ("pos", "vel", "vh"):
+        assert np.array_equal(g[k][:, :3], c[k][:, :3]), k
+    assert np.array_equal(g["density"], c["density"])
+    assert np.array_equal(g["ids"], ck["bid"][c["ids"]])
+    if solver == "pcisph":
+        assert fs.stats()["total_iterations"] == o.stats()["total_iterations"]
+    o.close()
+    fs.close()
+
+
+def test_xml_export_resumes_on_the_oracle(ctx, oracle, tmp_path):
+    """The C++ host API's ParticleExporter_XML file after 10 steps, parsed
+    here and fed to the oracle, reaches after 10 more steps the positions
+    and velocities of the straight 20-step C++ run, by particle index."""
+    import re
+    import subprocess
+    from test_gpu_host_api import EXE, read_record
+    straight, half, xml = tmp_path / "a.bin", tmp_path / "h.bin", tmp_path / "s.xml"
+    for args in (["--steps", "20", "--out", str(straight)],
+                 ["--steps", "10", "--out", str(half), "--export-xml", str(xml)]):
+        r = subprocess.run([str(EXE), "--side", "10", "--config", *args], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    prm, n, _, x_ref, v_ref, _ = read_record(straight)
+    text = xml.read_text()
+    idx = np.array([int(m) for m in re.findall(r'<Particle index="(\d+)"', text)])
+    vec = lambda tag: np.array(re.findall(rf'<{tag} X="([^"]+)" Y="([^"]+)" Z="([^"]+)"', text),
+                               np.float64).astype(np.float32)
+    x, v, vh = vec("position"), vec("velocity"), vec("velocityInterAp")
+    assert len(idx) == n and x.shape == (n, 3)
+    pad = lambda a: np.concatenate([a, np.zeros((n, 1), np.float32)], axis=1)
+    o = oracle_from_storage(oracle, prm, pad(x), pad(v), pad(vh))
+    o.update(10)
+    c = o.state()
+    orig = idx[c["ids"]]                      # oracle id = position in the file
+    order = np.argsort(orig)
+    assert np.array_equal(c["pos"][order, :3], x_ref[:, :3])
+    assert np.array_equal(c["vel"][order, :3], v_ref[:, :3])
+    o.close()

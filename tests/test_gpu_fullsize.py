@@ -1,8 +1,10 @@
 """Full-size workloads against their references: the DEM double dam break
 (2,097,152 fluid particles, 512 spheres, lattice walls) for two steps
-against the oracle, and the 64-scene batch (2.26 M particles) against the
-same scenes run one by one -- bit for bit, so the maximum sizes exercise the
-same code paths the small cases pin."""
+against the oracle, the 64-scene batch (2.26 M particles) against the same
+scenes run one by one and three of them against the oracle, and the
+262,144-particle PCISPH dam break for 1000 steps against the oracle -- bit
+for bit, so the maximum sizes exercise the same code paths the small cases
+pin."""
 import numpy as np
 import pytest
 
@@ -51,7 +53,10 @@ def test_dem_config3_full_two_steps(ctx, oracle):
     o.close()
 
 
-def test_batch_config5_full_two_steps(ctx):
+def test_batch_config5_full_two_steps(ctx, oracle):
+    """Every scene of the 64-scene batch equals the scene run alone; the
+    smallest, the median and the largest scene also equal the CPU oracle
+    run on that scene directly."""
     scs = scenes.batched_scenes(num_scenes=64, warm_start=WARM_NONE)
     assert 2_000_000 < sum(s.n for s in scs) < 2_600_000
     b = BatchSystem(ctx, scs)
@@ -65,4 +70,40 @@ def test_batch_config5_full_two_steps(ctx):
         same(b.state(i), fs.state(), f"scene {i}")
         assert b.stats(i)["total_iterations"] == fs.stats()["total_iterations"]
         fs.close()
+    by_size = sorted(range(len(scs)), key=lambda i: scs[i].n)
+    for i in (by_size[0], by_size[len(by_size) // 2], by_size[-1]):
+        sc = scs[i]
+        o = oracle.OracleSystem(sc.params, sc.n)
+        o.set_particles(sc.pos4, sc.vel4, sc.vh4)
+        o.update(2)
+        same(b.state(i), o.state(), f"scene {i} (n={sc.n}) against the oracle")
+        assert b.stats(i)["total_iterations"] == o.stats()["total_iterations"]
+        o.close()
     b.close()
+
+
+def test_pcisph_config2_1000_steps_against_oracle(ctx, oracle):
+    """The 262,144-particle PCISPH dam break for 1000 steps (tolerance loop,
+    retries as configured) against the oracle on the same input: bit for bit
+    at steps 1, 10, 100 and 1000, with the same corrector iterations and
+    retries.  (The workload's contract is 1e-5 relative per step and 1e-3
+    after 1000 steps; equality is stronger.)"""
+    sc = scenes.pcisph_dam_break()
+    assert sc.n == 262_144
+    g = FluidSystem(ctx, sc.params, sc.n)
+    o = oracle.OracleSystem(sc.params, sc.n)
+    for s_ in (g, o):
+        s_.set_particles(sc.pos4, sc.vel4, sc.vh4)
+    done = 0
+    for upto in (1, 10, 100, 1000):
+        g.update(upto - done)
+        o.update(upto - done)
+        done = upto
+        gs, os_ = g.stats(), o.stats()
+        for k in ("total_iterations", "retried_steps", "substeps", "flags"):
+            assert gs[k] == os_[k], (upto, k, gs[k], os_[k])
+        assert gs["worst_error"] == np.float32(os_["worst_error"])
+        same(g.state(), o.state(), f"step {upto}")
+        print(f"config 2 step {upto}: bitwise, {gs['total_iterations']} iterations", flush=True)
+    g.close()
+    o.close()
