@@ -55,6 +55,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# batch steps whose per-scene latency is recorded (events after every force
+# pass); the others pace their queue without events
+LATENCY_EVERY = 4
 
 # Algorithmic (compulsory) bytes per particle per launch of each kernel:
 # FP32 streams, each array read and written once, neighbour gathers assumed
@@ -91,6 +94,8 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the batched scenes run one by one through the single-system step")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the 2000-step full-horizon block")
     ap.add_argument("--no-extras", action="store_true",
@@ -225,18 +230,12 @@ def full_run(ctx, sc, model=None):
 
 
 def trace_average(kernel: str, workload: str):
-    """Average duration (us) of a kernel in the committed rocprofv3 summary of
-    this workload: its corrector replay on the final state (what the
-    roofline times) when the summary has one, else its full dispatches."""
+    """Full-dispatch average (us) of a kernel over the timed run in the
+    committed rocprofv3 summary of this workload, if there is one."""
     f = ROOT / "profiles" / f"kernels_{workload}.txt"
     if not f.exists():
         return None
     text = f.read_text()
-    i = text.find("corrector passes replayed")
-    if i >= 0:
-        m = re.search(rf"^\S*k_{kernel}\S*.*?\s+(\d+)\s+([0-9.]+)\s*$", text[i:], flags=re.M)
-        if m:
-            return float(m.group(2))
     i = text.find("full dispatches")
     if i < 0:
         return None
@@ -318,7 +317,7 @@ def batched_leg(ctx, args, model):
         b.update(args.batch_warmup)
         tuned = b.autotune(steps=2)
         b.set_graphs(graphs)
-        b.set_latency(True)
+        b.set_latency(True, every=LATENCY_EVERY)
         ctx.sync()
         t0 = time.perf_counter()
         b.update(args.batch_steps)
@@ -342,8 +341,11 @@ def batched_leg(ctx, args, model):
             "latency_p50_ms": round(lat["p50_ms"], 4),
             "latency_p99_ms": round(lat["p99_ms"], 4),
             "latency_samples": lat["samples"],
+            "latency_every": LATENCY_EVERY,
             "max_density_error": worst,
             "flags": flags,
+            "retried_scene_steps": int(sum(b.stats(i)["retried_steps"] for i in range(b.num_scenes))),
+            "substeps": int(sum(b.stats(i)["substeps"] for i in range(b.num_scenes))),
             "scenes_over_tolerance": over,
             "within_tolerance": over == 0 and (flags & 0x9) == 0,
             "block_sizes": tuned,
@@ -356,11 +358,12 @@ def batched_leg(ctx, args, model):
     # the same scenes and steps through the plain single-system path, one
     # scene after another: the baseline the batched, warm-started serving
     # path replaces
-    seq = plain_sequential(ctx, args)
-    out["plain_sequential"] = seq
-    for name in ("plain", "learned"):
-        if name in out:
-            out[f"{name}_batched_vs_plain_sequential"] = round(out[name]["value"] / seq["value"], 3)
+    if not args.no_sequential:
+        seq = plain_sequential(ctx, args)
+        out["plain_sequential"] = seq
+        for name in ("plain", "learned"):
+            if name in out:
+                out[f"{name}_batched_vs_plain_sequential"] = round(out[name]["value"] / seq["value"], 3)
     out["scenes"] = args.batch_scenes
     out["particles"] = sum(sc.n for sc in scenes.batched_scenes(num_scenes=args.batch_scenes))
     out["steps"] = args.batch_steps
@@ -513,19 +516,23 @@ def main():
                for k, (t, c) in timing.items() if c}
     roofline = None
     if pcisph and args.steps:
-        # the dominant kernel of the step by sampled time; its duration from
-        # the corrector passes replayed back to back on the final state
+        # the dominant kernel of the step by sampled time; its average launch
+        # duration over the timed region from the HIP events around its
+        # launches (every T-th step), the corrector passes replayed back to
+        # back on the final state beside it
         t = sys_.time_corrector(reps=50)
         pair = ("pci_force", "pci_density")
-        if all(k in kernels for k in pair):
-            dom = max(pair, key=lambda k: kernels[k]["total_ms"])
-        else:
-            dom = max(pair, key=lambda k: t[k])
-        roofline = roofline_of(dom, sc.n, t[dom], sc.name)
-        if dom in kernels:
-            roofline["in_step_event_us"] = round(kernels[dom]["avg_us"], 3)
+        have = all(k in kernels for k in pair)
+        dom = max(pair, key=(lambda k: kernels[k]["total_ms"]) if have else (lambda k: t[k]))
         other = "pci_density" if dom == "pci_force" else "pci_force"
-        roofline["other_kernel"] = roofline_of(other, sc.n, t[other], sc.name)
+        us = (lambda k: kernels[k]["avg_us"]) if have else (lambda k: t[k])
+        roofline = roofline_of(dom, sc.n, us(dom), sc.name)
+        roofline["source"] = ("HIP events around every launch of every "
+                              f"{args.timing_every}th timed step" if have else
+                              "corrector passes replayed on the final state")
+        roofline["replay_final_state_us"] = round(t[dom], 3)
+        roofline["other_kernel"] = roofline_of(other, sc.n, us(other), sc.name)
+        roofline["other_kernel"]["replay_final_state_us"] = round(t[other], 3)
     elif kernels:
         dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
         if dom in ALG_BYTES:

@@ -139,12 +139,13 @@ typedef struct psb_sph_params {
                                which may be redone as two of dt/4 (delta
                                quartered), down to dt / 2^retry_levels; the
                                step still advances dt.  0 disables (the
-                               reference's fixed step).  At most 4.  In
-                               config mode a corrector whose error, after
-                               min_iterations and above the tolerance, grows
-                               past twice the smallest error of its earlier
-                               iterations stops there as failed (it diverges:
-                               iterating on would only build pressure).       */
+                               reference's fixed step).  At most 4.  A
+                               substep's corrector whose error, after
+                               min_iterations, grows past twice the smallest
+                               error of its earlier iterations and past four
+                               times the tolerance stops there as failed (it
+                               diverges: iterating on would only build
+                               pressure).                                     */
 } psb_sph_params;
 
 #define PSB_WALL_CLAMP      0  /* box walls act only through the collider (clamp) */
@@ -765,8 +766,11 @@ psb_status psb_batch_set_block_sizes(psb_batch b, const int32_t* block_sizes);
  * graph per later iteration, captured on first use.                        */
 psb_status psb_batch_set_graphs(psb_batch b, int32_t on);
 /* Per-scene step latency (from the step's start to the end of the force
- * pass that finished that scene), collected over steps while enabled.      */
-psb_status psb_batch_set_latency(psb_batch b, int32_t on);
+ * pass that finished that scene), collected on every `every`-th step while
+ * enabled (0: off, 1: every step).  Sampled steps record an event after every
+ * force pass, which delays the next density pass by a few microseconds; the
+ * other steps pace their queue by polling host-mapped memory.               */
+psb_status psb_batch_set_latency(psb_batch b, int32_t every);
 psb_status psb_batch_get_latency(psb_batch b, double* p50_ms, double* p99_ms, double* max_ms,
                                  uint64_t* samples);
 

@@ -73,6 +73,7 @@ inline double rsqrt_pair(double x) { return 1.0 / std::sqrt(x); }
 constexpr float kMinPairD2 = 0x1p-96f;
 // the corrector's divergence stop: error above this factor times its smallest so far
 constexpr float kDivergeFactor = 2.0f;
+constexpr float kDivergeFloor = 4.0f;   // ... and above this times the tolerance
 
 // ---------------------------------------------------------------------------
 // Derived constants.  Computed in double from the float parameters, then
@@ -813,14 +814,16 @@ template <class R> struct System {
       max_it = std::min(4 * prm.max_iterations, 255);
     int it = 0;
     R err = 0;
-    // smallest error of the earlier iterations: a config-mode corrector that
-    // grows past twice it (after min_it, above the tolerance) stops, failed
+    // smallest error of the earlier iterations: a retry substep's corrector
+    // that grows past twice it and past 4x the tolerance (after min_it) stops,
+    // failed
     R min_prev = std::numeric_limits<R>::infinity();
     while (true) {
       err = pci_density(*cur, it == 0 && feat_now, it == 0 && mlp);
       if (it == 0 && mlp) mlp_pressure(*cur);
       it++;
-      const bool diverged = !ref && it >= min_it && err > (R)prm.tolerance &&
+      const bool diverged = !ref && level > 0 && it >= min_it &&
+                            err > (R)kDivergeFloor * (R)prm.tolerance &&
                             err > (R)kDivergeFactor * min_prev;
       bool fin = (it >= min_it && (ref || err <= (R)prm.tolerance)) || it >= max_it || diverged;
       min_prev = std::min(min_prev, err);

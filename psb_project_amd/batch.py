@@ -86,8 +86,9 @@ class BatchSystem:
     def set_graphs(self, on: bool) -> None:
         check(lib().psb_batch_set_graphs(self.handle, 1 if on else 0))
 
-    def set_latency(self, on: bool) -> None:
-        check(lib().psb_batch_set_latency(self.handle, 1 if on else 0))
+    def set_latency(self, on, every: int = 1) -> None:
+        """Record per-scene step latencies on every `every`-th step."""
+        check(lib().psb_batch_set_latency(self.handle, int(every) if on else 0))
 
     def latency(self) -> dict:
         p50, p99, mx = C.c_double(), C.c_double(), C.c_double()

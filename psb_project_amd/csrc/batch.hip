@@ -18,6 +18,7 @@
 // memory, the host stops queueing, and speculative launches of finished
 // scenes return at their first instruction.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -64,7 +65,8 @@ struct psb_batch_s {
   PciStatus* st = nullptr;
   ErrSlots es{};                  // spread per-iteration error maxima            // [S]
   uint32_t* done = nullptr;
-  int32_t* h_final = nullptr;         // host-mapped: iteration at which every scene finished
+  int32_t* h_final = nullptr;         // host-mapped: [0] iteration at which every scene finished,
+                                      // [1] the last force pass that made its decision
   int32_t* d_final = nullptr;
   PciStatus* h_stat = nullptr;        // pinned [2][S] status copies (latency, stats)
   psb_mlp mlp = nullptr;
@@ -80,7 +82,8 @@ struct psb_batch_s {
   hipEvent_t ev_start[2] = {};
   hipEvent_t ev_copy[2] = {};
   std::vector<hipEvent_t> ev_iter[2];
-  // latency
+  // latency: recorded on every lat_every-th step (0: off)
+  int lat_every = 0;
   bool lat_on = false;
   int pending = -1;                   // parity of a step whose latencies are not yet read
   std::vector<double> lat_ms;
@@ -177,6 +180,7 @@ static void buffers(psb_batch b, PciBuffers& pb, PciOut& o) {
   o = PciOut{};
   o.bx = b->bx; o.bv = b->bv; o.bvh = b->bvh; o.bid = b->bid; o.bp = b->bp; o.flags = b->flags;
   o.host_final = b->d_final;
+  o.host_progress = b->d_final + 1;
   o.done = b->done;
   o.num_scenes = b->S;
   o.es = b->es;
@@ -339,6 +343,7 @@ static psb_status batch_level_consts(psb_batch b, int level, Consts* out) {
   p.relaxation = (b->prm.relaxation > 0.f ? b->prm.relaxation : 1.0f) * scale;
   PSB_TRY(make_consts(p, out, nullptr));
   out->retry_ok = level < b->prm.retry_levels ? 1 : 0;
+  out->diverge_stop = 1;
   if (level == b->prm.retry_levels) out->max_it = retry_max_iterations(b->prm);
   return PSB_OK;
 }
@@ -432,7 +437,7 @@ static psb_status retry_attempt(psb_batch b, int level, bool from_a) {
   PSB_TRY(launch_pci_nonpressure_batch(rmap(b, R, BP_NONPRESSURE), rshape(b, R, BP_NONPRESSURE),
                                        b->n, c, b->prm.surface_tension != 0.f, false, nb.count,
                                        nb.list, b->ax, b->av, b->rho, b->avh, b->pres, pb, o, st));
-  *b->h_final = -1;
+  b->h_final[0] = b->h_final[1] = -1;
   for (int k = 0; k < c.max_it; ++k) {
     float4* cur = (k & 1) ? b->xp1 : b->xp0;
     float4* nxt = (k & 1) ? b->xp0 : b->xp1;
@@ -473,14 +478,37 @@ static psb_status retry_redo(psb_batch b, const std::vector<uint32_t>& scenes, i
   return PSB_OK;
 }
 
+// Until force pass `it` has made its decision (the first workgroup of every
+// unfinished scene stores `it` into host-mapped memory) or every scene has
+// finished: host polling instead of an event after every pass, whose
+// completion signal held the next density pass back.  A queue that stays
+// slow past 200 ms is waited for with a stream synchronisation.
+static psb_status wait_progress(psb_batch b, int it) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 1;; ++spin) {
+    if (__atomic_load_n(b->h_final + 1, __ATOMIC_ACQUIRE) >= it ||
+        __atomic_load_n(b->h_final, __ATOMIC_ACQUIRE) >= 0)
+      return PSB_OK;
+    if ((spin & 4095u) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+      PSB_HIP(hipStreamSynchronize(b->ctx->stream));
+      return PSB_OK;
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 static psb_status batch_step(psb_batch b, int64_t t) {
   hipStream_t st = b->ctx->stream;
   const int par = (int)(t & 1);
   const int min_it = b->c.min_it, max_it = b->c.max_it;
-  if (b->lat_on) PSB_HIP(hipEventRecord(b->ev_start[par], st));
-  *b->h_final = -1;
+  // latency steps time every iteration with events; the others pace the
+  // queue by the host-mapped progress word
+  const bool lat = b->lat_on && b->lat_every > 0 && t % b->lat_every == 0;
+  if (lat) PSB_HIP(hipEventRecord(b->ev_start[par], st));
+  b->h_final[0] = b->h_final[1] = -1;
   PSB_TRY(run_prologue(b));
-  PSB_HIP(hipEventRecord(b->ev_iter[par][min_it - 1], st));
+  if (lat) PSB_HIP(hipEventRecord(b->ev_iter[par][min_it - 1], st));
   if (b->lat_on && b->pending >= 0) {
     PSB_TRY(read_latency(b, b->pending));
     b->pending = -1;
@@ -488,16 +516,20 @@ static psb_status batch_step(psb_batch b, int64_t t) {
   int final_known = -1;
   for (int k = min_it; k < max_it; ++k) {
     PSB_TRY(run_iter(b, k));
-    PSB_HIP(hipEventRecord(b->ev_iter[par][k], st));
     // iteration k is queued; wait for k-1 and stop queueing once every scene finished
-    PSB_HIP(hipEventSynchronize(b->ev_iter[par][k - 1]));
+    if (lat) {
+      PSB_HIP(hipEventRecord(b->ev_iter[par][k], st));
+      PSB_HIP(hipEventSynchronize(b->ev_iter[par][k - 1]));
+    } else {
+      PSB_TRY(wait_progress(b, k - 1));
+    }
     const int32_t f = __atomic_load_n(b->h_final, __ATOMIC_ACQUIRE);
     if (f >= 0) { final_known = f; break; }
   }
   // finished by the last queued iteration: let its host-mapped store land
   // before the next step resets the flag
   if (final_known < 0) PSB_HIP(hipStreamSynchronize(st));
-  if (b->lat_on) {
+  if (lat) {
     // (before any retry: a second half's grid build resets every scene's
     // final iteration)
     PSB_HIP(hipMemcpyAsync(b->h_stat + (size_t)par * b->S, b->st, sizeof(PciStatus) * b->S,
@@ -509,12 +541,15 @@ static psb_status batch_step(psb_batch b, int64_t t) {
     std::vector<uint32_t> all(b->S), failed;
     for (uint32_t s = 0; s < b->S; ++s) all[s] = s;
     PSB_TRY(failed_scenes(b, all, &failed));
-    b->lat_retried[par].assign(b->S, 0);
+    if (lat) b->lat_retried[par].assign(b->S, 0);
     if (!failed.empty()) {
-      for (uint32_t s : failed) { b->retried[s] += 1; b->lat_retried[par][s] = 1; }
+      for (uint32_t s : failed) {
+        b->retried[s] += 1;
+        if (lat) b->lat_retried[par][s] = 1;
+      }
       PSB_TRY(retry_redo(b, failed, 1));
     }
-    PSB_HIP(hipEventRecord(b->ev_retry_end[par], st));
+    if (lat) PSB_HIP(hipEventRecord(b->ev_retry_end[par], st));
   }
   if (b->tuning) PSB_TRY(collect_tuning(b));
   return PSB_OK;
@@ -542,7 +577,7 @@ static psb_status batch_alloc(psb_batch b) {
   PSB_TRY(A((void**)&b->dsc, sizeof(BatchScene) * b->S));
   PSB_HIP(hipHostMalloc((void**)&b->h_final, 64, hipHostMallocMapped | hipHostMallocCoherent));
   PSB_HIP(hipHostGetDevicePointer((void**)&b->d_final, b->h_final, 0));
-  *b->h_final = -1;
+  b->h_final[0] = b->h_final[1] = -1;
   PSB_HIP(hipHostMalloc((void**)&b->h_stat, 2 * sizeof(PciStatus) * b->S, hipHostMallocDefault));
   PSB_HIP(hipHostMalloc((void**)&b->h_rstat, sizeof(PciStatus) * b->S, hipHostMallocDefault));
   for (int p = 0; p < 2; ++p) PSB_HIP(hipEventCreate(&b->ev_retry_end[p]));
@@ -762,9 +797,10 @@ psb_status psb_batch_set_graphs(psb_batch b, int32_t on) {
   return PSB_OK;
 }
 
-psb_status psb_batch_set_latency(psb_batch b, int32_t on) {
-  PSB_ARG(b, "null batch");
-  b->lat_on = on != 0;
+psb_status psb_batch_set_latency(psb_batch b, int32_t every) {
+  PSB_ARG(b && every >= 0, "bad argument");
+  b->lat_every = every;
+  b->lat_on = every != 0;
   b->pending = -1;
   b->lat_ms.clear();
   return PSB_OK;

@@ -20,10 +20,13 @@ struct PciStatus {
   uint32_t err[kMaxIterSlots];     // float bits: max error of D(k)
 };
 
-// A config-mode corrector stops as failed once its error, after
-// min_iterations and above the tolerance, exceeds this factor times the
-// smallest error of its earlier iterations.
+// The corrector of a retry substep (config mode, p = 0 start) stops as
+// failed once its error, after min_iterations, exceeds both kDivergeFactor
+// times the smallest error of its earlier iterations and kDivergeFloor times
+// the tolerance.  First attempts keep iterating to max_iterations: a warm
+// start whose first error is small may bounce above twice it and converge.
 constexpr float kDivergeFactor = 2.0f;
+constexpr float kDivergeFloor = 4.0f;
 
 // Per-step reset of one scene's status and error slots, by one workgroup.
 __device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, uint32_t* done,
@@ -81,6 +84,10 @@ struct PciOut {
   uint32_t* flags;
   // optional host-mapped slot that receives the final iteration index
   int32_t* host_final;
+  // optional host-mapped word: the first workgroup of force pass k stores k
+  // (after host_final, when k is final), so the host can pace its queue of
+  // iterations without recording an event after every pass
+  int32_t* host_progress;
   // batches: scenes finished this step; host_final is written by the last
   uint32_t* done;
   uint32_t num_scenes;

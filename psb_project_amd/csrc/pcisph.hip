@@ -382,7 +382,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
     const int it = k + 1;
     // smallest error of iterations 0 .. k-1 (+inf at k = 0)
     const float minp = __uint_as_float(st->min_err[(k + 1) & 1]);
-    const bool diverged = !REF && it >= c.min_it && err > c.tolerance && err > kDivergeFactor * minp;
+    const bool diverged = !REF && c.diverge_stop && it >= c.min_it &&
+                          err > kDivergeFloor * c.tolerance && err > kDivergeFactor * minp;
     fin = (it >= c.min_it && (REF || err <= c.tolerance)) || it >= c.max_it || diverged;
     if (lead && !REF) st->min_err[k & 1] = __float_as_uint(fminf(minp, err));
     if (fin && lead) {
@@ -406,6 +407,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
         __hip_atomic_store(o.host_final, hv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }
     }
+    if (lead && o.host_progress)
+      __hip_atomic_store(o.host_progress, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   __shared__ float4 hx[M::kHalo ? kHaloCap : 1];
   uint32_t staged = 0;

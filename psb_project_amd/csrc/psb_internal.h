@@ -91,6 +91,9 @@ struct Consts {
   // fused system step with params.retry_levels), so the force pass leaves
   // NOT_CONVERGED and the worst error to the attempt that is kept
   int retry_ok;
+  // 1: a retry substep, whose corrector stops as failed when it diverges
+  // (kDivergeFactor / kDivergeFloor)
+  int diverge_stop;
   // warm-start features: pressure scale and feature normalisers
   float pscale, inv_ps, h_over_ps, dt_over_h, e_scale;
   // lattice walls (PSB_WALL_LATTICE): implicit boundary particles at

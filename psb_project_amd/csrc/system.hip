@@ -13,6 +13,7 @@
 // (SphSystem.cpp:333,341-344) disappear: predictions always start from A,
 // and the host reads state only when asked.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -90,7 +91,6 @@ struct psb_system_s {
   std::vector<int> step_final;    // host-known final iteration per step of this update
   double t_total[PH_COUNT] = {};
   int64_t t_count[PH_COUNT] = {};
-  std::vector<hipEvent_t> ev_iter;
   // rigid DEM spheres (boundary particles after the fluid)
   uint32_t n_fluid = 0;
   uint32_t n_spheres = 0, per_sphere = 0;
@@ -183,17 +183,13 @@ static psb_status sys_alloc(psb_system s) {
   PSB_TRY(A((void**)&s->nlist, 4 * N * (size_t)s->prm.max_neighbors));
   PSB_TRY(A((void**)&s->flags, 64));
   PSB_TRY(A((void**)&s->st, sizeof(PciStatus)));
-  PSB_HIP(hipHostMalloc((void**)&s->h_final, 64 * sizeof(int32_t),
+  // [slot]: final iteration of an attempt; [64 + slot]: its force-pass progress
+  PSB_HIP(hipHostMalloc((void**)&s->h_final, 128 * sizeof(int32_t),
                         hipHostMallocMapped | hipHostMallocCoherent));
   PSB_HIP(hipHostGetDevicePointer((void**)&s->d_final, s->h_final, 0));
-  for (int i = 0; i < 64; ++i) s->h_final[i] = -1;
+  for (int i = 0; i < 128; ++i) s->h_final[i] = -1;
   PSB_HIP(hipMemsetAsync(s->flags, 0, 64, s->ctx->stream));
   PSB_HIP(hipMemsetAsync(s->st, 0, sizeof(PciStatus), s->ctx->stream));
-  for (int k = 0; k < kMaxIterSlots; ++k) {
-    hipEvent_t e;
-    PSB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    s->ev_iter.push_back(e);
-  }
   return PSB_OK;
 }
 
@@ -266,8 +262,32 @@ static psb_status level_consts(psb_system s, int level, Consts* out) {
   out->coll = s->c.coll;
   out->ncoll = s->c.ncoll;
   out->retry_ok = level < s->prm.retry_levels ? 1 : 0;
+  out->diverge_stop = 1;
   if (level == s->prm.retry_levels) out->max_it = retry_max_iterations(s->prm);
   return PSB_OK;
+}
+
+// Until the force pass of iteration `it` of the attempt in `slot` has made
+// its convergence decision (its first workgroup stores `it` into host-mapped
+// memory, after the final iteration when there is one) or the attempt has
+// finished: host polling instead of an event recorded after every pass, whose
+// completion signal held the next density pass back by ~6 us.  A queue that
+// stays slow past 200 ms is waited for with a stream synchronisation, after
+// which one of the two words is set.
+static psb_status wait_progress(psb_system s, int slot, int it) {
+  const int32_t* prog = s->h_final + 64 + slot;
+  const int32_t* fin = s->h_final + slot;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 1;; ++spin) {
+    if (__atomic_load_n(prog, __ATOMIC_ACQUIRE) >= it || __atomic_load_n(fin, __ATOMIC_ACQUIRE) >= 0)
+      return PSB_OK;
+    if ((spin & 4095u) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+      PSB_HIP(hipStreamSynchronize(s->ctx->stream));
+      return PSB_OK;
+    }
+    __builtin_ia32_pause();
+  }
 }
 
 // One attempt at a PCISPH step with constants c.  from_a: restart from the
@@ -326,6 +346,7 @@ static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool f
   PciOut o{};
   o.bx = s->bx; o.bv = s->bv; o.bvh = s->bvh; o.bid = s->bid; o.bp = s->bp; o.flags = s->flags;
   o.host_final = ref ? nullptr : s->d_final + slot;
+  o.host_progress = ref ? nullptr : s->d_final + 64 + slot;
   o.es = s->es;
   const bool surf = ref || s->prm.surface_tension != 0.f;
   {
@@ -343,7 +364,7 @@ static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool f
   }
   const int min_it = c.min_it, max_it = c.max_it;
   int final_known = -1;
-  if (!ref) s->h_final[slot] = -1;
+  if (!ref) s->h_final[slot] = s->h_final[64 + slot] = -1;
   for (int k = 0; k < max_it; ++k) {
     {
       Timed t(s, PH_PCI_DENSITY, att, k);
@@ -366,10 +387,10 @@ static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool f
     }
     std::swap(cur, nxt);
     if (ref) continue;
-    if (k + 1 >= min_it) PSB_HIP(hipEventRecord(s->ev_iter[k], st));
     if (k >= min_it) {
-      // iteration k is queued; wait for k-1 and stop queueing once it finished
-      PSB_HIP(hipEventSynchronize(s->ev_iter[k - 1]));
+      // iteration k is queued; wait for the force pass of k-1 to start and
+      // stop queueing once it finished the step
+      PSB_TRY(wait_progress(s, slot, k - 1));
       int32_t f = __atomic_load_n(&s->h_final[slot], __ATOMIC_ACQUIRE);
       if (f >= 0) { final_known = f; break; }
     }
@@ -512,7 +533,6 @@ psb_status psb_system_destroy(psb_system s) {
                   (void*)s->d_sf, (void*)s->d_soff, (void*)s->d_fb, (void*)s->d_coll})
     if (p) hipFree(p);
   for (auto e : s->ev) hipEventDestroy(e);
-  for (auto e : s->ev_iter) hipEventDestroy(e);
   if (s->grid) psb_finalize_grid(s->grid);
   delete s;
   return PSB_OK;

@@ -168,13 +168,24 @@ def test_batch_learned_warm_start_matches_single(ctx):
 def test_batch_latency_samples(ctx):
     scs = small_batch(WARM_NONE, False, n=4, seed=81)
     b = BatchSystem(ctx, scs)
+    ref = BatchSystem(ctx, scs)
     b.set_all()
+    ref.set_all()
     b.set_latency(True)
     b.update(5)
     lat = b.latency()
     assert lat["samples"] == 5 * len(scs)
     assert 0 < lat["p50_ms"] <= lat["p99_ms"] <= lat["max_ms"]
+    # every 3rd step sampled (steps 5..11 of the batch: 6 and 9); the others
+    # run without events, with the same results
+    b.set_latency(True, every=3)
+    b.update(7)
+    assert b.latency()["samples"] == 2 * len(scs)
+    ref.update(12)
+    for i in range(len(scs)):
+        assert_same(b.state(i), ref.state(i), f"scene {i}")
     b.close()
+    ref.close()
 
 
 @pytest.mark.parametrize("graphs", [False, True])
