@@ -757,6 +757,15 @@ psb_status psb_batch_set_mlp(psb_batch b, psb_mlp m);
  * per workgroup, negated for two threads per particle.  Results do not depend
  * on the choice.                                                            */
 psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes);
+/* Kernel durations of the corrector passes over every scene of the batch on
+ * its current state (no counterpart in the reference; a measurement helper
+ * for the benchmark, as psb_system_time_corrector): the density and force
+ * passes of iteration 0, every scene active, each launched `reps` times back
+ * to back between two HIP events; microseconds per launch.  Needs a batch
+ * that has stepped; the replays rewrite only scratch, so the next update
+ * continues from the same state.                                           */
+psb_status psb_batch_time_corrector(psb_batch b, int32_t reps, double* us_density,
+                                    double* us_force);
 /* Set the launch shapes directly, in the form psb_batch_autotune returns
  * (e.g. a stored tuning): 64, 128, 256 or 512 per phase; -128, -256 or -512
  * for two threads per particle in pci_density and pci_force.               */

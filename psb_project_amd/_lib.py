@@ -355,6 +355,7 @@ SIGNATURES = {
     "psb_batch_get_particles": (i32, [vp, u32, vp, vp, vp, vp, vp, vp]),
     "psb_batch_set_mlp": (i32, [vp, vp]),
     "psb_batch_autotune": (i32, [vp, i32, C.POINTER(C.c_int32)]),
+    "psb_batch_time_corrector": (i32, [vp, i32, C.POINTER(f64), C.POINTER(f64)]),
     "psb_batch_set_block_sizes": (i32, [vp, C.POINTER(C.c_int32)]),
     "psb_batch_set_graphs": (i32, [vp, i32]),
     "psb_batch_set_latency": (i32, [vp, i32]),

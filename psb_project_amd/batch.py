@@ -77,6 +77,13 @@ class BatchSystem:
         check(lib().psb_batch_autotune(self.handle, int(steps), out))
         return dict(zip(BATCH_PHASES, [int(x) for x in out]))
 
+    def time_corrector(self, reps: int = 20) -> dict:
+        """Microseconds per launch of the corrector passes over every scene
+        (iteration 0 replayed back to back on the current state)."""
+        d, f = C.c_double(), C.c_double()
+        check(lib().psb_batch_time_corrector(self.handle, int(reps), C.byref(d), C.byref(f)))
+        return {"pci_density": d.value, "pci_force": f.value}
+
     def set_block_sizes(self, sizes: dict) -> None:
         """Launch shapes per phase, as autotune returns them (negative: two
         threads per particle in pci_density / pci_force)."""

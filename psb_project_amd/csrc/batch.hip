@@ -889,6 +889,55 @@ psb_status psb_batch_set_block_sizes(psb_batch b, const int32_t* block_sizes) {
   return PSB_OK;
 }
 
+psb_status psb_batch_time_corrector(psb_batch b, int32_t reps, double* us_density,
+                                    double* us_force) {
+  PSB_ARG(b && reps > 0 && us_density && us_force, "bad argument");
+  if (b->step_counter == 0) { set_error("psb_batch_time_corrector before a step"); return PSB_ERR_STATE; }
+  PSB_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  // iteration 0 of a corrector that never finalises, every scene active: the
+  // force pass writes the next prediction only
+  Consts c = b->c;
+  c.min_it = 2;
+  c.max_it = kMaxIterSlots - 1;
+  c.retry_ok = 0;
+  c.diverge_stop = 0;
+  PciBuffers pb; PciOut o;
+  buffers(b, pb, o);
+  pb.feat = nullptr;
+  o.host_final = nullptr;
+  o.host_progress = nullptr;
+  o.done = nullptr;
+  psb_neighbors nb{b->ncount, b->nlist, (uint32_t)b->prm.max_neighbors, b->n};
+  PSB_TRY(launch_pci_retry_reset(b->st, st, b->es, nullptr, b->S));
+  hipEvent_t e[3];
+  for (auto& x : e) PSB_HIP(hipEventCreate(&x));
+  auto run = [&]() -> psb_status {
+    PSB_HIP(hipEventRecord(e[0], st));
+    for (int r = 0; r < reps; ++r)
+      PSB_TRY(launch_pci_density_batch(bmap(b, BP_DENSITY), bshape(b, BP_DENSITY), b->n, c,
+                                       nb.count, nb.list, b->xp0, b->pres, b->rho_pred, b->dl,
+                                       b->st, 0, nullptr, 0, st, b->lanes[BP_DENSITY], b->nlist16,
+                                       b->es));
+    PSB_HIP(hipEventRecord(e[1], st));
+    for (int r = 0; r < reps; ++r)
+      PSB_TRY(launch_pci_force_batch(bmap(b, BP_FORCE), bshape(b, BP_FORCE), b->n, c, nb.count,
+                                     nb.list, b->xp0, b->xp1, b->rho_pred, b->pres, pb, b->st, 0, 0,
+                                     o, st, b->lanes[BP_FORCE], b->nlist16));
+    PSB_HIP(hipEventRecord(e[2], st));
+    PSB_HIP(hipEventSynchronize(e[2]));
+    float md = 0.f, mf = 0.f;
+    PSB_HIP(hipEventElapsedTime(&md, e[0], e[1]));
+    PSB_HIP(hipEventElapsedTime(&mf, e[1], e[2]));
+    *us_density = 1e3 * (double)md / reps;
+    *us_force = 1e3 * (double)mf / reps;
+    return PSB_OK;
+  };
+  const psb_status stt = run();
+  for (auto& x : e) hipEventDestroy(x);
+  return stt;
+}
+
 psb_status psb_batch_autotune(psb_batch b, int32_t steps, int32_t* block_sizes) {
   PSB_ARG(b && steps >= 1, "bad argument");
   for (uint32_t s = 0; s < b->S; ++s)

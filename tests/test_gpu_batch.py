@@ -226,3 +226,25 @@ def test_batch_errors(ctx):
     with pytest.raises(PsbError):
         b.set_particles(5, scs[0].pos4)    # scene out of range
     b.close()
+
+
+def test_batch_time_corrector_keeps_the_run(ctx):
+    """The corrector replay over every scene times both passes and rewrites
+    only scratch: the next steps equal those of a batch never replayed."""
+    scs = small_batch(WARM_NONE, False, n=3, seed=84)
+    a = BatchSystem(ctx, scs)
+    b = BatchSystem(ctx, scs)
+    a.set_all()
+    b.set_all()
+    with pytest.raises(PsbError):
+        b.time_corrector(reps=2)          # before a step
+    a.update(4)
+    b.update(4)
+    t = b.time_corrector(reps=5)
+    assert t["pci_density"] > 0 and t["pci_force"] > 0
+    a.update(4)
+    b.update(4)
+    for i in range(len(scs)):
+        assert_same(b.state(i), a.state(i), f"scene {i}")
+    a.close()
+    b.close()

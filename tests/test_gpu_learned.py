@@ -134,7 +134,11 @@ def test_learned_warm_start_cuts_iterations(ctx):
     pressures of 8 seeded 64^3 dam breaks (200 steps each, every 16th
     particle, 4 epochs), then run 2 held-out seeds: the learned start needs at
     least 25 % fewer corrector iterations than the zero start, every step
-    still within the 1 % tolerance (the corrector is unchanged)."""
+    still within the 1 % tolerance (the corrector is unchanged).  This covers
+    200-step held-out horizons at half the benchmark's training data; the
+    benchmark's full protocol reaches about 40 % there (41-46 % over its last
+    runs) and about 21 % over the full 2000-step dam break, whose far-wall
+    impact lies outside the training horizon."""
     def make(i):
         return scenes.pcisph_dam_break(scene_id=200 + i)
     model, summary = learned.train_and_evaluate(ctx, make, range(8), range(8, 10), steps=200,

@@ -81,6 +81,40 @@ def test_grid_build_bitwise(ctx, oracle, name):
     assert np.array_equal(r["xs"], x[perm])
 
 
+def test_create_grid_follows_create_grid(ctx, oracle):
+    """psb_create_grid takes createGrid's arguments (centre, extents, float
+    cell size; UniformGrid.cu:18-31): Min = centre - extent/2 kept as float,
+    nbCells = ceil(extent / (double)sizeCell).  0.005f is just below 0.005,
+    so 0.64 m gives 129 cells, not the decimal 128.  The grid it makes then
+    sorts particles exactly as the oracle does with that descriptor."""
+    from psb_project_amd._lib import GridDesc
+    cx, cy, cz, L, W, D, cell = 0.1, 0.3, -0.2, 0.64, 0.32, 0.16, np.float32(0.005)
+    g = C.c_void_p()
+    check(lib().psb_create_grid(ctx.handle, cx, cy, cz, L, W, D, float(cell), 8192, C.byref(g)))
+    try:
+        gd = GridDesc()
+        check(lib().psb_grid_get_desc(g, C.byref(gd)))
+        want_dims = [int(np.ceil(e / np.float64(cell))) for e in (L, W, D)]
+        assert list(gd.dims) == want_dims == [129, 65, 33]
+        want_min = np.float32([cx - L / 2, cy - W / 2, cz - D / 2])
+        assert np.array_equal(np.float32(list(gd.min)), want_min)
+        assert np.float32(gd.cell_size) == cell
+        rng = np.random.default_rng(12)
+        x = np.zeros((8192, 4), np.float32)
+        x[:, :3] = want_min + rng.uniform(0, 1, (8192, 3)) * np.float32([L, W, D])
+        dpos = ctx.to_device(x)
+        check(lib().psb_store_particles(g, dpos.ptr, x.shape[0]))
+        hash_, perm, start = _grid_arrays(ctx, g, x.shape[0], int(np.prod(want_dims)))
+        h, p, s, _ = oracle.grid_build(x, gd)
+        assert np.array_equal(hash_, h)
+        assert np.array_equal(perm, p)
+        assert np.array_equal(start, s)
+        with pytest.raises(PsbError):
+            check(lib().psb_create_grid(ctx.handle, cx, cy, cz, L, W, D, 0.0, 16, C.byref(C.c_void_p())))
+    finally:
+        lib().psb_finalize_grid(g)
+
+
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_neighbors_bitwise(ctx, oracle, name):
     sc = SCENES[name]()
@@ -568,7 +602,7 @@ def test_pcisph_retry_with_spheres_bitwise(ctx, oracle):
     run, each substep moves the spheres by its own dt."""
     from psb_project_amd.system import FluidSystem
     sc = scenes.double_dam_break_dem(block=(12, 12, 8), box=(0.24, 0.12, 0.12), num_spheres=6,
-                                     max_it=3, tolerance=0.002)
+                                     max_it=3, tolerance=0.002, dt=5e-4)
     sc.params.retry_levels = 2
     m = sc.meta
     cap = sc.n + 93 * 6
