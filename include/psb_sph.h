@@ -17,8 +17,10 @@
  *     priority) rather than the legacy default stream; functions that hand
  *     results to the host synchronise that stream themselves;
  *   - the per-particle material arrays of the reference (mass, radius, k,
- *     restDensity, viscosity, threshold, surfaceTension) are uniform per
- *     system and passed once in psb_sph_params;
+ *     restDensity, viscosity, threshold, surfaceTension) become one material
+ *     per system, passed once in psb_sph_params, or (reference mode) a table
+ *     of up to PSB_MAX_MATERIALS materials with one index per particle
+ *     (psb_system_set_materials);
  *   - capacity, grid domain and neighbour-list width are arguments instead of
  *     the constants 65,536 / 1 m^3 / 200 / 64 of the reference.
  *
@@ -452,6 +454,30 @@ psb_status psb_system_clear_colliders(psb_system sys);
 
 /* Number of rigid spheres (psb_system_set_spheres or a loaded checkpoint).  */
 uint32_t psb_system_num_spheres(psb_system sys);
+/* Per-particle materials, the reference's per-particle arrays mass,
+ * interactionRadius, gasStiffness, restDensity, viscosity, threshold and
+ * surfaceTension (SphSystem.cpp:157-167, PciSphSystem.cpp:103-113, filled per
+ * particle by Emitter::addParticle, Emitter.cpp:141-175).  Particle i of
+ * the last psb_system_set_particles takes materials[index[i]] (n = that
+ * particle count); the fields read from each material are mass,
+ * support_radius, gas_stiffness, rest_density, viscosity, threshold and
+ * surface_tension, everything else comes from the system's parameters.
+ * Pairs mix as the reference's kernels do: WCSPH takes the neighbour's
+ * radius and mass (SphKernel.cu:17-21, 51-71), PCISPH max(h_i, h_j) and the
+ * neighbour's mass (PciSphSystem_Kernel.cu:51, 79, 101, 138), both the mean
+ * viscosity; particle i's neighbours are those within its own radius
+ * (UniformGridKernel.cu:82).  Reference mode only (WCSPH with
+ * PSB_EOS_LINEAR); every support_radius must be at most the system's, which
+ * sizes the neighbour search.  The steps then run separate one-thread-per-
+ * particle kernels (PCISPH: the reference's three corrector iterations).
+ * count = 0 returns to the system's one material; set_particles,
+ * set_state, set_spheres and load_checkpoint clear the table too. */
+#define PSB_MAX_MATERIALS 8
+psb_status psb_system_set_materials(psb_system sys, uint32_t count,
+                                    const psb_sph_params* materials,
+                                    const uint8_t* index, uint32_t n);
+/* Materials in use (0: the system's one material). */
+uint32_t psb_system_num_materials(psb_system sys);
 /* SPHSystem::update / PCI_SPHSystem::update, `steps` times. */
 psb_status psb_system_update(psb_system sys, int32_t steps);
 psb_status psb_system_get_stats(psb_system sys, psb_step_stats* out);

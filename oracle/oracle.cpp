@@ -635,6 +635,65 @@ Mlp make_mlp(const float* w, const float* mu, const float* sd, bool bf) {
 // ---------------------------------------------------------------------------
 // System: sorted state of one scene, advanced by WCSPH or PCISPH steps.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Per-particle materials (reference mode): SphSystem.cpp:157-167 and
+// PciSphSystem.cpp:103-113 pack each particle's material into per-particle
+// arrays; the kernels mix them per pair.  WCSPH takes the neighbour's radius
+// and mass (SphKernel.cu:17-21, 51-71), PCISPH max(h_i, h_j) and the
+// neighbour's mass (PciSphSystem_Kernel.cu:51, 79, 101, 138), both the mean
+// viscosity (SphKernel.cu:62, PciSphSystem_Kernel.cu:144); the search keeps
+// j when |x_i - x_j| <= h_i (UniformGridKernel.cu:82).  Constants per
+// (material of i, material of j), computed in double and rounded once.
+// ---------------------------------------------------------------------------
+constexpr int kMaxMat = PSB_MAX_MATERIALS;
+template <class R> struct MatOwnR { R h2, rho0, inv_rho0, k, l, sigma, beta; };
+template <class R> struct MatPairR { R h, h2, h2x3, mc, cs, csm, cvm, csf, mu; };
+template <class R> struct MatTabR {
+  MatOwnR<R> own[kMaxMat];
+  MatPairR<R> pair[kMaxMat * kMaxMat];
+};
+
+// 0 on success, 1 on an invalid material.
+template <class R>
+int make_mat_tab(const psb_sph_params& sys, const psb_sph_params* mats, uint32_t count,
+                 MatTabR<R>& t) {
+  if (!mats || count < 1 || count > (uint32_t)kMaxMat) return 1;
+  const double pi = 3.14159265358979323846, dt = sys.dt;
+  for (uint32_t a = 0; a < count; ++a) {
+    const psb_sph_params& q = mats[a];
+    if (!(q.support_radius > 0.f && q.support_radius <= sys.support_radius)) return 1;
+    if (!(q.mass > 0.f && q.rest_density > 0.f)) return 1;
+    const double h = q.support_radius, m = q.mass, rho0 = q.rest_density;
+    MatOwnR<R>& o = t.own[a];
+    o.h2 = (R)(h * h);
+    o.rho0 = (R)rho0;
+    o.inv_rho0 = (R)(1.0 / rho0);
+    o.k = (R)q.gas_stiffness;
+    o.l = (R)q.threshold;
+    o.sigma = (R)q.surface_tension;
+    o.beta = (R)(2.0 * std::pow(m * dt / rho0, 2));
+  }
+  for (uint32_t a = 0; a < count; ++a)
+    for (uint32_t b = 0; b < count; ++b) {
+      const psb_sph_params &qa = mats[a], &qb = mats[b];
+      const double hk = sys.solver == PSB_SOLVER_PCISPH
+                            ? (double)std::max(qa.support_radius, qb.support_radius)
+                            : (double)qb.support_radius;
+      const double mb = qb.mass;
+      MatPairR<R>& P = t.pair[a * kMaxMat + b];
+      P.h = (R)hk;
+      P.h2 = (R)(hk * hk);
+      P.h2x3 = (R)(3.0 * hk * hk);
+      P.mc = (R)(mb * 315.0 / (64.0 * pi * std::pow(hk, 9)));
+      P.cs = (R)(-45.0 / (pi * std::pow(hk, 6)));
+      P.csm = (R)(mb * (-45.0 / (pi * std::pow(hk, 6))));
+      P.cvm = (R)(mb * 45.0 / (pi * std::pow(hk, 6)));
+      P.csf = (R)(mb * (-945.0 / (32.0 * pi * std::pow(hk, 9))));
+      P.mu = (R)(((double)qa.viscosity + (double)qb.viscosity) / 2.0);
+    }
+  return 0;
+}
+
 template <class R> struct System {
   psb_sph_params prm;
   Derived<R> d;
@@ -1267,11 +1326,276 @@ template <class R> struct System {
     }
   }
 
+  // ---- per-particle materials ----------------------------------------------
+  uint32_t nmat = 0;               // 0: the system's one material
+  MatTabR<R> mt{};
+  std::vector<uint8_t> mat_by_id, ms;
+
+  const MatPairR<R>& mpair(uint32_t a, uint32_t b) const { return mt.pair[a * kMaxMat + b]; }
+
+  // Lists over the system radius, then each particle's cut to its own
+  // material's radius at the start positions (order kept).
+  void mixed_lists() {
+    build_grid();
+    neighbors_density();
+    ms.assign(n, 0);
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const uint32_t a = mat_by_id[Aid[p]];
+      ms[p] = (uint8_t)a;
+      const V4<R> xi = Ax[p];
+      uint32_t kept = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        const uint32_t j = list[(uint64_t)k * n + p];
+        const V4<R> xj = Ax[j];
+        R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        if (dot3(dx, dy, dz, dx, dy, dz) <= mt.own[a].h2) list[(uint64_t)(kept++) * n + p] = j;
+      }
+      count[p] = kept;
+    }
+  }
+
+  R mixed_density(uint32_t p, const std::vector<V4<R>>& X) const {
+    const V4<R> xi = X[p];
+    R S = 0;
+    for (uint32_t k = 0; k < count[p]; ++k) {
+      const uint32_t j = list[(uint64_t)k * n + p];
+      const MatPairR<R>& P = mpair(ms[p], ms[j]);
+      const V4<R> xj = X[j];
+      R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      R w = P.h2 - dot3(dx, dy, dz, dx, dy, dz);
+      S = fmaR((w * w) * w, P.mc, S);
+    }
+    return S;
+  }
+
+  // internalForces / compute_Viscosity_SurfaceTension_Force_PCI pair sums on
+  // the start positions (Ax.w = p / rho^2 when press, Av.w = 1 / rho).
+  PairSums<R> mixed_sums(uint32_t p, bool press) const {
+    PairSums<R> s{};
+    const V4<R> xi = Ax[p], vi = Av[p];
+    for (uint32_t k = 0; k < count[p]; ++k) {
+      const uint32_t j = list[(uint64_t)k * n + p];
+      const V4<R> xj = Ax[j], vj = Av[j];
+      R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      R d2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (!(d2 > (R)0)) continue;
+      const MatPairR<R>& P = mpair(ms[p], ms[j]);
+      R dd = std::sqrt(d2);
+      R invd = (R)1 / dd;
+      R hd = P.h - dd;
+      R rinvj = vj.w;
+      if (press) {
+        R c = (P.csm * (hd * hd)) * (xi.w + xj.w);
+        s.fp[0] = fmaR(dx * invd, c, s.fp[0]);
+        s.fp[1] = fmaR(dy * invd, c, s.fp[1]);
+        s.fp[2] = fmaR(dz * invd, c, s.fp[2]);
+      }
+      R lv = ((P.cvm * hd) * rinvj) * P.mu;
+      s.fv[0] = fmaR(vj.x - vi.x, lv, s.fv[0]);
+      s.fv[1] = fmaR(vj.y - vi.y, lv, s.fv[1]);
+      s.fv[2] = fmaR(vj.z - vi.z, lv, s.fv[2]);
+      R aa = P.h2 - d2;
+      R bb = P.h2x3 - (R)7 * d2;
+      R cj = P.csf * rinvj;
+      s.fs = fmaR(aa * bb, cj, s.fs);
+      R an = (aa * aa) * cj;
+      s.nn[0] = fmaR(dx, an, s.nn[0]);
+      s.nn[1] = fmaR(dy, an, s.nn[1]);
+      s.nn[2] = fmaR(dz, an, s.nn[2]);
+    }
+    return s;
+  }
+
+  static void mixed_surface(const PairSums<R>& s, R l, R sigma, R fS[3]) {
+    R ln = std::sqrt(dot3(s.nn[0], s.nn[1], s.nn[2], s.nn[0], s.nn[1], s.nn[2]));
+    if (ln >= l) {
+      R tt = s.fs * (-sigma);
+      R r = (R)1 / ln;
+      fS[0] = (s.nn[0] * tt) * r; fS[1] = (s.nn[1] * tt) * r; fS[2] = (s.nn[2] * tt) * r;
+    } else {
+      fS[0] = fS[1] = fS[2] = 0;
+    }
+  }
+
+  // SPHSystem::update with per-particle materials: densityEvaluation with
+  // the neighbour's radius and mass, p = k_i (rho - rho0_i)
+  // (SphKernel.cu:9-27), internalForces (:32-91), gravity, leapfrog, box,
+  // interpolation.
+  void mixed_wcsph_step() {
+    mixed_lists();
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const MatOwnR<R>& o = mt.own[ms[p]];
+      const R r = mixed_density(p, Ax);
+      const R pr = o.k * (r - o.rho0);
+      rho[p] = r;
+      pres[p] = pr;
+      Ax[p].w = pr / (r * r);
+      Av[p].w = (R)1 / r;
+    }
+    uint32_t zf = 0;
+#pragma omp parallel for schedule(static) reduction(| : zf) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const MatOwnR<R>& o = mt.own[ms[p]];
+      const PairSums<R> s = mixed_sums(p, true);
+      const R ri = rho[p];
+      const R cpf = -ri;
+      R fS[3];
+      mixed_surface(s, o.l, o.sigma, fS);
+      R F0 = (s.fv[0] + fS[0]) + s.fp[0] * cpf;
+      R F1 = (s.fv[1] + fS[1]) + s.fp[1] * cpf;
+      R F2 = (s.fv[2] + fS[2]) + s.fp[2] * cpf;
+      if (!(ri > (R)0)) zf |= PSB_FLAG_ZERO_DENSITY;
+      F0 = F0 + d.gx * ri; F1 = F1 + d.gy * ri; F2 = F2 + d.gz * ri;
+      const R inv = (R)1 / ri;
+      V4<R> vh = Avh[p], x = Ax[p];
+      vh.x = vh.x + (F0 * inv) * d.dt;
+      vh.y = vh.y + (F1 * inv) * d.dt;
+      vh.z = vh.z + (F2 * inv) * d.dt;
+      x.x = x.x + vh.x * d.dt; x.y = x.y + vh.y * d.dt; x.z = x.z + vh.z * d.dt;
+      finish_particle(p, x, vh);
+      Bp[p] = pres[p];
+    }
+    flags |= zf;
+    last_iters = 0; last_err = 0;
+  }
+
+  // PCI_SPHSystem::update with per-particle materials: pci_SPH_update
+  // (PciSphSystem.cu:8-68) with compute_Density_PCI, compute_Pressure_PCI,
+  // compute_Pressure_Force_PCI and compute_Viscosity_SurfaceTension_Force_PCI
+  // (PciSphSystem_Kernel.cu:36-170) on max(h_i, h_j), three iterations.
+  void mixed_pcisph_step() {
+    mixed_lists();
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const R r = mixed_density(p, Ax);
+      rho[p] = r;
+      pres[p] = 0;
+      Av[p].w = (R)1 / r;
+    }
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const MatOwnR<R>& o = mt.own[ms[p]];
+      const PairSums<R> s = mixed_sums(p, false);
+      R fS[3];
+      mixed_surface(s, o.l, (R)0.0728f, fS);
+      const R ri = rho[p];
+      const R fa0 = (R)0 + s.fv[0], fa1 = ((R)-9.81f * ri) + s.fv[1], fa2 = (R)0 + s.fv[2];
+      fa[p] = {fa0, fa1, fa2, 0};
+      fs4[p] = {fS[0], fS[1], fS[2], 0};
+      const R F0 = (fa0 + (R)0) + fS[0], F1 = (fa1 + (R)0) + fS[1], F2 = (fa2 + (R)0) + fS[2];
+      const R inv = (R)1 / ri;
+      const V4<R> x0 = Ax[p], vh0 = Avh[p];
+      const R v0 = vh0.x + (F0 * inv) * d.dt, v1 = vh0.y + (F1 * inv) * d.dt,
+              v2 = vh0.z + (F2 * inv) * d.dt;
+      xp0[p] = {x0.x + v0 * d.dt, x0.y + v1 * d.dt, x0.z + v2 * d.dt, 0};
+    }
+    std::vector<V4<R>>* cur = &xp0;
+    std::vector<V4<R>>* nxt = &xp1;
+    constexpr int kIters = 3;
+    R err = 0;
+    for (int it = 0; it < kIters; ++it) {
+      err = mixed_pci_density(*cur);
+      mixed_pci_force(*cur, *nxt, it == kIters - 1);
+      std::swap(cur, nxt);
+    }
+    last_iters = kIters; last_err = err;
+    total_iters += kIters;
+    worst_err = std::max(worst_err, err);
+    iter_hist.push_back(kIters); err_hist.push_back(err);
+  }
+
+  R mixed_pci_density(std::vector<V4<R>>& X) {
+    R emax = 0;
+#pragma omp parallel for schedule(static) reduction(max : emax) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const V4<R> xi = X[p];
+      R S = 0, W1x = 0, W1y = 0, W1z = 0, W2 = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        const uint32_t j = list[(uint64_t)k * n + p];
+        const V4<R> xj = X[j];
+        const MatPairR<R>& P = mpair(ms[p], ms[j]);
+        R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        R w = P.h2 - d2;
+        S = fmaR((w * w) * w, P.mc, S);
+        if (d2 > (R)0) {
+          R dd = std::sqrt(d2);
+          R invd = (R)1 / dd;
+          R hd = P.h - dd;
+          R g = P.cs * (hd * hd);
+          R ux = (dx * invd) * g, uy = (dy * invd) * g, uz = (dz * invd) * g;
+          W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
+          W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+        }
+      }
+      const MatOwnR<R>& o = mt.own[ms[p]];
+      const R r = S;
+      const R e = r - o.rho0;
+      R pr = pres[p];
+      if (r > o.rho0 && W2 > (R)0) pr = pr + e / (o.beta * (dot3(W1x, W1y, W1z, W1x, W1y, W1z) + W2));
+      pres[p] = pr;
+      rho_pred[p] = r;
+      X[p].w = pr / (r * r);
+      emax = std::max(emax, (e > (R)0 ? e : (R)0) * o.inv_rho0);
+    }
+    return emax;
+  }
+
+  void mixed_pci_force(const std::vector<V4<R>>& X, std::vector<V4<R>>& Xn, bool fin) {
+#pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
+    for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
+      const uint32_t p = (uint32_t)kk;
+      const V4<R> xi = X[p];
+      R sx = 0, sy = 0, sz = 0;
+      for (uint32_t k = 0; k < count[p]; ++k) {
+        const uint32_t j = list[(uint64_t)k * n + p];
+        const V4<R> xj = X[j];
+        R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        R d2 = dot3(dx, dy, dz, dx, dy, dz);
+        if (!(d2 > (R)0)) continue;
+        const MatPairR<R>& P = mpair(ms[p], ms[j]);
+        R dd = std::sqrt(d2);
+        R invd = (R)1 / dd;
+        R hd = P.h - dd;
+        R c = (P.csm * (hd * hd)) * (xi.w + xj.w);
+        sx = fmaR(dx * invd, c, sx); sy = fmaR(dy * invd, c, sy); sz = fmaR(dz * invd, c, sz);
+      }
+      const R rp = rho_pred[p];
+      const R cpf = -rp;
+      const V4<R> f = fa[p], s = fs4[p];
+      const R F0 = (f.x + sx * cpf) + s.x, F1 = (f.y + sy * cpf) + s.y, F2 = (f.z + sz * cpf) + s.z;
+      const R inv = (R)1 / rp;
+      const V4<R> x0 = Ax[p], vh0 = Avh[p];
+      V4<R> vh = {vh0.x + (F0 * inv) * d.dt, vh0.y + (F1 * inv) * d.dt, vh0.z + (F2 * inv) * d.dt, 0};
+      V4<R> x = {x0.x + vh.x * d.dt, x0.y + vh.y * d.dt, x0.z + vh.z * d.dt, 0};
+      if (!fin) {
+        Xn[p] = x;
+      } else {
+        finish_particle(p, x, vh);
+        Bp[p] = pres[p];
+      }
+    }
+  }
+
   void update(int steps) {
     flags = 0; total_iters = 0; worst_err = 0; steps_done = 0; max_nbr_seen = 0;
     retried_steps = 0; substeps = 0;
     for (int s = 0; s < steps; ++s) {
-      if (prm.solver == PSB_SOLVER_PCISPH) pcisph_step(); else wcsph_step();
+      if (nmat) {
+        if (prm.solver == PSB_SOLVER_PCISPH) mixed_pcisph_step(); else mixed_wcsph_step();
+      } else if (prm.solver == PSB_SOLVER_PCISPH) {
+        pcisph_step();
+      } else {
+        wcsph_step();
+      }
       steps_done++;
     }
   }
@@ -1288,6 +1612,7 @@ int set_particles(System<R>* s, uint32_t n, const float* pos4, const float* vel4
   const uint32_t nb = s->n_spheres * s->per_sphere;
   if ((uint64_t)n + nb > s->cap) return 3;
   s->n = n;
+  s->nmat = 0;   // per-particle materials refer to the previous particles
   for (uint32_t i = 0; i < n; ++i) {
     s->Bx[i] = {(R)pos4[4 * i], (R)pos4[4 * i + 1], (R)pos4[4 * i + 2], 0};
     s->Bv[i] = vel4 ? V4<R>{(R)vel4[4 * i], (R)vel4[4 * i + 1], (R)vel4[4 * i + 2], 0} : V4<R>{0, 0, 0, 0};
@@ -1776,6 +2101,26 @@ int oracle_system_set_particles(void* hp, uint32_t n, const float* pos4, const f
   Handle* h = (Handle*)hp;
   if (h->prec == 64) return set_particles(h->d, n, pos4, vel4, vh4);
   return set_particles(h->f, n, pos4, vel4, vh4);
+}
+
+// Per-particle materials (psb_system_set_materials): 0 on success, 1 on an
+// invalid argument (as the library refuses it).
+int oracle_system_set_materials(void* hp, uint32_t count, const psb_sph_params* mats,
+                                const uint8_t* index, uint32_t n) {
+  Handle* h = (Handle*)hp;
+  auto set = [&](auto* s) -> int {
+    if (count == 0) { s->nmat = 0; return 0; }
+    if (s->prm.mode != PSB_MODE_REFERENCE) return 1;
+    if (s->prm.solver != PSB_SOLVER_PCISPH && s->prm.eos != PSB_EOS_LINEAR) return 1;
+    if (!index || n != s->n || s->n_spheres) return 1;
+    for (uint32_t i = 0; i < n; ++i)
+      if (index[i] >= count) return 1;
+    if (make_mat_tab(s->prm, mats, count, s->mt)) return 1;
+    s->mat_by_id.assign(index, index + n);
+    s->nmat = count;
+    return 0;
+  };
+  return h->prec == 64 ? set(h->d) : set(h->f);
 }
 
 uint32_t oracle_system_count(void* hp) {

@@ -292,6 +292,8 @@ SIGNATURES = {
     "psb_system_set_particles_device": (i32, [vp, u32, vp, vp, vp]),
     "psb_system_count": (u32, [vp]),
     "psb_system_num_spheres": (u32, [vp]),
+    "psb_system_set_materials": (i32, [vp, u32, vp, vp, u32]),
+    "psb_system_num_materials": (u32, [vp]),
     "psb_system_add_collider": (i32, [vp, vp]),
     "psb_surface_default_params": (i32, [vp]),
     "psb_surface_create": (i32, [vp, C.POINTER(vp)]),

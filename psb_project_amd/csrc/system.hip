@@ -21,6 +21,7 @@
 #include "bodies.h"
 #include "mlp.h"
 #include "pcisph.h"
+#include "mixed.h"
 #include "psb_internal.h"
 
 namespace psb {
@@ -106,6 +107,12 @@ struct psb_system_s {
   int32_t cus = 0;               // compute units of the device
   std::vector<ColliderDev> colliders;   // after the box, in order
   ColliderDev* d_coll = nullptr;
+  // per-particle materials (reference mode): table, material of each id,
+  // material of each sorted particle; nmat = 0: the system's one material
+  MatTable* d_tbl = nullptr;
+  uint8_t* mat_by_id = nullptr;
+  uint8_t* ms = nullptr;
+  uint32_t nmat = 0;
 };
 
 namespace psb {
@@ -217,6 +224,54 @@ static psb_status wcsph_step(psb_system s, int step) {
     PSB_TRY(launch_wcsph_step_forces(s->n, s->c, s->prm.surface_tension != 0.f, s->ncount,
                                      s->nlist, s->ax, s->av, s->rho, s->pres, s->avh, s->aid, s->bx,
                                      s->bv, s->bvh, s->bid, s->bp, s->flags, st));
+  }
+  return PSB_OK;
+}
+
+// ---- per-particle materials (reference mode) -------------------------------
+static MixArgs mix_args(psb_system s) {
+  MixArgs a{};
+  a.n = s->n; a.c = s->c; a.tbl = s->d_tbl; a.cnt = s->ncount; a.list = s->nlist;
+  a.mat_by_id = s->mat_by_id; a.ms = s->ms;
+  a.ax = s->ax; a.av = s->av; a.avh = s->avh; a.aid = s->aid;
+  a.bx = s->bx; a.bv = s->bv; a.bvh = s->bvh; a.bid = s->bid; a.bp = s->bp;
+  a.rho = s->rho; a.pres = s->pres; a.rho_pred = s->rho_pred; a.fa = s->fa; a.fs = s->fs;
+  a.flags = s->flags; a.st = s->st;
+  return a;
+}
+
+// Lists over the system's radius (the largest material's), then each
+// particle's cut to its own material's radius (UniformGridKernel.cu:82).
+static psb_status mixed_lists(psb_system s, int step) {
+  PSB_TRY(grid_phase(s, step));
+  psb_neighbors nb{s->ncount, s->nlist, (uint32_t)s->prm.max_neighbors, s->n};
+  Timed t(s, PH_NEIGHBORS, step);
+  PSB_TRY(launch_neighbors(s->grid, s->ax, s->n, s->c.h2, s->gd.R, nb, s->rho, s->c.mc,
+                           s->flags, s->flags + 1, s->ctx->stream));
+  return launch_mix_prepare(mix_args(s), s->ctx->stream);
+}
+
+// SPHSystem::update with per-particle materials (SphSystem.cpp:273-297).
+static psb_status mixed_wcsph_step(psb_system s, int step) {
+  PSB_TRY(mixed_lists(s, step));
+  Timed t(s, PH_FORCES, step);
+  return launch_mix_wcsph(mix_args(s), s->ctx->stream);
+}
+
+// PCI_SPHSystem::update with per-particle materials: pci_SPH_update's three
+// predict-correct iterations (PciSphSystem.cu:8-68), collision, interpolation.
+static psb_status mixed_pcisph_step(psb_system s, int step) {
+  PSB_TRY(mixed_lists(s, step));
+  const MixArgs a = mix_args(s);
+  hipStream_t st = s->ctx->stream;
+  PSB_TRY(launch_mix_pci_prologue(a, s->xp0, st));
+  constexpr int kIters = 3;
+  float4* cur = s->xp0;
+  float4* nxt = s->xp1;
+  for (int it = 0; it < kIters; ++it) {
+    PSB_TRY(launch_mix_pci_density(a, cur, it, st));
+    PSB_TRY(launch_mix_pci_force(a, cur, nxt, it == kIters - 1, kIters, st));
+    std::swap(cur, nxt);
   }
   return PSB_OK;
 }
@@ -526,7 +581,7 @@ psb_status psb_system_destroy(psb_system s) {
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
                   s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat, s->es.base,
-                  s->tiles, s->halo_src};
+                  s->tiles, s->halo_src, s->d_tbl, s->mat_by_id, s->ms};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
   for (void* p : {(void*)s->d_sc[0], (void*)s->d_sc[1], (void*)s->d_sv[0], (void*)s->d_sv[1],
@@ -593,6 +648,7 @@ static psb_status append_spheres(psb_system s, uint32_t n_fluid) {
   s->sphere_cur = 0;
   s->n = n_fluid + nb;
   s->c.n_fluid = n_fluid;
+  s->nmat = 0;
   return PSB_OK;
 }
 
@@ -620,6 +676,7 @@ static psb_status upload_state(psb_system s, uint32_t n, const std::vector<float
   s->n = n;
   PSB_TRY(append_spheres(s, n));
   s->has_state = true;
+  s->nmat = 0;   // per-particle materials refer to the previous particles
   s->stepped = false;
   std::memset(&s->stats, 0, sizeof(s->stats));
   return PSB_OK;
@@ -678,6 +735,7 @@ psb_status psb_system_set_particles_device(psb_system s, uint32_t n, const float
   s->n = n;
   PSB_TRY(append_spheres(s, n));
   s->has_state = true;
+  s->nmat = 0;   // per-particle materials refer to the previous particles
   s->stepped = false;
   std::memset(&s->stats, 0, sizeof(s->stats));
   return PSB_OK;
@@ -701,8 +759,14 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   s->stats.retried_steps = 0;
   s->stats.substeps = 0;
   for (int k = 0; k < steps && s->n > 0; ++k) {
-    if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(pcisph_step(s, k));
-    else PSB_TRY(wcsph_step(s, k));
+    if (s->nmat) {
+      if (s->prm.solver == PSB_SOLVER_PCISPH) PSB_TRY(mixed_pcisph_step(s, k));
+      else PSB_TRY(mixed_wcsph_step(s, k));
+    } else if (s->prm.solver == PSB_SOLVER_PCISPH) {
+      PSB_TRY(pcisph_step(s, k));
+    } else {
+      PSB_TRY(wcsph_step(s, k));
+    }
     s->step_counter += 1;
     if (s->recs.size() > 4096) PSB_TRY(collect_timing(s));
   }
@@ -731,6 +795,37 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   if (steps > 0) s->stepped = true;
   return PSB_OK;
 }
+
+psb_status psb_system_set_materials(psb_system s, uint32_t count, const psb_sph_params* materials,
+                                    const uint8_t* index, uint32_t n) {
+  PSB_ARG(s, "null system");
+  if (count == 0) { s->nmat = 0; return PSB_OK; }
+  PSB_ARG(s->prm.mode == PSB_MODE_REFERENCE, "per-particle materials need PSB_MODE_REFERENCE");
+  PSB_ARG(s->prm.solver == PSB_SOLVER_PCISPH || s->prm.eos == PSB_EOS_LINEAR,
+          "per-particle materials with WCSPH need PSB_EOS_LINEAR");
+  if (!s->has_state) {
+    set_error("psb_system_set_materials before psb_system_set_particles");
+    return PSB_ERR_STATE;
+  }
+  PSB_ARG(index && n == s->n, "one material index per particle of the last set_particles");
+  PSB_ARG(s->n_spheres == 0, "per-particle materials and rigid spheres do not combine");
+  for (uint32_t i = 0; i < n; ++i) PSB_ARG(index[i] < count, "material index out of range");
+  MatTable tbl;
+  PSB_TRY(make_mat_table(s->prm, materials, count, &tbl));
+  PSB_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const size_t N = s->cap ? s->cap : 1;
+  if (!s->d_tbl) PSB_HIP(hipMalloc((void**)&s->d_tbl, sizeof(MatTable)));
+  if (!s->mat_by_id) PSB_HIP(hipMalloc((void**)&s->mat_by_id, N));
+  if (!s->ms) PSB_HIP(hipMalloc((void**)&s->ms, N));
+  PSB_HIP(hipMemcpyAsync(s->d_tbl, &tbl, sizeof(MatTable), hipMemcpyHostToDevice, st));
+  if (n) PSB_HIP(hipMemcpyAsync(s->mat_by_id, index, n, hipMemcpyHostToDevice, st));
+  PSB_HIP(hipStreamSynchronize(st));
+  s->nmat = count;
+  return PSB_OK;
+}
+
+uint32_t psb_system_num_materials(psb_system s) { return s ? s->nmat : 0; }
 
 psb_status psb_system_get_stats(psb_system s, psb_step_stats* out) {
   PSB_ARG(s && out, "null argument");
@@ -1303,6 +1398,7 @@ psb_status psb_system_load_checkpoint(psb_system s, const char* path) {
   s->stepped = h.stepped != 0;
   s->stats = stats;
   s->has_state = true;
+  s->nmat = 0;   // per-particle materials refer to the previous particles
   return PSB_OK;
 }
 

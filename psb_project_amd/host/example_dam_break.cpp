@@ -9,8 +9,13 @@
 //       particle files read by ParticleLoader_XML::load / ParticleLoader_Txt,
 //       one JSON line per particle
 //   example_dam_break --check-materials A.xml B.xml         (no GPU needed)
-//       two EmitterBox blocks, one per material, through the one-material
-//       check of SPHSystem::init: prints "uniform" or the refusal (exit 3)
+//       two EmitterBox blocks, one per material, grouped as SPHSystem::init
+//       groups them: prints "materials K particles N"
+//   example_dam_break --two-fluid A.xml B.xml --steps K --out FILE
+//       a block of each material side by side in a reference-mode
+//       PCI_SPHSystem (per-particle materials), K steps; FILE holds the
+//       record below followed by the material count, the materials
+//       (psb_sph_params each) and one material index byte per particle
 //   example_dam_break --steps K --side N --out FILE [--config]
 //       runs K steps of an N^3 block (spacing 5 mm) and writes the binary
 //       record: params (psb_sph_params), n, initial float4 positions, final
@@ -50,7 +55,7 @@ static void print_material(const SimulationData_SPHSystem& m) {
 
 int main(int argc, char** argv) {
   std::string material, save, out, export_xml, export_txt, checkpoint, resume_xml, resume;
-  std::string print_xml, print_txt, surface_out, check_a, check_b;
+  std::string print_xml, print_txt, surface_out, check_a, check_b, two_a, two_b;
   bool print = false, config = false, colliders = false, spheres = false, batch = false;
   int steps = 10, side = 12;
   for (int i = 1; i < argc; ++i) {
@@ -75,6 +80,7 @@ int main(int argc, char** argv) {
     else if (a == "--print-txt") print_txt = next();
     else if (a == "--surface") surface_out = next();
     else if (a == "--check-materials") { check_a = next(); check_b = next(); }
+    else if (a == "--two-fluid") { two_a = next(); two_b = next(); }
     else { std::fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
   }
   if (!check_a.empty()) {
@@ -86,16 +92,64 @@ int main(int argc, char** argv) {
     std::vector<SPHParticle*> ps = EmitterBox({0.1f, 0.1f, 0.1f}, 0.05f, 0.05f, 0.05f, &ma).emitParticles();
     std::vector<SPHParticle*> pb = EmitterBox({0.3f, 0.1f, 0.1f}, 0.05f, 0.05f, 0.05f, &mb).emitParticles();
     ps.insert(ps.end(), pb.begin(), pb.end());
-    int rc = 0;
-    try {
-      checkUniformMaterial(ps, "SPHSystem::init");
-      std::printf("uniform %zu\n", ps.size());
-    } catch (const std::invalid_argument& e) {
-      std::printf("refused: %s\n", e.what());
-      rc = 3;
-    }
+    const MaterialSet m = collectMaterials(ps, "SPHSystem::init");
+    std::printf("materials %zu particles %zu\n", m.mats.size(), ps.size());
     for (auto* q : ps) delete q;
-    return rc;
+    return 0;
+  }
+  if (!two_a.empty()) {
+    SimulationData_PCI_SPHSystem ma, mb;
+    if (!ma.loadConfiguration(two_a.c_str()) || !mb.loadConfiguration(two_b.c_str())) {
+      std::fprintf(stderr, "cannot read the materials\n");
+      return 1;
+    }
+    PCI_SPHSystem sys;   // reference mode: the reference's three corrector iterations
+    sys.setDt(0.005);
+    sys.addForce(new ForceExt_Constante(Vector3(0, -1, 0), 9.81f));
+    sys.addObjectCollision(BoxCollision::fromBounds(Vector3(0, 0, 0), Vector3(1.2f, 0.8f, 0.8f), 0.f));
+    std::vector<SPHParticle*> ps = EmitterBox({0.25f, 0.2f, 0.4f}, 0.4f, 0.3f, 0.4f, &ma).emitParticles();
+    std::vector<SPHParticle*> pb = EmitterBox({0.65f, 0.2f, 0.4f}, 0.4f, 0.3f, 0.4f, &mb).emitParticles();
+    ps.insert(ps.end(), pb.begin(), pb.end());
+    const MaterialSet m = collectMaterials(ps, "two-fluid");
+    const uint32_t n = (uint32_t)ps.size();
+    std::vector<float> x0(4 * (size_t)n, 0.f);
+    for (uint32_t i = 0; i < n; ++i) {
+      const Vector3 q = ps[i]->getPos();
+      x0[4 * i] = q.x; x0[4 * i + 1] = q.y; x0[4 * i + 2] = q.z;
+    }
+    sys.init(ps);
+    for (int k = 0; k < steps; ++k) sys.update();
+    const psb_step_stats st = sys.stats();
+    std::printf("two-fluid materials %u particles %u steps %d iterations %d\n",
+                psb_system_num_materials(sys.handle()), n, steps, st.iterations);
+    if (out.empty()) return 0;
+    std::vector<SPHParticle*> got = sys.getParticles();
+    FILE* f = std::fopen(out.c_str(), "wb");
+    if (!f) return 1;
+    const psb_sph_params prm = sys.params();
+    std::fwrite(&prm, sizeof(prm), 1, f);
+    std::fwrite(&n, 4, 1, f);
+    std::fwrite(x0.data(), 4, x0.size(), f);
+    std::vector<float> x(4 * (size_t)n, 0.f), v(4 * (size_t)n, 0.f);
+    for (uint32_t i = 0; i < n; ++i) {
+      const Vector3 q = got[i]->getPos(), u = got[i]->getVel();
+      x[4 * i] = q.x; x[4 * i + 1] = q.y; x[4 * i + 2] = q.z;
+      v[4 * i] = u.x; v[4 * i + 1] = u.y; v[4 * i + 2] = u.z;
+    }
+    std::fwrite(x.data(), 4, x.size(), f);
+    std::fwrite(v.data(), 4, v.size(), f);
+    const int32_t it = st.iterations;
+    std::fwrite(&it, 4, 1, f);
+    const uint32_t count = (uint32_t)m.mats.size();
+    std::fwrite(&count, 4, 1, f);
+    for (const SimulationData_SPHSystem* d : m.mats) {
+      psb_sph_params q = prm;
+      setMaterial(q, d);
+      std::fwrite(&q, sizeof(q), 1, f);
+    }
+    std::fwrite(m.index.data(), 1, m.index.size(), f);
+    std::fclose(f);
+    return 0;
   }
   SimulationData_PCI_SPHSystem water;
   if (!material.empty() && !water.loadConfiguration(material.c_str())) {

@@ -345,31 +345,52 @@ class EmitterBox {
 };
 
 // The reference packs every particle's own material into per-particle
-// arrays (SphSystem.cpp:157-167, Emitter.cpp:141-175) and mixes them per pair
-// (h = max(h_i, h_j), (mu_i + mu_j) / 2: PciSphSystem_Kernel.cu:51,79,101,138,144).
-// The C-ABI carries one material per system, so a particle set that mixes
-// materials is refused (std::invalid_argument) instead of running with
-// particle 0's material.  Particles sharing one SimulationData object, or
-// objects with equal values, form one material.
-inline void checkUniformMaterial(const std::vector<SPHParticle*>& ps, const char* who) {
-  if (ps.empty()) return;
-  const SimulationData_SPHSystem* d = ps[0]->getData();
-  for (size_t k = 1; k < ps.size(); ++k) {
-    const SimulationData_SPHSystem* e = ps[k]->getData();
-    if (e == d) continue;
-    if (e->getParticleMass() != d->getParticleMass() ||
-        e->getSupportRadius() != d->getSupportRadius() ||
-        e->getRestDensity() != d->getRestDensity() ||
-        e->getGasStiffness() != d->getGasStiffness() ||
-        e->getViscosity() != d->getViscosity() ||
-        e->getSurfaceTension() != d->getSurfaceTension() ||
-        e->getThreshold() != d->getThreshold() ||
-        e->getParticleRadius() != d->getParticleRadius())
-      throw std::invalid_argument(
-          std::string(who) + ": particle " + std::to_string(k) +
-          " has a different material than particle 0; one material per system is supported "
-          "(split the scene into one system per material)");
+// arrays (SphSystem.cpp:157-167, PciSphSystem.cpp:103-113, Emitter.cpp:141-175)
+// and mixes them per pair (PciSphSystem_Kernel.cu:51,79,101,138,144,
+// SphKernel.cu:17-71).  collectMaterials groups the particles by material:
+// particles sharing one SimulationData object, or objects with equal
+// values, form one material.  One material runs the system's fast path;
+// several (up to PSB_MAX_MATERIALS, reference mode) go to
+// psb_system_set_materials with one index per particle.
+struct MaterialSet {
+  std::vector<const SimulationData_SPHSystem*> mats;   // distinct, first-use order
+  std::vector<uint8_t> index;                          // per particle
+};
+inline bool sameMaterial(const SimulationData_SPHSystem* a, const SimulationData_SPHSystem* b) {
+  return a == b ||
+         (a->getParticleMass() == b->getParticleMass() &&
+          a->getSupportRadius() == b->getSupportRadius() &&
+          a->getRestDensity() == b->getRestDensity() &&
+          a->getGasStiffness() == b->getGasStiffness() &&
+          a->getViscosity() == b->getViscosity() &&
+          a->getSurfaceTension() == b->getSurfaceTension() &&
+          a->getThreshold() == b->getThreshold());
+}
+inline MaterialSet collectMaterials(const std::vector<SPHParticle*>& ps, const char* who) {
+  MaterialSet m;
+  m.index.resize(ps.size());
+  for (size_t k = 0; k < ps.size(); ++k) {
+    const SimulationData_SPHSystem* d = ps[k]->getData();
+    size_t j = 0;
+    while (j < m.mats.size() && !sameMaterial(m.mats[j], d)) ++j;
+    if (j == m.mats.size()) {
+      if (m.mats.size() == PSB_MAX_MATERIALS)
+        throw std::invalid_argument(std::string(who) + ": more than " +
+                                    std::to_string(PSB_MAX_MATERIALS) + " materials");
+      m.mats.push_back(d);
+    }
+    m.index[k] = (uint8_t)j;
   }
+  return m;
+}
+inline void setMaterial(psb_sph_params& p, const SimulationData_SPHSystem* d) {
+  p.mass = d->getParticleMass();
+  p.support_radius = d->getSupportRadius();
+  p.rest_density = d->getRestDensity();
+  p.gas_stiffness = d->getGasStiffness();
+  p.viscosity = d->getViscosity();
+  p.surface_tension = d->getSurfaceTension();
+  p.threshold = d->getThreshold();
 }
 
 // ---------------------------------------------------------------------------
@@ -423,17 +444,19 @@ class SPHSystem : public System {
   void init(std::vector<SPHParticle*> ps) override {
     particles = std::move(ps);
     if (particles.empty()) throw std::runtime_error("SPHSystem::init without particles");
-    const SimulationData_SPHSystem* d = particles[0]->getData();
-    checkUniformMaterial(particles, "SPHSystem::init");
+    const MaterialSet ms = collectMaterials(particles, "SPHSystem::init");
+    if (ms.mats.size() > 1 && prm.mode != PSB_MODE_REFERENCE)
+      throw std::invalid_argument("SPHSystem::init: particles of several materials need "
+                                  "PSB_MODE_REFERENCE");
+    // the system carries the material with the largest support radius (it
+    // sizes the neighbour search); the box clamp takes particle 0's radius
+    // (SphSystem.cpp:326)
+    const SimulationData_SPHSystem* d = ms.mats[0];
+    for (const SimulationData_SPHSystem* e : ms.mats)
+      if (e->getSupportRadius() > d->getSupportRadius()) d = e;
     prm.solver = solver();
-    prm.mass = d->getParticleMass();
-    prm.support_radius = d->getSupportRadius();
-    prm.rest_density = d->getRestDensity();
-    prm.gas_stiffness = d->getGasStiffness();
-    prm.viscosity = d->getViscosity();
-    prm.surface_tension = d->getSurfaceTension();
-    prm.threshold = d->getThreshold();
-    prm.particle_radius = d->getParticleRadius();
+    setMaterial(prm, d);
+    prm.particle_radius = particles[0]->getData()->getParticleRadius();
     prm.dt = (float)dt;
     Vector3 g;
     for (auto& f : forces) g = g + f->getDirection() * f->getAmplitude();
@@ -478,6 +501,12 @@ class SPHSystem : public System {
     }
     check(psb_system_set_particles(sys, n, x.data(), x.data(), v.data(), v.data(), v.data(),
                                    v.data()));
+    if (ms.mats.size() > 1) {
+      std::vector<psb_sph_params> table(ms.mats.size(), prm);
+      for (size_t k = 0; k < ms.mats.size(); ++k) setMaterial(table[k], ms.mats[k]);
+      check(psb_system_set_materials(sys, (uint32_t)table.size(), table.data(), ms.index.data(),
+                                     n));
+    }
     dirty = false;
   }
 

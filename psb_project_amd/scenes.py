@@ -282,3 +282,56 @@ def double_dam_break_dem(block=(128, 128, 64), s: float = 0.005, box=(2.56, 1.28
             "damping": 2.0 * damping_ratio * math.sqrt(spring * mass / 2.0),
             "sphere_mass": mass}
     return Scene("double_dam_break_dem", p, pos, meta=meta)
+
+
+# The reference's fluid materials (Ressources/FluidMaterial/Water.xml and
+# Mucus.xml): particle_mass, supportRadius, restDensity, gasStiffness,
+# viscosity, surfaceTension, threshold.
+WATER = dict(mass=0.02, support_radius=0.0457, rest_density=998.29, gas_stiffness=3.0,
+             viscosity=3.5, surface_tension=0.0728, threshold=7.065)
+MUCUS = dict(mass=0.04, support_radius=0.0726, rest_density=1000.0, gas_stiffness=5.0,
+             viscosity=36.0, surface_tension=6.0, threshold=5.0)
+
+
+def material(base: Params, **fields) -> Params:
+    """A copy of `base` with the given material fields."""
+    p = base.copy()
+    for k, v in fields.items():
+        setattr(p, k, v)
+    return p
+
+
+def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.03,
+                    seed: int = 0, dt: float = 0.005) -> Scene:
+    """Two touching blocks in reference mode, water (x-min block) and mucus,
+    each n_side^3 on a lattice of spacing s, in a 1.0 x 0.6 x 0.6 m box.
+
+    The system's parameters carry the larger support radius (mucus, which
+    sizes the neighbour search); meta["materials"] holds [water, mucus] and
+    meta["index"] each particle's material (0 for the first block)."""
+    p = default_params()
+    p.mode = MODE_REFERENCE
+    p.solver = solver
+    p.eos = EOS_LINEAR
+    for k, v in MUCUS.items():
+        setattr(p, k, v)
+    p.particle_radius = 0.5 * s
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    p.dt = dt
+    box = (1.0, 0.6, 0.6)
+    p.box_enabled = 1
+    for a in range(3):
+        p.box_min[a] = 0.0
+        p.box_max[a] = box[a]
+    p.elast = 0.0
+    p.grid = grid_desc((0.0, 0.0, 0.0), box, MUCUS["support_radius"] / 2)
+    p.max_neighbors = 160
+    p.lattice_spacing = s
+    p.min_iterations = p.max_iterations = 3
+    o = 0.05
+    a = lattice(n_side, s, origin=(o, o, o), jitter=0.02, seed=2000 + seed)
+    b = lattice(n_side, s, origin=(o + n_side * s, o, o), jitter=0.02, seed=3000 + seed)
+    pos = np.concatenate([a, b])
+    index = np.repeat(np.arange(2, dtype=np.uint8), [a.shape[0], b.shape[0]])
+    mats = [material(p, **WATER), material(p, **MUCUS)]
+    return Scene("two_fluid", p, pos, meta={"materials": mats, "index": index, "s": s})

@@ -57,6 +57,19 @@ class FluidSystem:
         check(lib().psb_system_set_particles(self.handle, n, ptr(arrs[1]), ptr(arrs[0]),
                                              ptr(arrs[3]), ptr(arrs[2]), ptr(arrs[4]), ptr(arrs[5])))
 
+    def set_materials(self, materials, index) -> None:
+        """Per-particle materials (reference mode): particle i of the last
+        set_particles takes materials[index[i]]; an empty list returns to
+        the system's one material."""
+        mats = (Params * max(len(materials), 1))(*[m.copy() for m in materials])
+        idx = np.ascontiguousarray(index, np.uint8)
+        check(lib().psb_system_set_materials(self.handle, len(materials), C.addressof(mats),
+                                             idx.ctypes.data if idx.size else None, idx.size))
+
+    @property
+    def num_materials(self) -> int:
+        return int(lib().psb_system_num_materials(self.handle))
+
     @property
     def n(self) -> int:
         return int(lib().psb_system_count(self.handle))

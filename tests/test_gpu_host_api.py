@@ -195,3 +195,37 @@ def test_cpp_surface_matches_python_binding(ctx, tmp_path):
     assert np.array_equal(g["triangles"], tris)
     s.close()
     fs.close()
+
+
+def test_cpp_two_materials_match_python_binding(ctx, tmp_path):
+    """A water block and a mucus block (the reference's Water.xml and
+    Mucus.xml) in one reference-mode PCI_SPHSystem: the host API builds the
+    per-particle material table, and the run equals FluidSystem with the
+    same table, bit for bit."""
+    gold = Path(__file__).resolve().parent / "golden" / "materials"
+    out = tmp_path / "two.bin"
+    r = subprocess.run([str(EXE), "--two-fluid", str(gold / "Water.xml"), str(gold / "Mucus.xml"),
+                        "--steps", "10", "--out", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "materials 2" in r.stdout
+    prm, n, x0, x, v, it = read_record(out)
+    raw = out.read_bytes()
+    off = C.sizeof(Params) + 4 + 3 * 16 * n + 4
+    count = int(np.frombuffer(raw, np.uint32, 1, off)[0]); off += 4
+    mats = [Params.from_buffer_copy(raw[off + k * C.sizeof(Params):off + (k + 1) * C.sizeof(Params)])
+            for k in range(count)]
+    off += count * C.sizeof(Params)
+    index = np.frombuffer(raw, np.uint8, n, off)
+    assert count == 2 and set(np.unique(index)) == {0, 1}
+    assert mats[0].support_radius != mats[1].support_radius
+    fs = FluidSystem(ctx, prm, n)
+    fs.set_particles(x0)
+    fs.set_materials(mats, index)
+    fs.update(10)
+    st = fs.state()
+    order = np.argsort(st["ids"])
+    assert np.array_equal(st["pos"][order, :3], x[:, :3])
+    assert np.array_equal(st["vel"][order, :3], v[:, :3])
+    assert fs.stats()["iterations"] == it == 3
+    fs.close()

@@ -132,13 +132,13 @@ def test_checkpoint_info_rejects_other_files(tmp_path):
         FluidSystem.checkpoint_info(tmp_path / "missing.ckpt")
 
 
-def test_mixed_materials_are_refused(exe):
-    """SPHSystem::init refuses particles of two materials (the reference mixes
-    per-particle materials per pair; the C-ABI carries one per system)."""
+def test_mixed_materials_are_grouped(exe):
+    """SPHSystem::init groups the particles by material (the reference packs
+    one material per particle): two different files give two materials, the
+    same file twice gives one."""
     r = subprocess.run([str(exe), "--check-materials", str(GOLD / "Water.xml"),
                         str(GOLD / "Mucus.xml")], capture_output=True, text=True)
-    assert r.returncode == 3
-    assert "different material" in r.stdout
+    assert r.returncode == 0 and r.stdout.startswith("materials 2 "), r.stdout
     r = subprocess.run([str(exe), "--check-materials", str(GOLD / "Water.xml"),
                         str(GOLD / "Water.xml")], capture_output=True, text=True)
-    assert r.returncode == 0 and r.stdout.startswith("uniform")
+    assert r.returncode == 0 and r.stdout.startswith("materials 1 "), r.stdout
