@@ -185,15 +185,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
   }
 }
 
-// Forward + backward over a minibatch; gradients accumulate (atomicAdd)
-// into grad[4801]; loss sum into loss[0].  Samples idx[b0 .. b0 + nb) of the
+// Per-wave partial gradients: row w of part holds wave w's 4801 gradient
+// entries and its loss sum at kOffLoss; k_mlp_reduce adds the rows in wave
+// order, so a training run gives the same network every time.
+constexpr int kOffLoss = kNumParams;
+constexpr int kPartRow = kNumParams + 1;
+
+// Forward + backward over a minibatch; each wave stores its gradient sums
+// (and loss sum) into its row of part.  Samples idx[b0 .. b0 + nb) of the
 // dataset (feature SoA [8][ns] + target[ns]); MSE on normalised targets.
 __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ feat,
                                                    const float* __restrict__ target, uint64_t ns,
                                                    const uint32_t* __restrict__ idx, uint32_t nb,
                                                    const float* __restrict__ w, MlpFrags f,
-                                                   MlpNorm nm, float* __restrict__ grad,
-                                                   float* __restrict__ loss) {
+                                                   MlpNorm nm, float* __restrict__ part) {
   __shared__ uint16_t lds_all[4 * 16 * kLdsRow];
   const int wave = threadIdx.x >> 6;
   uint16_t* lds = lds_all + wave * 16 * kLdsRow;
@@ -302,17 +307,19 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ fea
     for (int mt = 0; mt < 4; ++mt)
       gW1[mt] = mfma(pack4(dz1[mt][0], dz1[mt][1], dz1[mt][2], dz1[mt][3]), Bx, gW1[mt]);
   }
-  // ---- flush: reduce over the 4 lane groups where needed, then atomics ----
-  float* gw1p = grad + kOffW1;
-  float* gw2p = grad + kOffW2;
+  // ---- flush: reduce over the 4 lane groups where needed; every entry of
+  // the wave's row is written by exactly one lane ----
+  float* row = part + (size_t)wave_id * kPartRow;
+  float* gw1p = row + kOffW1;
+  float* gw2p = row + kOffW2;
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int out = mt * 16 + 4 * g + j;
-      if (r < kIn) atomicAdd(&gw1p[out * kIn + r], gW1[mt][j]);
+      if (r < kIn) gw1p[out * kIn + r] = gW1[mt][j];
 #pragma unroll
-      for (int nk = 0; nk < 4; ++nk) atomicAdd(&gw2p[out * kHid + nk * 16 + r], gW2[mt][nk][j]);
+      for (int nk = 0; nk < 4; ++nk) gw2p[out * kHid + nk * 16 + r] = gW2[mt][nk][j];
     }
   }
 #pragma unroll
@@ -322,17 +329,29 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ fea
     b += __shfl_xor(b, 16, 64); b += __shfl_xor(b, 32, 64);
     c += __shfl_xor(c, 16, 64); c += __shfl_xor(c, 32, 64);
     if (g == 0) {
-      atomicAdd(&grad[kOffB1 + nt * 16 + r], a);
-      atomicAdd(&grad[kOffB2 + nt * 16 + r], b);
-      atomicAdd(&grad[kOffW3 + nt * 16 + r], c);
+      row[kOffB1 + nt * 16 + r] = a;
+      row[kOffB2 + nt * 16 + r] = b;
+      row[kOffW3 + nt * 16 + r] = c;
     }
   }
   gb3 += __shfl_xor(gb3, 16, 64); gb3 += __shfl_xor(gb3, 32, 64);
   lsum += __shfl_xor(lsum, 16, 64); lsum += __shfl_xor(lsum, 32, 64);
   if (l == 0) {
-    atomicAdd(&grad[kOffB3], gb3);
-    atomicAdd(loss, lsum);
+    row[kOffB3] = gb3;
+    row[kOffLoss] = lsum;
   }
+}
+
+// grad[k] += sum over waves (in wave order) of part[w][k]; loss = the loss sum.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, uint32_t rows,
+                                                    float* __restrict__ grad,
+                                                    float* __restrict__ loss) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (uint32_t)kPartRow) return;
+  float acc = 0.f;
+  for (uint32_t w = 0; w < rows; ++w) acc += part[(size_t)w * kPartRow + k];
+  if (k < (uint32_t)kNumParams) grad[k] = grad[k] + acc;
+  else *loss = acc;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters; zeroes grad.
@@ -361,6 +380,8 @@ struct psb_mlp_s {
   float* m1 = nullptr;
   float* m2 = nullptr;
   float* loss = nullptr;   // [1]
+  float* part = nullptr;   // per-wave gradient rows of a training step
+  uint32_t part_rows = 0;
   MlpFrags frags{};
   void* frag_mem = nullptr;
   MlpNorm norm{};
@@ -440,6 +461,7 @@ psb_status psb_mlp_destroy(psb_mlp m) {
   hipStreamSynchronize(m->ctx->stream);
   hipFree(m->w); hipFree(m->grad); hipFree(m->m1); hipFree(m->m2); hipFree(m->loss);
   hipFree(m->frag_mem);
+  if (m->part) hipFree(m->part);
   delete m;
   return PSB_OK;
 }
@@ -489,12 +511,23 @@ psb_status psb_mlp_train_step(psb_mlp m, const float* features, const float* tar
                               float lr, int32_t apply_adam, float* loss_out) {
   PSB_ARG(m && features && targets && batch_idx && batch > 0, "null argument");
   hipStream_t s = m->ctx->stream;
-  PSB_HIP(hipMemsetAsync(m->loss, 0, 4, s));
   const uint32_t tiles = (batch + 15) / 16;
   uint32_t blocks = (tiles + 3) / 4;
   if (blocks > 512) blocks = 512;
+  const uint32_t rows = 4 * blocks;
+  if (rows > m->part_rows) {
+    PSB_HIP(hipStreamSynchronize(s));
+    if (m->part) PSB_HIP(hipFree(m->part));
+    m->part = nullptr;
+    m->part_rows = 0;
+    PSB_HIP(hipMalloc((void**)&m->part, (size_t)rows * kPartRow * sizeof(float)));
+    m->part_rows = rows;
+  }
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, s, features, targets, num_samples,
-                     batch_idx, batch, m->w, m->frags, m->norm, m->grad, m->loss);
+                     batch_idx, batch, m->w, m->frags, m->norm, m->part);
+  PSB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartRow + 255) / 256), dim3(256), 0, s, m->part, rows,
+                     m->grad, m->loss);
   PSB_HIP(hipGetLastError());
   if (apply_adam) {
     m->adam_t += 1;

@@ -63,6 +63,28 @@ def test_mfma_gradients_match_oracle(ctx, oracle):
         assert err < 1e-2, f"{name}: relative error {err}"
 
 
+def test_training_is_deterministic(ctx):
+    """Every wave stores its gradient sums in its own row and one kernel adds
+    the rows in wave order, so two training runs on the same data from the
+    same initial network end bit for bit equal (the loss too)."""
+    rng = np.random.default_rng(11)
+    ns, batch = 40000, 8192
+    feat = rng.normal(size=(8, ns)).astype(np.float32)
+    target = rng.normal(size=ns).astype(np.float32)
+    df, dt = ctx.to_device(feat), ctx.to_device(target)
+    idx = [ctx.to_device(np.random.default_rng(b).choice(ns, batch, replace=False)
+                         .astype(np.uint32)) for b in range(6)]
+    runs = []
+    for _ in range(2):
+        m = learned.PressureMLP(ctx, seed=4)
+        losses = [m.train_step(df.ptr, dt.ptr, ns, di.ptr, batch) for di in idx]
+        runs.append((m.params(), losses))
+        m.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    assert not np.array_equal(runs[0][0], learned.PressureMLP(ctx, seed=4).params())
+
+
 def test_features_and_recording_bitwise(ctx, oracle):
     from psb_project_amd.system import FluidSystem
     sc = scenes.pcisph_dam_break(n_side=16)
