@@ -775,9 +775,11 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
     const uint32_t sh = lane << 4;   // this lane's half of each 32-bit pair of entries
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     if (ng) w = g4[p];
+    // the prefetch index is clamped to the last group: an unconditional load
+    // leaves no conditional value to carry (no register copies per group)
     for (uint32_t g = 0; g < nfull; ++g) {
       const uint4 cur = w;
-      if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
+      w = g4[(uint64_t)min(g + 1, ng - 1) * n + p];
       const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
       float4 xq[4];
 #pragma unroll
@@ -842,9 +844,10 @@ __device__ __forceinline__ void for_pairs2(uint32_t p, uint32_t n, uint32_t m, c
     const uint32_t ng = (m + kL16Group - 1) / kL16Group;
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     if (ng) w = g4[p];
+    // prefetch index clamped to the last group (an unconditional load)
     for (uint32_t g = 0; g < ng; ++g) {
       const uint4 cur = w;
-      if (g + 1 < ng) w = g4[(uint64_t)(g + 1) * n + p];
+      w = g4[(uint64_t)min(g + 1, ng - 1) * n + p];
       const uint32_t e[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
                              cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
       const uint32_t k = g * kL16Group;
