@@ -342,16 +342,24 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ fea
   }
 }
 
-// grad[k] += sum over waves (in wave order) of part[w][k]; loss = the loss sum.
+// grad[k] += sum over the waves' rows of part[w][k]; loss = the loss sum.
+// Fixed order: row w goes to accumulator w % 8 (eight loads in flight), the
+// accumulators are added pairwise.
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, uint32_t rows,
                                                     float* __restrict__ grad,
                                                     float* __restrict__ loss) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (uint32_t)kPartRow) return;
-  float acc = 0.f;
-  for (uint32_t w = 0; w < rows; ++w) acc += part[(size_t)w * kPartRow + k];
-  if (k < (uint32_t)kNumParams) grad[k] = grad[k] + acc;
-  else *loss = acc;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  uint32_t w = 0;
+  for (; w + 8 <= rows; w += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(w + u) * kPartRow + k];
+  }
+  for (; w < rows; ++w) acc[w & 7u] += part[(size_t)w * kPartRow + k];
+  const float sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  if (k < (uint32_t)kNumParams) grad[k] = grad[k] + sum;
+  else *loss = sum;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters; zeroes grad.
