@@ -117,6 +117,26 @@ def read_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
+def read_issue_limits(kernel: str, workload: str):
+    """VALU issue and texture-data (TD) busy fractions of a kernel's dispatch
+    from the committed rocprofv3 counter summary, if there is one: the
+    limits that stand beside the HBM fraction for the neighbour passes."""
+    f = ROOT / "profiles" / f"counters_corrector_{workload}.txt"
+    if not f.exists():
+        return None
+    text = f.read_text()
+    m = re.search(rf"^\S*k_{kernel}<false.*?(?=^\S|\Z)", text, flags=re.M | re.S)
+    if not m:
+        return None
+    block = m.group(0)
+    v = re.search(r"VALU issue [0-9.]+ us = (\d+)% of the dispatch", block)
+    t = re.search(r"TD busy (\d+)% per CU", block)
+    if not (v and t):
+        return None
+    return {"valu_issue_frac": int(v.group(1)) / 100, "td_busy_frac": int(t.group(1)) / 100,
+            "source": f"profiles/{f.name}"}
+
+
 def host_cpu():
     """CPU model and core counts of this host (lscpu)."""
     import subprocess
@@ -254,6 +274,9 @@ def roofline_of(kernel: str, n: int, us: float, workload: str, trace: bool = Tru
          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
          "traffic": read_traffic(kernel, workload), "alg_bytes_per_launch": alg,
          "avg_launch_us": round(us, 3)}
+    lim = read_issue_limits(kernel, workload) if trace else None
+    if lim:
+        r["issue_limits"] = lim
     tr = trace_average(kernel, workload) if trace else None
     if tr:
         r["trace_avg_us"] = tr
