@@ -266,6 +266,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
     const int x0 = max(cx - R, 0), x1 = min(cx + R, g.nx - 1);
     float S = 0.f;
     bool wide = false;
+    // grouped 32-bit list of a system whose compact offsets have overflowed
+    // (then in place of the compact list, which its readers no longer take)
+    uint32_t* const l32g =
+        (!tiles && c16.list32g && c16.want32 && *c16.want32) ? c16.list32g : nullptr;
     // staged tiles: the compact entry is the LDS slot q + sb of the row
     const HaloTile* T = tiles ? tiles + p / kHaloTile : nullptr;
     const bool hok = T && T->ok;
@@ -281,7 +285,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_neighbors(M map, const float
       }
       if (acc && cnt < cap) {
         nlist[(uint64_t)cnt * n + p] = q;
-        if (c16.list16) {
+        if (l32g) l32g[l32g_at(cnt, p, n)] = q;
+        else if (c16.list16) {
           const int d = (int)(q - p);
           const bool fits = d >= -kCompactReach && d <= kCompactReach;
           const uint16_t e = hok ? (uint16_t)((int32_t)q + sb) : (uint16_t)(d + 32768);

@@ -13,7 +13,10 @@ struct PciStatus {
   uint32_t worst_err;              // float bits: largest final error since reset
   unsigned long long total_iters;  // iterations since reset
   uint32_t wide;                   // 1: some compact neighbour offset overflowed this step
-  uint32_t pad;
+  // 1 once a step's compact offsets overflowed (set by the next step's
+  // reset, then kept): the neighbour search also writes the grouped 32-bit
+  // list, which the corrector passes read while `wide` is set
+  uint32_t want32;
   // float bits: smallest error of iterations 0..k, written by F(k) into
   // [k & 1] and read by F(k + 1) from there (each launch reads the other slot)
   uint32_t min_err[2];
@@ -39,7 +42,10 @@ __device__ __forceinline__ void pci_reset_scene(PciStatus* st0, uint32_t scene, 
               (k % kMaxIterSlots)] = 0u;
   if (threadIdx.x == 0) {
     st->final_iter = -1;
-    if (!keep_wide) st->wide = 0u;
+    if (!keep_wide) {
+      if (st->wide) st->want32 = 1u;
+      st->wide = 0u;
+    }
     st->min_err[0] = st->min_err[1] = 0x7F800000u;   // +inf
   }
   if (done && scene == 0 && threadIdx.x == 0) *done = 0u;

@@ -162,9 +162,10 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_nonpressure(
       for (int k = 1; k < 8; ++k) b.feat[(size_t)k * n + p] = 0.f;
     return;
   }
-  // the compact list unless this scene's offsets overflowed it
+  // the compact list unless this scene's offsets overflowed it (or, once
+  // they have, the search writes the grouped 32-bit list instead)
   const PciStatus* stc = b.st ? b.st + map.scene_of_block() : nullptr;
-  const bool c16 = b.list16 && stc && stc->wide == 0u;
+  const bool c16 = b.list16 && stc && stc->wide == 0u && !(M::kWideList && c.list32g && stc->want32);
   const uint32_t m = cnt[p];
   PairSums s;
   float dl = c.delta;
@@ -293,6 +294,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_density(
     };
     if (staged)
       for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, n, staged - 1}, hx, pair);
+    else if (M::kWideList && list16 && c.list32g && st->want32)
+      for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32G{c.list32g, n}, X, pair);
     else if (list16 && st->wide == 0u)
       for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16, n}, X, pair);
     else
@@ -456,6 +459,8 @@ __global__ __launch_bounds__(M::kMaxThreads) void k_pci_force(
   };
   if (staged)
     for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, ListSlot{list16, n, staged - 1}, hx, pair);
+  else if (M::kWideList && list16 && c.list32g && st->want32)
+    for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List32G{c.list32g, n}, X, pair);
   else if (list16 && st->wide == 0u)
     for_neighbors_split<L, kUnroll>(p, n, cnt[p], lane, List16{list16, n}, X, pair);
   else

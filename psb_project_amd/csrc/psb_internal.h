@@ -108,6 +108,9 @@ struct Consts {
   // colliders applied after the box, in order (device array)
   const ColliderDev* coll;
   int ncoll;
+  // grouped 32-bit neighbour list (l32g_at) of a single system whose compact
+  // offsets overflow, or null
+  const uint32_t* list32g;
 };
 
 psb_status make_consts(const psb_sph_params& p, Consts* c, GridDev* g);
@@ -163,6 +166,7 @@ struct SingleMapT {
   static constexpr int kMaxThreads = 256;
   static constexpr int kLanes = LANES;
   static constexpr bool kHalo = false;
+  static constexpr bool kWideList = true;   // may read the grouped 32-bit list
   __device__ __forceinline__ bool locate(uint32_t n, uint32_t& p, uint32_t& s, bool& lead) const {
     p = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LANES;
     s = 0;
@@ -214,6 +218,7 @@ struct TileMap {
   static constexpr int kMaxThreads = kHaloTile;
   static constexpr int kLanes = 1;
   static constexpr bool kHalo = true;
+  static constexpr bool kWideList = false;
   const HaloTile* __restrict__ tiles;
   const uint32_t* __restrict__ src;  // [tile][kHaloCap]: sorted index staged in each slot
   __device__ __forceinline__ uint32_t tile() const { return xcd_block(blockIdx.x, gridDim.x); }
@@ -236,6 +241,7 @@ struct BatchMapT {
   static constexpr int kMaxThreads = 512;
   static constexpr int kLanes = LANES;
   static constexpr bool kHalo = false;
+  static constexpr bool kWideList = false;
   const BatchScene* __restrict__ sc;
   const uint16_t* __restrict__ blk_scene;   // scene of each block, for this block size
   int bsel;                                 // particles per block = 64 << bsel
@@ -667,16 +673,38 @@ constexpr int kL16Group = 8;
 inline size_t l16_bytes(size_t n, uint32_t cap) {
   return 2u * (size_t)kL16Group * n * ((cap + kL16Group - 1) / kL16Group);
 }
+// Grouped 32-bit list: absolute indices, four consecutive entries of one
+// particle in one 16-byte word, [k / 4][p][k % 4] (l32g_at), so a wave's
+// load of one group is contiguous and a reader takes 8 entries in two
+// loads; the corrector passes read it where the compact offsets overflow.
+// Allocated in whole groups of 8 (the readers' unit).
+__host__ __device__ __forceinline__ uint64_t l32g_at(uint32_t k, uint32_t p, uint32_t n) {
+  return ((uint64_t)(k >> 2) * n + p) * 4u + (k & 3u);
+}
+inline size_t l32g_bytes(size_t n, uint32_t cap) {
+  return 4u * 8u * n * ((cap + 7u) / 8u);
+}
 struct List32 {
   static constexpr bool kGrouped = false;
+  static constexpr bool kGrouped32 = false;
   const uint32_t* __restrict__ l;
   uint32_t n;
   __device__ __forceinline__ uint32_t at(uint32_t k, uint32_t p) const {
     return l[(uint64_t)k * n + p];
   }
 };
+struct List32G {
+  static constexpr bool kGrouped = false;
+  static constexpr bool kGrouped32 = true;
+  const uint32_t* __restrict__ l;
+  uint32_t n;
+  __device__ __forceinline__ uint32_t at(uint32_t k, uint32_t p) const {
+    return l[l32g_at(k, p, n)];
+  }
+};
 struct List16 {
   static constexpr bool kGrouped = true;
+  static constexpr bool kGrouped32 = false;
   const uint16_t* __restrict__ l;
   uint32_t n;
   __device__ __forceinline__ uint32_t decode(uint32_t e, uint32_t p) const {
@@ -691,6 +719,7 @@ constexpr int kCompactReach = 32767;
 // LDS slots of a staged tile (clamped to the staged count)
 struct ListSlot {
   static constexpr bool kGrouped = true;
+  static constexpr bool kGrouped32 = false;
   const uint16_t* __restrict__ l;
   uint32_t n;
   uint32_t last;
@@ -708,6 +737,10 @@ struct Compact16 {
   uint16_t* list16 = nullptr;
   uint32_t* wide = nullptr;
   uint32_t wide_stride = 0;
+  // grouped 32-bit list (l32g_at), written instead of list16 when *want32 is
+  // set and no tile is staged (single system)
+  uint32_t* list32g = nullptr;
+  const uint32_t* want32 = nullptr;
 };
 
 // Neighbour sums of the corrector passes in their canonical order: two
@@ -798,6 +831,67 @@ __device__ __forceinline__ void for_neighbors_split(uint32_t p, uint32_t n, uint
         xq[u] = gather4(X, j < r ? list.decode(__builtin_amdgcn_ubfe(wd[u], sh, 16), p)
                                  : list.safe(p));
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (2u * u + lane < r) f(0, xq[u]);
+    }
+  } else if constexpr (LANES == 1 && LI::kGrouped32) {
+    // the grouped 32-bit list: 8 absolute indices in two 16-byte loads, the
+    // next group's loads issued before the current group's gathers
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
+    const uint32_t ng = (m + 7u) / 8u, nfull = m / 8u;
+    uint4 wa = make_uint4(0u, 0u, 0u, 0u), wb = wa;
+    if (ng) { wa = g4[p]; wb = g4[(uint64_t)n + p]; }
+    for (uint32_t g = 0; g < nfull; ++g) {
+      const uint4 ca = wa, cb = wb;
+      if (g + 1 < ng) {
+        wa = g4[(uint64_t)(2u * g + 2u) * n + p];
+        wb = g4[(uint64_t)(2u * g + 3u) * n + p];
+      }
+      const uint32_t e[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+      float4 xq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xq[u] = gather4(X, e[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f(u & 1, xq[u]);
+    }
+    if (nfull < ng) {
+      const uint32_t e[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+      const uint32_t r = m - nfull * 8u;   // 1..7 entries left
+      float4 xq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xq[u] = gather4(X, (uint32_t)u < r ? e[u] : p);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if ((uint32_t)u < r) f(u & 1, xq[u]);
+    }
+  } else if constexpr (LANES == 2 && LI::kGrouped32) {
+    // two lanes on the grouped 32-bit list: lane l takes entries k = l
+    // (mod 2) of each group of 8 (4 gathers per group and lane)
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(list.l);
+    const uint32_t ng = (m + 7u) / 8u, nfull = m / 8u;
+    uint4 wa = make_uint4(0u, 0u, 0u, 0u), wb = wa;
+    if (ng) { wa = g4[p]; wb = g4[(uint64_t)n + p]; }
+    for (uint32_t g = 0; g < nfull; ++g) {
+      const uint4 ca = wa, cb = wb;
+      const uint32_t gn = 2u * min(g + 1, ng - 1);
+      wa = g4[(uint64_t)gn * n + p];
+      wb = g4[(uint64_t)(gn + 1u) * n + p];
+      const uint32_t e[4] = {lane ? ca.y : ca.x, lane ? ca.w : ca.z, lane ? cb.y : cb.x,
+                             lane ? cb.w : cb.z};
+      float4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xq[u] = gather4(X, e[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f(0, xq[u]);
+    }
+    if (nfull < ng) {
+      const uint32_t e[4] = {lane ? wa.y : wa.x, lane ? wa.w : wa.z, lane ? wb.y : wb.x,
+                             lane ? wb.w : wb.z};
+      const uint32_t r = m - nfull * 8u;   // 1..7 entries left
+      float4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xq[u] = gather4(X, 2u * u + lane < r ? e[u] : p);
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (2u * u + lane < r) f(0, xq[u]);

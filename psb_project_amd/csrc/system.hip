@@ -70,6 +70,7 @@ struct psb_system_s {
   uint32_t* ncount = nullptr;
   uint32_t* nlist = nullptr;
   uint16_t* nlist16 = nullptr;     // compact twin of nlist for the corrector passes (PCISPH)
+  uint32_t* nlist32g = nullptr;    // grouped 32-bit twin, used where compact offsets overflow
   bool compact = true;
   // neighbour rows staged in LDS by the corrector passes (single-lane, compact list)
   bool halo = false;
@@ -183,6 +184,11 @@ static psb_status sys_alloc(psb_system s) {
     }
     PSB_TRY(A((void**)&s->xp0, 16 * N)); PSB_TRY(A((void**)&s->xp1, 16 * N));
     PSB_TRY(A((void**)&s->nlist16, l16_bytes(N, (uint32_t)s->prm.max_neighbors)));
+    // offsets can overflow the compact list only beyond 32,768 particles
+    if (N > (size_t)kCompactReach + 1) {
+      PSB_TRY(A((void**)&s->nlist32g, l32g_bytes(N, (uint32_t)s->prm.max_neighbors)));
+      s->c.list32g = s->nlist32g;
+    }
     s->es.stride = err_slot_stride(1);
     PSB_TRY(A((void**)&s->es.base, 4 * (size_t)kErrSlots * s->es.stride));
   }
@@ -316,6 +322,7 @@ static psb_status level_consts(psb_system s, int level, Consts* out) {
   out->n_fluid = s->c.n_fluid;
   out->coll = s->c.coll;
   out->ncoll = s->c.ncoll;
+  out->list32g = s->c.list32g;
   out->retry_ok = level < s->prm.retry_levels ? 1 : 0;
   out->diverge_stop = 1;
   if (level == s->prm.retry_levels) out->max_it = retry_max_iterations(s->prm);
@@ -375,6 +382,8 @@ static psb_status pcisph_attempt(psb_system s, int step, const Consts& c, bool f
   if (s->compact && s->nlist16) {
     c16.list16 = s->nlist16;
     c16.wide = &s->st->wide;
+    c16.list32g = s->nlist32g;
+    c16.want32 = &s->st->want32;
   }
   const uint16_t* l16 = c16.list16;
   // halo staging: single lane, compact list
@@ -580,7 +589,8 @@ psb_status psb_system_destroy(psb_system s) {
   hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->ax, s->av, s->avh, s->bx, s->bv, s->bvh, s->aid, s->bid, s->ap, s->bp,
                   s->rho, s->pres, s->rho_pred, s->fa, s->fs, s->anp, s->xp0, s->xp1, s->dl,
-                  s->ncount, s->nlist, s->nlist16, s->flags, s->st, s->feat, s->es.base,
+                  s->ncount, s->nlist, s->nlist16, s->nlist32g, s->flags, s->st, s->feat,
+                  s->es.base,
                   s->tiles, s->halo_src, s->d_tbl, s->mat_by_id, s->ms};
   for (void* p : ptrs) if (p) hipFree(p);
   if (s->h_final) hipHostFree(s->h_final);
