@@ -71,6 +71,7 @@ struct psb_system_s {
   uint32_t* nlist = nullptr;
   uint16_t* nlist16 = nullptr;     // compact twin of nlist for the corrector passes (PCISPH)
   uint32_t* nlist32g = nullptr;    // grouped 32-bit twin, used where compact offsets overflow
+  bool grouped32 = false;          // the last update call ended on the grouped 32-bit list
   bool compact = true;
   // neighbour rows staged in LDS by the corrector passes (single-lane, compact list)
   bool halo = false;
@@ -284,12 +285,15 @@ static psb_status mixed_pcisph_step(psb_system s, int step) {
 
 // Threads per particle in the density (D) and force (F) passes, as set, or
 // by default two while one lane per particle would leave at most 4
-// wavefronts per SIMD (n <= 1024 x CUs), else one.  Measured on MI355X,
-// event-timed: 262,144 particles, D 16.9 -> 16.4 us and F 19.3 -> 18.2 us
-// with two; 2.1 M (DEM scene) two lanes win in D early (184 -> 172 us) but
-// lose later in the run (46.6 -> 61.9 us after 400 steps) and in F (213 ->
-// 220 us): the second lane duplicates the per-particle work.  LDS staging
-// runs with one lane.
+// wavefronts per SIMD (n <= 1024 x CUs) or once the system reads the grouped
+// 32-bit list (its last update call ended with it), else one.  Measured on
+// MI355X, event-timed: 262,144 particles, D 16.9 -> 16.4 us and F 19.3 ->
+// 18.2 us with two.  2.1 M (DEM scene) on the ungrouped 32-bit list two
+// lanes won in D early (184 -> 172 us) but lost later in the run (46.6 ->
+// 61.9 us after 400 steps) and in F (213 -> 220 us); on the grouped list
+// they win throughout: steps 305-405, D 195.5 -> 160.7 us, F 200.4 -> 179.3
+// us, 4.89 -> 4.40 ms per step (early steps 1.834 against 1.848 ms).  LDS
+// staging runs with one lane.
 struct Lanes {
   int d, f;
 };
@@ -299,7 +303,8 @@ static Lanes corrector_lanes(psb_system s) {
     return Lanes{l, l};
   }
   if (s->halo) return Lanes{1, 1};
-  const int l = s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus ? 2 : 1;
+  const bool small = s->cus > 0 && (uint64_t)s->n <= 1024ull * (uint64_t)s->cus;
+  const int l = small || s->grouped32 ? 2 : 1;
   return Lanes{l, l};
 }
 
@@ -792,6 +797,7 @@ psb_status psb_system_update(psb_system s, int32_t steps) {
   s->stats.max_neighbors_seen = (int32_t)fl[1];
   s->stats.steps = steps;
   if (s->prm.solver == PSB_SOLVER_PCISPH && steps > 0) {
+    s->grouped32 = hs.want32 != 0u && s->nlist32g && s->compact;
     s->stats.iterations = hs.last_iters;
     std::memcpy(&s->stats.max_error, &hs.last_err, 4);
     s->stats.total_iterations = (int64_t)hs.total_iters;
