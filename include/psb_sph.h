@@ -592,7 +592,9 @@ psb_status psb_system_get_arrays(psb_system sys, psb_system_arrays* out);
  * the neighbour search also writes 16-bit offsets q - p, which the
  * per-iteration density and pressure-force passes read instead of the
  * 32-bit indices; a step where some offset exceeds 32767 falls back to the
- * 32-bit list by itself.  Results do not depend on it.                     */
+ * 32-bit list by itself, and from the next step on the search writes a
+ * grouped 32-bit list (four indices per 16-byte word) in its place.
+ * Results do not depend on it.                                             */
 psb_status psb_system_set_compact_list(psb_system sys, int32_t on);
 
 /* Neighbour rows staged in LDS by the PCISPH corrector passes (off by
@@ -607,8 +609,9 @@ psb_status psb_system_set_halo(psb_system sys, int32_t on);
 /* Threads per particle in the PCISPH corrector passes: 1, 2 (the neighbour
  * sums split into even and odd list positions, one lane each), or 0 (the
  * default: 2 while one lane per particle would leave at most 4 wavefronts
- * per SIMD, i.e. n <= 1024 x compute units, else 1; 1 with LDS staging
- * on).  Results do not depend on it: the sums always run in the even + odd
+ * per SIMD, i.e. n <= 1024 x compute units, or once the previous update
+ * call ended on the grouped 32-bit list, else 1; 1 with LDS staging on).
+ * Results do not depend on it: the sums always run in the even + odd
  * order.                                                                   */
 psb_status psb_system_set_corrector_lanes(psb_system sys, int32_t lanes);
 
