@@ -101,6 +101,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the surface-extraction and checkpoint timings")
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM double dam break")
+    ap.add_argument("--no-materials", action="store_true",
+                    help="skip the per-particle-material (water + mucus) reference-mode leg")
     ap.add_argument("--dem-steps", type=int, default=1000)
     ap.add_argument("--dem-warmup", type=int, default=5)
     ap.add_argument("--dem-all-starts", action="store_true",
@@ -521,6 +523,31 @@ def state_file_leg(ctx, fs, sc, reps: int = 5):
     return out
 
 
+def materials_leg(ctx, n_side: int = 32, steps: int = 100):
+    """Reference-mode steps with per-particle materials: a water block and a
+    mucus block (the reference's Water.xml / Mucus.xml), n_side^3 each, both
+    solvers, 5 untimed steps then `steps` timed."""
+    from psb_project_amd import scenes
+    from psb_project_amd._lib import SOLVER_PCISPH, SOLVER_WCSPH
+    from psb_project_amd.system import FluidSystem
+    out = {}
+    for name, solver in (("pcisph", SOLVER_PCISPH), ("wcsph", SOLVER_WCSPH)):
+        sc = scenes.two_fluid_scene(solver, n_side=n_side)
+        fs = FluidSystem(ctx, sc.params, sc.n)
+        fs.set_particles(sc.pos4)
+        fs.set_materials(sc.meta["materials"], sc.meta["index"])
+        fs.update(5)
+        elapsed, st = timed_run(ctx, fs, steps)
+        out[name] = {"value": round(sc.n * steps / elapsed, 1),
+                     "ms_per_step": round(1e3 * elapsed / steps, 4),
+                     "particles": sc.n, "materials": fs.num_materials, "steps": steps,
+                     "flags": st["flags"]}
+        if solver == SOLVER_PCISPH:
+            out[name]["iters_per_step"] = st["total_iterations"] / steps
+        fs.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -610,6 +637,10 @@ def main():
         dem = dem_leg(ctx, args, model)
     if model is not None:
         model.close()
+    materials = None
+    if not args.no_materials:
+        print("bench: per-particle materials (reference mode)", file=sys.stderr, flush=True)
+        materials = materials_leg(ctx)
     cpu = None
     if not args.no_cpu_baseline and args.steps:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -643,6 +674,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "dem": dem,
+        "materials": materials,
         "state_files": extras,
         "kernels": {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"]}
                     for k, v in kernels.items()},

@@ -304,7 +304,8 @@ def material(base: Params, **fields) -> Params:
 def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.03,
                     seed: int = 0, dt: float = 0.005) -> Scene:
     """Two touching blocks in reference mode, water (x-min block) and mucus,
-    each n_side^3 on a lattice of spacing s, in a 1.0 x 0.6 x 0.6 m box.
+    each n_side^3 on a lattice of spacing s, in a box of at least 1.0 x 0.6 x
+    0.6 m (grown to hold larger blocks with 0.1 m to spare).
 
     The system's parameters carry the larger support radius (mucus, which
     sizes the neighbour search); meta["materials"] holds [water, mucus] and
@@ -318,7 +319,9 @@ def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.0
     p.particle_radius = 0.5 * s
     p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
     p.dt = dt
-    box = (1.0, 0.6, 0.6)
+    o = 0.05
+    box = (max(1.0, o + 2 * n_side * s + 0.1), max(0.6, o + n_side * s + 0.1),
+           max(0.6, o + n_side * s + 0.1))
     p.box_enabled = 1
     for a in range(3):
         p.box_min[a] = 0.0
@@ -328,7 +331,6 @@ def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.0
     p.max_neighbors = 160
     p.lattice_spacing = s
     p.min_iterations = p.max_iterations = 3
-    o = 0.05
     a = lattice(n_side, s, origin=(o, o, o), jitter=0.02, seed=2000 + seed)
     b = lattice(n_side, s, origin=(o + n_side * s, o, o), jitter=0.02, seed=3000 + seed)
     pos = np.concatenate([a, b])
