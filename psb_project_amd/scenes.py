@@ -302,10 +302,12 @@ def material(base: Params, **fields) -> Params:
 
 
 def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.03,
-                    seed: int = 0, dt: float = 0.005) -> Scene:
+                    seed: int = 0, dt: float = 0.005, rest_spacing: bool = False) -> Scene:
     """Two touching blocks in reference mode, water (x-min block) and mucus,
     each n_side^3 on a lattice of spacing s, in a box of at least 1.0 x 0.6 x
-    0.6 m (grown to hold larger blocks with 0.1 m to spare).
+    0.6 m (grown to hold larger blocks with 0.1 m to spare).  rest_spacing:
+    each block on its material's own spacing (m / rho0)^(1/3) instead, so
+    that both start near their rest density.
 
     The system's parameters carry the larger support radius (mucus, which
     sizes the neighbour search); meta["materials"] holds [water, mucus] and
@@ -320,8 +322,13 @@ def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.0
     p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
     p.dt = dt
     o = 0.05
-    box = (max(1.0, o + 2 * n_side * s + 0.1), max(0.6, o + n_side * s + 0.1),
-           max(0.6, o + n_side * s + 0.1))
+    sa, sb = s, s
+    if rest_spacing:
+        sa = (WATER["mass"] / WATER["rest_density"]) ** (1.0 / 3.0)
+        sb = (MUCUS["mass"] / MUCUS["rest_density"]) ** (1.0 / 3.0)
+    smax = max(sa, sb)
+    box = (max(1.0, o + n_side * (sa + sb) + 0.1), max(0.6, o + n_side * smax + 0.1),
+           max(0.6, o + n_side * smax + 0.1))
     p.box_enabled = 1
     for a in range(3):
         p.box_min[a] = 0.0
@@ -331,8 +338,8 @@ def two_fluid_scene(solver: int = SOLVER_PCISPH, n_side: int = 8, s: float = 0.0
     p.max_neighbors = 160
     p.lattice_spacing = s
     p.min_iterations = p.max_iterations = 3
-    a = lattice(n_side, s, origin=(o, o, o), jitter=0.02, seed=2000 + seed)
-    b = lattice(n_side, s, origin=(o + n_side * s, o, o), jitter=0.02, seed=3000 + seed)
+    a = lattice(n_side, sa, origin=(o, o, o), jitter=0.02, seed=2000 + seed)
+    b = lattice(n_side, sb, origin=(o + n_side * sa, o, o), jitter=0.02, seed=3000 + seed)
     pos = np.concatenate([a, b])
     index = np.repeat(np.arange(2, dtype=np.uint8), [a.shape[0], b.shape[0]])
     mats = [material(p, **WATER), material(p, **MUCUS)]
