@@ -21,8 +21,10 @@ substeps, the worst final error and the flags.
 
 Then the learned warm start is trained the way it is meant to be used
 (untimed): plain PCISPH records converged pressures on 16 seeded scenes of
-the same shape for 200 steps, the MLP is trained on the GPU, and 4 held-out
-seeds compare plain, previous-pressure and learned starts.  The same K steps
+the same shape, 200 steps each, the windows spread over the 2000-step
+horizon (starting at steps 0, 120, ..., 1800), the MLP is trained on the
+GPU, and 4 held-out seeds compare plain, previous-pressure and learned
+starts over their first 200 steps.  The same K steps
 of the workload are then timed again with the learned start.
 
 Then the batched serving workload: 64 independent scenes (log-uniform
@@ -90,6 +92,9 @@ def parse():
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=200)
     ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-horizon", type=int, default=None,
+                    help="spread the training windows over this many steps "
+                         "(default: the workload's horizon, 2000; 0: all from step 0)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=300)
@@ -292,11 +297,13 @@ def learned_leg(ctx, sc, args):
     from psb_project_amd._lib import WARM_MLP
     from psb_project_amd.system import FluidSystem
     n_side = sc.meta["n_side"]
+    horizon = (int(sc.meta.get("steps", 2000)) if args.train_horizon is None
+               else args.train_horizon)
     t0 = time.perf_counter()
     model, summary = learned.train_and_evaluate(
         ctx, lambda i: scenes.pcisph_dam_break(n_side=n_side, scene_id=100 + i),
         range(args.train_seeds), range(args.train_seeds, args.train_seeds + args.eval_seeds),
-        steps=args.train_steps, epochs=args.train_epochs)
+        steps=args.train_steps, epochs=args.train_epochs, horizon=horizon)
     train_s = time.perf_counter() - t0
     p = sc.params.copy()
     p.warm_start = WARM_MLP
@@ -317,6 +324,7 @@ def learned_leg(ctx, sc, args):
         "heldout": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                         for kk, vv in v.items()} for k, v in ev.items()},
         "train": {"scenes": args.train_seeds, "steps": args.train_steps,
+                  "window_starts": summary["window_starts"],
                   "samples": summary["samples"], "epochs": summary["epochs"],
                   "mse_per_epoch": [round(x, 5) for x in summary["mse_per_epoch"]],
                   "seconds": round(train_s, 1)},

@@ -141,18 +141,24 @@ class Dataset:
 
 
 def record(ctx: Context, scenes_list, steps: int, dataset: Dataset, stride: int = 16,
-           warmup: int = 0):
+           warmup=0):
     """Run plain PCISPH on each scene and record (features, converged
-    pressure) of every `stride`-th particle per step.  Returns iterations/step."""
+    pressure) of every `stride`-th particle per step, after `warmup` plain
+    steps that are not recorded (one count for every scene, or one per
+    scene).  Returns iterations/step."""
     from .system import FluidSystem
+    scenes_list = list(scenes_list)
+    warm = [int(warmup)] * len(scenes_list) if np.isscalar(warmup) else [int(w) for w in warmup]
+    if len(warm) != len(scenes_list):
+        raise ValueError(f"{len(warm)} warm-up counts for {len(scenes_list)} scenes")
     iters = []
-    for sc in scenes_list:
+    for sc, w in zip(scenes_list, warm):
         p = sc.params.copy()
         p.warm_start = WARM_NONE
         fs = FluidSystem(ctx, p, sc.n)
         fs.set_particles(sc.pos4, sc.vel4, sc.vh4)
-        if warmup:
-            fs.update(warmup)
+        if w:
+            fs.update(w)
         check(lib().psb_system_set_recorder(fs.handle, dataset.handle, int(stride)))
         fs.update(steps)
         iters.append(fs.stats()["total_iterations"] / steps)
@@ -254,18 +260,32 @@ def evaluate(ctx: Context, scenes_list, steps: int, model: PressureMLP | None,
     return out
 
 
+def window_starts(count: int, steps: int, horizon: int | None) -> list[int]:
+    """First recorded step of each training scene: every scene from step 0
+    without a horizon; with one, `count` windows of `steps` steps spread
+    evenly over [0, horizon), the first at 0 and the last ending at the
+    horizon (16 windows of 200 over 2000 steps start every 120 steps)."""
+    if not horizon or horizon <= steps or count < 2:
+        return [0] * count
+    span = horizon - steps
+    return [(i * span) // (count - 1) for i in range(count)]
+
+
 def train_and_evaluate(ctx: Context, make_scene, train_ids, eval_ids, steps: int = 200,
                        stride: int = 16, epochs: int = 4, batch: int = 8192, lr: float = 1e-3,
-                       seed: int = 1, log=None, eval_steps: int | None = None):
+                       seed: int = 1, log=None, eval_steps: int | None = None,
+                       horizon: int | None = None):
     """Record plain-PCISPH converged pressures on the training scenes (`steps`
-    steps each, every `stride`-th particle), train the network on the GPU,
-    then compare warm starts on the held-out scenes over `eval_steps` steps
-    (default: `steps`).  Returns (model, summary)."""
+    steps each, every `stride`-th particle; with `horizon`, the scenes'
+    windows spread over the first `horizon` steps, `window_starts`), train the
+    network on the GPU, then compare warm starts on the held-out scenes over
+    their first `eval_steps` steps (default: `steps`).  Returns (model, summary)."""
     eval_steps = steps if eval_steps is None else eval_steps
     train_scenes = [make_scene(i) for i in train_ids]
+    starts = window_starts(len(train_scenes), steps, horizon)
     n_per = sum((sc.n + stride - 1) // stride for sc in train_scenes) * steps
     ds = Dataset(ctx, n_per)
-    plain_iters = record(ctx, train_scenes, steps, ds, stride=stride)
+    plain_iters = record(ctx, train_scenes, steps, ds, stride=stride, warmup=starts)
     from .system import FluidSystem
     probe = FluidSystem(ctx, train_scenes[0].params, train_scenes[0].n)
     pscale = probe.pressure_scale
@@ -274,7 +294,7 @@ def train_and_evaluate(ctx: Context, make_scene, train_ids, eval_ids, steps: int
     samples = ds.count
     ds.close()
     summary = {"train_scenes": list(train_ids), "eval_scenes": list(eval_ids), "steps": steps,
-               "eval_steps": eval_steps, "stride": stride,
+               "window_starts": starts, "eval_steps": eval_steps, "stride": stride,
                "samples": int(samples), "epochs": epochs, "lr": lr, "batch": batch,
                "train_plain_iters_per_step": float(np.mean(plain_iters)),
                "mse_per_epoch": [float(x) for x in losses]}
@@ -293,6 +313,8 @@ def main(argv=None) -> int:
     ap.add_argument("--n-side", type=int, default=64)
     ap.add_argument("--stride", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--horizon", type=int, default=0,
+                    help="spread the training windows over this many steps (0: all from step 0)")
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=None, help="write the trained model (.npz)")
@@ -307,7 +329,7 @@ def main(argv=None) -> int:
     model, summary = train_and_evaluate(
         ctx, make_scene, range(a.train_seeds), range(a.train_seeds, a.train_seeds + a.eval_seeds),
         steps=a.steps, stride=a.stride, epochs=a.epochs, batch=a.batch, lr=a.lr,
-        log=lambda m: print(m, flush=True))
+        log=lambda m: print(m, flush=True), horizon=a.horizon or None)
     if a.out:
         model.save(a.out)
     text = json.dumps(summary, indent=1)
