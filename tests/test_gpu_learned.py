@@ -112,6 +112,33 @@ def test_features_and_recording_bitwise(ctx, oracle):
     assert g.pressure_scale == pytest.approx(o.pressure_scale, rel=1e-6)
 
 
+def test_recording_windows_start_where_asked(ctx):
+    """record() with one warm-up count per scene: the second scene's samples
+    are exactly those of a system stepped 5 steps unrecorded, then 3
+    recorded; the first scene's start at step 0."""
+    from psb_project_amd.system import FluidSystem
+    sc = scenes.pcisph_dam_break(n_side=16)
+    stride, steps = 4, 3
+    per = -(-sc.n // stride) * steps
+    ds = learned.Dataset(ctx, 2 * per)
+    learned.record(ctx, [sc, sc], steps, ds, stride=stride, warmup=[0, 5])
+    assert ds.count == 2 * per
+    feat, tgt = ds.download()
+    for warm, block in ((0, slice(0, per)), (5, slice(per, 2 * per))):
+        ref = learned.Dataset(ctx, per)
+        g = FluidSystem(ctx, sc.params, sc.n)
+        g.set_particles(sc.pos4, sc.vel4, sc.vh4)
+        if warm:
+            g.update(warm)
+        g.set_recorder(ref, stride=stride)
+        g.update(steps)
+        g.close()
+        rf, rt = ref.download()
+        assert np.array_equal(feat[:, block], rf) and np.array_equal(tgt[block], rt), warm
+        ref.close()
+    ds.close()
+
+
 def test_training_reduces_loss(ctx):
     sc = scenes.pcisph_dam_break(n_side=16)
     ds = learned.Dataset(ctx, 400_000)

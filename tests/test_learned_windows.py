@@ -20,3 +20,10 @@ def test_windows_from_the_start_without_a_longer_horizon(count, steps, horizon):
 def test_windows_uneven_span_stays_inside():
     w = window_starts(7, 100, 1000)
     assert w[0] == 0 and w[-1] == 900 and w == sorted(w)
+
+
+def test_record_refuses_a_warmup_list_of_the_wrong_length():
+    from psb_project_amd import learned, scenes
+    sc = scenes.pcisph_dam_break(n_side=8)
+    with pytest.raises(ValueError, match="warm-up counts"):
+        learned.record(None, [sc, sc], 3, None, warmup=[1])
