@@ -72,6 +72,9 @@ ALG_BYTES = {
     "density_epilogue": 12,
     "wcsph_forces": 116,      # forces 44 + integrate/box/interpolate 72
     "grid_build": 72,         # hash 20 + counting sort / ranges 4 + reorder 48 (+ 8 n_cells/N)
+    # per-particle materials (one thread per particle, material index 1 B)
+    "mix_pci_density": 29,    # x* 12 + material 1 + p read/write 8 + rho* 4 + p/rho^2 4
+    "mix_pci_force": 80,      # x*,p 16 + rho* 4 + a_np 12 + a_s 12 + x 12 + v_half 12 + write x* 12
 }
 
 
@@ -550,12 +553,25 @@ def materials_leg(ctx, n_side: int = 32, steps: int = 100):
     fs.set_particles(sc.pos4)
     fs.set_materials(sc.meta["materials"], sc.meta["index"])
     fs.update(5)
+    fs.reset_timing()
+    fs.set_timing(True, every=4)
     elapsed, st = timed_run(ctx, fs, steps)
+    timing = fs.timing()
+    fs.set_timing(False)
+    kernels = {k: {"avg_us": round(1e3 * t / c, 3), "launches": c}
+               for k, (t, c) in timing.items() if c}
     out = {"value": round(sc.n * steps / elapsed, 1),
            "ms_per_step": round(1e3 * elapsed / steps, 4),
            "particles": sc.n, "materials": fs.num_materials, "steps": steps,
            "dt": sc.params.dt, "flags": st["flags"],
-           "iters_per_step": st["total_iterations"] / steps}
+           "iters_per_step": st["total_iterations"] / steps, "kernels": kernels}
+    if "mix_pci_force" in kernels:
+        # HIP events around the launches of every 4th timed step
+        out["roofline"] = roofline_of("mix_pci_force", sc.n, kernels["mix_pci_force"]["avg_us"],
+                                      sc.name, trace=False)
+        if "mix_pci_density" in kernels:
+            out["roofline"]["other_kernel"] = roofline_of(
+                "mix_pci_density", sc.n, kernels["mix_pci_density"]["avg_us"], sc.name, trace=False)
     fs.close()
     return out
 

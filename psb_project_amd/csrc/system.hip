@@ -28,12 +28,13 @@ namespace psb {
 
 enum Phase {
   PH_GRID, PH_NEIGHBORS, PH_EPILOGUE, PH_FORCES, PH_NONPRESSURE, PH_PCI_DENSITY, PH_PCI_FORCE,
-  PH_MLP, PH_RECORD, PH_COUNT
+  PH_MLP, PH_RECORD, PH_MIX_PROLOGUE, PH_MIX_DENSITY, PH_MIX_FORCE, PH_COUNT
 };
 // grid_build = hash/count + 3-kernel scan + scatter + stable rank/reorder
 static const char* kPhaseNames[PH_COUNT] = {
     "grid_build", "neighbors_density", "density_epilogue", "wcsph_forces", "pci_nonpressure",
-    "pci_density", "pci_force", "mlp_infer", "record"};
+    "pci_density", "pci_force", "mlp_infer", "record", "mix_pci_prologue", "mix_pci_density",
+    "mix_pci_force"};
 
 psb_status grid_create(psb_context ctx, const psb_grid_desc& d, uint32_t capacity, psb_grid* out);
 
@@ -271,12 +272,19 @@ static psb_status mixed_pcisph_step(psb_system s, int step) {
   PSB_TRY(mixed_lists(s, step));
   const MixArgs a = mix_args(s);
   hipStream_t st = s->ctx->stream;
-  PSB_TRY(launch_mix_pci_prologue(a, s->xp0, st));
+  {
+    Timed t(s, PH_MIX_PROLOGUE, step);
+    PSB_TRY(launch_mix_pci_prologue(a, s->xp0, st));
+  }
   constexpr int kIters = 3;
   float4* cur = s->xp0;
   float4* nxt = s->xp1;
   for (int it = 0; it < kIters; ++it) {
-    PSB_TRY(launch_mix_pci_density(a, cur, it, st));
+    {
+      Timed t(s, PH_MIX_DENSITY, step);
+      PSB_TRY(launch_mix_pci_density(a, cur, it, st));
+    }
+    Timed t(s, PH_MIX_FORCE, step);
     PSB_TRY(launch_mix_pci_force(a, cur, nxt, it == kIters - 1, kIters, st));
     std::swap(cur, nxt);
   }
