@@ -1511,31 +1511,39 @@ template <class R> struct System {
     iter_hist.push_back(kIters); err_hist.push_back(err);
   }
 
+  // The mixed corrector passes' sums: four partial sums over the list
+  // entries k = 0, 1, 2, 3 (mod 4), each in increasing k, added as
+  // (s0 + s1) + (s2 + s3) (the kernels' four lanes per particle).
+  static R mix_total(const R (&a)[4]) { return (a[0] + a[1]) + (a[2] + a[3]); }
+
   R mixed_pci_density(std::vector<V4<R>>& X) {
     R emax = 0;
 #pragma omp parallel for schedule(static) reduction(max : emax) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       const V4<R> xi = X[p];
-      R S = 0, W1x = 0, W1y = 0, W1z = 0, W2 = 0;
+      R S4[4] = {}, X4[4] = {}, Y4[4] = {}, Z4[4] = {}, V4s[4] = {};
       for (uint32_t k = 0; k < count[p]; ++k) {
+        const uint32_t a = k & 3u;
         const uint32_t j = list[(uint64_t)k * n + p];
         const V4<R> xj = X[j];
         const MatPairR<R>& P = mpair(ms[p], ms[j]);
         R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         R d2 = dot3(dx, dy, dz, dx, dy, dz);
         R w = P.h2 - d2;
-        S = fmaR((w * w) * w, P.mc, S);
+        S4[a] = fmaR((w * w) * w, P.mc, S4[a]);
         if (d2 > (R)0) {
           R dd = std::sqrt(d2);
           R invd = (R)1 / dd;
           R hd = P.h - dd;
           R g = P.cs * (hd * hd);
           R ux = (dx * invd) * g, uy = (dy * invd) * g, uz = (dz * invd) * g;
-          W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
-          W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
+          X4[a] = X4[a] + ux; Y4[a] = Y4[a] + uy; Z4[a] = Z4[a] + uz;
+          V4s[a] = V4s[a] + dot3(ux, uy, uz, ux, uy, uz);
         }
       }
+      const R S = mix_total(S4), W1x = mix_total(X4), W1y = mix_total(Y4), W1z = mix_total(Z4),
+              W2 = mix_total(V4s);
       const MatOwnR<R>& o = mt.own[ms[p]];
       const R r = S;
       const R e = r - o.rho0;
@@ -1554,8 +1562,9 @@ template <class R> struct System {
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       const V4<R> xi = X[p];
-      R sx = 0, sy = 0, sz = 0;
+      R sx4[4] = {}, sy4[4] = {}, sz4[4] = {};
       for (uint32_t k = 0; k < count[p]; ++k) {
+        const uint32_t a = k & 3u;
         const uint32_t j = list[(uint64_t)k * n + p];
         const V4<R> xj = X[j];
         R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1566,8 +1575,11 @@ template <class R> struct System {
         R invd = (R)1 / dd;
         R hd = P.h - dd;
         R c = (P.csm * (hd * hd)) * (xi.w + xj.w);
-        sx = fmaR(dx * invd, c, sx); sy = fmaR(dy * invd, c, sy); sz = fmaR(dz * invd, c, sz);
+        sx4[a] = fmaR(dx * invd, c, sx4[a]);
+        sy4[a] = fmaR(dy * invd, c, sy4[a]);
+        sz4[a] = fmaR(dz * invd, c, sz4[a]);
       }
+      const R sx = mix_total(sx4), sy = mix_total(sy4), sz = mix_total(sz4);
       const R rp = rho_pred[p];
       const R cpf = -rp;
       const V4<R> f = fa[p], s = fs4[p];

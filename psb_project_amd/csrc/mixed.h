@@ -20,6 +20,9 @@
 namespace psb {
 
 constexpr int kMaxMaterials = PSB_MAX_MATERIALS;
+// threads per particle in the mixed PCISPH corrector passes (a power of two
+// dividing 64: a particle's lanes share one wavefront)
+constexpr uint32_t kMixLanes = 4;
 
 // The particle's own material.
 struct MatOwn {
@@ -67,6 +70,7 @@ struct MixArgs {
   float4* fa; float4* fs;
   uint32_t* flags;
   PciStatus* st;
+  ErrSlots es;                        // per-iteration error slots (PCISPH systems)
 };
 
 // After the neighbour search over the system radius: each particle's

@@ -1,8 +1,10 @@
 // mixed.hip -- reference-mode WCSPH and PCISPH steps with per-particle
 // materials (a table of at most kMaxMaterials, one index per particle).
 //
-// One thread per particle over the 32-bit list; the material table (3.3 KB)
-// is copied into LDS by every workgroup.  Canonical FP32 order (the oracle
+// One thread per particle over the 32-bit list, four in the PCISPH corrector
+// passes (k_mix_pci_density / k_mix_pci_force: lane l sums the entries
+// k = l mod 4, the partial sums added as (s0 + s1) + (s2 + s3)); the
+// material table (3.3 KB) is copied into LDS by every workgroup.  Canonical FP32 order (the oracle
 // runs the same expressions; fma only where written):
 //   list:      j kept when dot3(d) <= own[a].h2 at the start positions
 //   density:   S = fma((w * w) * w, P.mc, S), w = P.h2 - d2 (no clamp)
@@ -41,6 +43,9 @@ __global__ __launch_bounds__(256) void k_mix_prepare(MixArgs a) {
   const uint32_t p = particle_index();
   if (p == 0)
     for (int k = 0; k < kMaxIterSlots; ++k) a.st->err[k] = 0u;
+  if (a.es.base)
+    for (uint32_t k = p; k < (uint32_t)(kErrSlots * kMaxIterSlots); k += gridDim.x * blockDim.x)
+      a.es.base[(size_t)(k / kMaxIterSlots) * a.es.stride + k % kMaxIterSlots] = 0u;
   if (p >= a.n) return;
   const uint32_t m = a.mat_by_id[a.aid[p]];
   a.ms[p] = (uint8_t)m;
@@ -223,19 +228,27 @@ __global__ __launch_bounds__(256) void k_mix_pci_nonpressure(MixArgs a, float4* 
 
 // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
 // on predicted positions.
+// The corrector passes take kMixLanes threads per particle: lane l sums the
+// list entries k = l (mod 4) in increasing k and the four partial sums are
+// added as (s0 + s1) + (s2 + s3), the same bits on every lane of the group.
+__device__ __forceinline__ float mix_total(float v) {
+  const float x = v + __shfl_xor(v, 1, 64);
+  return x + __shfl_xor(x, 2, 64);
+}
+
 __global__ __launch_bounds__(256) void k_mix_pci_density(MixArgs a, float4* __restrict__ X,
                                                          int it) {
   __shared__ MatTable t;
   load_table(a.tbl, t);
-  const uint32_t p = particle_index();
-  // every lane reaches the wave maximum below; lanes past n add nothing
+  const uint32_t g = particle_index();
+  const uint32_t p = g / kMixLanes, lane = g % kMixLanes;
   const bool valid = p < a.n;
   const uint32_t pp = valid ? p : 0u;
   const uint32_t mi = a.ms[pp];
   const float4 xi = X[pp];
   float S = 0.f, W1x = 0.f, W1y = 0.f, W1z = 0.f, W2 = 0.f;
   const uint32_t c = valid ? a.cnt[p] : 0u;
-  for (uint32_t k = 0; k < c; ++k) {
+  for (uint32_t k = lane; k < c; k += kMixLanes) {
     const uint32_t j = a.list[(uint64_t)k * a.n + p];
     const float4 xj = X[j];
     const MatPair& P = pair_of(t, mi, a.ms[j]);
@@ -247,14 +260,17 @@ __global__ __launch_bounds__(256) void k_mix_pci_density(MixArgs a, float4* __re
       float dd, invd;
       dist_and_inverse(d2, dd, invd);
       const float hd = P.h - dd;
-      const float g = P.cs * (hd * hd);
-      const float ux = (dx * invd) * g, uy = (dy * invd) * g, uz = (dz * invd) * g;
+      const float gw = P.cs * (hd * hd);
+      const float ux = (dx * invd) * gw, uy = (dy * invd) * gw, uz = (dz * invd) * gw;
       W1x = W1x + ux; W1y = W1y + uy; W1z = W1z + uz;
       W2 = W2 + dot3(ux, uy, uz, ux, uy, uz);
     }
   }
+  S = mix_total(S);
+  W1x = mix_total(W1x); W1y = mix_total(W1y); W1z = mix_total(W1z);
+  W2 = mix_total(W2);
   uint32_t err = 0u;
-  if (valid) {
+  if (valid && lane == 0u) {
     const MatOwn& o = t.own[mi];
     const float r = S;
     const float e = r - o.rho0;
@@ -266,33 +282,43 @@ __global__ __launch_bounds__(256) void k_mix_pci_density(MixArgs a, float4* __re
     X[p].w = pr / (r * r);
     err = __float_as_uint((e > 0.f ? e : 0.f) * o.inv_rho0);
   }
-  // errors are >= 0, so their bit patterns order as the floats do
+  // one atomic per wave, spread over kErrSlots words (4 lanes per particle
+  // make 4x the waves; on one word their atomics serialised)
   err = wave_max_u(err);
-  if ((threadIdx.x & 63u) == 0u && err) atomicMax(&a.st->err[it], err);
+  if ((threadIdx.x & 63u) == 0u && err)
+    atomicMax(a.es.base ? &a.es.base[(size_t)(blockIdx.x % kErrSlots) * a.es.stride + it]
+                        : &a.st->err[it],
+              err);
 }
 
-// compute_Pressure_Force_PCI (PciSphSystem_Kernel.cu:88-114) +
-// integrateSPH_LeapFrog_Forces from the start state (restore_Pos_Vel_PCI).
 __global__ __launch_bounds__(256) void k_mix_pci_force(MixArgs a, const float4* __restrict__ X,
                                                        float4* __restrict__ Xn, int fin,
                                                        int iters) {
   __shared__ MatTable t;
   load_table(a.tbl, t);
-  const uint32_t p = particle_index();
-  if (fin && p == 0) {
+  const uint32_t g = particle_index();
+  const uint32_t p = g / kMixLanes, lane = g % kMixLanes;
+  if (fin && g == 0) {
     PciStatus* st = a.st;
-    const uint32_t e = st->err[iters - 1];
+    uint32_t e = st->err[iters - 1];
+    if (a.es.base) {
+      for (int k = 0; k < kErrSlots; ++k)
+        e = max(e, a.es.base[(size_t)k * a.es.stride + iters - 1]);
+      st->err[iters - 1] = e;
+    }
     st->last_iters = iters;
     st->last_err = e;
     st->total_iters += (unsigned long long)iters;
     if (e > st->worst_err) st->worst_err = e;
   }
+  // the four lanes of a particle share one wave (n * 4 threads, groups of
+  // four aligned), so a whole group leaves here together before the shuffles
   if (p >= a.n) return;
   const uint32_t mi = a.ms[p];
   const float4 xi = X[p];
   float sx = 0.f, sy = 0.f, sz = 0.f;
   const uint32_t c = a.cnt[p];
-  for (uint32_t k = 0; k < c; ++k) {
+  for (uint32_t k = lane; k < c; k += kMixLanes) {
     const uint32_t j = a.list[(uint64_t)k * a.n + p];
     const float4 xj = X[j];
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -307,6 +333,10 @@ __global__ __launch_bounds__(256) void k_mix_pci_force(MixArgs a, const float4* 
     sy = __builtin_fmaf(dy * invd, cc, sy);
     sz = __builtin_fmaf(dz * invd, cc, sz);
   }
+  sx = mix_total(sx);
+  sy = mix_total(sy);
+  sz = mix_total(sz);
+  if (lane != 0u) return;
   const float rp = a.rho_pred[p];
   const float cpf = -rp;
   const float4 f = a.fa[p], fs = a.fs[p];
@@ -397,7 +427,8 @@ psb_status launch_mix_pci_prologue(const MixArgs& a, float4* xp, hipStream_t s) 
 psb_status launch_mix_pci_density(const MixArgs& a, float4* X, int it, hipStream_t s) {
   PSB_ARG(it >= 0 && it < kMaxIterSlots, "iteration out of range");
   if (!a.n) return PSB_OK;
-  hipLaunchKernelGGL(k_mix_pci_density, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a, X, it);
+  hipLaunchKernelGGL(k_mix_pci_density, dim3(blocks_for((uint64_t)a.n * kMixLanes)), dim3(kBlock),
+                     0, s, a, X, it);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
@@ -405,8 +436,8 @@ psb_status launch_mix_pci_density(const MixArgs& a, float4* X, int it, hipStream
 psb_status launch_mix_pci_force(const MixArgs& a, const float4* X, float4* Xn, bool fin,
                                 int iters, hipStream_t s) {
   PSB_ARG(iters >= 1 && iters <= kMaxIterSlots, "iteration count out of range");
-  hipLaunchKernelGGL(k_mix_pci_force, dim3(blocks_for(std::max(a.n, 1u))), dim3(kBlock), 0, s, a,
-                     X, Xn, fin ? 1 : 0, iters);
+  hipLaunchKernelGGL(k_mix_pci_force, dim3(blocks_for((uint64_t)std::max(a.n, 1u) * kMixLanes)),
+                     dim3(kBlock), 0, s, a, X, Xn, fin ? 1 : 0, iters);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }

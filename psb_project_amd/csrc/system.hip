@@ -244,7 +244,7 @@ static MixArgs mix_args(psb_system s) {
   a.ax = s->ax; a.av = s->av; a.avh = s->avh; a.aid = s->aid;
   a.bx = s->bx; a.bv = s->bv; a.bvh = s->bvh; a.bid = s->bid; a.bp = s->bp;
   a.rho = s->rho; a.pres = s->pres; a.rho_pred = s->rho_pred; a.fa = s->fa; a.fs = s->fs;
-  a.flags = s->flags; a.st = s->st;
+  a.flags = s->flags; a.st = s->st; a.es = s->es;
   return a;
 }
 
