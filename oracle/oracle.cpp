@@ -1356,26 +1356,30 @@ template <class R> struct System {
     }
   }
 
-  R mixed_density(uint32_t p, const std::vector<V4<R>>& X) const {
+  // lanes = 4: the sums split as in mix_total (the PCISPH passes); 1: one
+  // sum in list order (WCSPH)
+  R mixed_density(uint32_t p, const std::vector<V4<R>>& X, uint32_t lanes = 1) const {
     const V4<R> xi = X[p];
-    R S = 0;
+    R S4[4] = {};
     for (uint32_t k = 0; k < count[p]; ++k) {
+      const uint32_t a = k % lanes;
       const uint32_t j = list[(uint64_t)k * n + p];
       const MatPairR<R>& P = mpair(ms[p], ms[j]);
       const V4<R> xj = X[j];
       R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       R w = P.h2 - dot3(dx, dy, dz, dx, dy, dz);
-      S = fmaR((w * w) * w, P.mc, S);
+      S4[a] = fmaR((w * w) * w, P.mc, S4[a]);
     }
-    return S;
+    return lanes == 1 ? S4[0] : mix_total(S4);
   }
 
   // internalForces / compute_Viscosity_SurfaceTension_Force_PCI pair sums on
   // the start positions (Ax.w = p / rho^2 when press, Av.w = 1 / rho).
-  PairSums<R> mixed_sums(uint32_t p, bool press) const {
-    PairSums<R> s{};
+  PairSums<R> mixed_sums(uint32_t p, bool press, uint32_t lanes = 1) const {
+    PairSums<R> q[4] = {};
     const V4<R> xi = Ax[p], vi = Av[p];
     for (uint32_t k = 0; k < count[p]; ++k) {
+      PairSums<R>& s = q[k % lanes];
       const uint32_t j = list[(uint64_t)k * n + p];
       const V4<R> xj = Ax[j], vj = Av[j];
       R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1405,6 +1409,22 @@ template <class R> struct System {
       s.nn[1] = fmaR(dy, an, s.nn[1]);
       s.nn[2] = fmaR(dz, an, s.nn[2]);
     }
+    if (lanes == 1) return q[0];
+    PairSums<R> s;
+    auto mix = [&](R PairSums<R>::*f) {
+      const R a[4] = {q[0].*f, q[1].*f, q[2].*f, q[3].*f};
+      return mix_total(a);
+    };
+    auto mix3 = [&](R (PairSums<R>::*f)[3], R out[3]) {
+      for (int c = 0; c < 3; ++c) {
+        const R a[4] = {(q[0].*f)[c], (q[1].*f)[c], (q[2].*f)[c], (q[3].*f)[c]};
+        out[c] = mix_total(a);
+      }
+    };
+    mix3(&PairSums<R>::fp, s.fp);
+    mix3(&PairSums<R>::fv, s.fv);
+    s.fs = mix(&PairSums<R>::fs);
+    mix3(&PairSums<R>::nn, s.nn);
     return s;
   }
 
@@ -1473,7 +1493,7 @@ template <class R> struct System {
 #pragma omp parallel for schedule(static) if(n >= kOmpMinItems)
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
-      const R r = mixed_density(p, Ax);
+      const R r = mixed_density(p, Ax, 4);
       rho[p] = r;
       pres[p] = 0;
       Av[p].w = (R)1 / r;
@@ -1482,7 +1502,7 @@ template <class R> struct System {
     for (int64_t kk = 0; kk < (int64_t)n; ++kk) {
       const uint32_t p = (uint32_t)kk;
       const MatOwnR<R>& o = mt.own[ms[p]];
-      const PairSums<R> s = mixed_sums(p, false);
+      const PairSums<R> s = mixed_sums(p, false, 4);
       R fS[3];
       mixed_surface(s, o.l, (R)0.0728f, fS);
       const R ri = rho[p];

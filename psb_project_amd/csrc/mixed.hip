@@ -1,9 +1,9 @@
 // mixed.hip -- reference-mode WCSPH and PCISPH steps with per-particle
 // materials (a table of at most kMaxMaterials, one index per particle).
 //
-// One thread per particle over the 32-bit list, four in the PCISPH corrector
-// passes (k_mix_pci_density / k_mix_pci_force: lane l sums the entries
-// k = l mod 4, the partial sums added as (s0 + s1) + (s2 + s3)); the
+// One thread per particle over the 32-bit list in the WCSPH step, four in
+// the PCISPH passes (lane l sums the entries k = l mod 4, the partial sums
+// added as (s0 + s1) + (s2 + s3)); the
 // material table (3.3 KB) is copied into LDS by every workgroup.  Canonical FP32 order (the oracle
 // runs the same expressions; fma only where written):
 //   list:      j kept when dot3(d) <= own[a].h2 at the start positions
@@ -62,13 +62,24 @@ __global__ __launch_bounds__(256) void k_mix_prepare(MixArgs a) {
   a.cnt[p] = kept;
 }
 
-// Density over positions X (start or predicted).
+// The PCISPH passes take kMixLanes threads per particle: lane l sums the
+// list entries k = l (mod 4) in increasing k and the four partial sums are
+// added as (s0 + s1) + (s2 + s3), the same bits on every lane of the group.
+__device__ __forceinline__ float mix_total(float v) {
+  const float x = v + __shfl_xor(v, 1, 64);
+  return x + __shfl_xor(x, 2, 64);
+}
+
+// Density over positions X (start or predicted); L lanes per particle (1 or
+// kMixLanes, this thread being `lane` of them).
+template <uint32_t L>
 __device__ __forceinline__ float mixed_density(const MatTable& t, const MixArgs& a, uint32_t p,
-                                               uint32_t mi, const float4* __restrict__ X) {
+                                               uint32_t lane, uint32_t mi,
+                                               const float4* __restrict__ X) {
   const float4 xi = X[p];
   float S = 0.f;
   const uint32_t c = a.cnt[p];
-  for (uint32_t k = 0; k < c; ++k) {
+  for (uint32_t k = lane; k < c; k += L) {
     const uint32_t j = a.list[(uint64_t)k * a.n + p];
     const float4 xj = X[j];
     const MatPair& P = pair_of(t, mi, a.ms[j]);
@@ -76,7 +87,7 @@ __device__ __forceinline__ float mixed_density(const MatTable& t, const MixArgs&
     const float w = P.h2 - dot3(dx, dy, dz, dx, dy, dz);
     S = __builtin_fmaf((w * w) * w, P.mc, S);
   }
-  return S;
+  return L == 1 ? S : mix_total(S);
 }
 
 struct MixSums {
@@ -85,9 +96,9 @@ struct MixSums {
 
 // Pair sums of internalForces / compute_Viscosity_SurfaceTension_Force_PCI
 // on the start positions (x.w = p / rho^2 when PRESS, v.w = 1 / rho).
-template <bool PRESS>
+template <bool PRESS, uint32_t L>
 __device__ __forceinline__ MixSums mixed_sums(const MatTable& t, const MixArgs& a, uint32_t p,
-                                              uint32_t mi) {
+                                              uint32_t lane, uint32_t mi) {
   MixSums s;
   s.fp[0] = s.fp[1] = s.fp[2] = 0.f;
   s.fv[0] = s.fv[1] = s.fv[2] = 0.f;
@@ -95,7 +106,7 @@ __device__ __forceinline__ MixSums mixed_sums(const MatTable& t, const MixArgs& 
   s.nn[0] = s.nn[1] = s.nn[2] = 0.f;
   const float4 xi = a.ax[p], vi = a.av[p];
   const uint32_t c = a.cnt[p];
-  for (uint32_t k = 0; k < c; ++k) {
+  for (uint32_t k = lane; k < c; k += L) {
     const uint32_t j = a.list[(uint64_t)k * a.n + p];
     const float4 xj = a.ax[j], vj = a.av[j];
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -125,6 +136,14 @@ __device__ __forceinline__ MixSums mixed_sums(const MatTable& t, const MixArgs& 
     s.nn[1] = __builtin_fmaf(dy, an, s.nn[1]);
     s.nn[2] = __builtin_fmaf(dz, an, s.nn[2]);
   }
+  if (L > 1) {
+    for (int i = 0; i < 3; ++i) {
+      if (PRESS) s.fp[i] = mix_total(s.fp[i]);
+      s.fv[i] = mix_total(s.fv[i]);
+      s.nn[i] = mix_total(s.nn[i]);
+    }
+    s.fs = mix_total(s.fs);
+  }
   return s;
 }
 
@@ -147,7 +166,7 @@ __global__ __launch_bounds__(256) void k_mix_wcsph_density(MixArgs a) {
   const uint32_t p = particle_index();
   if (p >= a.n) return;
   const uint32_t mi = a.ms[p];
-  const float r = mixed_density(t, a, p, mi, a.ax);
+  const float r = mixed_density<1>(t, a, p, 0u, mi, a.ax);
   const float pr = t.own[mi].k * (r - t.own[mi].rho0);
   a.rho[p] = r;
   a.pres[p] = pr;
@@ -163,7 +182,7 @@ __global__ __launch_bounds__(256) void k_mix_wcsph_forces(MixArgs a) {
   const uint32_t p = particle_index();
   if (p >= a.n) return;
   const uint32_t mi = a.ms[p];
-  const MixSums s = mixed_sums<true>(t, a, p, mi);
+  const MixSums s = mixed_sums<true, 1>(t, a, p, 0u, mi);
   const float ri = a.rho[p];
   const float cpf = -ri;
   float fS[3];
@@ -193,9 +212,11 @@ __global__ __launch_bounds__(256) void k_mix_wcsph_forces(MixArgs a) {
 __global__ __launch_bounds__(256) void k_mix_pci_density0(MixArgs a) {
   __shared__ MatTable t;
   load_table(a.tbl, t);
-  const uint32_t p = particle_index();
-  if (p >= a.n) return;
-  const float r = mixed_density(t, a, p, a.ms[p], a.ax);
+  const uint32_t g = particle_index();
+  const uint32_t p = g / kMixLanes, lane = g % kMixLanes;
+  if (p >= a.n) return;   // a particle's four lanes leave together
+  const float r = mixed_density<kMixLanes>(t, a, p, lane, a.ms[p], a.ax);
+  if (lane != 0u) return;
   a.rho[p] = r;
   a.pres[p] = 0.f;
   a.av[p].w = 1.0f / r;
@@ -207,10 +228,12 @@ __global__ __launch_bounds__(256) void k_mix_pci_density0(MixArgs a) {
 __global__ __launch_bounds__(256) void k_mix_pci_nonpressure(MixArgs a, float4* __restrict__ xp) {
   __shared__ MatTable t;
   load_table(a.tbl, t);
-  const uint32_t p = particle_index();
-  if (p >= a.n) return;
+  const uint32_t g = particle_index();
+  const uint32_t p = g / kMixLanes, lane = g % kMixLanes;
+  if (p >= a.n) return;   // a particle's four lanes leave together
   const uint32_t mi = a.ms[p];
-  const MixSums s = mixed_sums<false>(t, a, p, mi);
+  const MixSums s = mixed_sums<false, kMixLanes>(t, a, p, lane, mi);
+  if (lane != 0u) return;
   float fS[3];
   mixed_surface(s, t.own[mi].l, 0.0728f, fS);
   const float ri = a.rho[p];
@@ -228,14 +251,6 @@ __global__ __launch_bounds__(256) void k_mix_pci_nonpressure(MixArgs a, float4* 
 
 // compute_Density_PCI + compute_Pressure_PCI (PciSphSystem_Kernel.cu:36-85)
 // on predicted positions.
-// The corrector passes take kMixLanes threads per particle: lane l sums the
-// list entries k = l (mod 4) in increasing k and the four partial sums are
-// added as (s0 + s1) + (s2 + s3), the same bits on every lane of the group.
-__device__ __forceinline__ float mix_total(float v) {
-  const float x = v + __shfl_xor(v, 1, 64);
-  return x + __shfl_xor(x, 2, 64);
-}
-
 __global__ __launch_bounds__(256) void k_mix_pci_density(MixArgs a, float4* __restrict__ X,
                                                          int it) {
   __shared__ MatTable t;
@@ -417,9 +432,11 @@ psb_status launch_mix_wcsph(const MixArgs& a, hipStream_t s) {
 
 psb_status launch_mix_pci_prologue(const MixArgs& a, float4* xp, hipStream_t s) {
   if (!a.n) return PSB_OK;
-  hipLaunchKernelGGL(k_mix_pci_density0, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_mix_pci_density0, dim3(blocks_for((uint64_t)a.n * kMixLanes)), dim3(kBlock),
+                     0, s, a);
   PSB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_mix_pci_nonpressure, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a, xp);
+  hipLaunchKernelGGL(k_mix_pci_nonpressure, dim3(blocks_for((uint64_t)a.n * kMixLanes)),
+                     dim3(kBlock), 0, s, a, xp);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
