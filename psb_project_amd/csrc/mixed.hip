@@ -40,26 +40,41 @@ __device__ __forceinline__ const MatPair& pair_of(const MatTable& t, uint32_t a,
 __global__ __launch_bounds__(256) void k_mix_prepare(MixArgs a) {
   __shared__ MatTable t;
   load_table(a.tbl, t);
-  const uint32_t p = particle_index();
-  if (p == 0)
+  const uint32_t g = particle_index();
+  if (g == 0)
     for (int k = 0; k < kMaxIterSlots; ++k) a.st->err[k] = 0u;
   if (a.es.base)
-    for (uint32_t k = p; k < (uint32_t)(kErrSlots * kMaxIterSlots); k += gridDim.x * blockDim.x)
+    for (uint32_t k = g; k < (uint32_t)(kErrSlots * kMaxIterSlots); k += gridDim.x * blockDim.x)
       a.es.base[(size_t)(k / kMaxIterSlots) * a.es.stride + k % kMaxIterSlots] = 0u;
-  if (p >= a.n) return;
+  // kMixLanes lanes per particle test four consecutive entries per round; a
+  // ballot over the group gives each kept entry its place, so the list is
+  // compacted in place in its own order (a round writes only to positions
+  // its group has already read)
+  const uint32_t p = g / kMixLanes, lane = g % kMixLanes;
+  if (p >= a.n) return;   // a particle's four lanes leave together
   const uint32_t m = a.mat_by_id[a.aid[p]];
-  a.ms[p] = (uint8_t)m;
+  if (lane == 0u) a.ms[p] = (uint8_t)m;
   const float h2 = t.own[m].h2;
   const float4 xi = a.ax[p];
   const uint32_t c = a.cnt[p];
+  const uint32_t first = (threadIdx.x & 63u) & ~(kMixLanes - 1u);
+  const uint32_t below = (1u << lane) - 1u;
   uint32_t kept = 0;
-  for (uint32_t k = 0; k < c; ++k) {
-    const uint32_t j = a.list[(uint64_t)k * a.n + p];
-    const float4 xj = a.ax[j];
-    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    if (dot3(dx, dy, dz, dx, dy, dz) <= h2) a.list[(uint64_t)(kept++) * a.n + p] = j;
+  for (uint32_t k0 = 0; k0 < c; k0 += kMixLanes) {
+    const uint32_t k = k0 + lane;
+    bool keep = false;
+    uint32_t j = 0;
+    if (k < c) {
+      j = a.list[(uint64_t)k * a.n + p];
+      const float4 xj = a.ax[j];
+      const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      keep = dot3(dx, dy, dz, dx, dy, dz) <= h2;
+    }
+    const uint32_t grp = (uint32_t)(__ballot(keep) >> first) & ((1u << kMixLanes) - 1u);
+    if (keep) a.list[(uint64_t)(kept + __popc(grp & below)) * a.n + p] = j;
+    kept += __popc(grp);
   }
-  a.cnt[p] = kept;
+  if (lane == 0u) a.cnt[p] = kept;
 }
 
 // The PCISPH passes take kMixLanes threads per particle: lane l sums the
@@ -416,7 +431,8 @@ psb_status make_mat_table(const psb_sph_params& sys, const psb_sph_params* mats,
 }
 
 psb_status launch_mix_prepare(const MixArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mix_prepare, dim3(blocks_for(std::max(a.n, 1u))), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_mix_prepare, dim3(blocks_for((uint64_t)std::max(a.n, 1u) * kMixLanes)),
+                     dim3(kBlock), 0, s, a);
   PSB_HIP(hipGetLastError());
   return PSB_OK;
 }
