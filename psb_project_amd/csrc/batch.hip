@@ -301,6 +301,11 @@ static psb_status run_iter(psb_batch b, int k) {
 // latencies of the step whose events carry parity `par`
 static psb_status read_latency(psb_batch b, int par) {
   PSB_HIP(hipEventSynchronize(b->ev_copy[par]));
+  // ev_copy precedes the step's retries; a redone scene's latency ends at
+  // ev_retry_end, which may still be ahead on the stream
+  const std::vector<char>& redo = b->lat_retried[par];
+  if (std::find(redo.begin(), redo.end(), (char)1) != redo.end())
+    PSB_HIP(hipEventSynchronize(b->ev_retry_end[par]));
   const PciStatus* hs = b->h_stat + (size_t)par * b->S;
   for (uint32_t s = 0; s < b->S; ++s) {
     if (!b->lat_retried[par].empty() && b->lat_retried[par][s]) {
